@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark of the PPI evaluation hot path on one MI355X.
+
+A step is one evaluation of the whole population over the whole dataset (one
+pass of the hot path: every program on every row, per-program error out).  The
+headline workload is the plain drop-in at scale ("regression": 30,000 seeded
+programs, depth <= 10, <= 128 symbols, over 100,000 x 16 N(0,1) rows, squared
+error, sum reduction).  Throughput is GPop/s = sum(size) * rows / seconds
+(ppi.cc:652), with programs and data already resident in HBM.
+
+Prints ONE JSON line.  Extra fields carry the other forms (PDP, PP) and the
+classification workload.  The CPU baseline is the oracle's restatement of the
+reference's sequential loop, timed on this host over a bounded sample.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ppi-auto_amd"))
+
+import ppiamd  # noqa: E402
+
+VALU_PEAK = 256 * 64 * 2.4e9      # non-packed FP32 VALU lane-ops/s: one op per node-evaluation
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_info():
+    model = platform.processor()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile=0, strategy=ppiamd.SEQ):
+    eng.configure(w.grammar.table, strategy=strategy, error_fn=w.error_fn, reducemax=w.reducemax,
+                  protected=w.protected, cost=w.cost, form=form, rows_per_lane=rows_per_lane,
+                  programs_per_tile=programs_per_tile)
+    eng.set_dataset(w.data)
+    eng.upload(w.pop)
+    if warmup > 0:
+        eng.time(warmup)
+    t0 = time.perf_counter()
+    ms = eng.time(steps)
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    return {"ms_per_step": ms, "wall_s": wall, "main_kernel_ms": st.main_kernel_ms,
+            "node_evals": st.node_evals, "executed_nodes": st.executed_nodes,
+            "rows_per_lane": st.rows_per_lane, "programs_per_tile": st.programs_per_tile,
+            "form": st.form}
+
+
+def cpu_baseline(w, budget_s, threads):
+    """Oracle (restated sequential path) on the first k programs x all rows."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle  # cpu_baseline leg only
+    cfg = oracle.make_cfg(ppiamd.SEQ, w.error_fn, w.reducemax, w.protected, w.cost)
+    k = 50
+    t, _ = oracle.time_seq(cfg, w.grammar.table, w.pop.subset(k), w.data, threads=threads)
+    rate = w.pop.subset(k).total_size * w.data.shape[0] / max(t, 1e-9)
+    k2 = int(min(w.pop.n, max(k, k * budget_s / max(t, 1e-6))))
+    sub = w.pop.subset(k2)
+    t2, _ = oracle.time_seq(cfg, w.grammar.table, sub, w.data, threads=threads)
+    rate = sub.total_size * w.data.shape[0] / t2
+    return rate, t2, k2
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline leg")
+    ap.add_argument("--quick", action="store_true", help="skip the extra forms and workloads")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    if ppiamd.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    log("building workload: regression (30,000 programs x 100,000 rows x 16 features)")
+    w = ppiamd.workload("regression")
+    eng = ppiamd.Engine(0)
+    log("timing FP form")
+    fp = bench_engine(eng, w, ppiamd.FORM_FP, a.steps, a.warmup)
+    value = fp["node_evals"] / (fp["ms_per_step"] * 1e-3) / 1e9
+    achieved = fp["node_evals"] / (fp["main_kernel_ms"] * 1e-3)
+    extra = {}
+    if not a.quick:
+        log("timing PDP form")
+        pdp = bench_engine(eng, w, ppiamd.FORM_PDP, max(1, a.steps // 5), 1)
+        extra["pdp_gpops"] = pdp["node_evals"] / (pdp["ms_per_step"] * 1e-3) / 1e9
+        log("timing PP form (2,000-program sample)")
+        wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(2000), w.data, w.error_fn)
+        pp = bench_engine(eng, wpp, ppiamd.FORM_PP, 1, 0)
+        extra["pp_gpops"] = pp["node_evals"] / (pp["ms_per_step"] * 1e-3) / 1e9
+        log("timing classification (COST, 30,000 programs x 150,000 rows)")
+        wc = ppiamd.workload("classification")
+        cl = bench_engine(eng, wc, ppiamd.FORM_FP, max(1, a.steps // 2), 1)
+        extra["classification_gpops"] = cl["node_evals"] / (cl["ms_per_step"] * 1e-3) / 1e9
+    eng.close()
+
+    log("CPU baseline (oracle restatement of seq_interpret)")
+    model, ncpu = host_info()
+    threads = min(ncpu or 1, 16)
+    cpu_rate, cpu_t, cpu_k = cpu_baseline(w, a.cpu_budget, threads)
+    seq_rate, seq_t, seq_k = cpu_baseline(w, a.cpu_budget, 1)
+
+    line = {
+        "metric": "GPop/s (program nodes x rows per second), plain drop-in evaluator",
+        "value": round(value, 3),
+        "unit": "GPop/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(fp["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded programs and dataset)",
+        "config": {"workload": "regression: 30,000 programs (depth<=10, <=128 symbols) x 100,000 rows x 16 "
+                               "features, EF=(X-Y)^2, sum, PROTECTED, sequential-path semantics",
+                   "form": "FP", "rows_per_lane": fp["rows_per_lane"],
+                   "programs_per_tile": fp["programs_per_tile"],
+                   "node_evals_per_step": fp["node_evals"]},
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4), "peak": VALU_PEAK / 1e12,
+                     "unit": "Tnode/s (1 node-evaluation = 1 FP32 VALU lane-op)",
+                     "frac": round(achieved / VALU_PEAK, 5), "traffic": None},
+        "cpu_baseline": {"value": round(cpu_rate / 1e9, 4), "unit": "GPop/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_k} programs x all 100,000 rows, {cpu_t:.1f} s, OpenMP over programs, "
+                                   f"host {model}"},
+        "cpu_baseline_1thread": {"value": round(seq_rate / 1e9, 4), "unit": "GPop/s", "cores": 1,
+                                 "sample": f"first {seq_k} programs x all rows, {seq_t:.1f} s"},
+        "vs_cpu": round(value / (cpu_rate / 1e9), 2),
+    }
+    line.update({k: round(v, 3) for k, v in extra.items()})
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

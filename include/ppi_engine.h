@@ -1,0 +1,186 @@
+/*
+ * ppi_engine.h — C ABI of the MI355X (gfx950) program-evaluation engine for PPI.
+ *
+ * The engine evaluates a population of grammar-generated prefix programs over a
+ * float32 dataset and returns one error per program, which is the hot path of
+ * PPI's src/interpreter (reference: src/interpreter/accelerator.{h,cc,cl} and
+ * src/interpreter/sequential.{h,cc}).  The C++ drop-in shim (acc_interpret_*,
+ * seq_interpret_*) is a thin layer over these entry points; see
+ * include/ppi_interpreter.h.
+ *
+ * Conventions
+ *   - Plain C types only; no exceptions cross the ABI.  Every entry point
+ *     returns a ppi_status; ppi_last_error() gives a message for the calling
+ *     thread.
+ *   - Host arrays are owned by the caller.  The engine copies what it needs.
+ *   - One engine object holds one dataset and one resident program set on one
+ *     device; it is not re-entrant (the reference keeps one static interpreter
+ *     per process, accelerator.cc:52).
+ */
+#ifndef PPI_ENGINE_H
+#define PPI_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPI_ABI_VERSION 1
+
+typedef enum ppi_status {
+    PPI_OK = 0,
+    PPI_E_ARG = 1,         /* invalid argument */
+    PPI_E_STATE = 2,       /* call out of order (no dataset, no programs, ...) */
+    PPI_E_PROGRAM = 3,     /* a program cannot be compiled (too many stack slots, ...) */
+    PPI_E_DEVICE = 4,      /* HIP runtime error */
+    PPI_E_NOMEM = 5,       /* host or device allocation failed */
+    PPI_E_UNSUPPORTED = 6, /* feature not available in this build */
+    PPI_E_NODEVICE = 7     /* no HIP device present */
+} ppi_status;
+
+/*
+ * Engine opcodes.  A grammar's Symbol values are mapped onto these by a symbol
+ * table (ppi_symbol).  Operator semantics follow src/interpreter/interpreter_core
+ * and src/interpreter/functions.h of the reference; operand order follows the
+ * interpreter's stack: for a binary operator the FIRST child in prefix order is
+ * the left operand (interpreter_core:48-51, stack[top] is the left operand).
+ */
+typedef enum ppi_op {
+    PPI_OP_UNKNOWN = 0,   /* unrecognised symbol: pushes NaN (sequential.cc:160-162) */
+    PPI_OP_CONST = 1,     /* T_CONST: pushes ephemeral[i] (sequential.cc:155-158) */
+    PPI_OP_ATTRIBUTE = 2, /* T_ATTRIBUTE: pushes input[row][(int)ephemeral[i]] (sequential.cc:150-153) */
+    PPI_OP_VALUE = 3,     /* named or integer constant (T_PI, T_0, T_N7, ...): pushes ppi_symbol.value */
+
+    /* ternary */
+    PPI_OP_IF_THEN_ELSE = 10,
+    /* binary */
+    PPI_OP_AND = 11, PPI_OP_OR = 12, PPI_OP_XOR = 13,
+    PPI_OP_GREATER = 14, PPI_OP_GREATEREQUAL = 15, PPI_OP_LESS = 16, PPI_OP_LESSEQUAL = 17,
+    PPI_OP_EQUAL = 18, PPI_OP_NOTEQUAL = 19,
+    PPI_OP_ADD = 20, PPI_OP_SUB = 21, PPI_OP_MULT = 22, PPI_OP_MEAN = 23,
+    PPI_OP_MAX = 24, PPI_OP_MIN = 25, PPI_OP_MOD = 26, PPI_OP_POW = 27, PPI_OP_DIV = 28,
+    /* unary */
+    PPI_OP_NOT = 40, PPI_OP_ABS = 41, PPI_OP_POW2 = 42, PPI_OP_POW3 = 43, PPI_OP_POW4 = 44,
+    PPI_OP_POW5 = 45, PPI_OP_NEG = 46, PPI_OP_ROUND = 47, PPI_OP_CEIL = 48, PPI_OP_FLOOR = 49,
+    PPI_OP_EXP = 50, PPI_OP_EXP10 = 51, PPI_OP_EXP2 = 52, PPI_OP_SIN = 53, PPI_OP_COS = 54,
+    PPI_OP_STEP = 55, PPI_OP_SIGN = 56, PPI_OP_LOGISTIC = 57, PPI_OP_GAMMA = 58, PPI_OP_GAUSS = 59,
+    PPI_OP_SQRT = 60, PPI_OP_LOG = 61, PPI_OP_LOG10 = 62, PPI_OP_LOG2 = 63, PPI_OP_TAN = 64,
+
+    PPI_OP_COUNT_ = 65
+} ppi_op;
+
+/* One symbol-table entry: grammar Symbol value -> engine opcode (+ value for PPI_OP_VALUE). */
+typedef struct ppi_symbol {
+    int32_t symbol;
+    int32_t op;
+    float value;
+} ppi_symbol;
+
+/* Per-row error function ERROR(X,Y) (definitions.h:18, CMakeLists.txt:19-33). */
+typedef enum ppi_error_fn {
+    PPI_EF_ABS = 0,        /* fabs((X)-(Y))                       (reference default) */
+    PPI_EF_SQUARED = 1,    /* ((X)-(Y))*((X)-(Y)) */
+    PPI_EF_NEQ = 2,        /* ((X)!=(Y))                          (problem/iris/README.md) */
+    PPI_EF_COST = 3,       /* COST[(int)X][(int)Y]                (default when COST is given) */
+    PPI_EF_COST_CLAMP = 4  /* COST[(int)fminf(fmaxf(X,0),N-1)][(int)Y] */
+} ppi_error_fn;
+
+/* Result semantics: which reference path's error vector and ranking to reproduce. */
+typedef enum ppi_strategy {
+    PPI_STRATEGY_SEQ = 0,  /* seq_interpret (sequential.cc:117-209) */
+    PPI_STRATEGY_PP = 1,   /* evaluate_pp + host epilogue (accelerator.cl:7-86, accelerator.cc:843-889) */
+    PPI_STRATEGY_FP = 2,   /* evaluate_dp ("DP", fitness-parallel) + host reduction (accelerator.cl:88-171, accelerator.cc:756-840) */
+    PPI_STRATEGY_PDP = 3   /* evaluate_pdp (accelerator.cl:173-268) */
+} ppi_strategy;
+
+/* Kernel form used to compute the result (any form can serve any strategy's semantics). */
+typedef enum ppi_form {
+    PPI_FORM_AUTO = 0,  /* autotuned */
+    PPI_FORM_PP = 1,    /* one lane per program, rows sequential */
+    PPI_FORM_FP = 2,    /* row tiles x program chunks, rows across lanes, programs wave-uniform */
+    PPI_FORM_PDP = 3    /* one workgroup per program, lanes over rows */
+} ppi_form;
+
+typedef struct ppi_config {
+    int32_t strategy;       /* ppi_strategy */
+    int32_t form;           /* ppi_form */
+    int32_t error_fn;       /* ppi_error_fn */
+    int32_t reducemax;      /* REDUCEMAX: max(EF*nlin) instead of the sum (definitions.h:21) */
+    int32_t protected_ops;  /* PROTECTED (functions.h:99-193) */
+    int32_t native_math;    /* NATIVE: fast float transcendentals instead of the sequential path's
+                               double-rounded ones (functions.h:19-57) */
+    int32_t cost_n;         /* COST matrix is cost_n x cost_n, row-major in cost[] */
+    const float* cost;
+    /* Reference launch geometry that shapes result semantics (accelerator.cc:272-355).
+       <= 0 selects what acc_interpret_init computes on this device. */
+    int32_t max_local_size; /* -cl-mls */
+    int32_t compute_units;  /* CL_DEVICE_MAX_COMPUTE_UNITS */
+    /* Tuning; 0 = autotune. */
+    int32_t rows_per_lane;  /* 1, 2, 4 or 8 */
+    int32_t programs_per_tile;
+} ppi_config;
+
+typedef struct ppi_stats {
+    double upload_ms;       /* host pack + H2D of the last program set */
+    double kernel_ms;       /* device time of the last evaluation (all kernels, HIP events) */
+    double main_kernel_ms;  /* device time of the evaluation kernel alone */
+    double download_ms;     /* D2H of the results */
+    double node_evals;      /* sum(size) * nlin of the last evaluation (ppi.cc:588,652) */
+    double executed_nodes;  /* tape instructions actually executed * rows */
+    int32_t form;           /* form that ran */
+    int32_t rows_per_lane;
+    int32_t programs_per_tile;
+    int32_t max_slots;      /* register stack slots used by the deepest program */
+} ppi_stats;
+
+typedef struct ppi_engine ppi_engine;
+
+/* Library / device management. */
+int32_t ppi_abi_version(void);
+const char* ppi_last_error(void);
+ppi_status ppi_device_count(int32_t* count);
+
+/* acc_interpret_init equivalent (accelerator.cc:557-648): create, configure, stage dataset. */
+ppi_status ppi_engine_create(ppi_engine** out, int32_t device);
+void ppi_engine_destroy(ppi_engine* eng);
+ppi_status ppi_engine_configure(ppi_engine* eng, const ppi_config* cfg,
+                                const ppi_symbol* table, int32_t ntable);
+/* Dataset as the reference receives it: float** input, input[row][col], y = column ncol-1
+   (main.cc:32-122, accelerator.cc:363-430).  Copied once. */
+ppi_status ppi_engine_set_dataset(ppi_engine* eng, const float* const* input, int32_t nlin, int32_t ncol);
+/* Same, from one row-major block of nlin*ncol floats. */
+ppi_status ppi_engine_set_dataset_rowmajor(ppi_engine* eng, const float* data, int32_t nlin, int32_t ncol);
+
+/* Program upload (the H2D half of acc_interpret, accelerator.cc:668-684): phenotype,
+   ephemeral and size exactly as ppi.cc:312-314 lays them out (program i occupies
+   [i*stride, i*stride + size[i])).  Programs are compiled into a packed device tape. */
+ppi_status ppi_engine_upload(ppi_engine* eng, const int32_t* phenotype, const float* ephemeral,
+                             const int32_t* size, int32_t stride, int32_t n_programs);
+
+/* Launch the evaluation of the resident programs (asynchronous; accelerator.cc:694-733). */
+ppi_status ppi_engine_launch(ppi_engine* eng);
+/* Wait for the launch and fetch results (accelerator.cc:743-889):
+     vector[i] = the strategy's error of program i including alpha*size[i];
+     if index != NULL and best_size != NULL: index[0..*best_size) = best programs by the
+     strategy's ranking rule (*best_size may be lowered as the reference does). */
+ppi_status ppi_engine_wait(ppi_engine* eng, float alpha, float* vector,
+                           int32_t* index, int32_t* best_size);
+/* upload + launch + wait. */
+ppi_status ppi_engine_evaluate(ppi_engine* eng, const int32_t* phenotype, const float* ephemeral,
+                               const int32_t* size, int32_t stride, int32_t n_programs, float alpha,
+                               float* vector, int32_t* index, int32_t* best_size);
+/* Prediction mode (ppp -pred; accelerator.cl:72-75): out[row] = value of program 0 on every row. */
+ppi_status ppi_engine_predict(ppi_engine* eng, const int32_t* phenotype, const float* ephemeral,
+                              int32_t size, float* out);
+ppi_status ppi_engine_get_stats(const ppi_engine* eng, ppi_stats* out);
+
+/* Device-resident timing loop used by benchmarks: run `iters` evaluations of the resident
+   programs back to back on the engine stream, return mean device ms per evaluation. */
+ppi_status ppi_engine_time(ppi_engine* eng, int32_t iters, double* ms_per_eval);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPI_ENGINE_H */

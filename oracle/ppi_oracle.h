@@ -1,0 +1,79 @@
+/*
+ * ppi_oracle.h — CPU oracle for the PPI evaluation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
+ * algorithm (src/interpreter/sequential.cc, accelerator.cl, accelerator.cc,
+ * interpreter_core, functions.h, util/Util.h) used to check the HIP engine.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The engine never links or calls it.
+ *
+ * Programs use the same table-driven encoding as the engine (ppi_symbol in
+ * include/ppi_engine.h): a grammar Symbol value is looked up in the table to
+ * find its opcode.
+ */
+#ifndef PPI_ORACLE_H
+#define PPI_ORACLE_H
+
+#include <stdint.h>
+#include "../include/ppi_engine.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_cfg {
+    int32_t strategy;      /* ppi_strategy */
+    int32_t error_fn;      /* ppi_error_fn */
+    int32_t reducemax;
+    int32_t protected_ops;
+    int32_t cost_n;
+    const float* cost;
+    int32_t dp_local;      /* evaluate_dp work-group size (accelerator.cc:304-313) */
+    int32_t pdp_local;     /* evaluate_pdp work-group size (accelerator.cc:316-329) */
+    int32_t best_local;    /* best_individual work-group size (accelerator.cc:344-355) */
+} oracle_cfg;
+
+/* Error vector + ranking, exactly as the reference path named by cfg->strategy.
+   data is row-major nlin x ncol with y in column ncol-1.  Returns 0, or -1 on bad input. */
+int oracle_interpret(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                     const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                     int32_t stride, int32_t n_programs,
+                     const float* data, int32_t nlin, int32_t ncol,
+                     float alpha, float* vector, int32_t* index, int32_t* best_size,
+                     int32_t ppp_mode, int32_t prediction_mode);
+
+/* Value of one program on one row (float, reference semantics). */
+float oracle_eval_row(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                      const int32_t* phenotype, const float* ephemeral, int32_t size,
+                      const float* row, int32_t ncol);
+
+/* util::PickNBest (Util.h:158-173): best[k] = k-th smallest error, ties -> largest index. */
+void oracle_pick_n_best(int32_t n, int32_t* best, int32_t num, const float* errors, const int32_t* indices);
+
+/* FP64 evaluation: fitness64[i] = sum_rows EF64 / nlin (or max) with every operator in double,
+   non-finite handling as the sequential path.  alpha*size is NOT added. */
+int oracle_fitness_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                       const int32_t* phenotype, const double* ephemeral64, const int32_t* size,
+                       int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
+                       double* fitness64);
+
+/* FP64 forward-mode gradient of fitness64 (sum EF / nlin) with respect to every T_CONST of each
+   program, in increasing phenotype position.  grad is n_programs x max_consts; n_consts[i] is
+   the number of constants found (extra ones beyond max_consts are ignored). */
+int oracle_grad_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                    const int32_t* phenotype, const double* ephemeral64, const int32_t* size,
+                    int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
+                    int32_t max_consts, double* fitness64, double* grad, int32_t* n_consts);
+
+/* The sequential path timed on `threads` host threads (threads <= 1: the reference's own
+   single-threaded loop).  Returns elapsed seconds, or a negative value on error. */
+double oracle_time_seq(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                       const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                       int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
+                       float alpha, float* vector, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

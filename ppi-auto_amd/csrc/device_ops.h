@@ -1,0 +1,177 @@
+// device_ops.h — operator semantics on gfx950.
+//
+// EXACT mode reproduces the reference's sequential path, which is the oracle:
+// sequential.cc includes functions.h and interpreter_core in a C++ unit whose
+// sin/cos/exp/log/pow/fmod/sinh/tan/exp10/exp2/log10/log2 calls resolve to the
+// double routines, so every transcendental is evaluated in double and rounded
+// to float when it is stored on the float stack.  Here the same happens with the
+// ROCm device library's double routines.  Basic arithmetic is float with no FMA
+// contraction (the file is compiled with -ffp-contract=off).
+//
+// NATIVE mode mirrors the reference's -DNATIVE OpenCL build (functions.h:19-57,
+// native_* built-ins): fast float intrinsics, not bit-compatible with the
+// sequential path.
+#ifndef PPI_DEVICE_OPS_H
+#define PPI_DEVICE_OPS_H
+
+#include <hip/hip_runtime.h>
+
+#include "tape.h"
+
+namespace ppi {
+namespace dev {
+
+// Float constants of the reference's symbol header / functions.h:19-57.
+constexpr float kPI_F = 3.14159274101257f;
+constexpr float kPI_2_F = 1.57079637050629f;
+constexpr float kE_F = 2.71828174591064f;
+
+__device__ __forceinline__ float b2f(bool b) { return b ? 1.0f : 0.0f; }
+
+template <bool N> __device__ __forceinline__ float f_exp(float x) {
+    if constexpr (N) return __expf(x);
+    else return (float)exp((double)x);
+}
+template <bool N> __device__ __forceinline__ float f_exp10(float x) {
+    if constexpr (N) return __exp10f(x);
+    else return (float)exp10((double)x);
+}
+template <bool N> __device__ __forceinline__ float f_exp2(float x) {
+    if constexpr (N) return exp2f(x);
+    else return (float)exp2((double)x);
+}
+template <bool N> __device__ __forceinline__ float f_sin(float x) {
+    if constexpr (N) return __sinf(x);
+    else return (float)sin((double)x);
+}
+template <bool N> __device__ __forceinline__ float f_cos(float x) {
+    if constexpr (N) return __cosf(x);
+    else return (float)cos((double)x);
+}
+// functions.h:115-129 (SQRT): float(sqrt(double(x))) == sqrtf(x) exactly.
+template <bool P, bool N> __device__ __forceinline__ float f_sqrt(float x) {
+    if (P && x < 0.0f) return 1.0f;
+    return sqrtf(x);
+}
+// functions.h:131-177 (LOG, LOG10, LOG2).
+template <bool P, bool N> __device__ __forceinline__ float f_log(float x) {
+    if (P && x < 1.0f) return 1.0f;
+    if constexpr (N) return __logf(x);
+    else return (float)log((double)x);
+}
+template <bool P, bool N> __device__ __forceinline__ float f_log10(float x) {
+    if (P && x < 1.0f) return 1.0f;
+    if constexpr (N) return __log10f(x);
+    else return (float)log10((double)x);
+}
+template <bool P, bool N> __device__ __forceinline__ float f_log2(float x) {
+    if (P && x < 1.0f) return 1.0f;
+    if constexpr (N) return __log2f(x);
+    else return (float)log2((double)x);
+}
+// functions.h:99-113 (DIV).
+template <bool P, bool N> __device__ __forceinline__ float f_div(float x, float y) {
+    if (P && y == 0.0f) return 1.0f;
+    if constexpr (N) return __fdividef(x, y);
+    else return x / y;
+}
+// functions.h:179-193 (TAN).
+template <bool P, bool N> __device__ __forceinline__ float f_tan(float x) {
+    if (P && (fabsf(x) == kPI_2_F || fabsf(x) == 3.0f * kPI_2_F)) return 1.0f;
+    if constexpr (N) return __tanf(x);
+    else return (float)tan((double)x);
+}
+template <bool N> __device__ __forceinline__ float f_pow(float x, float y) {
+    if constexpr (N) return __powf(x, y);
+    else return (float)pow((double)x, (double)y);
+}
+
+// interpreter_core:1-151, unary operators.
+template <bool P, bool N> __device__ __forceinline__ float unary(uint32_t u, float a) {
+    switch (u) {
+    case U_NOT: return b2f(!(a != 0.0f));
+    case U_ABS: return fabsf(a);
+    case U_POW2: return a * a;
+    case U_POW3: return a * a * a;
+    case U_POW4: return a * a * a * a;
+    case U_POW5: return a * a * a * a * a;
+    case U_NEG: return -a;
+    case U_ROUND: return roundf(a);
+    case U_CEIL: return ceilf(a);
+    case U_FLOOR: return floorf(a);
+    case U_EXP: return f_exp<N>(a);
+    case U_EXP10: return f_exp10<N>(a);
+    case U_EXP2: return f_exp2<N>(a);
+    case U_SIN: return f_sin<N>(a);
+    case U_COS: return f_cos<N>(a);
+    case U_STEP: return b2f(a >= 0.0f);
+    case U_SIGN: return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f);
+    case U_LOGISTIC: return 1.0f / (1.0f + f_exp<N>(-a));
+    case U_GAMMA: {
+        // interpreter_core:127-129: sinh and pow are double routines in the reference.
+        float s1;
+        if constexpr (N) s1 = f_sqrt<P, N>(a * sinhf(1.0f / a));
+        else s1 = f_sqrt<P, N>((float)((double)a * sinh((double)(1.0f / a))));
+        float base = (a / kE_F) * s1;
+        float s2 = f_sqrt<P, N>(2.0f * kPI_F / a);
+        if constexpr (N) return __powf(base, a) * s2;
+        else return (float)(pow((double)base, (double)a) * (double)s2);
+    }
+    case U_GAUSS: return f_exp<N>(-a * a);
+    case U_SQRT: return f_sqrt<P, N>(a);
+    case U_LOG: return f_log<P, N>(a);
+    case U_LOG10: return f_log10<P, N>(a);
+    case U_LOG2: return f_log2<P, N>(a);
+    default: return f_tan<P, N>(a);
+    }
+}
+
+// Binary operators; `a` is the accumulator, `b` the folded or parked operand.
+// Non-R forms compute f(a, b); R forms compute f(b, a).
+template <bool P, bool N> __device__ __forceinline__ float binary(uint32_t op, float a, float b) {
+    switch (op) {
+    case B_ADD: return a + b;
+    case B_SUB: return a - b;
+    case B_RSUB: return b - a;
+    case B_MULT: return a * b;
+    case B_DIV: return f_div<P, N>(a, b);
+    case B_RDIV: return f_div<P, N>(b, a);
+    case B_MEAN: return (a + b) / 2.0f;
+    case B_MAX: return fmaxf(a, b);
+    case B_RMAX: return fmaxf(b, a);
+    case B_MIN: return fminf(a, b);
+    case B_RMIN: return fminf(b, a);
+    case B_MOD: return fmodf(a, b);
+    case B_RMOD: return fmodf(b, a);
+    case B_POW: return f_pow<N>(a, b);
+    case B_RPOW: return f_pow<N>(b, a);
+    case B_AND: return b2f((a != 0.0f) && (b != 0.0f));
+    case B_OR: return b2f((a != 0.0f) || (b != 0.0f));
+    case B_XOR: return b2f((a != 0.0f) != (b != 0.0f));
+    case B_GT: return b2f(a > b);
+    case B_GE: return b2f(a >= b);
+    case B_LT: return b2f(a < b);
+    case B_LE: return b2f(a <= b);
+    case B_EQ: return b2f(a == b);
+    default: return b2f(a != b);
+    }
+}
+
+// ERROR(X,Y) (definitions.h:18) for the built-in error functions.
+__device__ __forceinline__ float error_fn(int ef, float X, float Y, const float* cost, int n) {
+    switch (ef) {
+    case PPI_EF_ABS: return fabsf(X - Y);
+    case PPI_EF_SQUARED: return (X - Y) * (X - Y);
+    case PPI_EF_NEQ: return b2f(X != Y);
+    default: {
+        float xx = (ef == PPI_EF_COST_CLAMP) ? fminf(fmaxf(X, 0.0f), (float)(n - 1)) : X;
+        bool ok = (xx > -1.0f) && (xx < (float)n) && (Y > -1.0f) && (Y < (float)n);
+        return ok ? cost[(int)xx * n + (int)Y] : __builtin_nanf("");
+    }
+    }
+}
+
+}  // namespace dev
+}  // namespace ppi
+
+#endif

@@ -1,0 +1,564 @@
+// engine.cc — runtime of the PPI evaluation engine and its C ABI (include/ppi_engine.h).
+//
+// Replaces the OpenCL host of the reference (src/interpreter/accelerator.cc):
+// device selection, dataset staging (once), per-generation program upload,
+// kernel launch, result download and ranking.  One engine = one HIP device,
+// one stream, one dataset, one resident program set.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ppi_engine.h"
+#include "engine_internal.h"
+#include "kernels.h"
+#include "tape.h"
+
+namespace ppi {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+#define PPI_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                      \
+            return PPI_E_DEVICE;                                                               \
+        }                                                                                      \
+    } while (0)
+
+template <typename T>
+static ppi_status ensure(T** p, size_t* cap, size_t n) {
+    if (n <= *cap && *p) return PPI_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t want = std::max<size_t>(n, 1);
+    hipError_t e = hipMalloc((void**)p, want * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc(") + std::to_string(want * sizeof(T)) + " B): " + hipGetErrorString(e));
+        *p = nullptr;
+        return PPI_E_NOMEM;
+    }
+    *cap = want;
+    return PPI_OK;
+}
+
+template <typename T>
+static ppi_status ensure_host(T** p, size_t* cap, size_t n) {
+    if (n <= *cap && *p) return PPI_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t want = std::max<size_t>(n, 1);
+    hipError_t e = hipHostMalloc((void**)p, want * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        *p = nullptr;
+        return PPI_E_NOMEM;
+    }
+    *cap = want;
+    return PPI_OK;
+}
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// util::PickNBest (Util.h:158-173): k-th best = k-th smallest error, ties to the
+// larger index; `indices` maps positions to program indices when given.
+void pick_n_best(int n, int32_t* best, int num, const float* errors, const int32_t* indices) {
+    std::vector<int32_t> ord((size_t)num);
+    std::iota(ord.begin(), ord.end(), 0);
+    auto better = [&](int32_t a, int32_t b) {
+        if (errors[a] < errors[b]) return true;
+        if (errors[b] < errors[a]) return false;
+        return a > b;
+    };
+    n = std::min(n, num);
+    std::partial_sort(ord.begin(), ord.begin() + n, ord.end(), better);
+    for (int k = 0; k < n; ++k) best[k] = indices ? indices[ord[k]] : ord[k];
+}
+
+// best_individual (accelerator.cl:270-313) over groups of `local2`, then host
+// PickNBest over the group winners (accelerator.cc:866-883).
+void best_acc(const float* vec, int P, int local2, int32_t* index, int32_t* best_size) {
+    int ng = (P + local2 - 1) / local2;
+    std::vector<float> PB((size_t)ng), lb((size_t)local2);
+    std::vector<int32_t> PI((size_t)ng), li((size_t)local2);
+    int np2 = 1;
+    while (np2 < local2) np2 <<= 1;
+    for (int g = 0; g < ng; ++g) {
+        for (int l = 0; l < local2; ++l) {
+            int gl = g * local2 + l;
+            if (gl < P) { lb[l] = vec[gl]; li[l] = gl; }
+            else { lb[l] = FLT_MAX; li[l] = -1; }
+        }
+        for (int s = np2 / 2; s > 0; s /= 2)
+            for (int l = 0; l < s; ++l)
+                if (l + s < local2 && lb[l] > lb[l + s]) { lb[l] = lb[l + s]; li[l] = li[l + s]; }
+        PB[g] = lb[0];
+        PI[g] = li[0];
+    }
+    if (*best_size > ng) *best_size = ng;
+    pick_n_best(*best_size, index, ng, PB.data(), PI.data());
+}
+
+}  // namespace ppi
+
+using namespace ppi;
+
+// Geometry the reference's acc_interpret_init computes for this device (accelerator.cc:272-355).
+static void reference_geometry(ppi_engine* E) {
+    const int cu = E->cfg.compute_units > 0 ? E->cfg.compute_units : E->dev_cu;
+    int mls1 = E->cfg.max_local_size > 0 ? E->cfg.max_local_size
+                                          : (int)std::min<size_t>((size_t)E->dev_max_wg, E->dev_lds / 4);
+    int mls2 = (int)std::min<size_t>((size_t)E->dev_max_wg, E->dev_lds / 8);
+    if (E->nlin > 0) {
+        E->dp_local = std::max(1, std::min(mls1, (int)std::ceil(E->nlin / (float)cu)));
+        E->dp_groups = (E->nlin + E->dp_local - 1) / E->dp_local;
+        E->pdp_local = E->nlin < mls1 ? E->nlin : mls1;
+    }
+    if (E->nprog > 0) E->best_local = std::max(1, std::min(mls2, (int)std::ceil(E->nprog / (float)cu)));
+}
+
+static bool plan(ppi_engine* E, std::string* why) {
+    const CompileStats& cs = E->tape.stats;
+    int need_f = cs.max_col + 1;
+    int F = need_f <= 8 ? 8 : (need_f <= 16 ? 16 : (need_f <= 32 ? 32 : 0));
+    int S = cs.max_slots <= 8 ? 8 : 16;
+    if (cs.max_slots > 16) {
+        *why = "program needs more than 16 register slots";
+        return false;
+    }
+    int form = E->cfg.form;
+    if (form == PPI_FORM_AUTO) form = PPI_FORM_FP;
+    int R = E->cfg.rows_per_lane > 0 ? E->cfg.rows_per_lane : (form == PPI_FORM_PDP ? 2 : 4);
+    auto shape_supported = [form](int r, int f, int s) {
+        return form == PPI_FORM_PDP ? pdp_shape_supported(r, f, s) : ppi::shape_supported(r, f, s);
+    };
+    if (!shape_supported(R, F, S)) {
+        // Nearest supported shape: keep F and S, lower R; then widen F.
+        const int Rs[4] = {8, 4, 2, 1};
+        bool found = false;
+        for (int fi = 0; fi < 2 && !found; ++fi) {
+            int FF = fi == 0 ? F : (F == 0 ? 0 : 32);
+            for (int k = 0; k < 4 && !found; ++k)
+                if (Rs[k] <= R && shape_supported(Rs[k], FF, S)) { R = Rs[k]; F = FF; found = true; }
+            for (int k = 3; k >= 0 && !found; --k)
+                if (shape_supported(Rs[k], FF, S)) { R = Rs[k]; F = FF; found = true; }
+        }
+        if (!found && shape_supported(1, 0, 16)) { R = 1; F = 0; S = 16; found = true; }
+        if (!found) { *why = "no kernel shape"; return false; }
+    }
+    E->form = form;
+    E->R = R;
+    E->F = F;
+    E->S = S;
+    // Programs per FP tile: enough workgroups to fill 256 CUs several times over.
+    int64_t rows_per_block = 256LL * R;
+    int64_t nrb = (E->nlin + rows_per_block - 1) / rows_per_block;
+    int chunk = E->cfg.programs_per_tile;
+    if (chunk <= 0) {
+        int64_t target = (int64_t)E->dev_cu * 16;
+        int64_t per = (E->nprog * nrb + target - 1) / target;
+        chunk = (int)std::max<int64_t>(8, std::min<int64_t>(kMaxChunk, per));
+    }
+    E->chunk = std::min(chunk, kMaxChunk);
+    E->nrb = (int)nrb;
+    return true;
+}
+
+extern "C" {
+
+int32_t ppi_abi_version(void) { return PPI_ABI_VERSION; }
+
+const char* ppi_last_error(void) { return g_last_error.c_str(); }
+
+ppi_status ppi_device_count(int32_t* count) {
+    if (!count) { set_error("count is NULL"); return PPI_E_ARG; }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; set_error(hipGetErrorString(e)); return PPI_E_NODEVICE; }
+    *count = n;
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_create(ppi_engine** out, int32_t device) {
+    if (!out) { set_error("out is NULL"); return PPI_E_ARG; }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device");
+        return PPI_E_NODEVICE;
+    }
+    if (device < 0) device = 0;
+    if (device >= n) {
+        set_error("valid device range: [0, " + std::to_string(n - 1) + "]");
+        return PPI_E_ARG;
+    }
+    ppi_engine* E = new ppi_engine();
+    E->device = device;
+    hipError_t e = hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&E->ev[k]);
+    if (e != hipSuccess) {
+        set_error(std::string("device init: ") + hipGetErrorString(e));
+        delete E;
+        return PPI_E_DEVICE;
+    }
+    E->dev_cu = prop.multiProcessorCount;
+    E->dev_max_wg = prop.maxThreadsPerBlock;
+    E->dev_lds = prop.sharedMemPerBlock;
+    E->arch = prop.gcnArchName;
+    std::memset(&E->cfg, 0, sizeof(E->cfg));
+    E->cfg.strategy = PPI_STRATEGY_SEQ;
+    E->cfg.error_fn = PPI_EF_ABS;
+    *out = E;
+    return PPI_OK;
+}
+
+void ppi_engine_destroy(ppi_engine* E) {
+    if (!E) return;
+    (void)hipSetDevice(E->device);
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    void* dev[] = {E->dX, E->dMeta, E->dCode, E->dPartial, E->dBadLo, E->dBadHi, E->dRaw, E->dVec,
+                   E->dOrder, E->dCost, E->dPred};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    if (E->hRaw) (void)hipHostFree(E->hRaw);
+    if (E->hVec) (void)hipHostFree(E->hVec);
+    for (auto& ev : E->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+}
+
+ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable) {
+    if (!E || !cfg) { set_error("engine or config is NULL"); return PPI_E_ARG; }
+    if (cfg->strategy < PPI_STRATEGY_SEQ || cfg->strategy > PPI_STRATEGY_PDP) { set_error("bad strategy"); return PPI_E_ARG; }
+    if (cfg->form < PPI_FORM_AUTO || cfg->form > PPI_FORM_PDP) { set_error("bad form"); return PPI_E_ARG; }
+    if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
+    bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
+    if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs cost_n > 0 and cost"); return PPI_E_ARG; }
+    if (cfg->rows_per_lane != 0 && cfg->rows_per_lane != 1 && cfg->rows_per_lane != 2 && cfg->rows_per_lane != 4 &&
+        cfg->rows_per_lane != 8) {
+        set_error("rows_per_lane must be 0, 1, 2, 4 or 8");
+        return PPI_E_ARG;
+    }
+    if (cfg->programs_per_tile < 0 || cfg->programs_per_tile > kMaxChunk) {
+        set_error("programs_per_tile must be in [0, " + std::to_string(kMaxChunk) + "]");
+        return PPI_E_ARG;
+    }
+    if (ntable < 0 || (ntable > 0 && !table)) { set_error("bad symbol table"); return PPI_E_ARG; }
+    for (int32_t k = 0; k < ntable; ++k) {
+        int op = table[k].op;
+        bool ok = (op >= PPI_OP_UNKNOWN && op <= PPI_OP_VALUE) || (op >= PPI_OP_IF_THEN_ELSE && op <= PPI_OP_DIV) ||
+                  (op >= PPI_OP_NOT && op <= PPI_OP_TAN);
+        if (!ok) { set_error("symbol table entry " + std::to_string(k) + ": unknown opcode " + std::to_string(op)); return PPI_E_ARG; }
+    }
+    (void)hipSetDevice(E->device);
+    E->cfg = *cfg;
+    E->cost.assign(cfg->cost, cfg->cost + (cost ? (size_t)cfg->cost_n * cfg->cost_n : 0));
+    E->cfg.cost = nullptr;
+    if (cost) {
+        size_t cap = E->cap_cost;
+        ppi_status s = ensure(&E->dCost, &cap, E->cost.size());
+        if (s) return s;
+        E->cap_cost = cap;
+        PPI_HIP(hipMemcpy(E->dCost, E->cost.data(), E->cost.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    E->symtab.build(table, ntable);
+    E->configured = true;
+    E->nprog = 0;  // programs compile against the table: re-upload after reconfiguring
+    reference_geometry(E);
+    return PPI_OK;
+}
+
+static ppi_status stage_dataset(ppi_engine* E, int32_t nlin, int32_t ncol, const float* const* rows, const float* block) {
+    if (nlin <= 0 || ncol <= 0) { set_error("dataset needs nlin > 0 and ncol > 0"); return PPI_E_ARG; }
+    (void)hipSetDevice(E->device);
+    const int64_t ldx = ((int64_t)nlin + 2047) / 2048 * 2048;
+    std::vector<float> colmajor((size_t)ldx * ncol, 0.0f);
+    for (int32_t c = 0; c < ncol; ++c) {
+        float* dst = colmajor.data() + (size_t)c * ldx;
+        for (int32_t r = 0; r < nlin; ++r) dst[r] = rows ? rows[r][c] : block[(size_t)r * ncol + c];
+    }
+    size_t cap = E->cap_X;
+    ppi_status s = ensure(&E->dX, &cap, colmajor.size());
+    if (s) return s;
+    E->cap_X = cap;
+    PPI_HIP(hipMemcpy(E->dX, colmajor.data(), colmajor.size() * sizeof(float), hipMemcpyHostToDevice));
+    E->nlin = nlin;
+    E->ncol = ncol;
+    E->ldx = ldx;
+    E->nprog = 0;  // attribute bounds depend on ncol: re-upload programs
+    reference_geometry(E);
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_set_dataset(ppi_engine* E, const float* const* input, int32_t nlin, int32_t ncol) {
+    if (!E || !input) { set_error("engine or input is NULL"); return PPI_E_ARG; }
+    return stage_dataset(E, nlin, ncol, input, nullptr);
+}
+
+ppi_status ppi_engine_set_dataset_rowmajor(ppi_engine* E, const float* data, int32_t nlin, int32_t ncol) {
+    if (!E || !data) { set_error("engine or data is NULL"); return PPI_E_ARG; }
+    return stage_dataset(E, nlin, ncol, nullptr, data);
+}
+
+ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                             int32_t stride, int32_t n) {
+    if (!E) { set_error("engine is NULL"); return PPI_E_ARG; }
+    if (!E->configured) { set_error("configure the engine first"); return PPI_E_STATE; }
+    if (!E->dX) { set_error("set a dataset first"); return PPI_E_STATE; }
+    if (n <= 0 || stride <= 0 || !phenotype || !ephemeral || !size) { set_error("bad program arrays"); return PPI_E_ARG; }
+    (void)hipSetDevice(E->device);
+    double t0 = now_ms();
+    std::string err;
+    int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (!compile_population_mt(E->symtab, phenotype, ephemeral, size, stride, n, E->ncol, false, threads, &E->tape, &err)) {
+        set_error(err);
+        return PPI_E_PROGRAM;
+    }
+    E->tape.code.push_back(Ins{H_LOAD_C, 0});  // look-ahead padding
+    E->nprog = n;
+    reference_geometry(E);
+    std::string why;
+    if (!plan(E, &why)) { set_error(why); return PPI_E_PROGRAM; }
+    ppi_status s;
+    size_t cap;
+    cap = E->cap_meta; if ((s = ensure(&E->dMeta, &cap, E->tape.meta.size()))) return s; E->cap_meta = cap;
+    cap = E->cap_code; if ((s = ensure(&E->dCode, &cap, E->tape.code.size()))) return s; E->cap_code = cap;
+    cap = E->cap_raw; if ((s = ensure(&E->dRaw, &cap, (size_t)n))) return s;
+    cap = E->cap_raw; if ((s = ensure(&E->dVec, &cap, (size_t)n))) return s; E->cap_raw = cap;
+    cap = E->cap_bad; if ((s = ensure(&E->dBadLo, &cap, (size_t)n))) return s;
+    cap = E->cap_bad; if ((s = ensure(&E->dBadHi, &cap, (size_t)n))) return s; E->cap_bad = cap;
+    cap = E->cap_hres; if ((s = ensure_host(&E->hRaw, &cap, (size_t)n))) return s;
+    cap = E->cap_hres; if ((s = ensure_host(&E->hVec, &cap, (size_t)n))) return s; E->cap_hres = cap;
+    if (E->form == PPI_FORM_FP) {
+        cap = E->cap_partial;
+        if ((s = ensure(&E->dPartial, &cap, (size_t)E->nrb * n))) return s;
+        E->cap_partial = cap;
+    }
+    PPI_HIP(hipMemcpyAsync(E->dMeta, E->tape.meta.data(), E->tape.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice, E->stream));
+    PPI_HIP(hipMemcpyAsync(E->dCode, E->tape.code.data(), E->tape.code.size() * sizeof(Ins), hipMemcpyHostToDevice, E->stream));
+    if (E->form == PPI_FORM_PP) {
+        std::vector<int32_t> order((size_t)n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return E->tape.meta[a].ncode > E->tape.meta[b].ncode;
+        });
+        cap = E->cap_order;
+        if ((s = ensure(&E->dOrder, &cap, (size_t)n))) return s;
+        E->cap_order = cap;
+        PPI_HIP(hipMemcpyAsync(E->dOrder, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, E->stream));
+    }
+    PPI_HIP(hipStreamSynchronize(E->stream));
+    E->stats.upload_ms = now_ms() - t0;
+    E->stats.node_evals = (double)E->tape.stats.total_size * E->nlin;
+    E->stats.executed_nodes = (double)E->tape.stats.total_code * E->nlin;
+    E->stats.max_slots = E->tape.stats.max_slots;
+    return PPI_OK;
+}
+
+static EvalArgs make_args(ppi_engine* E, float alpha) {
+    EvalArgs A{};
+    A.meta = E->dMeta;
+    A.code = E->dCode;
+    A.X = E->dX;
+    A.ldx = E->ldx;
+    A.nlin = E->nlin;
+    A.ncol = E->ncol;
+    A.ycol = E->ncol - 1;
+    A.nprog = E->nprog;
+    A.chunk = E->chunk;
+    A.strategy = E->cfg.strategy;
+    A.error_fn = E->cfg.error_fn;
+    A.reducemax = E->cfg.reducemax;
+    A.cost = E->dCost;
+    A.cost_n = E->cfg.cost_n;
+    A.pdp_lanes = std::max(1, E->pdp_local);
+    A.dp_groups = E->dp_groups;
+    A.alpha = alpha;
+    A.order = E->dOrder;
+    A.partial = E->dPartial;
+    A.bad_lo = E->dBadLo;
+    A.bad_hi = E->dBadHi;
+    A.raw = E->dRaw;
+    A.vec = E->dVec;
+    return A;
+}
+
+// Enqueue one evaluation of the resident programs on the engine stream.
+static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
+    EvalArgs A = make_args(E, alpha);
+    const bool P = E->cfg.protected_ops != 0, N = E->cfg.native_math != 0;
+    PPI_HIP(launch_fill_u32(E->dBadLo, 0xffffffffu, E->nprog, E->stream));
+    PPI_HIP(launch_fill_u32(E->dBadHi, 0u, E->nprog, E->stream));
+    if (time_main) PPI_HIP(hipEventRecord(E->ev[2], E->stream));
+    if (E->form == PPI_FORM_FP) {
+        PPI_HIP(launch_fp(A, E->R, E->F, E->S, P, N, E->stream));
+        if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
+        PPI_HIP(launch_finalize(A, E->nrb, E->stream));
+    } else if (E->form == PPI_FORM_PDP) {
+        PPI_HIP(launch_pdp(A, E->R, E->F, E->S, P, N, E->stream));
+        if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
+    } else {
+        PPI_HIP(launch_pp(A, P, N, E->stream));
+        if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
+    }
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_launch(ppi_engine* E) {
+    if (!E) { set_error("engine is NULL"); return PPI_E_ARG; }
+    if (E->nprog <= 0) { set_error("upload programs first"); return PPI_E_STATE; }
+    (void)hipSetDevice(E->device);
+    PPI_HIP(hipEventRecord(E->ev[0], E->stream));
+    ppi_status s = enqueue_eval(E, 0.0f, true);
+    if (s) return s;
+    PPI_HIP(hipMemcpyAsync(E->hRaw, E->dRaw, (size_t)E->nprog * sizeof(float), hipMemcpyDeviceToHost, E->stream));
+    PPI_HIP(hipEventRecord(E->ev[1], E->stream));
+    E->launched = true;
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_wait(ppi_engine* E, float alpha, float* vector, int32_t* index, int32_t* best_size) {
+    if (!E) { set_error("engine is NULL"); return PPI_E_ARG; }
+    if (!E->launched) { set_error("nothing launched"); return PPI_E_STATE; }
+    E->launched = false;
+    PPI_HIP(hipEventSynchronize(E->ev[1]));
+    float ms = 0.0f;
+    PPI_HIP(hipEventElapsedTime(&ms, E->ev[0], E->ev[1]));
+    E->stats.kernel_ms = ms;
+    PPI_HIP(hipEventElapsedTime(&ms, E->ev[2], E->ev[3]));
+    E->stats.main_kernel_ms = ms;
+    E->stats.form = E->form;
+    E->stats.rows_per_lane = E->R;
+    E->stats.programs_per_tile = E->chunk;
+    double t0 = now_ms();
+    const int n = E->nprog;
+    // The kernels ran with alpha = 0; the alpha*size term is added here in the
+    // reference's own order and precision (host epilogues, accelerator.cc:801,849).
+    const float fn = (float)E->nlin;
+    for (int i = 0; i < n; ++i) {
+        const float raw = E->hRaw[i];
+        const ProgMeta& m = E->tape.meta[i];
+        const float as = alpha * (float)m.size;
+        float v;
+        switch (E->cfg.strategy) {
+        case PPI_STRATEGY_PP:
+        case PPI_STRATEGY_PDP:
+            v = raw + as;
+            break;
+        default:
+            // SEQ / FP: the kernel wrote sum/nlin + 0*size (or FLT_MAX); recover the
+            // alpha-inclusive value exactly as sum/nlin + alpha*size.
+            v = (raw == FLT_MAX || (m.flags & kEmpty)) ? raw : raw + as;
+            break;
+        }
+        E->hVec[i] = v;
+        (void)fn;
+    }
+    if (vector) std::memcpy(vector, E->hVec, (size_t)n * sizeof(float));
+    if (index && best_size && *best_size > 0) {
+        if (E->cfg.strategy == PPI_STRATEGY_SEQ) {
+            if (*best_size > n) *best_size = n;
+            pick_n_best(*best_size, index, n, E->hVec, nullptr);
+        } else if (E->cfg.strategy == PPI_STRATEGY_FP) {
+            best_acc(E->hVec, n, E->best_local, index, best_size);
+        } else {
+            best_acc(E->hRaw, n, E->best_local, index, best_size);
+        }
+    }
+    E->stats.download_ms = now_ms() - t0;
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_evaluate(ppi_engine* E, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                               int32_t stride, int32_t n, float alpha, float* vector, int32_t* index,
+                               int32_t* best_size) {
+    ppi_status s = ppi_engine_upload(E, phenotype, ephemeral, size, stride, n);
+    if (s) return s;
+    s = ppi_engine_launch(E);
+    if (s) return s;
+    return ppi_engine_wait(E, alpha, vector, index, best_size);
+}
+
+ppi_status ppi_engine_predict(ppi_engine* E, const int32_t* phenotype, const float* ephemeral, int32_t size, float* out) {
+    if (!E || !phenotype || !ephemeral || !out || size < 0) { set_error("bad arguments"); return PPI_E_ARG; }
+    if (!E->configured || !E->dX) { set_error("configure the engine and set a dataset first"); return PPI_E_STATE; }
+    (void)hipSetDevice(E->device);
+    Tape t;
+    std::string err;
+    int32_t sz = size;
+    int32_t stride = std::max(1, size);
+    if (!compile_population(E->symtab, phenotype, ephemeral, &sz, stride, 1, E->ncol, false, &t, &err)) {
+        set_error(err);
+        return PPI_E_PROGRAM;
+    }
+    if (t.stats.max_slots > 16) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
+    t.code.push_back(Ins{H_LOAD_C, 0});
+    ppi_status s;
+    size_t cap;
+    cap = E->cap_meta; if ((s = ensure(&E->dMeta, &cap, 1))) return s; E->cap_meta = cap;
+    cap = E->cap_code; if ((s = ensure(&E->dCode, &cap, t.code.size()))) return s; E->cap_code = cap;
+    cap = E->cap_pred; if ((s = ensure(&E->dPred, &cap, (size_t)E->nlin))) return s; E->cap_pred = cap;
+    PPI_HIP(hipMemcpyAsync(E->dMeta, t.meta.data(), sizeof(ProgMeta), hipMemcpyHostToDevice, E->stream));
+    PPI_HIP(hipMemcpyAsync(E->dCode, t.code.data(), t.code.size() * sizeof(Ins), hipMemcpyHostToDevice, E->stream));
+    E->nprog = 0;  // the resident program set was replaced
+    EvalArgs A = make_args(E, 0.0f);
+    A.nprog = 1;
+    int F = E->ncol <= 32 ? 32 : 0;
+    PPI_HIP(launch_predict(A, F, 16, E->cfg.protected_ops != 0, E->cfg.native_math != 0, E->dPred, E->stream));
+    PPI_HIP(hipMemcpyAsync(out, E->dPred, (size_t)E->nlin * sizeof(float), hipMemcpyDeviceToHost, E->stream));
+    PPI_HIP(hipStreamSynchronize(E->stream));
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_get_stats(const ppi_engine* E, ppi_stats* out) {
+    if (!E || !out) { set_error("engine or out is NULL"); return PPI_E_ARG; }
+    *out = E->stats;
+    out->form = E->form;
+    out->rows_per_lane = E->R;
+    out->programs_per_tile = E->chunk;
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_time(ppi_engine* E, int32_t iters, double* ms_per_eval) {
+    if (!E || !ms_per_eval || iters <= 0) { set_error("bad arguments"); return PPI_E_ARG; }
+    if (E->nprog <= 0) { set_error("upload programs first"); return PPI_E_STATE; }
+    (void)hipSetDevice(E->device);
+    PPI_HIP(hipEventRecord(E->ev[0], E->stream));
+    double main_ms = 0.0;
+    for (int32_t k = 0; k < iters; ++k) {
+        ppi_status s = enqueue_eval(E, 0.0f, true);
+        if (s) return s;
+        PPI_HIP(hipEventSynchronize(E->ev[3]));
+        float ms = 0.0f;
+        PPI_HIP(hipEventElapsedTime(&ms, E->ev[2], E->ev[3]));
+        main_ms += ms;
+    }
+    PPI_HIP(hipEventRecord(E->ev[1], E->stream));
+    PPI_HIP(hipEventSynchronize(E->ev[1]));
+    float ms = 0.0f;
+    PPI_HIP(hipEventElapsedTime(&ms, E->ev[0], E->ev[1]));
+    *ms_per_eval = ms / iters;
+    E->stats.main_kernel_ms = main_ms / iters;
+    E->stats.kernel_ms = ms / iters;
+    return PPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// engine_internal.h — state of one engine object (opaque behind include/ppi_engine.h).
+#ifndef PPI_ENGINE_INTERNAL_H
+#define PPI_ENGINE_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ppi_engine.h"
+#include "tape.h"
+
+namespace ppi {
+void set_error(const std::string& s);
+void pick_n_best(int n, int32_t* best, int num, const float* errors, const int32_t* indices);
+void best_acc(const float* vec, int P, int local2, int32_t* index, int32_t* best_size);
+}  // namespace ppi
+
+struct ppi_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // eval begin/end, main kernel begin/end
+    std::string arch;
+    int dev_cu = 0;
+    int dev_max_wg = 0;
+    size_t dev_lds = 0;
+
+    bool configured = false;
+    ppi_config cfg{};
+    std::vector<float> cost;
+    ppi::SymbolTable symtab;
+
+    // dataset, column-major with rows padded to ldx
+    int32_t nlin = 0, ncol = 0;
+    int64_t ldx = 0;
+    float* dX = nullptr;
+    size_t cap_X = 0;
+
+    // resident programs
+    ppi::Tape tape;
+    int32_t nprog = 0;
+    ppi::ProgMeta* dMeta = nullptr;
+    size_t cap_meta = 0;
+    ppi::Ins* dCode = nullptr;
+    size_t cap_code = 0;
+    int32_t* dOrder = nullptr;
+    size_t cap_order = 0;
+
+    // work and result buffers
+    float* dPartial = nullptr;
+    size_t cap_partial = 0;
+    uint32_t* dBadLo = nullptr;
+    uint32_t* dBadHi = nullptr;
+    size_t cap_bad = 0;
+    float* dRaw = nullptr;
+    float* dVec = nullptr;
+    size_t cap_raw = 0;
+    float* dCost = nullptr;
+    size_t cap_cost = 0;
+    float* dPred = nullptr;
+    size_t cap_pred = 0;
+    float* hRaw = nullptr;
+    float* hVec = nullptr;
+    size_t cap_hres = 0;
+
+    // reference geometry (accelerator.cc:272-355) and the launch plan
+    int dp_local = 1, dp_groups = 1, pdp_local = 1, best_local = 1;
+    int form = PPI_FORM_FP, R = 4, F = 16, S = 8, chunk = 64, nrb = 1;
+
+    bool launched = false;
+    ppi_stats stats{};
+};
+
+#endif

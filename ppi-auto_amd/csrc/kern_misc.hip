@@ -1,0 +1,62 @@
+// Generated instantiation unit: dispatch, PP form, finalize, predict.
+#define PPI_KERNELS_MISC 1
+#include "kernels_impl.h"
+
+namespace ppi {
+hipError_t launch_fp_r1(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
+hipError_t launch_fp_r2(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
+hipError_t launch_fp_r4(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
+hipError_t launch_fp_r8(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
+
+bool shape_supported(int R, int F, int S) {
+    return (R == 1 && F == 32 && S == 16) ||
+           (R == 1 && F == 0 && S == 16) ||
+           (R == 2 && F == 8 && S == 8) ||
+           (R == 2 && F == 16 && S == 8) ||
+           (R == 2 && F == 32 && S == 8) ||
+           (R == 2 && F == 0 && S == 8) ||
+           (R == 2 && F == 32 && S == 16) ||
+           (R == 4 && F == 8 && S == 8) ||
+           (R == 4 && F == 16 && S == 8) ||
+           (R == 4 && F == 32 && S == 8) ||
+           (R == 4 && F == 0 && S == 8) ||
+           (R == 8 && F == 8 && S == 8) ||
+           (R == 8 && F == 16 && S == 8);
+}
+
+hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st) {
+    switch (R) {
+    case 1: return launch_fp_r1(A, F, S, P, N, st);
+    case 2: return launch_fp_r2(A, F, S, P, N, st);
+    case 4: return launch_fp_r4(A, F, S, P, N, st);
+    case 8: return launch_fp_r8(A, F, S, P, N, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st) {
+    hipLaunchKernelGGL(dev::finalize_kernel, dim3((unsigned)((A.nprog + 255) / 256)), dim3(256), 0, st, A, nrb);
+    return hipGetLastError();
+}
+
+hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st) {
+    dim3 grid((unsigned)((A.nprog + 255) / 256));
+    if (P && N) hipLaunchKernelGGL((dev::pp_kernel<true, true>), grid, dim3(256), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::pp_kernel<true, false>), grid, dim3(256), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::pp_kernel<false, true>), grid, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((dev::pp_kernel<false, false>), grid, dim3(256), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st) {
+    if (F == 32 && S == 16) return launch::launch_predict_t<32, 16>(A, P, N, out, st);
+    if (F == 0 && S == 16) return launch::launch_predict_t<0, 16>(A, P, N, out, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dev::fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+}  // namespace ppi

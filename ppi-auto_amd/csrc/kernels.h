@@ -1,0 +1,53 @@
+// kernels.h — launch interface between the engine runtime and the HIP kernels.
+#ifndef PPI_KERNELS_H
+#define PPI_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tape.h"
+
+namespace ppi {
+
+constexpr int kMaxChunk = 256;  // programs per FP tile (LDS reduction buffer)
+
+// Everything a kernel needs, passed by value.
+struct EvalArgs {
+    const ProgMeta* meta;
+    const Ins* code;
+    const float* X;       // column-major dataset, column c at X + c*ldx, rows padded
+    int64_t ldx;
+    int32_t nlin;
+    int32_t ncol;
+    int32_t ycol;         // ncol - 1
+    int32_t nprog;
+    int32_t chunk;        // programs per FP tile
+    int32_t strategy;     // ppi_strategy: result semantics
+    int32_t error_fn;
+    int32_t reducemax;
+    const float* cost;
+    int32_t cost_n;
+    int32_t pdp_lanes;    // reference evaluate_pdp work-group size (bad-lane bookkeeping)
+    int32_t dp_groups;    // reference evaluate_dp work-group count (empty-program rule)
+    float alpha;
+    const int32_t* order; // PP form: lane -> program (length-sorted), may be null
+    float* partial;       // FP form: [row_block][nprog]
+    uint32_t* bad_lo;     // per program: min bad-row key (0xffffffff = none)
+    uint32_t* bad_hi;     // per program: max bad-row key
+    float* raw;           // per program: kernel-side error (before alpha*size for PP/PDP)
+    float* vec;           // per program: reported error
+};
+
+bool shape_supported(int R, int F, int S);
+bool pdp_shape_supported(int R, int F, int S);
+hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
+hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
+hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st);
+hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st);
+hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+
+}  // namespace ppi
+
+#endif

@@ -1,0 +1,508 @@
+// kernels_impl.h — HIP kernels of the PPI evaluation engine for gfx950.
+//
+// Forms (reference: accelerator.cl):
+//   FP  (fp_tile_kernel)   row tiles x program chunks.  A wave owns 64*R rows whose
+//                          features sit in VGPRs; it walks a chunk of programs whose
+//                          instructions are wave-uniform (scalar loads, scalar
+//                          branches).  Per-(program, row-block) partials are reduced
+//                          by finalize_kernel in a fixed order.  (evaluate_dp, :88-171)
+//   PDP (pdp_kernel)       one workgroup per program, waves stride over row tiles,
+//                          result written directly.  (evaluate_pdp, :173-268)
+//   PP  (pp_kernel)        one lane per program, rows in order; the only form whose
+//                          sums follow the reference's row order exactly.  (evaluate_pp, :7-86)
+#ifndef PPI_KERNELS_IMPL_H
+#define PPI_KERNELS_IMPL_H
+
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "device_ops.h"
+#include "kernels.h"
+
+namespace ppi {
+namespace dev {
+
+constexpr int kWaves = 4;              // waves per workgroup in FP / PDP forms
+constexpr int kBlock = 64 * kWaves;
+
+template <int K> struct VecT { typedef float type __attribute__((ext_vector_type(K))); };
+
+// ---------------------------------------------------------------------------
+// interpreter core: one program over R rows per lane, program wave-uniform.
+// ---------------------------------------------------------------------------
+template <int R, int F, int S, bool P, bool N>
+struct Core {
+    typedef typename VecT<(F > 0 ? F : 1)>::type VF;
+    typedef typename VecT<S>::type VS;
+
+    // Attribute column `col` of row r: VGPR tile (F > 0) or global memory (F == 0).
+    static __device__ __forceinline__ float feat(const VF (&xf)[R], const float* __restrict__ X, int64_t ldx,
+                                                 const int (&row)[R], int r, uint32_t col) {
+        if constexpr (F > 0) return xf[r][col];
+        else return X[(int64_t)col * ldx + row[r]];
+    }
+
+    static __device__ __forceinline__ void push(VS (&stk)[R], const float (&acc)[R], uint32_t slot) {
+        // Slot index is wave-uniform; a scalar switch keeps every write a plain v_mov.
+        switch (slot) {
+#define PPI_PUSH_CASE(K) \
+    case K: if constexpr (K < S) { _Pragma("unroll") for (int r = 0; r < R; ++r) stk[r][K] = acc[r]; } break;
+            PPI_PUSH_CASE(0) PPI_PUSH_CASE(1) PPI_PUSH_CASE(2) PPI_PUSH_CASE(3)
+            PPI_PUSH_CASE(4) PPI_PUSH_CASE(5) PPI_PUSH_CASE(6) PPI_PUSH_CASE(7)
+            PPI_PUSH_CASE(8) PPI_PUSH_CASE(9) PPI_PUSH_CASE(10) PPI_PUSH_CASE(11)
+            PPI_PUSH_CASE(12) PPI_PUSH_CASE(13) PPI_PUSH_CASE(14) PPI_PUSH_CASE(15)
+#undef PPI_PUSH_CASE
+        default: break;
+        }
+    }
+
+    static __device__ __forceinline__ void run(const Ins* __restrict__ code, uint32_t n, const VF (&xf)[R],
+                                               const float* __restrict__ X, int64_t ldx, const int (&row)[R],
+                                               float (&acc)[R], VS (&stk)[R]) {
+        uint32_t nc = __builtin_amdgcn_readfirstlane(code[0].code);
+        uint32_t na = __builtin_amdgcn_readfirstlane(code[0].arg);
+        for (uint32_t pc = 0; pc < n; ++pc) {
+            const uint32_t c = nc, a = na;
+            // One instruction of look-ahead; the tape carries a padding entry at its end.
+            nc = __builtin_amdgcn_readfirstlane(code[pc + 1].code);
+            na = __builtin_amdgcn_readfirstlane(code[pc + 1].arg);
+            const uint32_t h = c & 0xffu;
+            const float cv = __uint_as_float(a);
+            if (h >= H_BIN_C) {
+                float b[R];
+                uint32_t op;
+                if (h < H_BIN_A) {
+                    op = h - H_BIN_C;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) b[r] = cv;
+                } else if (h < H_BIN_S) {
+                    op = h - H_BIN_A;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) b[r] = feat(xf, X, ldx, row, r, a);
+                } else {
+                    op = h - H_BIN_S;
+                    const uint32_t slot = (c >> 8) & 31u;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
+                }
+                switch (op) {
+#define PPI_BCASE(K)                                                           \
+    case K:                                                                    \
+        _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = binary<P, N>(K, acc[r], b[r]); \
+        break;
+                    PPI_BCASE(B_ADD) PPI_BCASE(B_SUB) PPI_BCASE(B_RSUB) PPI_BCASE(B_MULT)
+                    PPI_BCASE(B_DIV) PPI_BCASE(B_RDIV) PPI_BCASE(B_MEAN) PPI_BCASE(B_MAX)
+                    PPI_BCASE(B_MIN) PPI_BCASE(B_MOD) PPI_BCASE(B_RMOD) PPI_BCASE(B_POW)
+                    PPI_BCASE(B_RPOW) PPI_BCASE(B_AND) PPI_BCASE(B_OR) PPI_BCASE(B_XOR)
+                    PPI_BCASE(B_GT) PPI_BCASE(B_GE) PPI_BCASE(B_LT) PPI_BCASE(B_LE)
+                    PPI_BCASE(B_EQ) PPI_BCASE(B_NE) PPI_BCASE(B_RMAX) PPI_BCASE(B_RMIN)
+#undef PPI_BCASE
+                default: break;
+                }
+            } else if (h >= H_UN) {
+                switch (h - H_UN) {
+#define PPI_UCASE(K)                                                           \
+    case K:                                                                    \
+        _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = unary<P, N>(K, acc[r]); \
+        break;
+                    PPI_UCASE(U_NOT) PPI_UCASE(U_ABS) PPI_UCASE(U_POW2) PPI_UCASE(U_POW3)
+                    PPI_UCASE(U_POW4) PPI_UCASE(U_POW5) PPI_UCASE(U_NEG) PPI_UCASE(U_ROUND)
+                    PPI_UCASE(U_CEIL) PPI_UCASE(U_FLOOR) PPI_UCASE(U_EXP) PPI_UCASE(U_EXP10)
+                    PPI_UCASE(U_EXP2) PPI_UCASE(U_SIN) PPI_UCASE(U_COS) PPI_UCASE(U_STEP)
+                    PPI_UCASE(U_SIGN) PPI_UCASE(U_LOGISTIC) PPI_UCASE(U_GAMMA) PPI_UCASE(U_GAUSS)
+                    PPI_UCASE(U_SQRT) PPI_UCASE(U_LOG) PPI_UCASE(U_LOG10) PPI_UCASE(U_LOG2)
+                    PPI_UCASE(U_TAN)
+#undef PPI_UCASE
+                default: break;
+                }
+            } else if (h == H_LOAD_C) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = cv;
+            } else if (h == H_LOAD_A) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = feat(xf, X, ldx, row, r, a);
+            } else if (h == H_PUSH_C) {
+                push(stk, acc, (c >> 8) & 31u);
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = cv;
+            } else if (h == H_PUSH_A) {
+                push(stk, acc, (c >> 8) & 31u);
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = feat(xf, X, ldx, row, r, a);
+            } else {
+                // H_TERN: IF_THEN_ELSE (interpreter_core:1-4): (bool)cond ? then : else.
+                const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
+                const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
+                    const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
+                    const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
+                    const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
+                    acc[r] = (cd != 0.0f) ? th : el;
+                }
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ bool finite_f(float x) { return !(isinf(x) || isnan(x)); }
+
+// Per-program result from (reduced value, bad-row record).  Restates the
+// reference's epilogues: sequential.cc:135-189, accelerator.cl:64-84 +
+// accelerator.cc:849 (PP), accelerator.cl:157-171 + accelerator.cc:773-805 (DP),
+// accelerator.cl:236-267 + accelerator.cc:849 (PDP).
+__device__ __forceinline__ void semantics(const EvalArgs& A, float total, uint32_t flags, uint32_t kmin,
+                                          uint32_t kmax, int32_t size, float* raw_out, float* vec_out) {
+    const float fn = (float)A.nlin;
+    const float as = A.alpha * (float)size;
+    float raw, vec;
+    const bool bad = kmin != 0xffffffffu;
+    if (flags & kEmpty) {
+        if (A.strategy == PPI_STRATEGY_FP) {
+            vec = (A.dp_groups == 1 && !A.reducemax) ? FLT_MAX / fn : FLT_MAX;
+            raw = vec;
+        } else {
+            raw = FLT_MAX;
+            vec = (A.strategy == PPI_STRATEGY_SEQ) ? FLT_MAX : raw + as;
+        }
+    } else if (bad) {
+        if (A.strategy == PPI_STRATEGY_PDP) {
+            if (A.reducemax) raw = FLT_MAX / fn;
+            else if (kmin != kmax) raw = FLT_MAX;                 // >= 2 lanes hold MAXFLOAT: inf
+            else raw = (total >= 0x1.0p103f) ? FLT_MAX : FLT_MAX / fn;  // MAXFLOAT absorbs the rest
+            vec = raw + as;
+        } else if (A.strategy == PPI_STRATEGY_PP) {
+            raw = FLT_MAX / fn;
+            vec = raw + as;
+        } else {
+            vec = FLT_MAX / fn + as;
+            raw = vec;
+        }
+    } else {
+        if (A.strategy == PPI_STRATEGY_PP || A.strategy == PPI_STRATEGY_PDP) {
+            raw = finite_f(total) ? total / fn : FLT_MAX;
+            vec = raw + as;
+        } else {
+            vec = finite_f(total) ? total / fn + as : FLT_MAX;
+            raw = vec;
+        }
+    }
+    *raw_out = raw;
+    *vec_out = vec;
+}
+
+// Row-tile setup shared by FP and PDP: rows, validity, features, target.
+template <int R, int F, int S, bool P, bool N>
+__device__ __forceinline__ void load_tile(const EvalArgs& A, int64_t row0, int lane,
+                                          typename Core<R, F, S, P, N>::VF (&xf)[R], int (&row)[R],
+                                          float (&y)[R], bool (&valid)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        row[r] = (int)(row0 + r * 64 + lane);
+        valid[r] = row[r] < A.nlin;
+        y[r] = A.X[(int64_t)A.ycol * A.ldx + row[r]];
+    }
+    if constexpr (F > 0) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            const bool have = f < A.ncol;
+#pragma unroll
+            for (int r = 0; r < R; ++r) xf[r][f] = have ? A.X[(int64_t)f * A.ldx + row[r]] : 0.0f;
+        }
+    }
+}
+
+// Error of the R rows of one lane: accumulate finite errors, record bad rows.
+template <int R>
+__device__ __forceinline__ void accumulate(const EvalArgs& A, const float (&acc)[R], const float (&y)[R],
+                                           const bool (&valid)[R], const int (&row)[R], float& part,
+                                           uint32_t& kmin, uint32_t& kmax) {
+    const float fn = (float)A.nlin;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!valid[r]) continue;
+        const float e = error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n);
+        if (!finite_f(e)) {
+            const uint32_t key = (uint32_t)row[r] % (uint32_t)A.pdp_lanes;
+            kmin = min(kmin, key);
+            kmax = max(kmax, key);
+        } else if (A.reducemax) {
+            part = fmaxf(part, e * fn);
+        } else {
+            part += e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP form
+// ---------------------------------------------------------------------------
+template <int R, int F, int S, bool P, bool N>
+__global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
+    typedef Core<R, F, S, P, N> C;
+    __shared__ float red[kWaves][kMaxChunk];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int rb = blockIdx.x;
+    const int p0 = blockIdx.y * A.chunk;
+    const int p1 = min(p0 + A.chunk, A.nprog);
+    const int64_t row0 = ((int64_t)rb * kWaves + wave) * (64 * R);
+
+    typename C::VF xf[R];
+    int row[R];
+    float y[R];
+    bool valid[R];
+    load_tile<R, F, S, P, N>(A, row0, lane, xf, row, y, valid);
+
+    for (int p = p0; p < p1; ++p) {
+        const ProgMeta m = A.meta[p];
+        float part = 0.0f;
+        uint32_t kmin = 0xffffffffu, kmax = 0u;
+        if (!(m.flags & kEmpty)) {
+            float acc[R];
+            typename C::VS stk[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) { acc[r] = 0.0f; stk[r] = 0.0f; }
+            C::run(A.code + m.code_off, m.ncode, xf, A.X, A.ldx, row, acc, stk);
+            accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
+        }
+        part = A.reducemax ? wave_max(part) : wave_sum(part);
+        if (__any(kmin != 0xffffffffu)) {
+            kmin = wave_umin(kmin);
+            kmax = wave_umax(kmax);
+            if (lane == 0) {
+                atomicMin(&A.bad_lo[p], kmin);
+                atomicMax(&A.bad_hi[p], kmax);
+            }
+        }
+        if (lane == 0) red[wave][p - p0] = part;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < p1 - p0; t += kBlock) {
+        float v = red[0][t];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) v = A.reducemax ? fmaxf(v, red[w][t]) : v + red[w][t];
+        A.partial[(int64_t)rb * A.nprog + p0 + t] = v;
+    }
+}
+
+#ifdef PPI_KERNELS_MISC
+__global__ void finalize_kernel(EvalArgs A, int nrb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.nprog) return;
+    float total = 0.0f;
+    for (int rb = 0; rb < nrb; ++rb) {
+        const float v = A.partial[(int64_t)rb * A.nprog + p];
+        total = A.reducemax ? fmaxf(total, v) : total + v;
+    }
+    const ProgMeta m = A.meta[p];
+    semantics(A, total, m.flags, A.bad_lo[p], A.bad_hi[p], m.size, &A.raw[p], &A.vec[p]);
+}
+#endif  // PPI_KERNELS_MISC
+
+// ---------------------------------------------------------------------------
+// PDP form: one workgroup per program.
+// ---------------------------------------------------------------------------
+template <int R, int F, int S, bool P, bool N>
+__global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
+    typedef Core<R, F, S, P, N> C;
+    __shared__ float red[kWaves];
+    __shared__ uint32_t rlo[kWaves], rhi[kWaves];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int p = blockIdx.x;
+    const ProgMeta m = A.meta[p];
+    float part = 0.0f;
+    uint32_t kmin = 0xffffffffu, kmax = 0u;
+    if (!(m.flags & kEmpty)) {
+        const int64_t step = (int64_t)kWaves * 64 * R;
+        for (int64_t row0 = (int64_t)wave * 64 * R; row0 < A.nlin; row0 += step) {
+            typename C::VF xf[R];
+            int row[R];
+            float y[R];
+            bool valid[R];
+            load_tile<R, F, S, P, N>(A, row0, lane, xf, row, y, valid);
+            float acc[R];
+            typename C::VS stk[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) { acc[r] = 0.0f; stk[r] = 0.0f; }
+            C::run(A.code + m.code_off, m.ncode, xf, A.X, A.ldx, row, acc, stk);
+            accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
+        }
+    }
+    part = A.reducemax ? wave_max(part) : wave_sum(part);
+    kmin = wave_umin(kmin);
+    kmax = wave_umax(kmax);
+    if (lane == 0) { red[wave] = part; rlo[wave] = kmin; rhi[wave] = kmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = red[0];
+        uint32_t lo = rlo[0], hi = rhi[0];
+        for (int w = 1; w < kWaves; ++w) {
+            v = A.reducemax ? fmaxf(v, red[w]) : v + red[w];
+            lo = min(lo, rlo[w]);
+            hi = max(hi, rhi[w]);
+        }
+        semantics(A, v, m.flags, lo, hi, m.size, &A.raw[p], &A.vec[p]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PP form: one lane per program, rows in order (divergent dispatch).
+// ---------------------------------------------------------------------------
+template <bool P, bool N>
+__global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
+    __shared__ float stk[kMaxSlots][256];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= A.nprog) return;
+    const int p = A.order ? A.order[t] : t;
+    const ProgMeta m = A.meta[p];
+    float part = 0.0f;
+    uint32_t kmin = 0xffffffffu, kmax = 0u;
+    const bool stop_at_bad = A.strategy != PPI_STRATEGY_PDP;
+    const float fn = (float)A.nlin;
+    if (!(m.flags & kEmpty)) {
+        const Ins* code = A.code + m.code_off;
+        for (int row = 0; row < A.nlin; ++row) {
+            float acc = 0.0f;
+            for (uint32_t pc = 0; pc < m.ncode; ++pc) {
+                const uint32_t c = code[pc].code, a = code[pc].arg;
+                const uint32_t h = c & 0xffu;
+                const float cv = __uint_as_float(a);
+                if (h >= H_BIN_C) {
+                    float b;
+                    uint32_t op;
+                    if (h < H_BIN_A) { op = h - H_BIN_C; b = cv; }
+                    else if (h < H_BIN_S) { op = h - H_BIN_A; b = A.X[(int64_t)a * A.ldx + row]; }
+                    else { op = h - H_BIN_S; b = stk[(c >> 8) & 31u][threadIdx.x]; }
+                    acc = binary<P, N>(op, acc, b);
+                } else if (h >= H_UN) {
+                    acc = unary<P, N>(h - H_UN, acc);
+                } else if (h == H_LOAD_C) {
+                    acc = cv;
+                } else if (h == H_LOAD_A) {
+                    acc = A.X[(int64_t)a * A.ldx + row];
+                } else if (h == H_PUSH_C || h == H_PUSH_A) {
+                    stk[(c >> 8) & 31u][threadIdx.x] = acc;
+                    acc = (h == H_PUSH_C) ? cv : A.X[(int64_t)a * A.ldx + row];
+                } else {
+                    const float v1 = stk[(c >> 8) & 31u][threadIdx.x], v2 = stk[(c >> 13) & 31u][threadIdx.x];
+                    const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
+                    const float cd = wc == 0 ? acc : (wc == 1 ? v1 : v2);
+                    const float th = wt == 0 ? acc : (wt == 1 ? v1 : v2);
+                    const float el = we == 0 ? acc : (we == 1 ? v1 : v2);
+                    acc = (cd != 0.0f) ? th : el;
+                }
+            }
+            const float e = error_fn(A.error_fn, acc, A.X[(int64_t)A.ycol * A.ldx + row], A.cost, A.cost_n);
+            if (!finite_f(e)) {
+                const uint32_t key = (uint32_t)row % (uint32_t)A.pdp_lanes;
+                kmin = min(kmin, key);
+                kmax = max(kmax, key);
+                if (stop_at_bad) break;  // sequential.cc:173 / accelerator.cl:64
+            } else if (A.reducemax) {
+                part = (e * fn > part) ? e * fn : part;
+            } else {
+                part += e;
+            }
+        }
+    }
+    semantics(A, part, m.flags, kmin, kmax, m.size, &A.raw[p], &A.vec[p]);
+}
+
+// ---------------------------------------------------------------------------
+// prediction (ppp -pred): one program, one row per lane.
+// ---------------------------------------------------------------------------
+template <int F, int S, bool P, bool N>
+__global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out) {
+    typedef Core<1, F, S, P, N> C;
+    const int lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6)) * 64;
+    typename C::VF xf[1];
+    int row[1];
+    float y[1];
+    bool valid[1];
+    load_tile<1, F, S, P, N>(A, row0, lane, xf, row, y, valid);
+    const ProgMeta m = A.meta[0];
+    float acc[1] = {__builtin_nanf("")};
+    typename C::VS stk[1];
+    stk[0] = 0.0f;
+    if (!(m.flags & kEmpty)) C::run(A.code + m.code_off, m.ncode, xf, A.X, A.ldx, row, acc, stk);
+    if (valid[0]) out[row[0]] = acc[0];
+}
+
+#ifdef PPI_KERNELS_MISC
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+#endif  // PPI_KERNELS_MISC
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// launcher templates
+// ---------------------------------------------------------------------------
+namespace launch {
+
+template <int R, int F, int S>
+hipError_t launch_fp_t(const EvalArgs& A, bool P, bool N, hipStream_t st) {
+    const int64_t rows_per_block = (int64_t)dev::kBlock * R;
+    dim3 grid((unsigned)((A.nlin + rows_per_block - 1) / rows_per_block), (unsigned)((A.nprog + A.chunk - 1) / A.chunk));
+    if (P && N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, false>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, false>), grid, dim3(dev::kBlock), 0, st, A);
+    return hipGetLastError();
+}
+
+template <int R, int F, int S>
+hipError_t launch_pdp_t(const EvalArgs& A, bool P, bool N, hipStream_t st) {
+    dim3 grid((unsigned)A.nprog);
+    if (P && N) hipLaunchKernelGGL((dev::pdp_kernel<R, F, S, true, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::pdp_kernel<R, F, S, true, false>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::pdp_kernel<R, F, S, false, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::pdp_kernel<R, F, S, false, false>), grid, dim3(dev::kBlock), 0, st, A);
+    return hipGetLastError();
+}
+
+template <int F, int S>
+hipError_t launch_predict_t(const EvalArgs& A, bool P, bool N, float* out, hipStream_t st) {
+    dim3 grid((unsigned)((A.nlin + dev::kBlock - 1) / dev::kBlock));
+    if (P && N) hipLaunchKernelGGL((dev::predict_kernel<F, S, true, true>), grid, dim3(dev::kBlock), 0, st, A, out);
+    else if (P) hipLaunchKernelGGL((dev::predict_kernel<F, S, true, false>), grid, dim3(dev::kBlock), 0, st, A, out);
+    else if (N) hipLaunchKernelGGL((dev::predict_kernel<F, S, false, true>), grid, dim3(dev::kBlock), 0, st, A, out);
+    else hipLaunchKernelGGL((dev::predict_kernel<F, S, false, false>), grid, dim3(dev::kBlock), 0, st, A, out);
+    return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace ppi
+
+#endif

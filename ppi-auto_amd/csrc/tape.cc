@@ -1,0 +1,397 @@
+// tape.cc — prefix program -> Sethi-Ullman accumulator tape (see tape.h).
+#include "tape.h"
+
+#include <algorithm>
+#include <cstring>
+#include <thread>
+
+namespace ppi {
+
+void SymbolTable::build(const ppi_symbol* t, int32_t n) {
+    entries.assign(t, t + n);
+    direct.assign(8192, 0);
+    other.clear();
+    for (int32_t k = 0; k < n; ++k) {
+        int32_t s = t[k].symbol - 10000;
+        if (s >= 0 && s < 8192) {
+            if (!direct[s]) direct[s] = k + 1;
+        } else {
+            other.emplace_back(t[k].symbol, k);
+        }
+    }
+    std::stable_sort(other.begin(), other.end(),
+                     [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+                         return a.first < b.first;
+                     });
+}
+
+const ppi_symbol* SymbolTable::find(int32_t symbol) const {
+    int32_t s = symbol - 10000;
+    if (s >= 0 && s < 8192) return direct[s] ? &entries[direct[s] - 1] : nullptr;
+    auto it = std::lower_bound(other.begin(), other.end(), std::make_pair(symbol, INT32_MIN));
+    if (it != other.end() && it->first == symbol) return &entries[it->second];
+    return nullptr;
+}
+
+namespace {
+
+int arity(int32_t op) {
+    if (op == PPI_OP_IF_THEN_ELSE) return 3;
+    if (op >= PPI_OP_AND && op <= PPI_OP_DIV) return 2;
+    if (op >= PPI_OP_NOT && op <= PPI_OP_TAN) return 1;
+    return 0;
+}
+
+uint32_t unop_of(int32_t op) {
+    switch (op) {
+    case PPI_OP_NOT: return U_NOT;     case PPI_OP_ABS: return U_ABS;
+    case PPI_OP_POW2: return U_POW2;   case PPI_OP_POW3: return U_POW3;
+    case PPI_OP_POW4: return U_POW4;   case PPI_OP_POW5: return U_POW5;
+    case PPI_OP_NEG: return U_NEG;     case PPI_OP_ROUND: return U_ROUND;
+    case PPI_OP_CEIL: return U_CEIL;   case PPI_OP_FLOOR: return U_FLOOR;
+    case PPI_OP_EXP: return U_EXP;     case PPI_OP_EXP10: return U_EXP10;
+    case PPI_OP_EXP2: return U_EXP2;   case PPI_OP_SIN: return U_SIN;
+    case PPI_OP_COS: return U_COS;     case PPI_OP_STEP: return U_STEP;
+    case PPI_OP_SIGN: return U_SIGN;   case PPI_OP_LOGISTIC: return U_LOGISTIC;
+    case PPI_OP_GAMMA: return U_GAMMA; case PPI_OP_GAUSS: return U_GAUSS;
+    case PPI_OP_SQRT: return U_SQRT;   case PPI_OP_LOG: return U_LOG;
+    case PPI_OP_LOG10: return U_LOG10; case PPI_OP_LOG2: return U_LOG2;
+    default: return U_TAN;
+    }
+}
+
+// Binary operator with the accumulator as the left operand (rev=false) or as the
+// right operand (rev=true); the other operand is the folded/parked value.
+uint32_t binop_of(int32_t op, bool rev) {
+    switch (op) {
+    case PPI_OP_AND: return B_AND;
+    case PPI_OP_OR: return B_OR;
+    case PPI_OP_XOR: return B_XOR;
+    case PPI_OP_GREATER: return rev ? B_LT : B_GT;
+    case PPI_OP_GREATEREQUAL: return rev ? B_LE : B_GE;
+    case PPI_OP_LESS: return rev ? B_GT : B_LT;
+    case PPI_OP_LESSEQUAL: return rev ? B_GE : B_LE;
+    case PPI_OP_EQUAL: return B_EQ;
+    case PPI_OP_NOTEQUAL: return B_NE;
+    case PPI_OP_ADD: return B_ADD;
+    case PPI_OP_SUB: return rev ? B_RSUB : B_SUB;
+    case PPI_OP_MULT: return B_MULT;
+    case PPI_OP_MEAN: return B_MEAN;
+    case PPI_OP_MAX: return rev ? B_RMAX : B_MAX;
+    case PPI_OP_MIN: return rev ? B_RMIN : B_MIN;
+    case PPI_OP_MOD: return rev ? B_RMOD : B_MOD;
+    case PPI_OP_POW: return rev ? B_RPOW : B_POW;
+    default: return rev ? B_RDIV : B_DIV;
+    }
+}
+
+struct Node {
+    int32_t op;       // ppi_op of an operator, or -1 for a leaf
+    int32_t ch[3];
+    uint32_t arg;     // leaf: float bits or column
+    int8_t leaf_attr; // leaf: 1 = attribute column, 0 = constant
+    int32_t pos;      // phenotype position (for T_CONST bookkeeping)
+    bool is_tconst;
+    int16_t regs;
+};
+
+uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+struct Compiler {
+    const SymbolTable& st;
+    int32_t ncol;
+    bool want_const_pos;
+    std::vector<Node> nodes;
+    std::vector<int32_t> vstack;
+    struct Frame { int32_t node, base, phase, o[3]; };
+    std::vector<Frame> frames;
+
+    Compiler(const SymbolTable& s, int32_t nc, bool w) : st(s), ncol(nc), want_const_pos(w) {}
+
+    // Returns slots used, -1 if invalid (underflow / empty stack).
+    int compile(const int32_t* ph, const float* eph, int32_t size, Tape* out) {
+        nodes.clear();
+        vstack.clear();
+        for (int32_t i = size - 1; i >= 0; --i) {
+            const ppi_symbol* s = st.find(ph[i]);
+            int32_t op = s ? s->op : PPI_OP_UNKNOWN;
+            int ar = arity(op);
+            Node n{};
+            n.pos = i;
+            n.regs = 0;
+            if (ar == 0) {
+                n.op = -1;
+                n.leaf_attr = 0;
+                n.is_tconst = false;
+                if (op == PPI_OP_CONST) {
+                    n.arg = fbits(eph[i]);
+                    n.is_tconst = true;
+                } else if (op == PPI_OP_ATTRIBUTE) {
+                    float c = eph[i];
+                    if (c > -1.0f && c < (float)ncol) {
+                        n.leaf_attr = 1;
+                        n.arg = (uint32_t)(int32_t)c;
+                    } else {
+                        n.arg = fbits(__builtin_nanf(""));
+                    }
+                } else if (op == PPI_OP_VALUE) {
+                    n.arg = fbits(s->value);
+                } else {
+                    n.arg = fbits(__builtin_nanf(""));
+                }
+            } else {
+                if ((int)vstack.size() < ar) return -1;
+                n.op = op;
+                for (int k = 0; k < ar; ++k) { n.ch[k] = vstack.back(); vstack.pop_back(); }
+                // Sethi-Ullman register need.
+                if (ar == 1) {
+                    n.regs = nodes[n.ch[0]].regs;
+                } else if (ar == 2) {
+                    const Node& a = nodes[n.ch[0]];
+                    const Node& b = nodes[n.ch[1]];
+                    bool la = a.op < 0, lb = b.op < 0;
+                    if (la && lb) n.regs = 0;
+                    else if (la) n.regs = b.regs;
+                    else if (lb) n.regs = a.regs;
+                    else n.regs = (int16_t)(a.regs == b.regs ? a.regs + 1 : std::max(a.regs, b.regs));
+                } else {
+                    int16_t r[3] = {nodes[n.ch[0]].regs, nodes[n.ch[1]].regs, nodes[n.ch[2]].regs};
+                    std::sort(r, r + 3, [](int16_t x, int16_t y) { return x > y; });
+                    n.regs = (int16_t)std::max<int>(r[0], std::max<int>(1 + r[1], 2 + r[2]));
+                }
+            }
+            nodes.push_back(n);
+            vstack.push_back((int32_t)nodes.size() - 1);
+        }
+        if (vstack.empty()) return -1;
+        return emit(vstack.back(), out);
+    }
+
+    int pending_save = -1;
+    int max_slot = 0;
+
+    void put(Tape* out, uint32_t code, uint32_t arg, uint32_t cpos) {
+        out->code.push_back(Ins{code, arg});
+        if (want_const_pos) out->const_pos.push_back(cpos);
+    }
+
+    void emit_load(const Node& n, Tape* out) {
+        uint32_t cpos = n.is_tconst ? (uint32_t)n.pos + 1 : 0;
+        if (pending_save >= 0) {
+            uint32_t h = n.leaf_attr ? H_PUSH_A : H_PUSH_C;
+            put(out, h | ((uint32_t)pending_save << 8), n.arg, cpos);
+            max_slot = std::max(max_slot, pending_save + 1);
+            pending_save = -1;
+        } else {
+            put(out, n.leaf_attr ? H_LOAD_A : H_LOAD_C, n.arg, cpos);
+        }
+    }
+
+    void emit_fold(uint32_t bop, const Node& leaf, Tape* out) {
+        uint32_t cpos = leaf.is_tconst ? (uint32_t)leaf.pos + 1 : 0;
+        put(out, (leaf.leaf_attr ? H_BIN_A : H_BIN_C) + bop, leaf.arg, cpos);
+    }
+
+    int emit(int32_t root, Tape* out) {
+        pending_save = -1;
+        max_slot = 0;
+        frames.clear();
+        frames.push_back(Frame{root, 0, 0, {0, 0, 0}});
+        while (!frames.empty()) {
+            Frame& f = frames.back();
+            const Node& n = nodes[f.node];
+            if (n.op < 0) {
+                emit_load(n, out);
+                frames.pop_back();
+                continue;
+            }
+            int ar = arity(n.op);
+            if (ar == 1) {
+                if (f.phase == 0) {
+                    f.phase = 1;
+                    int32_t c = n.ch[0], b = f.base;
+                    frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+                } else {
+                    put(out, H_UN + unop_of(n.op), 0, 0);
+                    frames.pop_back();
+                }
+                continue;
+            }
+            if (ar == 2) {
+                const Node& L = nodes[n.ch[0]];
+                const Node& R = nodes[n.ch[1]];
+                bool lleaf = L.op < 0, rleaf = R.op < 0;
+                if (lleaf && rleaf) {
+                    emit_load(L, out);
+                    emit_fold(binop_of(n.op, false), R, out);
+                    frames.pop_back();
+                } else if (lleaf || rleaf) {
+                    if (f.phase == 0) {
+                        f.phase = 1;
+                        int32_t c = lleaf ? n.ch[1] : n.ch[0], b = f.base;
+                        frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+                    } else {
+                        // acc holds the computed child; fold the leaf on its side.
+                        emit_fold(binop_of(n.op, /*rev=*/lleaf), lleaf ? L : R, out);
+                        frames.pop_back();
+                    }
+                } else {
+                    bool left_first = L.regs >= R.regs;
+                    if (f.phase == 0) {
+                        f.phase = 1;
+                        int32_t c = left_first ? n.ch[0] : n.ch[1], b = f.base;
+                        frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+                    } else if (f.phase == 1) {
+                        f.phase = 2;
+                        pending_save = f.base;
+                        int32_t c = left_first ? n.ch[1] : n.ch[0], b = f.base + 1;
+                        frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+                    } else {
+                        // r[base] holds the first-evaluated child, acc the second.
+                        put(out, H_BIN_S + binop_of(n.op, /*rev=*/left_first) + ((uint32_t)f.base << 8), 0, 0);
+                        frames.pop_back();
+                    }
+                }
+                continue;
+            }
+            // ternary: IF_THEN_ELSE(cond=ch0, then=ch1, else=ch2)
+            if (f.phase == 0) {
+                int32_t ord[3] = {0, 1, 2};
+                std::stable_sort(ord, ord + 3, [&](int32_t x, int32_t y) {
+                    return nodes[n.ch[x]].regs > nodes[n.ch[y]].regs;
+                });
+                for (int k = 0; k < 3; ++k) f.o[k] = ord[k];
+                f.phase = 1;
+                int32_t c = n.ch[ord[0]], b = f.base;
+                frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+            } else if (f.phase == 1) {
+                f.phase = 2;
+                pending_save = f.base;
+                int32_t c = n.ch[f.o[1]], b = f.base + 1;
+                frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+            } else if (f.phase == 2) {
+                f.phase = 3;
+                pending_save = f.base + 1;
+                int32_t c = n.ch[f.o[2]], b = f.base + 2;
+                frames.push_back(Frame{c, b, 0, {0, 0, 0}});
+            } else {
+                // where[role]: 1 = r[slot] (first), 2 = r[slot2] (second), 0 = acc (third)
+                uint32_t where[3];
+                where[f.o[0]] = 1;
+                where[f.o[1]] = 2;
+                where[f.o[2]] = 0;
+                uint32_t code = H_TERN | ((uint32_t)f.base << 8) | ((uint32_t)(f.base + 1) << 13) |
+                                (where[0] << 18) | (where[1] << 20) | (where[2] << 22);
+                put(out, code, 0, 0);
+                frames.pop_back();
+            }
+        }
+        return max_slot;
+    }
+};
+
+}  // namespace
+
+bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
+                        const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
+                        Tape* out, std::string* err) {
+    Compiler c(st, ncol, want_const_pos);
+    out->meta.reserve(out->meta.size() + (size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        ProgMeta m{};
+        m.code_off = (uint32_t)out->code.size();
+        m.size = size[i];
+        int32_t sz = size[i];
+        if (sz < 0 || sz > stride) {
+            if (err) *err = "program " + std::to_string(i) + ": size " + std::to_string(sz) +
+                            " outside [0, " + std::to_string(stride) + "]";
+            return false;
+        }
+        if (sz == 0) {
+            m.flags = kEmpty;
+            m.ncode = 0;
+            out->stats.n_empty++;
+            out->meta.push_back(m);
+            continue;
+        }
+        int slots = c.compile(phenotype + (size_t)i * stride, ephemeral + (size_t)i * stride, sz, out);
+        if (slots < 0) {
+            // Underflowing program (undefined in the reference): evaluates to NaN.
+            out->code.resize(m.code_off);
+            if (want_const_pos) out->const_pos.resize(m.code_off);
+            out->code.push_back(Ins{H_LOAD_C, 0x7fc00000u});
+            if (want_const_pos) out->const_pos.push_back(0);
+            m.flags = kInvalid;
+            slots = 0;
+            out->stats.n_invalid++;
+        }
+        if (slots > kMaxSlots) {
+            if (err) *err = "program " + std::to_string(i) + " needs " + std::to_string(slots) +
+                            " register slots (max " + std::to_string(kMaxSlots) + ")";
+            return false;
+        }
+        m.ncode = (uint32_t)out->code.size() - m.code_off;
+        m.flags |= (uint32_t)slots << 8;
+        out->stats.total_size += sz;
+        out->stats.total_code += m.ncode;
+        out->stats.max_slots = std::max(out->stats.max_slots, slots);
+        out->stats.max_code = std::max<int32_t>(out->stats.max_code, (int32_t)m.ncode);
+        out->meta.push_back(m);
+    }
+    for (size_t k = 0; k < out->code.size(); ++k) {
+        uint32_t h = out->code[k].code & 0xffu;
+        if (h == H_LOAD_A || h == H_PUSH_A || (h >= H_BIN_A && h < H_BIN_S))
+            out->stats.max_col = std::max<int32_t>(out->stats.max_col, (int32_t)out->code[k].arg);
+    }
+    return true;
+}
+
+bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
+                           const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
+                           int threads, Tape* out, std::string* err) {
+    out->meta.clear();
+    out->code.clear();
+    out->const_pos.clear();
+    out->stats = CompileStats{};
+    if (threads <= 1 || n < 4096)
+        return compile_population(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err);
+    std::vector<Tape> parts((size_t)threads);
+    std::vector<std::string> errs((size_t)threads);
+    std::vector<char> ok((size_t)threads, 1);
+    std::vector<std::thread> pool;
+    int32_t chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        int32_t lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) continue;
+        pool.emplace_back([&, t, lo, hi] {
+            ok[t] = compile_population(st, phenotype + (size_t)lo * stride, ephemeral + (size_t)lo * stride,
+                                       size + lo, stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t]);
+        });
+    }
+    for (auto& th : pool) th.join();
+    for (int t = 0; t < threads; ++t)
+        if (!ok[t]) {
+            if (err) *err = errs[t] + " (in chunk starting at " + std::to_string(t * chunk) + ")";
+            return false;
+        }
+    size_t ncode = 0;
+    for (auto& p : parts) ncode += p.code.size();
+    out->code.reserve(ncode);
+    out->meta.reserve((size_t)n);
+    for (auto& p : parts) {
+        uint32_t base = (uint32_t)out->code.size();
+        for (auto m : p.meta) { m.code_off += base; out->meta.push_back(m); }
+        out->code.insert(out->code.end(), p.code.begin(), p.code.end());
+        if (want_const_pos) out->const_pos.insert(out->const_pos.end(), p.const_pos.begin(), p.const_pos.end());
+        out->stats.total_size += p.stats.total_size;
+        out->stats.total_code += p.stats.total_code;
+        out->stats.max_slots = std::max(out->stats.max_slots, p.stats.max_slots);
+        out->stats.max_code = std::max(out->stats.max_code, p.stats.max_code);
+        out->stats.n_invalid += p.stats.n_invalid;
+        out->stats.n_empty += p.stats.n_empty;
+        out->stats.max_col = std::max(out->stats.max_col, p.stats.max_col);
+    }
+    return true;
+}
+
+}  // namespace ppi

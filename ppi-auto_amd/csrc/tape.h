@@ -1,0 +1,119 @@
+// tape.h — compiles PPI prefix programs into the engine's device tape.
+//
+// A PPI program is a prefix expression evaluated by scanning it backwards with
+// a stack (sequential.cc:143-164); the value is the final top of stack, i.e. the
+// expression rooted at position 0.  Operators are pure, so any evaluation order
+// of independent subtrees gives bit-identical results.  The compiler therefore
+// rebuilds the expression tree and emits accumulator code in Sethi-Ullman order:
+//
+//   * the running value lives in one register (the accumulator);
+//   * a terminal operand of an operator is folded into the operator's instruction
+//     (acc = acc - x[3], acc = 2.5 * acc, ...), so leaves cost no instruction;
+//   * when both operands are computed, the first one is parked in a register
+//     stack slot whose index is fixed at compile time; Sethi-Ullman ordering
+//     keeps the slot count at most log2(#leaves) + 1 (+1 per ternary level).
+//
+// Everything below the root that the reference evaluates but never reads (extra
+// stack entries left under the final top) is dead and is not emitted.
+#ifndef PPI_TAPE_H
+#define PPI_TAPE_H
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ppi_engine.h"
+
+namespace ppi {
+
+// Instruction: 64 bits.  `code` = handler (bits 0-7) | slot (bits 8-12) |
+// slot2 (bits 13-17) | ternary roles (bits 18-23); `arg` = float bits of a
+// constant operand, or an attribute column.
+struct Ins {
+    uint32_t code;
+    uint32_t arg;
+};
+
+// Device binary operators.  R* variants take the folded/parked operand as the
+// LEFT operand (the accumulator is the right one).
+enum BinOp : uint32_t {
+    B_ADD = 0, B_SUB, B_RSUB, B_MULT, B_DIV, B_RDIV, B_MEAN, B_MAX, B_MIN, B_MOD, B_RMOD,
+    B_POW, B_RPOW, B_AND, B_OR, B_XOR, B_GT, B_GE, B_LT, B_LE, B_EQ, B_NE, B_RMAX, B_RMIN,
+    B_COUNT
+};
+
+// Device unary operators.
+enum UnOp : uint32_t {
+    U_NOT = 0, U_ABS, U_POW2, U_POW3, U_POW4, U_POW5, U_NEG, U_ROUND, U_CEIL, U_FLOOR,
+    U_EXP, U_EXP10, U_EXP2, U_SIN, U_COS, U_STEP, U_SIGN, U_LOGISTIC, U_GAMMA, U_GAUSS,
+    U_SQRT, U_LOG, U_LOG10, U_LOG2, U_TAN,
+    U_COUNT
+};
+
+// Handlers (bits 0-7 of Ins::code).
+enum Handler : uint32_t {
+    H_LOAD_C = 0,   // acc = c
+    H_LOAD_A = 1,   // acc = x[col]
+    H_PUSH_C = 2,   // r[slot] = acc; acc = c
+    H_PUSH_A = 3,   // r[slot] = acc; acc = x[col]
+    H_TERN = 4,     // acc = select(cond, then, else) over {acc, r[slot], r[slot2]}
+    H_UN = 8,       // + UnOp:        acc = f(acc)
+    H_BIN_C = 40,   // + BinOp:       acc = f(acc, c)
+    H_BIN_A = 72,   // + BinOp:       acc = f(acc, x[col])
+    H_BIN_S = 104,  // + BinOp:       acc = f(acc, r[slot])
+    H_END = 136
+};
+
+constexpr int kMaxSlots = 16;
+
+// Per-program header on the device (16 bytes).
+struct ProgMeta {
+    uint32_t code_off;  // first instruction in the tape
+    uint32_t ncode;     // instructions
+    int32_t size;       // original phenotype size (for alpha*size and GPop accounting)
+    uint32_t flags;     // kEmpty | kInvalid; bits 8-15: slots used
+};
+enum : uint32_t { kEmpty = 1u, kInvalid = 2u };
+
+struct SymbolTable {
+    // symbol -> ppi_symbol, for symbols in [10000, 10000 + 8192) by direct index,
+    // others through a sorted list.
+    std::vector<int32_t> direct;  // index+1 into entries, 0 = none
+    std::vector<ppi_symbol> entries;
+    std::vector<std::pair<int32_t, int32_t>> other;  // (symbol, index) sorted
+    void build(const ppi_symbol* t, int32_t n);
+    const ppi_symbol* find(int32_t symbol) const;
+};
+
+struct CompileStats {
+    int64_t total_size = 0;     // sum of phenotype sizes
+    int64_t total_code = 0;     // sum of tape instructions
+    int32_t max_slots = 0;
+    int32_t max_code = 0;
+    int32_t n_invalid = 0;
+    int32_t n_empty = 0;
+    int32_t max_col = -1;       // largest attribute column referenced
+};
+
+struct Tape {
+    std::vector<ProgMeta> meta;
+    std::vector<Ins> code;
+    std::vector<uint32_t> const_pos;  // per instruction: phenotype position + 1 of a T_CONST operand, else 0
+    CompileStats stats;
+};
+
+// Compiles programs [0, n) into `out` (appending).  ncol bounds attribute columns
+// (an out-of-range column evaluates as NaN).  Returns false with `err` set if a
+// program needs more than kMaxSlots register slots.
+bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
+                        const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
+                        Tape* out, std::string* err);
+
+// Multi-threaded front end: splits the population over `threads` workers.
+bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
+                           const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
+                           int threads, Tape* out, std::string* err);
+
+}  // namespace ppi
+
+#endif

@@ -1,0 +1,395 @@
+"""Python binding (ctypes) of the PPI evaluation engine's C ABI, plus workload builders.
+
+The engine is the HIP library ``libppi_engine.so`` built in this directory
+(include/ppi_engine.h is its interface).  Nothing here evaluates programs: every
+evaluation goes through the HIP library, and loading fails loudly when the
+library is missing.  The workload builders produce the seeded populations and
+datasets of the benchmark configurations through ``libppi_workload.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+import sys
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "shim"))
+import grammar as _grammar  # noqa: E402
+
+ENGINE_LIB = os.path.join(HERE, "libppi_engine.so")
+WORKLOAD_LIB = os.path.join(HERE, "libppi_workload.so")
+
+# ppi_status
+OK, E_ARG, E_STATE, E_PROGRAM, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_NODEVICE = range(8)
+# ppi_strategy
+SEQ, PP, FP, PDP = 0, 1, 2, 3
+STRATEGIES = {"SEQ": SEQ, "PP": PP, "FP": FP, "DP": FP, "PDP": PDP}
+# ppi_form
+FORM_AUTO, FORM_PP, FORM_FP, FORM_PDP = 0, 1, 2, 3
+# ppi_error_fn
+EF_ABS, EF_SQUARED, EF_NEQ, EF_COST, EF_COST_CLAMP = range(5)
+
+OPS = _grammar.OPS
+
+
+class PpiError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ppi status {status}: {msg}")
+        self.status = status
+
+
+class Symbol(C.Structure):
+    _fields_ = [("symbol", C.c_int32), ("op", C.c_int32), ("value", C.c_float)]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("strategy", C.c_int32), ("form", C.c_int32), ("error_fn", C.c_int32), ("reducemax", C.c_int32),
+        ("protected_ops", C.c_int32), ("native_math", C.c_int32), ("cost_n", C.c_int32),
+        ("cost", C.POINTER(C.c_float)), ("max_local_size", C.c_int32), ("compute_units", C.c_int32),
+        ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("upload_ms", C.c_double), ("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
+        ("download_ms", C.c_double), ("node_evals", C.c_double), ("executed_nodes", C.c_double),
+        ("form", C.c_int32), ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32),
+        ("max_slots", C.c_int32),
+    ]
+
+
+_lib = None
+_wl = None
+
+ENGINE_SYMBOLS = [
+    "ppi_abi_version", "ppi_last_error", "ppi_device_count", "ppi_engine_create", "ppi_engine_destroy",
+    "ppi_engine_configure", "ppi_engine_set_dataset", "ppi_engine_set_dataset_rowmajor", "ppi_engine_upload",
+    "ppi_engine_launch", "ppi_engine_wait", "ppi_engine_evaluate", "ppi_engine_predict", "ppi_engine_get_stats",
+    "ppi_engine_time",
+]
+
+
+def lib() -> C.CDLL:
+    """Load the HIP engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ENGINE_LIB):
+            raise RuntimeError(f"{ENGINE_LIB} is missing: run `make -C {HERE}` (or __graft_entry__.build())")
+        L = C.CDLL(ENGINE_LIB)
+        P = C.POINTER
+        L.ppi_abi_version.restype = C.c_int32
+        L.ppi_last_error.restype = C.c_char_p
+        L.ppi_device_count.argtypes = [P(C.c_int32)]
+        L.ppi_engine_create.argtypes = [P(C.c_void_p), C.c_int32]
+        L.ppi_engine_destroy.argtypes = [C.c_void_p]
+        L.ppi_engine_destroy.restype = None
+        L.ppi_engine_configure.argtypes = [C.c_void_p, P(Config), P(Symbol), C.c_int32]
+        L.ppi_engine_set_dataset.argtypes = [C.c_void_p, P(P(C.c_float)), C.c_int32, C.c_int32]
+        L.ppi_engine_set_dataset_rowmajor.argtypes = [C.c_void_p, P(C.c_float), C.c_int32, C.c_int32]
+        L.ppi_engine_upload.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32, C.c_int32]
+        L.ppi_engine_launch.argtypes = [C.c_void_p]
+        L.ppi_engine_wait.argtypes = [C.c_void_p, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
+        L.ppi_engine_evaluate.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
+                                          C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
+        L.ppi_engine_predict.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32, P(C.c_float)]
+        L.ppi_engine_get_stats.argtypes = [C.c_void_p, P(Stats)]
+        L.ppi_engine_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+        for name in ENGINE_SYMBOLS:
+            if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy"):
+                getattr(L, name).restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def wl() -> C.CDLL:
+    global _wl
+    if _wl is None:
+        if not os.path.exists(WORKLOAD_LIB):
+            raise RuntimeError(f"{WORKLOAD_LIB} is missing: run `make -C {HERE}`")
+        L = C.CDLL(WORKLOAD_LIB)
+        P = C.POINTER
+        L.ppi_wl_grow.argtypes = [P(WlGrammar), C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  P(C.c_int32), P(C.c_float), P(C.c_int32)]
+        L.ppi_wl_grow.restype = C.c_int64
+        L.ppi_wl_exact.argtypes = [P(WlGrammar), C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   P(C.c_int32), P(C.c_float), P(C.c_int32)]
+        L.ppi_wl_exact.restype = C.c_int64
+        L.ppi_wl_dataset.argtypes = [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, P(C.c_float)]
+        L.ppi_wl_dataset.restype = C.c_int32
+        _wl = L
+    return _wl
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    st = lib().ppi_device_count(C.byref(n))
+    return n.value if st == OK else 0
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _check(st: int) -> None:
+    if st != OK:
+        raise PpiError(st, lib().ppi_last_error().decode(errors="replace"))
+
+
+class Engine:
+    """One engine on one HIP device (acc_interpret_init + acc_interpret equivalent)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().ppi_engine_create(C.byref(self._h), device))
+        self._cost = None
+
+    def close(self) -> None:
+        if self._h:
+            lib().ppi_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def configure(self, table: Sequence[Tuple[int, int, float]], strategy: int = SEQ, error_fn: int = EF_ABS,
+                  reducemax: bool = False, protected: bool = True, native: bool = False,
+                  cost: Optional[np.ndarray] = None, form: int = FORM_AUTO, max_local_size: int = 0,
+                  compute_units: int = 0, rows_per_lane: int = 0, programs_per_tile: int = 0) -> None:
+        cfg = Config()
+        cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
+        cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
+        if cost is not None:
+            self._cost = np.ascontiguousarray(cost, dtype=np.float32)
+            cfg.cost_n = self._cost.shape[0]
+            cfg.cost = _ptr(self._cost, C.c_float)
+        cfg.max_local_size, cfg.compute_units = max_local_size, compute_units
+        cfg.rows_per_lane, cfg.programs_per_tile = rows_per_lane, programs_per_tile
+        tab = (Symbol * max(1, len(table)))()
+        for k, (s, op, v) in enumerate(table):
+            tab[k].symbol, tab[k].op, tab[k].value = s, op, v
+        _check(lib().ppi_engine_configure(self._h, C.byref(cfg), tab, len(table)))
+
+    def set_dataset(self, data: np.ndarray) -> None:
+        """Row-major nlin x ncol float32 dataset; target in the last column."""
+        d = np.ascontiguousarray(data, dtype=np.float32)
+        _check(lib().ppi_engine_set_dataset_rowmajor(self._h, _ptr(d, C.c_float), d.shape[0], d.shape[1]))
+
+    def set_dataset_rows(self, data: np.ndarray) -> None:
+        """Same data passed as float** (one pointer per row), the reference's layout."""
+        d = np.ascontiguousarray(data, dtype=np.float32)
+        rows = (C.POINTER(C.c_float) * d.shape[0])()
+        base = d.ctypes.data
+        for r in range(d.shape[0]):
+            rows[r] = C.cast(base + r * d.shape[1] * 4, C.POINTER(C.c_float))
+        _check(lib().ppi_engine_set_dataset(self._h, rows, d.shape[0], d.shape[1]))
+
+    def upload(self, pop: "Population") -> None:
+        _check(lib().ppi_engine_upload(self._h, _ptr(pop.phenotype, C.c_int32), _ptr(pop.ephemeral, C.c_float),
+                                       _ptr(pop.size, C.c_int32), pop.stride, pop.n))
+
+    def launch(self) -> None:
+        _check(lib().ppi_engine_launch(self._h))
+
+    def wait(self, n: int, alpha: float = 0.0, best_size: int = 0):
+        vec = np.empty(n, dtype=np.float32)
+        idx = np.zeros(max(1, best_size), dtype=np.int32)
+        bs = C.c_int32(best_size)
+        _check(lib().ppi_engine_wait(self._h, alpha, _ptr(vec, C.c_float),
+                                     _ptr(idx, C.c_int32) if best_size else None,
+                                     C.byref(bs) if best_size else None))
+        return vec, idx[:bs.value] if best_size else None
+
+    def evaluate(self, pop: "Population", alpha: float = 0.0, best_size: int = 0):
+        self.upload(pop)
+        self.launch()
+        return self.wait(pop.n, alpha, best_size)
+
+    def predict(self, phenotype: np.ndarray, ephemeral: np.ndarray, size: int, nlin: int) -> np.ndarray:
+        ph = np.ascontiguousarray(phenotype, dtype=np.int32)
+        ep = np.ascontiguousarray(ephemeral, dtype=np.float32)
+        out = np.empty(nlin, dtype=np.float32)
+        _check(lib().ppi_engine_predict(self._h, _ptr(ph, C.c_int32), _ptr(ep, C.c_float), size,
+                                        _ptr(out, C.c_float)))
+        return out
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().ppi_engine_get_stats(self._h, C.byref(s)))
+        return s
+
+    def time(self, iters: int) -> float:
+        ms = C.c_double(0.0)
+        _check(lib().ppi_engine_time(self._h, iters, C.byref(ms)))
+        return ms.value
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+class WlGrammar(C.Structure):
+    _fields_ = [
+        ("binary_ops", C.POINTER(C.c_int32)), ("n_binary", C.c_int32),
+        ("unary_ops", C.POINTER(C.c_int32)), ("n_unary", C.c_int32),
+        ("t_attribute", C.c_int32), ("t_const", C.c_int32), ("n_attributes", C.c_int32),
+        ("const_lo", C.c_float), ("const_hi", C.c_float), ("p_const", C.c_float), ("p_leaf", C.c_float),
+        ("max_consts", C.c_int32),
+    ]
+
+
+@dataclasses.dataclass
+class Population:
+    phenotype: np.ndarray  # int32 [n*stride]
+    ephemeral: np.ndarray  # float32 [n*stride]
+    size: np.ndarray       # int32 [n]
+    stride: int
+
+    @property
+    def n(self) -> int:
+        return int(self.size.shape[0])
+
+    @property
+    def total_size(self) -> int:
+        return int(self.size.astype(np.int64).sum())
+
+    def subset(self, k: int) -> "Population":
+        return Population(self.phenotype[: k * self.stride].copy(), self.ephemeral[: k * self.stride].copy(),
+                          self.size[:k].copy(), self.stride)
+
+
+BINARY = ["+", "-", "*", "/"]
+UNARY = ["sin", "cos", "exp", "log", "sqrt"]
+
+
+def config_bnf(n_attributes: int, binary=BINARY, unary=UNARY, const: bool = True) -> str:
+    """The grammar of the benchmark populations, in PPI's BNF format."""
+    terms = " | ".join([f"attr{k}" for k in range(n_attributes)] + (["const"] if const else []))
+    return ("S = <expr>\n"
+            "P = <expr> ::= <bin> <expr> <expr> | <un> <expr> | <term>\n"
+            f"    <bin> ::= {' | '.join(binary)}\n"
+            f"    <un> ::= {' | '.join(unary)}\n"
+            f"    <term> ::= {terms}\n")
+
+
+class Grammar:
+    """Symbol numbering + engine table for a benchmark grammar."""
+
+    def __init__(self, n_attributes: int, binary=BINARY, unary=UNARY, const: bool = True):
+        self.bnf = config_bnf(n_attributes, binary, unary, const)
+        self.g = _grammar.parse_bnf(self.bnf)
+        self.values = self.g.values()
+        self.table = self.g.engine_table()
+        self.n_attributes = n_attributes
+        self.binary = [self.values[_grammar._terminal_name(t)] for t in binary]
+        self.unary = [self.values[_grammar._terminal_name(t)] for t in unary]
+        self.t_attribute = self.values["T_ATTRIBUTE"]
+        self.t_const = self.values.get("T_CONST", -1)
+
+    def wl(self, const_lo=-5.0, const_hi=5.0, p_const=0.3, p_leaf=0.3, max_consts=0):
+        g = WlGrammar()
+        self._b = np.array(self.binary, dtype=np.int32)
+        self._u = np.array(self.unary, dtype=np.int32)
+        g.binary_ops, g.n_binary = _ptr(self._b, C.c_int32), len(self.binary)
+        g.unary_ops, g.n_unary = _ptr(self._u, C.c_int32), len(self.unary)
+        g.t_attribute, g.t_const, g.n_attributes = self.t_attribute, self.t_const, self.n_attributes
+        g.const_lo, g.const_hi, g.p_const, g.p_leaf, g.max_consts = const_lo, const_hi, p_const, p_leaf, max_consts
+        return g
+
+
+def grow(gr: Grammar, n: int, seed: int, max_depth: int, max_symbols: int, stride: Optional[int] = None,
+         **kw) -> Population:
+    stride = stride or max_symbols
+    ph = np.zeros(n * stride, dtype=np.int32)
+    ep = np.zeros(n * stride, dtype=np.float32)
+    sz = np.zeros(n, dtype=np.int32)
+    g = gr.wl(**kw)
+    r = wl().ppi_wl_grow(C.byref(g), seed, n, stride, max_depth, max_symbols, _ptr(ph, C.c_int32),
+                         _ptr(ep, C.c_float), _ptr(sz, C.c_int32))
+    if r < 0:
+        raise ValueError("ppi_wl_grow failed")
+    return Population(ph, ep, sz, stride)
+
+
+def exact(gr: Grammar, n: int, seed: int, len_lo: int, len_hi: int, stride: Optional[int] = None, **kw) -> Population:
+    stride = stride or len_hi
+    ph = np.zeros(n * stride, dtype=np.int32)
+    ep = np.zeros(n * stride, dtype=np.float32)
+    sz = np.zeros(n, dtype=np.int32)
+    g = gr.wl(**kw)
+    r = wl().ppi_wl_exact(C.byref(g), seed, n, stride, len_lo, len_hi, _ptr(ph, C.c_int32),
+                          _ptr(ep, C.c_float), _ptr(sz, C.c_int32))
+    if r < 0:
+        raise ValueError("ppi_wl_exact failed")
+    return Population(ph, ep, sz, stride)
+
+
+DS_FORMULA_SMALL, DS_POLY_NORMAL, DS_CLUSTERS, DS_POLY_UNIFORM, DS_FORMULA_ADJOINT = 1, 2, 3, 4, 5
+
+
+def dataset(kind: int, seed: int, nlin: int, nfeat: int) -> np.ndarray:
+    out = np.zeros((nlin, nfeat + 1), dtype=np.float32)
+    if wl().ppi_wl_dataset(kind, seed, nlin, nfeat, _ptr(out, C.c_float)) != 0:
+        raise ValueError("ppi_wl_dataset failed")
+    return out
+
+
+COST4 = (1.0 - np.eye(4)).astype(np.float32)
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    grammar: Grammar
+    pop: Population
+    data: np.ndarray
+    error_fn: int
+    cost: Optional[np.ndarray] = None
+    reducemax: bool = False
+    protected: bool = True
+
+    @property
+    def node_evals(self) -> float:
+        return float(self.pop.total_size) * self.data.shape[0]
+
+
+def workload(name: str, n_programs: Optional[int] = None, nlin: Optional[int] = None, seed: int = 1) -> Workload:
+    """Seeded workloads of the benchmark configurations (sizes can be scaled down for tests).
+
+    small            1,000 programs, depth <= 6, <= 64 symbols, 1,000 x 4 U[-1,1],
+                     y = x0^2 - 0.5*x1*x2 + sin(x3), EF = |X-Y|
+    regression       30,000 programs, depth <= 10, <= 128 symbols, 16 attributes,
+                     100,000 x 16 N(0,1), y = random cubic + noise, EF = (X-Y)^2
+    classification   30,000 programs (depth <= 10, <= 128), 150,000 x 4 from 4 clusters,
+                     EF = COST[(int)clamp(X,0,3)][(int)Y] with a 0/1 cost matrix
+    """
+    if name == "small":
+        gr = Grammar(4)
+        pop = grow(gr, n_programs or 1000, seed, 6, 64)
+        data = dataset(DS_FORMULA_SMALL, seed + 1, nlin or 1000, 4)
+        return Workload(name, gr, pop, data, EF_ABS)
+    if name == "regression":
+        gr = Grammar(16)
+        pop = grow(gr, n_programs or 30000, seed, 10, 128)
+        data = dataset(DS_POLY_NORMAL, seed + 1, nlin or 100000, 16)
+        return Workload(name, gr, pop, data, EF_SQUARED)
+    if name in ("classification", "classification_max"):
+        gr = Grammar(4)
+        pop = grow(gr, n_programs or 30000, seed, 10, 128)
+        data = dataset(DS_CLUSTERS, seed + 1, nlin or 150000, 4)
+        return Workload(name, gr, pop, data, EF_COST_CLAMP, cost=COST4, reducemax=name.endswith("_max"))
+    raise ValueError(f"unknown workload {name!r}")

@@ -1,0 +1,69 @@
+"""Shared test helpers: hand-written programs, grammars with the full operator set."""
+import numpy as np
+
+import ppiamd
+
+ALL_BINARY = ["+", "-", "*", "/", "mean", "max", "min", "mod", "pow", "xor", "=", "!=", "<", "<=", ">", ">=",
+              "or", "and"]
+ALL_UNARY = ["abs", "sqrt", "pow2", "pow3", "pow4", "pow5", "neg", "round", "ceil", "floor", "exp", "exp10", "exp2",
+             "log", "log10", "log2", "sin", "cos", "tan", "step", "sign", "logistic", "gamma", "gauss", "not"]
+CONFIG_BINARY = ppiamd.BINARY
+CONFIG_UNARY = ppiamd.UNARY
+
+
+class Builder:
+    """Builds phenotype arrays from prefix token lists, e.g. ['-', 'x0', 1.5]."""
+
+    def __init__(self, n_attributes=4, extra_terminals=()):
+        bnf_terms = " | ".join([f"attr{k}" for k in range(n_attributes)] + ["const"] + list(extra_terminals))
+        bnf = ("S = <e>\nP = <e> ::= <b> <e> <e> | <u> <e> | if_then_else <e> <e> <e> | <t>\n"
+               f"    <b> ::= {' | '.join(ALL_BINARY)}\n    <u> ::= {' | '.join(ALL_UNARY)}\n    <t> ::= {bnf_terms}\n")
+        g = ppiamd._grammar.parse_bnf(bnf)
+        self.values = g.values()
+        self.table = g.engine_table()
+        self.t_attr = self.values["T_ATTRIBUTE"]
+        self.t_const = self.values["T_CONST"]
+
+    def sym(self, tok):
+        if tok == "ite":
+            return self.values["T_IF_THEN_ELSE"]
+        return self.values[ppiamd._grammar._terminal_name(tok)]
+
+    def encode(self, tokens):
+        ph, ep = [], []
+        for t in tokens:
+            if isinstance(t, (float, np.floating)):
+                ph.append(self.t_const); ep.append(float(t))
+            elif isinstance(t, str) and t.startswith("x") and t[1:].isdigit():
+                ph.append(self.t_attr); ep.append(float(int(t[1:])))
+            elif isinstance(t, tuple) and t[0] == "sym":
+                ph.append(int(t[1])); ep.append(float(t[2]) if len(t) > 2 else 0.0)
+            else:
+                ph.append(self.sym(t)); ep.append(0.0)
+        return ph, ep
+
+    def population(self, programs, stride=None):
+        enc = [self.encode(p) for p in programs]
+        stride = stride or max(1, max(len(p) for p, _ in enc))
+        n = len(enc)
+        ph = np.zeros(n * stride, dtype=np.int32)
+        ep = np.zeros(n * stride, dtype=np.float32)
+        sz = np.zeros(n, dtype=np.int32)
+        for i, (p, e) in enumerate(enc):
+            ph[i * stride:i * stride + len(p)] = p
+            ep[i * stride:i * stride + len(e)] = e
+            sz[i] = len(p)
+        return ppiamd.Population(ph, ep, sz, stride)
+
+
+def full_grammar(n_attributes):
+    return ppiamd.Grammar(n_attributes, binary=ALL_BINARY, unary=ALL_UNARY)
+
+
+def rel_close(a, b, rtol):
+    """Elementwise |a-b| <= rtol*max(|a|,|b|) with exact equality for non-finite / huge sentinels."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.maximum(np.abs(a), np.abs(b))
+    ok = (a == b) | (np.abs(a - b) <= rtol * scale)
+    return ok

@@ -1,0 +1,126 @@
+// Test-only host interpreter of the engine's compiled tape (no GPU involved).
+// Compiling the prefix programs with ppi::compile_population and executing the
+// tape here must give, bit for bit, the oracle's prefix-stack values: this pins
+// the compiler's tree rebuild, Sethi-Ullman ordering, operand orientation and
+// register-slot assignment on the CPU.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../ppi-auto_amd/csrc/tape.h"
+
+using namespace ppi;
+
+static float bf(bool b) { return b ? 1.0f : 0.0f; }
+static float dv(float x, float y, int P) { return (P && y == 0.0f) ? 1.0f : x / y; }
+static float sq(float x, int P) { return (P && x < 0.0f) ? 1.0f : (float)std::sqrt((double)x); }
+static float lg(float x, int P, int k) {
+    if (P && x < 1.0f) return 1.0f;
+    return k == 0 ? (float)std::log((double)x) : (k == 1 ? (float)std::log10((double)x) : (float)std::log2((double)x));
+}
+
+static float un(uint32_t u, float a, int P) {
+    const float PI_2 = 1.57079637050629f, PI = 3.14159274101257f, E = 2.71828174591064f;
+    switch (u) {
+    case U_NOT: return bf(!(a != 0.0f));
+    case U_ABS: return std::fabs(a);
+    case U_POW2: return a * a;
+    case U_POW3: return a * a * a;
+    case U_POW4: return a * a * a * a;
+    case U_POW5: return a * a * a * a * a;
+    case U_NEG: return -a;
+    case U_ROUND: return std::round(a);
+    case U_CEIL: return std::ceil(a);
+    case U_FLOOR: return std::floor(a);
+    case U_EXP: return (float)std::exp((double)a);
+    case U_EXP10: return (float)::exp10((double)a);
+    case U_EXP2: return (float)std::exp2((double)a);
+    case U_SIN: return (float)std::sin((double)a);
+    case U_COS: return (float)std::cos((double)a);
+    case U_STEP: return bf(a >= 0.0f);
+    case U_SIGN: return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f);
+    case U_LOGISTIC: return 1.0f / (1.0f + (float)std::exp((double)-a));
+    case U_GAMMA: {
+        float base = (a / E) * sq((float)((double)a * std::sinh((double)(1.0f / a))), P);
+        return (float)(std::pow((double)base, (double)a) * (double)sq(2.0f * PI / a, P));
+    }
+    case U_GAUSS: return (float)std::exp((double)(-a * a));
+    case U_SQRT: return sq(a, P);
+    case U_LOG: return lg(a, P, 0);
+    case U_LOG10: return lg(a, P, 1);
+    case U_LOG2: return lg(a, P, 2);
+    default:
+        if (P && (std::fabs(a) == PI_2 || std::fabs(a) == 3.0f * PI_2)) return 1.0f;
+        return (float)std::tan((double)a);
+    }
+}
+
+static float bin(uint32_t op, float a, float b, int P) {
+    switch (op) {
+    case B_ADD: return a + b;
+    case B_SUB: return a - b;
+    case B_RSUB: return b - a;
+    case B_MULT: return a * b;
+    case B_DIV: return dv(a, b, P);
+    case B_RDIV: return dv(b, a, P);
+    case B_MEAN: return (a + b) / 2.0f;
+    case B_MAX: return (float)std::fmax((double)a, (double)b);
+    case B_RMAX: return (float)std::fmax((double)b, (double)a);
+    case B_MIN: return (float)std::fmin((double)a, (double)b);
+    case B_RMIN: return (float)std::fmin((double)b, (double)a);
+    case B_MOD: return (float)std::fmod((double)a, (double)b);
+    case B_RMOD: return (float)std::fmod((double)b, (double)a);
+    case B_POW: return (float)std::pow((double)a, (double)b);
+    case B_RPOW: return (float)std::pow((double)b, (double)a);
+    case B_AND: return bf(a != 0.0f && b != 0.0f);
+    case B_OR: return bf(a != 0.0f || b != 0.0f);
+    case B_XOR: return bf((a != 0.0f) != (b != 0.0f));
+    case B_GT: return bf(a > b);
+    case B_GE: return bf(a >= b);
+    case B_LT: return bf(a < b);
+    case B_LE: return bf(a <= b);
+    case B_EQ: return bf(a == b);
+    default: return bf(a != b);
+    }
+}
+
+extern "C" int tape_eval(const ppi_symbol* table, int ntable, const int32_t* ph, const float* eph, const int32_t* size,
+                         int stride, int n, const float* data, int nlin, int ncol, int P, float* out,
+                         int32_t* slots, int32_t* ncode) {
+    SymbolTable st;
+    st.build(table, ntable);
+    Tape t;
+    std::string err;
+    if (!compile_population(st, ph, eph, size, stride, n, ncol, false, &t, &err)) return -1;
+    for (int i = 0; i < n; ++i) {
+        const ProgMeta& m = t.meta[i];
+        slots[i] = (int32_t)((m.flags >> 8) & 0xff);
+        ncode[i] = (int32_t)m.ncode;
+        for (int r = 0; r < nlin; ++r) {
+            const float* row = data + (size_t)r * ncol;
+            float acc = NAN, reg[kMaxSlots];
+            for (int k = 0; k < kMaxSlots; ++k) reg[k] = NAN;
+            if (m.flags & kEmpty) { out[(size_t)i * nlin + r] = NAN; continue; }
+            for (uint32_t pc = 0; pc < m.ncode; ++pc) {
+                const Ins in = t.code[m.code_off + pc];
+                const uint32_t h = in.code & 0xffu, s1 = (in.code >> 8) & 31u, s2 = (in.code >> 13) & 31u;
+                float cv;
+                std::memcpy(&cv, &in.arg, 4);
+                auto term = [&](bool attr) { return attr ? row[in.arg] : cv; };
+                if (h >= H_BIN_S) acc = bin(h - H_BIN_S, acc, reg[s1], P);
+                else if (h >= H_BIN_A) acc = bin(h - H_BIN_A, acc, term(true), P);
+                else if (h >= H_BIN_C) acc = bin(h - H_BIN_C, acc, term(false), P);
+                else if (h >= H_UN) acc = un(h - H_UN, acc, P);
+                else if (h == H_LOAD_C || h == H_LOAD_A) acc = term(h == H_LOAD_A);
+                else if (h == H_PUSH_C || h == H_PUSH_A) { reg[s1] = acc; acc = term(h == H_PUSH_A); }
+                else {
+                    const float v[3] = {acc, reg[s1], reg[s2]};
+                    const float c = v[(in.code >> 18) & 3u], th = v[(in.code >> 20) & 3u], el = v[(in.code >> 22) & 3u];
+                    acc = (c != 0.0f) ? th : el;
+                }
+            }
+            out[(size_t)i * nlin + r] = acc;
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded inputs.
+
+Bars (stated per check): bit-exact for COST/classification counts, for every
+non-finite/empty-program sentinel and for the PP form (which keeps the
+reference's row order); relative 1e-5 for float sums whose reduction order
+differs from the reference's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import ppiamd
+from helpers import Builder, full_grammar, rel_close
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = np.float32(1e-6)
+RTOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def eng():
+    if ppiamd.device_count() == 0:
+        pytest.fail("no HIP device visible")
+    e = ppiamd.Engine(0)
+    yield e
+    e.close()
+
+
+def run_both(eng, w, strategy, form, alpha=ALPHA, best_size=1, reducemax=None, native=False, rows_per_lane=0,
+             programs_per_tile=0, data=None, pop=None, table=None):
+    data = w.data if data is None else data
+    pop = w.pop if pop is None else pop
+    table = w.grammar.table if table is None else table
+    rm = w.reducemax if reducemax is None else reducemax
+    eng.configure(table, strategy=strategy, error_fn=w.error_fn, reducemax=rm, protected=w.protected,
+                  cost=w.cost, form=form, native=native, rows_per_lane=rows_per_lane,
+                  programs_per_tile=programs_per_tile, compute_units=256)
+    eng.set_dataset(data)
+    vec, idx = eng.evaluate(pop, alpha=alpha, best_size=best_size)
+    dpl, ppl, bpl = oracle.reference_geometry(data.shape[0], pop.n)
+    cfg = oracle.make_cfg(strategy, w.error_fn, rm, w.protected, w.cost, dpl, ppl, bpl)
+    ovec, oidx = oracle.interpret(cfg, table, pop, data, alpha=alpha, best_size=best_size)
+    return vec, idx, ovec, oidx
+
+
+def check_rank(vec, idx, ovec, oidx):
+    # The chosen program must be one whose oracle error equals the oracle's best.
+    assert np.float32(ovec[idx[0]]) == np.float32(ovec[oidx[0]]) or rel_close(ovec[idx[0]], ovec[oidx[0]], RTOL)
+
+
+@pytest.mark.parametrize("strategy", [ppiamd.SEQ, ppiamd.PP, ppiamd.FP, ppiamd.PDP])
+@pytest.mark.parametrize("form", [ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_PP])
+def test_small_all_strategies_forms(eng, strategy, form):
+    w = ppiamd.workload("small")
+    print(f"case small strategy={strategy} form={form}", flush=True)
+    vec, idx, ovec, oidx = run_both(eng, w, strategy, form)
+    ok = rel_close(vec, ovec, RTOL)
+    assert ok.all(), f"{(~ok).sum()} mismatches, first {np.nonzero(~ok)[0][:5]}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
+    if form == ppiamd.FORM_PP:
+        # PP keeps the reference's row order: bit-exact.
+        assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32))
+    check_rank(vec, idx, ovec, oidx)
+    if np.array_equal(vec.view(np.uint32), ovec.view(np.uint32)):
+        assert np.array_equal(idx, oidx)
+
+
+@pytest.mark.parametrize("reducemax", [False, True])
+@pytest.mark.parametrize("form", [ppiamd.FORM_FP, ppiamd.FORM_PDP])
+@pytest.mark.parametrize("strategy", [ppiamd.SEQ, ppiamd.PDP])
+def test_classification_bit_exact(eng, reducemax, form, strategy):
+    w = ppiamd.workload("classification", n_programs=3000, nlin=20000, seed=7)
+    print(f"case classification rm={reducemax} form={form} strategy={strategy}", flush=True)
+    vec, idx, ovec, oidx = run_both(eng, w, strategy, form, reducemax=reducemax)
+    assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32)), \
+        f"{(vec != ovec).sum()} differ: {vec[vec != ovec][:5]} vs {ovec[vec != ovec][:5]}"
+    assert np.array_equal(idx, oidx)
+
+
+@pytest.mark.parametrize("rows_per_lane", [1, 2, 4, 8])
+def test_regression_rows_per_lane(eng, rows_per_lane):
+    w = ppiamd.workload("regression", n_programs=1500, nlin=5000, seed=3)
+    print(f"case regression R={rows_per_lane}", flush=True)
+    vec, idx, ovec, oidx = run_both(eng, w, ppiamd.SEQ, ppiamd.FORM_FP, rows_per_lane=rows_per_lane,
+                                    programs_per_tile=32)
+    ok = rel_close(vec, ovec, RTOL)
+    assert ok.all(), f"{(~ok).sum()} mismatches: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
+    check_rank(vec, idx, ovec, oidx)
+
+
+@pytest.mark.parametrize("form", [ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_PP])
+def test_full_operator_set(eng, form):
+    """Every operator of interpreter_core on random programs, PROTECTED on and off."""
+    gr = full_grammar(6)
+    pop = ppiamd.grow(gr, 1200, 11, 7, 96, p_leaf=0.3)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 12, 3000, 6) * np.float32(3.0)
+    for prot in (True, False):
+        w = ppiamd.Workload("full", gr, pop, data, ppiamd.EF_ABS, protected=prot)
+        print(f"case full-ops form={form} protected={prot}", flush=True)
+        vec, idx, ovec, oidx = run_both(eng, w, ppiamd.SEQ, form)
+        ok = rel_close(vec, ovec, RTOL)
+        assert ok.all(), f"{(~ok).sum()} mismatches at {np.nonzero(~ok)[0][:5]}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
+
+
+def test_edge_programs(eng):
+    """Empty, invalid, unknown-symbol, out-of-range attribute, ternary and dead-code programs."""
+    b = Builder(4)
+    progs = [
+        ["-", "x0", "x1"],                      # SUB x0 x1 = x0 - x1
+        ["/", "x0", 0.0],                       # protected DIV -> 1
+        ["log", 2.0],                           # protected LOG(2) = log(2)
+        ["ite", ">", "x0", 0.0, "x1", "x2"],    # IF_THEN_ELSE
+        ["ite", "x3", "+", "x0", "x1", "*", "x2", "x3"],
+        ["x0", "x1", "x2"],                     # leftover stack entries: value is x0
+        ["+", "x0"],                            # underflow: NaN
+        [("sym", 4242)],                        # unknown symbol: NaN
+        [("sym", b.t_attr, 9.0)],               # attribute column out of range: NaN
+        [("sym", b.t_attr, 4.0)],               # column 4 = the target column itself
+        ["exp", "exp", "exp", "x0"],            # overflows on some rows
+        ["sqrt", "-", "x0", "x0"],              # sqrt(0)
+        ["mod", "x0", 0.0],                     # fmod(x, 0) = NaN
+        ["pow", "x0", 0.5],
+        ["gamma", "abs", "x1"],
+        ["tan", ("sym", b.values["T_CONST"], float(np.float32(1.57079637050629)))],
+    ]
+    pop = b.population(progs)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 5, 777, 4)
+    empty = ppiamd.Population(pop.phenotype.copy(), pop.ephemeral.copy(), pop.size.copy(), pop.stride)
+    empty.size[5] = 0
+    for strategy in (ppiamd.SEQ, ppiamd.PP, ppiamd.FP, ppiamd.PDP):
+        for form in (ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_PP):
+            for p in (pop, empty):
+                w = ppiamd.Workload("edge", None, p, data, ppiamd.EF_ABS)
+                print(f"case edge strategy={strategy} form={form}", flush=True)
+                vec, idx, ovec, oidx = run_both(eng, w, strategy, form, table=b.table)
+                ok = rel_close(vec, ovec, RTOL)
+                assert ok.all(), f"s={strategy} f={form}: {np.nonzero(~ok)[0]} {vec[~ok]} vs {ovec[~ok]}"
+
+
+def test_prediction_mode(eng):
+    b = Builder(4)
+    prog = ["+", "*", "x0", "x0", "sin", "x3"]
+    ph, ep = b.encode(prog)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 9, 1000, 4)
+    eng.configure(b.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_ABS, compute_units=256)
+    eng.set_dataset(data)
+    out = eng.predict(np.array(ph, np.int32), np.array(ep, np.float32), len(ph), data.shape[0])
+    pop = b.population([prog])
+    cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS)
+    ref, _ = oracle.interpret(cfg, b.table, pop, data, ppp_mode=1, prediction_mode=1)
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
