@@ -397,6 +397,35 @@ int oracle_interpret(const oracle_cfg* cfg, const ppi_symbol* table, int32_t nta
     return 0;
 }
 
+int oracle_exact_sums(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                      const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                      int32_t stride, int32_t nInd, const float* data, int32_t nlin, int32_t ncol,
+                      double* sums, int32_t* bad) {
+    if (!cfg || nlin <= 0 || ncol <= 0 || stride <= 0) return -1;
+    lut L;
+    lut_init(&L, table, ntable);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int32_t ind = 0; ind < nInd; ++ind) {
+        float* stack = (float*)malloc(sizeof(float) * (size_t)(stride + 1));
+        const int32_t* ph = phenotype + (size_t)ind * stride;
+        const float* eph = ephemeral + (size_t)ind * stride;
+        double acc = 0.0;
+        int b = size[ind] == 0;
+        for (int32_t r = 0; r < nlin && !b; ++r) {
+            const float* row = data + (size_t)r * ncol;
+            float err = ef_f32(cfg, eval_f32(&L, cfg->protected_ops, ph, eph, size[ind], row, ncol, stack),
+                               row[ncol - 1]);
+            if (!fin(err)) { b = 1; break; }
+            if (cfg->reducemax) { double v = (double)(err * (float)nlin); acc = v > acc ? v : acc; }
+            else acc += (double)err;
+        }
+        sums[ind] = acc;
+        bad[ind] = b;
+        free(stack);
+    }
+    return 0;
+}
+
 float oracle_eval_row(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
                       const int32_t* phenotype, const float* ephemeral, int32_t size,
                       const float* row, int32_t ncol) {

@@ -42,6 +42,16 @@ int oracle_interpret(const oracle_cfg* cfg, const ppi_symbol* table, int32_t nta
                      float alpha, float* vector, int32_t* index, int32_t* best_size,
                      int32_t ppp_mode, int32_t prediction_mode);
 
+/* Per-row float errors of the sequential path, summed exactly: sums[i] = sum over rows of
+   (double)ERROR(X,Y) with every row evaluated in float exactly as seq_interpret does
+   (max instead of sum under REDUCEMAX).  bad[i] = 1 if some row's error is non-finite.
+   This is the value the float sums of every path approximate; it pins reduction-order
+   effects independently of the order either path uses. */
+int oracle_exact_sums(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                      const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                      int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
+                      double* sums, int32_t* bad);
+
 /* Value of one program on one row (float, reference semantics). */
 float oracle_eval_row(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
                       const int32_t* phenotype, const float* ephemeral, int32_t size,

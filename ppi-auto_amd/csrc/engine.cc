@@ -135,7 +135,7 @@ static bool plan(ppi_engine* E, std::string* why) {
     const CompileStats& cs = E->tape.stats;
     int need_f = cs.max_col + 1;
     int F = need_f <= 8 ? 8 : (need_f <= 16 ? 16 : (need_f <= 32 ? 32 : 0));
-    int S = cs.max_slots <= 8 ? 8 : 16;
+    int S = cs.max_slots <= 12 ? 12 : 16;
     if (cs.max_slots > 16) {
         *why = "program needs more than 16 register slots";
         return false;
@@ -165,7 +165,7 @@ static bool plan(ppi_engine* E, std::string* why) {
     E->F = F;
     E->S = S;
     // Programs per FP tile: enough workgroups to fill 256 CUs several times over.
-    int64_t rows_per_block = 256LL * R;
+    int64_t rows_per_block = fp_rows_per_block(R);
     int64_t nrb = (E->nlin + rows_per_block - 1) / rows_per_block;
     int chunk = E->cfg.programs_per_tile;
     if (chunk <= 0) {
@@ -332,7 +332,7 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
         set_error(err);
         return PPI_E_PROGRAM;
     }
-    E->tape.code.push_back(Ins{H_LOAD_C, 0});  // look-ahead padding
+    E->tape.code.insert(E->tape.code.end(), kTapePad, Ins{H_END, kEmpty});  // window / look-ahead padding
     E->nprog = n;
     reference_geometry(E);
     std::string why;
@@ -510,7 +510,7 @@ ppi_status ppi_engine_predict(ppi_engine* E, const int32_t* phenotype, const flo
         return PPI_E_PROGRAM;
     }
     if (t.stats.max_slots > 16) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
-    t.code.push_back(Ins{H_LOAD_C, 0});
+    t.code.insert(t.code.end(), kTapePad, Ins{H_END, kEmpty});
     ppi_status s;
     size_t cap;
     cap = E->cap_meta; if ((s = ensure(&E->dMeta, &cap, 1))) return s; E->cap_meta = cap;

@@ -3,10 +3,10 @@
 
 namespace ppi {
 hipError_t launch_fp_r2(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st) {
-    if (F == 8 && S == 8) return launch::launch_fp_t<2, 8, 8>(A, P, N, st);
-    if (F == 16 && S == 8) return launch::launch_fp_t<2, 16, 8>(A, P, N, st);
-    if (F == 32 && S == 8) return launch::launch_fp_t<2, 32, 8>(A, P, N, st);
-    if (F == 0 && S == 8) return launch::launch_fp_t<2, 0, 8>(A, P, N, st);
+    if (F == 8 && S == 12) return launch::launch_fp_t<2, 8, 12>(A, P, N, st);
+    if (F == 16 && S == 12) return launch::launch_fp_t<2, 16, 12>(A, P, N, st);
+    if (F == 32 && S == 12) return launch::launch_fp_t<2, 32, 12>(A, P, N, st);
+    if (F == 0 && S == 12) return launch::launch_fp_t<2, 0, 12>(A, P, N, st);
     if (F == 32 && S == 16) return launch::launch_fp_t<2, 32, 16>(A, P, N, st);
     return hipErrorInvalidValue;
 }

@@ -3,10 +3,10 @@
 
 namespace ppi {
 hipError_t launch_fp_r4(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st) {
-    if (F == 8 && S == 8) return launch::launch_fp_t<4, 8, 8>(A, P, N, st);
-    if (F == 16 && S == 8) return launch::launch_fp_t<4, 16, 8>(A, P, N, st);
-    if (F == 32 && S == 8) return launch::launch_fp_t<4, 32, 8>(A, P, N, st);
-    if (F == 0 && S == 8) return launch::launch_fp_t<4, 0, 8>(A, P, N, st);
+    if (F == 8 && S == 12) return launch::launch_fp_t<4, 8, 12>(A, P, N, st);
+    if (F == 16 && S == 12) return launch::launch_fp_t<4, 16, 12>(A, P, N, st);
+    if (F == 32 && S == 12) return launch::launch_fp_t<4, 32, 12>(A, P, N, st);
+    if (F == 0 && S == 12) return launch::launch_fp_t<4, 0, 12>(A, P, N, st);
     return hipErrorInvalidValue;
 }
 }  // namespace ppi

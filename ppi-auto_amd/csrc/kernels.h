@@ -10,7 +10,14 @@
 
 namespace ppi {
 
-constexpr int kMaxChunk = 256;  // programs per FP tile (LDS reduction buffer)
+constexpr int kMaxChunk = 1024;  // programs per FP workgroup
+
+// Rows of one FP-form workgroup tile (64 lanes x R rows per lane, shared by its waves).
+inline constexpr int fp_rows_per_block(int R) { return 64 * R; }
+
+// Entries appended after the last program so that the FP form's instruction
+// window (two 64-entry blocks ahead of the current instruction) stays in bounds.
+constexpr int kTapePad = 192;
 
 // Everything a kernel needs, passed by value.
 struct EvalArgs {

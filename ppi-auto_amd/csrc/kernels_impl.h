@@ -19,6 +19,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 #include "device_ops.h"
 #include "kernels.h"
@@ -31,121 +32,169 @@ constexpr int kBlock = 64 * kWaves;
 
 template <int K> struct VecT { typedef float type __attribute__((ext_vector_type(K))); };
 
+// The tape and program headers are read-only for the whole launch and every
+// read is wave-uniform: reading them through the constant address space lets
+// the compiler issue scalar loads (s_load through the scalar cache) instead of
+// a vector load + wait per instruction.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T* p) {
+    return (ConstPtr<T>)(p);
+}
+
+__device__ __forceinline__ ProgMeta load_meta(const ProgMeta* meta, int i) {
+    ConstPtr<ProgMeta> q = as_const(meta) + i;
+    ProgMeta m;
+    m.code_off = q->code_off;
+    m.ncode = q->ncode;
+    m.size = q->size;
+    m.flags = q->flags;
+    return m;
+}
+
 // ---------------------------------------------------------------------------
-// interpreter core: one program over R rows per lane, program wave-uniform.
+// interpreter core: one tape instruction over R rows per lane, wave-uniform.
 // ---------------------------------------------------------------------------
-template <int R, int F, int S, bool P, bool N>
-struct Core {
+
+// Feature sources: where the attribute operands of a row tile come from.
+template <int R, int F>
+struct VgprFeat {  // tile held in VGPRs, one F-vector per row (PDP, prediction)
     typedef typename VecT<(F > 0 ? F : 1)>::type VF;
+    const VF (&xf)[R];
+    __device__ __forceinline__ void get(uint32_t col, float (&b)[R]) const {
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r] = xf[r][col];
+    }
+};
+template <int R>
+struct GlobalFeat {  // straight from the column-major dataset (more than 32 columns)
+    const float* __restrict__ X;
+    int64_t ldx;
+    const int (&row)[R];
+    __device__ __forceinline__ void get(uint32_t col, float (&b)[R]) const {
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r] = X[(int64_t)col * ldx + row[r]];
+    }
+};
+template <int R>
+struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP form)
+    const float* base;  // &tile[0][lane][0]
+    __device__ __forceinline__ void get(uint32_t col, float (&b)[R]) const {
+        const float* q = base + col * (64 * R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r] = q[r];
+    }
+};
+
+template <int R, int F>
+__device__ __forceinline__ auto make_feat(const EvalArgs& A, const float* lds_lane, const int (&row)[R]) {
+    if constexpr (F > 0) return LdsFeat<R>{lds_lane};
+    else return GlobalFeat<R>{A.X, A.ldx, row};
+}
+
+template <int R, int S, bool P, bool N>
+struct Core {
     typedef typename VecT<S>::type VS;
 
-    // Attribute column `col` of row r: VGPR tile (F > 0) or global memory (F == 0).
-    static __device__ __forceinline__ float feat(const VF (&xf)[R], const float* __restrict__ X, int64_t ldx,
-                                                 const int (&row)[R], int r, uint32_t col) {
-        if constexpr (F > 0) return xf[r][col];
-        else return X[(int64_t)col * ldx + row[r]];
-    }
-
+    // The slot index is wave-uniform.  With at least 9 elements per vector the
+    // backend indexes the register tuple directly (s_set_gpr_idx_on + one v_mov
+    // per row) instead of expanding into compare/select chains.
+    static_assert(S >= 9, "register stack vectors need >= 9 elements for indexed moves");
     static __device__ __forceinline__ void push(VS (&stk)[R], const float (&acc)[R], uint32_t slot) {
-        // Slot index is wave-uniform; a scalar switch keeps every write a plain v_mov.
-        switch (slot) {
-#define PPI_PUSH_CASE(K) \
-    case K: if constexpr (K < S) { _Pragma("unroll") for (int r = 0; r < R; ++r) stk[r][K] = acc[r]; } break;
-            PPI_PUSH_CASE(0) PPI_PUSH_CASE(1) PPI_PUSH_CASE(2) PPI_PUSH_CASE(3)
-            PPI_PUSH_CASE(4) PPI_PUSH_CASE(5) PPI_PUSH_CASE(6) PPI_PUSH_CASE(7)
-            PPI_PUSH_CASE(8) PPI_PUSH_CASE(9) PPI_PUSH_CASE(10) PPI_PUSH_CASE(11)
-            PPI_PUSH_CASE(12) PPI_PUSH_CASE(13) PPI_PUSH_CASE(14) PPI_PUSH_CASE(15)
-#undef PPI_PUSH_CASE
-        default: break;
-        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) stk[r][slot] = acc[r];
     }
 
-    static __device__ __forceinline__ void run(const Ins* __restrict__ code, uint32_t n, const VF (&xf)[R],
-                                               const float* __restrict__ X, int64_t ldx, const int (&row)[R],
-                                               float (&acc)[R], VS (&stk)[R]) {
-        uint32_t nc = __builtin_amdgcn_readfirstlane(code[0].code);
-        uint32_t na = __builtin_amdgcn_readfirstlane(code[0].arg);
-        for (uint32_t pc = 0; pc < n; ++pc) {
-            const uint32_t c = nc, a = na;
-            // One instruction of look-ahead; the tape carries a padding entry at its end.
-            nc = __builtin_amdgcn_readfirstlane(code[pc + 1].code);
-            na = __builtin_amdgcn_readfirstlane(code[pc + 1].arg);
-            const uint32_t h = c & 0xffu;
-            const float cv = __uint_as_float(a);
-            if (h >= H_BIN_C) {
-                float b[R];
-                uint32_t op;
-                if (h < H_BIN_A) {
-                    op = h - H_BIN_C;
+    // Executes instruction (c, a); H_END is handled by the caller.
+    template <class FeatT>
+    static __device__ __forceinline__ void step(uint32_t c, uint32_t a, const FeatT& ft, float (&acc)[R],
+                                                VS (&stk)[R]) {
+        const uint32_t h = c & 0xffu;
+        const float cv = __uint_as_float(a);
+        if (h >= H_BIN_C) {
+            float b[R];
+            uint32_t op;
+            if (h < H_BIN_A) {
+                op = h - H_BIN_C;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) b[r] = cv;
-                } else if (h < H_BIN_S) {
-                    op = h - H_BIN_A;
+                for (int r = 0; r < R; ++r) b[r] = cv;
+            } else if (h < H_BIN_S) {
+                op = h - H_BIN_A;
+                ft.get(a, b);
+            } else {
+                op = h - H_BIN_S;
+                const uint32_t slot = (c >> 8) & 31u;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) b[r] = feat(xf, X, ldx, row, r, a);
-                } else {
-                    op = h - H_BIN_S;
-                    const uint32_t slot = (c >> 8) & 31u;
-#pragma unroll
-                    for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
-                }
-                switch (op) {
+                for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
+            }
+            switch (op) {
 #define PPI_BCASE(K)                                                           \
     case K:                                                                    \
         _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = binary<P, N>(K, acc[r], b[r]); \
         break;
-                    PPI_BCASE(B_ADD) PPI_BCASE(B_SUB) PPI_BCASE(B_RSUB) PPI_BCASE(B_MULT)
-                    PPI_BCASE(B_DIV) PPI_BCASE(B_RDIV) PPI_BCASE(B_MEAN) PPI_BCASE(B_MAX)
-                    PPI_BCASE(B_MIN) PPI_BCASE(B_MOD) PPI_BCASE(B_RMOD) PPI_BCASE(B_POW)
-                    PPI_BCASE(B_RPOW) PPI_BCASE(B_AND) PPI_BCASE(B_OR) PPI_BCASE(B_XOR)
-                    PPI_BCASE(B_GT) PPI_BCASE(B_GE) PPI_BCASE(B_LT) PPI_BCASE(B_LE)
-                    PPI_BCASE(B_EQ) PPI_BCASE(B_NE) PPI_BCASE(B_RMAX) PPI_BCASE(B_RMIN)
+                PPI_BCASE(B_ADD) PPI_BCASE(B_SUB) PPI_BCASE(B_RSUB) PPI_BCASE(B_MULT)
+                PPI_BCASE(B_DIV) PPI_BCASE(B_RDIV) PPI_BCASE(B_MEAN) PPI_BCASE(B_MAX)
+                PPI_BCASE(B_MIN) PPI_BCASE(B_MOD) PPI_BCASE(B_RMOD) PPI_BCASE(B_POW)
+                PPI_BCASE(B_RPOW) PPI_BCASE(B_AND) PPI_BCASE(B_OR) PPI_BCASE(B_XOR)
+                PPI_BCASE(B_GT) PPI_BCASE(B_GE) PPI_BCASE(B_LT) PPI_BCASE(B_LE)
+                PPI_BCASE(B_EQ) PPI_BCASE(B_NE) PPI_BCASE(B_RMAX) PPI_BCASE(B_RMIN)
 #undef PPI_BCASE
-                default: break;
-                }
-            } else if (h >= H_UN) {
-                switch (h - H_UN) {
+            default: break;
+            }
+        } else if (h >= H_UN) {
+            switch (h - H_UN) {
 #define PPI_UCASE(K)                                                           \
     case K:                                                                    \
         _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = unary<P, N>(K, acc[r]); \
         break;
-                    PPI_UCASE(U_NOT) PPI_UCASE(U_ABS) PPI_UCASE(U_POW2) PPI_UCASE(U_POW3)
-                    PPI_UCASE(U_POW4) PPI_UCASE(U_POW5) PPI_UCASE(U_NEG) PPI_UCASE(U_ROUND)
-                    PPI_UCASE(U_CEIL) PPI_UCASE(U_FLOOR) PPI_UCASE(U_EXP) PPI_UCASE(U_EXP10)
-                    PPI_UCASE(U_EXP2) PPI_UCASE(U_SIN) PPI_UCASE(U_COS) PPI_UCASE(U_STEP)
-                    PPI_UCASE(U_SIGN) PPI_UCASE(U_LOGISTIC) PPI_UCASE(U_GAMMA) PPI_UCASE(U_GAUSS)
-                    PPI_UCASE(U_SQRT) PPI_UCASE(U_LOG) PPI_UCASE(U_LOG10) PPI_UCASE(U_LOG2)
-                    PPI_UCASE(U_TAN)
+                PPI_UCASE(U_NOT) PPI_UCASE(U_ABS) PPI_UCASE(U_POW2) PPI_UCASE(U_POW3)
+                PPI_UCASE(U_POW4) PPI_UCASE(U_POW5) PPI_UCASE(U_NEG) PPI_UCASE(U_ROUND)
+                PPI_UCASE(U_CEIL) PPI_UCASE(U_FLOOR) PPI_UCASE(U_EXP) PPI_UCASE(U_EXP10)
+                PPI_UCASE(U_EXP2) PPI_UCASE(U_SIN) PPI_UCASE(U_COS) PPI_UCASE(U_STEP)
+                PPI_UCASE(U_SIGN) PPI_UCASE(U_LOGISTIC) PPI_UCASE(U_GAMMA)
+                PPI_UCASE(U_GAUSS) PPI_UCASE(U_SQRT) PPI_UCASE(U_LOG)
+                PPI_UCASE(U_LOG10) PPI_UCASE(U_LOG2) PPI_UCASE(U_TAN)
 #undef PPI_UCASE
-                default: break;
-                }
-            } else if (h == H_LOAD_C) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = cv;
-            } else if (h == H_LOAD_A) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = feat(xf, X, ldx, row, r, a);
-            } else if (h == H_PUSH_C) {
-                push(stk, acc, (c >> 8) & 31u);
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = cv;
-            } else if (h == H_PUSH_A) {
-                push(stk, acc, (c >> 8) & 31u);
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = feat(xf, X, ldx, row, r, a);
-            } else {
-                // H_TERN: IF_THEN_ELSE (interpreter_core:1-4): (bool)cond ? then : else.
-                const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
-                const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
-                    const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
-                    const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
-                    const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
-                    acc[r] = (cd != 0.0f) ? th : el;
-                }
+            default: break;
             }
+        } else if (h == H_LOAD_C) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = cv;
+        } else if (h == H_LOAD_A) {
+            ft.get(a, acc);
+        } else if (h == H_PUSH_C) {
+            push(stk, acc, (c >> 8) & 31u);
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = cv;
+        } else if (h == H_PUSH_A) {
+            push(stk, acc, (c >> 8) & 31u);
+            ft.get(a, acc);
+        } else {
+            // H_TERN: IF_THEN_ELSE (interpreter_core:1-4): (bool)cond ? then : else.
+            const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
+            const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
+                const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
+                const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
+                const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
+                acc[r] = (cd != 0.0f) ? th : el;
+            }
+        }
+    }
+
+    // One program from scalar-loaded instructions (one instruction of look-ahead).
+    template <class FeatT>
+    static __device__ __forceinline__ void run(ConstPtr<Ins> code, uint32_t n, const FeatT& ft, float (&acc)[R],
+                                               VS (&stk)[R]) {
+        uint32_t nc = code[0].code, na = code[0].arg;
+        for (uint32_t pc = 0; pc < n; ++pc) {
+            const uint32_t c = nc, a = na;
+            nc = code[pc + 1].code;  // code[code_off + ncode] is the program's H_END
+            na = code[pc + 1].arg;
+            step(c, a, ft, acc, stk);
         }
     }
 };
@@ -217,17 +266,22 @@ __device__ __forceinline__ void semantics(const EvalArgs& A, float total, uint32
     *vec_out = vec;
 }
 
-// Row-tile setup shared by FP and PDP: rows, validity, features, target.
-template <int R, int F, int S, bool P, bool N>
-__device__ __forceinline__ void load_tile(const EvalArgs& A, int64_t row0, int lane,
-                                          typename Core<R, F, S, P, N>::VF (&xf)[R], int (&row)[R],
-                                          float (&y)[R], bool (&valid)[R]) {
+// Rows, validity and target of a lane's R rows of a tile starting at row0.
+template <int R>
+__device__ __forceinline__ void tile_rows(const EvalArgs& A, int64_t row0, int lane, int (&row)[R], float (&y)[R],
+                                          bool (&valid)[R]) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         row[r] = (int)(row0 + r * 64 + lane);
         valid[r] = row[r] < A.nlin;
         y[r] = A.X[(int64_t)A.ycol * A.ldx + row[r]];
     }
+}
+
+// Features of a lane's R rows into VGPRs (columns >= ncol read as 0).
+template <int R, int F>
+__device__ __forceinline__ void tile_features(const EvalArgs& A, const int (&row)[R],
+                                              typename VgprFeat<R, F>::VF (&xf)[R]) {
     if constexpr (F > 0) {
 #pragma unroll
         for (int f = 0; f < F; ++f) {
@@ -260,55 +314,101 @@ __device__ __forceinline__ void accumulate(const EvalArgs& A, const float (&acc)
     }
 }
 
+// Wave-reduce a lane partial and bad-row record; lane 0 stores them for program p.
+__device__ __forceinline__ void publish(const EvalArgs& A, int64_t slot, int p, int lane, float part, uint32_t kmin,
+                                        uint32_t kmax) {
+    part = A.reducemax ? wave_max(part) : wave_sum(part);
+    if (__any(kmin != 0xffffffffu)) {
+        kmin = wave_umin(kmin);
+        kmax = wave_umax(kmax);
+        if (lane == 0) {
+            atomicMin(&A.bad_lo[p], kmin);
+            atomicMax(&A.bad_hi[p], kmax);
+        }
+    }
+    if (lane == 0) A.partial[slot] = part;
+}
+
 // ---------------------------------------------------------------------------
 // FP form
+//
+// Workgroup = one tile of 64*R rows x one chunk of programs.  The tile's
+// features are staged once in LDS and shared by the 4 waves; each wave takes a
+// contiguous quarter of the chunk and streams through its tape.  Instructions
+// are fetched from a 64-entry window held across the wave's lanes (lane k holds
+// tape entry base+k, read back with v_readlane), refilled by one coalesced load
+// per 64 instructions and double-buffered, so no memory latency sits on the
+// dispatch path.  Programs end with H_END, where the wave reduces the program's
+// error over the tile and stores one partial.
 // ---------------------------------------------------------------------------
 template <int R, int F, int S, bool P, bool N>
 __global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
-    typedef Core<R, F, S, P, N> C;
-    __shared__ float red[kWaves][kMaxChunk];
+    typedef Core<R, S, P, N> C;
+    constexpr int FL = F > 0 ? F : 1;
+    __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int rb = blockIdx.x;
-    const int p0 = blockIdx.y * A.chunk;
-    const int p1 = min(p0 + A.chunk, A.nprog);
-    const int64_t row0 = ((int64_t)rb * kWaves + wave) * (64 * R);
+    const int64_t row0 = (int64_t)rb * (64 * R);
 
-    typename C::VF xf[R];
+    if constexpr (F > 0) {
+        for (int i = threadIdx.x; i < F * R * 64; i += kBlock) {
+            const int f = i / (R * 64), j = i - f * (R * 64);
+            tile[f][j & 63][j >> 6] = f < A.ncol ? A.X[(int64_t)f * A.ldx + row0 + j] : 0.0f;
+        }
+    }
     int row[R];
     float y[R];
     bool valid[R];
-    load_tile<R, F, S, P, N>(A, row0, lane, xf, row, y, valid);
-
-    for (int p = p0; p < p1; ++p) {
-        const ProgMeta m = A.meta[p];
-        float part = 0.0f;
-        uint32_t kmin = 0xffffffffu, kmax = 0u;
-        if (!(m.flags & kEmpty)) {
-            float acc[R];
-            typename C::VS stk[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) { acc[r] = 0.0f; stk[r] = 0.0f; }
-            C::run(A.code + m.code_off, m.ncode, xf, A.X, A.ldx, row, acc, stk);
-            accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
-        }
-        part = A.reducemax ? wave_max(part) : wave_sum(part);
-        if (__any(kmin != 0xffffffffu)) {
-            kmin = wave_umin(kmin);
-            kmax = wave_umax(kmax);
-            if (lane == 0) {
-                atomicMin(&A.bad_lo[p], kmin);
-                atomicMax(&A.bad_hi[p], kmax);
-            }
-        }
-        if (lane == 0) red[wave][p - p0] = part;
-    }
+    tile_rows<R>(A, row0, lane, row, y, valid);
     __syncthreads();
-    for (int t = threadIdx.x; t < p1 - p0; t += kBlock) {
-        float v = red[0][t];
-#pragma unroll
-        for (int w = 1; w < kWaves; ++w) v = A.reducemax ? fmaxf(v, red[w][t]) : v + red[w][t];
-        A.partial[(int64_t)rb * A.nprog + p0 + t] = v;
+
+    const int c0 = blockIdx.y * A.chunk;
+    const int c1 = min(c0 + A.chunk, A.nprog);
+    const int per = (c1 - c0 + kWaves - 1) / kWaves;
+    const int pa = c0 + wave * per;
+    const int pb = min(pa + per, c1);
+    if (pa >= pb) return;
+
+    const ProgMeta first = load_meta(A.meta, pa);
+    const ProgMeta last = load_meta(A.meta, pb - 1);
+    const int64_t gbeg = first.code_off;
+    const int64_t gend = (int64_t)last.code_off + last.ncode + 1;  // through the last H_END
+
+    typedef typename std::conditional<(F > 0), LdsFeat<R>, GlobalFeat<R>>::type FeatT;
+    FeatT ft = make_feat<R, F>(A, &tile[0][lane][0], row);
+
+    // Instruction window (tape is padded so base+127 stays in bounds).
+    int64_t base = gbeg & ~(int64_t)63;
+    const Ins* code = A.code;
+    uint32_t wc = code[base + lane].code, wa = code[base + lane].arg;
+    uint32_t nc = code[base + 64 + lane].code, na = code[base + 64 + lane].arg;
+
+    int p = pa;
+    float acc[R];
+    typename C::VS stk[R];
+    for (int64_t g = gbeg; g < gend; ++g) {
+        int k = (int)(g - base);
+        if (k == 64) {
+            base += 64;
+            k = 0;
+            wc = nc;
+            wa = na;
+            nc = code[base + 64 + lane].code;
+            na = code[base + 64 + lane].arg;
+        }
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)wc, k);
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)wa, k);
+        if ((c & 0xffu) == H_END) {
+            // H_END carries the program's flags in its argument.
+            float part = 0.0f;
+            uint32_t kmin = 0xffffffffu, kmax = 0u;
+            if (!(a & kEmpty)) accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
+            publish(A, (int64_t)rb * A.nprog + p, p, lane, part, kmin, kmax);
+            ++p;
+            continue;
+        }
+        C::step(c, a, ft, acc, stk);
     }
 }
 
@@ -331,28 +431,31 @@ __global__ void finalize_kernel(EvalArgs A, int nrb) {
 // ---------------------------------------------------------------------------
 template <int R, int F, int S, bool P, bool N>
 __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
-    typedef Core<R, F, S, P, N> C;
+    typedef Core<R, S, P, N> C;
     __shared__ float red[kWaves];
     __shared__ uint32_t rlo[kWaves], rhi[kWaves];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int p = blockIdx.x;
-    const ProgMeta m = A.meta[p];
+    const ProgMeta m = load_meta(A.meta, p);
     float part = 0.0f;
     uint32_t kmin = 0xffffffffu, kmax = 0u;
     if (!(m.flags & kEmpty)) {
         const int64_t step = (int64_t)kWaves * 64 * R;
         for (int64_t row0 = (int64_t)wave * 64 * R; row0 < A.nlin; row0 += step) {
-            typename C::VF xf[R];
             int row[R];
             float y[R];
             bool valid[R];
-            load_tile<R, F, S, P, N>(A, row0, lane, xf, row, y, valid);
+            tile_rows<R>(A, row0, lane, row, y, valid);
             float acc[R];
             typename C::VS stk[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) { acc[r] = 0.0f; stk[r] = 0.0f; }
-            C::run(A.code + m.code_off, m.ncode, xf, A.X, A.ldx, row, acc, stk);
+            if constexpr (F > 0) {
+                typename VgprFeat<R, F>::VF xf[R];
+                tile_features<R, F>(A, row, xf);
+                C::run(as_const(A.code) + m.code_off, m.ncode, VgprFeat<R, F>{xf}, acc, stk);
+            } else {
+                C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<R>{A.X, A.ldx, row}, acc, stk);
+            }
             accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
         }
     }
@@ -441,19 +544,25 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
 // ---------------------------------------------------------------------------
 template <int F, int S, bool P, bool N>
 __global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out) {
-    typedef Core<1, F, S, P, N> C;
+    typedef Core<1, S, P, N> C;
     const int lane = threadIdx.x & 63;
     const int64_t row0 = ((int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6)) * 64;
-    typename C::VF xf[1];
     int row[1];
     float y[1];
     bool valid[1];
-    load_tile<1, F, S, P, N>(A, row0, lane, xf, row, y, valid);
-    const ProgMeta m = A.meta[0];
+    tile_rows<1>(A, row0, lane, row, y, valid);
+    const ProgMeta m = load_meta(A.meta, 0);
     float acc[1] = {__builtin_nanf("")};
     typename C::VS stk[1];
-    stk[0] = 0.0f;
-    if (!(m.flags & kEmpty)) C::run(A.code + m.code_off, m.ncode, xf, A.X, A.ldx, row, acc, stk);
+    if (!(m.flags & kEmpty)) {
+        if constexpr (F > 0) {
+            typename VgprFeat<1, F>::VF xf[1];
+            tile_features<1, F>(A, row, xf);
+            C::run(as_const(A.code) + m.code_off, m.ncode, VgprFeat<1, F>{xf}, acc, stk);
+        } else {
+            C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<1>{A.X, A.ldx, row}, acc, stk);
+        }
+    }
     if (valid[0]) out[row[0]] = acc[0];
 }
 
@@ -473,7 +582,7 @@ namespace launch {
 
 template <int R, int F, int S>
 hipError_t launch_fp_t(const EvalArgs& A, bool P, bool N, hipStream_t st) {
-    const int64_t rows_per_block = (int64_t)dev::kBlock * R;
+    const int64_t rows_per_block = fp_rows_per_block(R);
     dim3 grid((unsigned)((A.nlin + rows_per_block - 1) / rows_per_block), (unsigned)((A.nprog + A.chunk - 1) / A.chunk));
     if (P && N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, true>), grid, dim3(dev::kBlock), 0, st, A);
     else if (P) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, false>), grid, dim3(dev::kBlock), 0, st, A);

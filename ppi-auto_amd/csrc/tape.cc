@@ -290,6 +290,11 @@ struct Compiler {
     }
 };
 
+void put_end(Tape* out, bool want_const_pos, uint32_t flags) {
+    out->code.push_back(Ins{H_END, flags});
+    if (want_const_pos) out->const_pos.push_back(0);
+}
+
 }  // namespace
 
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
@@ -310,6 +315,7 @@ bool compile_population(const SymbolTable& st, const int32_t* phenotype, const f
         if (sz == 0) {
             m.flags = kEmpty;
             m.ncode = 0;
+            put_end(out, want_const_pos, kEmpty);
             out->stats.n_empty++;
             out->meta.push_back(m);
             continue;
@@ -331,6 +337,7 @@ bool compile_population(const SymbolTable& st, const int32_t* phenotype, const f
             return false;
         }
         m.ncode = (uint32_t)out->code.size() - m.code_off;
+        put_end(out, want_const_pos, 0);
         m.flags |= (uint32_t)slots << 8;
         out->stats.total_size += sz;
         out->stats.total_code += m.ncode;

@@ -57,11 +57,12 @@ enum Handler : uint32_t {
     H_PUSH_C = 2,   // r[slot] = acc; acc = c
     H_PUSH_A = 3,   // r[slot] = acc; acc = x[col]
     H_TERN = 4,     // acc = select(cond, then, else) over {acc, r[slot], r[slot2]}
+    H_END = 5,      // end of program: code[code_off + ncode] of every program
     H_UN = 8,       // + UnOp:        acc = f(acc)
     H_BIN_C = 40,   // + BinOp:       acc = f(acc, c)
     H_BIN_A = 72,   // + BinOp:       acc = f(acc, x[col])
     H_BIN_S = 104,  // + BinOp:       acc = f(acc, r[slot])
-    H_END = 136
+    H_LIMIT = 136
 };
 
 constexpr int kMaxSlots = 16;
@@ -69,7 +70,7 @@ constexpr int kMaxSlots = 16;
 // Per-program header on the device (16 bytes).
 struct ProgMeta {
     uint32_t code_off;  // first instruction in the tape
-    uint32_t ncode;     // instructions
+    uint32_t ncode;     // instructions, excluding the H_END that follows them
     int32_t size;       // original phenotype size (for alpha*size and GPop accounting)
     uint32_t flags;     // kEmpty | kInvalid; bits 8-15: slots used
 };

@@ -67,3 +67,24 @@ def rel_close(a, b, rtol):
     scale = np.maximum(np.abs(a), np.abs(b))
     ok = (a == b) | (np.abs(a - b) <= rtol * scale)
     return ok
+
+
+def sums_agree(vec, ovec, exact_sum, bad, nlin, alpha, size, rtol=1e-5):
+    """Per-program agreement bar for float error sums whose reduction order differs.
+
+    A value passes if it is within rtol of the oracle's, or if it is at least as
+    close to the exact sum of the same per-row float errors (exact_sum / nlin +
+    alpha * size, computed in double) as the oracle's own sequential float sum is.
+    Programs with a non-finite row error or size 0 (FLT_MAX-based sentinels) and
+    sums that overflow must match exactly.
+    """
+    vec = np.asarray(vec, np.float64)
+    ovec = np.asarray(ovec, np.float64)
+    size = np.asarray(size, np.float64)
+    ex = np.asarray(exact_sum, np.float64) / nlin + np.float64(np.float32(alpha)) * size
+    close = rel_close(vec, ovec, rtol)
+    err_g = np.abs(vec - ex)
+    err_o = np.abs(ovec - ex)
+    better = err_g <= err_o + 1e-6 * np.abs(ex)
+    sentinel = (np.asarray(bad) != 0) | (size == 0) | (ovec >= 3.4e38) | (vec >= 3.4e38)
+    return np.where(sentinel, vec == ovec, close | better)

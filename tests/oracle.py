@@ -34,6 +34,10 @@ def lib():
                                        P(C.c_int32), C.c_int32, C.c_int32, P(C.c_float), C.c_int32, C.c_int32,
                                        C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32), C.c_int32, C.c_int32]
         L.oracle_interpret.restype = C.c_int
+        L.oracle_exact_sums.argtypes = [P(OracleCfg), P(ppiamd.Symbol), C.c_int32, P(C.c_int32), P(C.c_float),
+                                        P(C.c_int32), C.c_int32, C.c_int32, P(C.c_float), C.c_int32, C.c_int32,
+                                        P(C.c_double), P(C.c_int32)]
+        L.oracle_exact_sums.restype = C.c_int
         L.oracle_eval_row.argtypes = [P(OracleCfg), P(ppiamd.Symbol), C.c_int32, P(C.c_int32), P(C.c_float),
                                       C.c_int32, P(C.c_float), C.c_int32]
         L.oracle_eval_row.restype = C.c_float
@@ -102,6 +106,19 @@ def interpret(cfg, table, pop, data, alpha=0.0, best_size=0, ppp_mode=0, predict
                                C.byref(bs) if best_size else None, ppp_mode, prediction_mode)
     assert r == 0
     return out, (idx[:bs.value] if best_size else None)
+
+
+def exact_sums(cfg, table, pop, data):
+    """(exactly summed per-row float errors, bad flag) per program; see oracle_exact_sums."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    sums = np.zeros(pop.n, dtype=np.float64)
+    bad = np.zeros(pop.n, dtype=np.int32)
+    r = lib().oracle_exact_sums(C.byref(cfg), _table(table), len(table), _p(pop.phenotype, C.c_int32),
+                                _p(pop.ephemeral, C.c_float), _p(pop.size, C.c_int32), pop.stride, pop.n,
+                                _p(data, C.c_float), data.shape[0], data.shape[1], _p(sums, C.c_double),
+                                _p(bad, C.c_int32))
+    assert r == 0
+    return sums, bad
 
 
 def eval_row(cfg, table, phenotype, ephemeral, row):
