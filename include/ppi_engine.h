@@ -179,6 +179,51 @@ ppi_status ppi_engine_get_stats(const ppi_engine* eng, ppi_stats* out);
    programs back to back on the engine stream, return mean device ms per evaluation. */
 ppi_status ppi_engine_time(ppi_engine* eng, int32_t iters, double* ms_per_eval);
 
+/*
+ * Batched evaluation: many (population, dataset) pairs in one persistent launch.
+ * No counterpart in the reference (which evaluates one population on one dataset
+ * per call); it serves GE runs that evaluate many islands or problems on one GPU.
+ * Datasets are staged once; populations are compiled and appended; build() plans
+ * the work (autotuning rows per lane and work-item size when
+ * ppi_config.rows_per_lane is 0) and captures upload-free replay as a hipGraph.
+ */
+typedef struct ppi_batch ppi_batch;
+
+typedef struct ppi_batch_stats {
+    double graph_ms;        /* last replay: whole graph (reset, kernel, finalize, D2H), device time */
+    double kernel_ms;       /* persistent kernel alone (ppi_batch_time) */
+    double node_evals;      /* sum over pairs of sum(size) * nlin */
+    double build_ms;        /* compile + upload + plan + autotune + capture */
+    int32_t n_pairs;
+    int32_t n_items;
+    int32_t grid;           /* persistent workgroups */
+    int32_t rows_per_lane;  /* chosen R */
+    int32_t feature_slots;  /* F of the kernel shape (0 = attributes read from global memory) */
+    int32_t autotuned;      /* 1 if R was picked by measurement */
+    double tune_ms[4];      /* sampled ms per candidate R = 2, 4, 8 (0 = not tried) */
+} ppi_batch_stats;
+
+ppi_status ppi_batch_create(ppi_batch** out, int32_t device);
+void ppi_batch_destroy(ppi_batch* b);
+ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable);
+/* Row-major nlin x ncol dataset, target in the last column; returns its id. */
+ppi_status ppi_batch_add_dataset(ppi_batch* b, const float* data, int32_t nlin, int32_t ncol, int32_t* dataset_id);
+/* A population evaluated on dataset `dataset_id` (ppi.cc:312-314 layout); returns its id. */
+ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int32_t* phenotype,
+                                    const float* ephemeral, const int32_t* size, int32_t stride, int32_t n,
+                                    int32_t* pop_id);
+/* Drop all populations (datasets stay staged). */
+ppi_status ppi_batch_clear_populations(ppi_batch* b);
+ppi_status ppi_batch_build(ppi_batch* b);
+ppi_status ppi_batch_launch(ppi_batch* b);
+ppi_status ppi_batch_wait(ppi_batch* b);
+/* Results of one population after wait(), with the configured strategy's semantics and ranking. */
+ppi_status ppi_batch_results(ppi_batch* b, int32_t pop_id, float alpha, float* vector, int32_t* index,
+                             int32_t* best_size);
+/* `iters` graph replays back to back; mean device ms per replay (also times the kernel alone). */
+ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run);
+ppi_status ppi_batch_get_stats(const ppi_batch* b, ppi_batch_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,611 @@
+// batch.cc — batched evaluation of many (population, dataset) pairs in one
+// persistent launch (C ABI in include/ppi_engine.h, ppi_batch_*).
+//
+// Work items are (pair, row tile of 64*R rows, program range), where each pair's
+// programs are cut into ranges of roughly equal tape length so that every item
+// costs about the same; items are ordered longest first and pulled from an
+// atomic counter by a grid sized to the kernel's occupancy.  Rows per lane R is
+// picked by timing the candidates on an evenly spaced sample of the items.  The
+// replay (queue reset, persistent kernel, per-program epilogue, download) is
+// captured once as a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ppi_engine.h"
+#include "engine_internal.h"
+#include "kernels.h"
+#include "tape.h"
+
+using namespace ppi;
+
+namespace {
+
+#define B_HIP(call)                                                                            \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                      \
+            return PPI_E_DEVICE;                                                               \
+        }                                                                                      \
+    } while (0)
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Dataset {
+    int32_t nlin = 0, ncol = 0;
+    int64_t ldx = 0;
+    float* dX = nullptr;
+};
+
+struct Pop {
+    int32_t dataset = 0;
+    int32_t n = 0;
+    int32_t prog_base = 0;
+    int64_t total_size = 0;
+};
+
+template <typename T>
+ppi_status dalloc(T** p, size_t n) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        *p = nullptr;
+        return PPI_E_NOMEM;
+    }
+    return PPI_OK;
+}
+
+}  // namespace
+
+struct ppi_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int dev_cu = 0, dev_max_wg = 0;
+    size_t dev_lds = 0;
+
+    bool configured = false;
+    ppi_config cfg{};
+    std::vector<float> cost;
+    SymbolTable symtab;
+
+    std::vector<Dataset> datasets;
+    std::vector<Pop> pops;
+    Tape tape;                       // all populations, concatenated
+
+    // device state of the last build
+    ProgMeta* dMeta = nullptr;
+    Ins* dCode = nullptr;
+    PairDev* dPairs = nullptr;
+    WorkItem* dItems = nullptr;
+    int32_t* dPairOf = nullptr;
+    float* dPartial = nullptr;
+    uint32_t* dBadLo = nullptr;
+    uint32_t* dBadHi = nullptr;
+    float* dRaw = nullptr;
+    float* dVec = nullptr;
+    float* dCost = nullptr;
+    uint32_t* dCounter = nullptr;
+    float* hRaw = nullptr;
+    size_t hRawCap = 0;
+
+    std::vector<PairDev> pairs;
+    std::vector<WorkItem> items;
+    int R = 4, F = 32, S = 12, grid = 0;
+    bool P = false, N = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool built = false, launched = false;
+    ppi_batch_stats stats{};
+};
+
+namespace {
+
+void free_build(ppi_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    // (the queue counter lives as long as the batch object)
+    void* ptrs[] = {b->dMeta, b->dCode, b->dPairs, b->dItems, b->dPairOf, b->dPartial, b->dBadLo, b->dBadHi,
+                    b->dRaw, b->dVec};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    b->dMeta = nullptr; b->dCode = nullptr; b->dPairs = nullptr; b->dItems = nullptr; b->dPairOf = nullptr;
+    b->dPartial = nullptr; b->dBadLo = nullptr; b->dBadHi = nullptr; b->dRaw = nullptr; b->dVec = nullptr;
+    b->built = false;
+}
+
+int feature_slots(int max_col) {
+    int need = max_col + 1;
+    return need <= 16 ? 16 : (need <= 32 ? 32 : 0);
+}
+
+// Reference geometry of one pair (accelerator.cc:272-355), as the plain engine derives it.
+void pair_geometry(const ppi_batch* b, int32_t nlin, int32_t nprog, int32_t* pdp_lanes, int32_t* dp_groups,
+                   int32_t* best_local) {
+    const int cu = b->cfg.compute_units > 0 ? b->cfg.compute_units : b->dev_cu;
+    int mls1 = b->cfg.max_local_size > 0 ? b->cfg.max_local_size
+                                          : (int)std::min<size_t>((size_t)b->dev_max_wg, b->dev_lds / 4);
+    int mls2 = (int)std::min<size_t>((size_t)b->dev_max_wg, b->dev_lds / 8);
+    int dp_local = std::max(1, std::min(mls1, (int)std::ceil(nlin / (float)cu)));
+    *dp_groups = (nlin + dp_local - 1) / dp_local;
+    *pdp_lanes = std::max(1, nlin < mls1 ? nlin : mls1);
+    *best_local = std::max(1, std::min(mls2, (int)std::ceil(nprog / (float)cu)));
+}
+
+// Work items for rows per lane R: each pair's programs are cut into ranges whose
+// tape length (instructions + one per program for its epilogue) is about
+// `target`, and every range is crossed with every row tile of the pair.
+void plan_items(const ppi_batch* b, int R, double items_per_slot, std::vector<PairDev>* pairs,
+                std::vector<WorkItem>* items, int64_t* partials) {
+    pairs->clear();
+    items->clear();
+    double total = 0.0;
+    std::vector<int32_t> nrb(b->pops.size());
+    for (size_t k = 0; k < b->pops.size(); ++k) {
+        const Pop& p = b->pops[k];
+        const Dataset& d = b->datasets[p.dataset];
+        nrb[k] = (d.nlin + 64 * R - 1) / (64 * R);
+        double len = 0.0;
+        for (int32_t j = 0; j < p.n; ++j) len += b->tape.meta[p.prog_base + j].ncode + 1.0;
+        total += len * nrb[k];
+    }
+    const double slots = std::max(1.0, (double)b->grid * items_per_slot);
+    const double target = std::max(64.0, total / slots);
+    int64_t part = 0;
+    struct Cost { double c; WorkItem w; };
+    std::vector<Cost> costed;
+    for (size_t k = 0; k < b->pops.size(); ++k) {
+        const Pop& p = b->pops[k];
+        const Dataset& d = b->datasets[p.dataset];
+        PairDev pd{};
+        pd.X = d.dX;
+        pd.ldx = d.ldx;
+        pd.nlin = d.nlin;
+        pd.ncol = d.ncol;
+        pd.prog_base = p.prog_base;
+        pd.nprog = p.n;
+        pd.partial_base = part;
+        pd.nrb = nrb[k];
+        int32_t bl;
+        pair_geometry(b, d.nlin, p.n, &pd.pdp_lanes, &pd.dp_groups, &bl);
+        pairs->push_back(pd);
+        part += (int64_t)nrb[k] * p.n;
+        // Program ranges of about `target` instructions each (an item covers one row tile).
+        const double per_range = std::max(16.0, target);
+        int32_t j0 = 0;
+        double acc = 0.0;
+        for (int32_t j = 0; j < p.n; ++j) {
+            acc += b->tape.meta[p.prog_base + j].ncode + 1.0;
+            const bool cut = acc >= per_range || j + 1 == p.n;
+            if (cut) {
+                for (int32_t t = 0; t < nrb[k]; ++t) costed.push_back({acc, WorkItem{(int32_t)k, t, j0, j + 1}});
+                j0 = j + 1;
+                acc = 0.0;
+            }
+        }
+    }
+    std::stable_sort(costed.begin(), costed.end(), [](const Cost& x, const Cost& y) { return x.c > y.c; });
+    items->reserve(costed.size());
+    for (auto& c : costed) items->push_back(c.w);
+    *partials = part;
+}
+
+ppi_status upload_items(ppi_batch* b) {
+    ppi_status s;
+    if ((s = dalloc(&b->dPairs, b->pairs.size()))) return s;
+    if ((s = dalloc(&b->dItems, b->items.size()))) return s;
+    B_HIP(hipMemcpy(b->dPairs, b->pairs.data(), b->pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice));
+    B_HIP(hipMemcpy(b->dItems, b->items.data(), b->items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    return PPI_OK;
+}
+
+BatchArgs make_args(ppi_batch* b, int32_t n_items) {
+    BatchArgs B{};
+    EvalArgs& A = B.base;
+    A.meta = b->dMeta;
+    A.code = b->dCode;
+    A.chunk = 0;
+    A.strategy = b->cfg.strategy;
+    A.error_fn = b->cfg.error_fn;
+    A.reducemax = b->cfg.reducemax;
+    A.cost = b->dCost;
+    A.cost_n = b->cfg.cost_n;
+    A.alpha = 0.0f;
+    A.partial = b->dPartial;
+    A.bad_lo = b->dBadLo;
+    A.bad_hi = b->dBadHi;
+    A.raw = b->dRaw;
+    A.vec = b->dVec;
+    B.pairs = b->dPairs;
+    B.items = b->dItems;
+    B.n_items = n_items;
+    B.counter = b->dCounter;
+    return B;
+}
+
+int32_t total_programs(const ppi_batch* b) {
+    return b->pops.empty() ? 0 : b->pops.back().prog_base + b->pops.back().n;
+}
+
+// Queue reset, persistent kernel, epilogue (optionally the download).
+ppi_status enqueue_run(ppi_batch* b, int32_t n_items, bool with_finalize, bool with_download) {
+    const int32_t n = total_programs(b);
+    BatchArgs B = make_args(b, n_items);
+    B_HIP(launch_fill_u32(b->dBadLo, 0xffffffffu, n, b->stream));
+    B_HIP(launch_fill_u32(b->dBadHi, 0u, n, b->stream));
+    B_HIP(launch_fill_u32(b->dCounter, 0u, 1, b->stream));
+    B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
+    if (with_finalize) B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
+    if (with_download)
+        B_HIP(hipMemcpyAsync(b->hRaw, b->dRaw, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    return PPI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ppi_status ppi_batch_create(ppi_batch** out, int32_t device) {
+    if (!out) { set_error("out is NULL"); return PPI_E_ARG; }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { set_error("no HIP device"); return PPI_E_NODEVICE; }
+    if (device < 0 || device >= n) { set_error("bad device index"); return PPI_E_ARG; }
+    ppi_batch* b = new ppi_batch();
+    b->device = device;
+    hipDeviceProp_t prop;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dCounter, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        set_error(std::string("device init: ") + hipGetErrorString(e));
+        ppi_batch_destroy(b);
+        return PPI_E_DEVICE;
+    }
+    b->dev_cu = prop.multiProcessorCount;
+    b->dev_max_wg = prop.maxThreadsPerBlock;
+    b->dev_lds = prop.sharedMemPerBlock;
+    b->cfg.strategy = PPI_STRATEGY_SEQ;
+    b->cfg.error_fn = PPI_EF_SQUARED;
+    *out = b;
+    return PPI_OK;
+}
+
+void ppi_batch_destroy(ppi_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_build(b);
+    if (b->dCounter) (void)hipFree(b->dCounter);
+    for (auto& d : b->datasets)
+        if (d.dX) (void)hipFree(d.dX);
+    if (b->dCost) (void)hipFree(b->dCost);
+    if (b->hRaw) (void)hipHostFree(b->hRaw);
+    for (auto& ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable) {
+    if (!b || !cfg || ntable < 0 || (ntable > 0 && !table)) { set_error("bad arguments"); return PPI_E_ARG; }
+    if (cfg->strategy < PPI_STRATEGY_SEQ || cfg->strategy > PPI_STRATEGY_PDP) { set_error("bad strategy"); return PPI_E_ARG; }
+    if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
+    const bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
+    if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs a matrix"); return PPI_E_ARG; }
+    if (cfg->rows_per_lane != 0 && cfg->rows_per_lane != 2 && cfg->rows_per_lane != 4 && cfg->rows_per_lane != 8) {
+        set_error("batched rows_per_lane must be 0 (autotune), 2, 4 or 8");
+        return PPI_E_ARG;
+    }
+    (void)hipSetDevice(b->device);
+    b->cfg = *cfg;
+    b->cost.assign(cfg->cost, cfg->cost + (cost ? (size_t)cfg->cost_n * cfg->cost_n : 0));
+    b->cfg.cost = nullptr;
+    if (cost) {
+        ppi_status s = dalloc(&b->dCost, b->cost.size());
+        if (s) return s;
+        B_HIP(hipMemcpy(b->dCost, b->cost.data(), b->cost.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    b->symtab.build(table, ntable);
+    b->configured = true;
+    b->pops.clear();
+    b->tape = Tape{};
+    free_build(b);
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_add_dataset(ppi_batch* b, const float* data, int32_t nlin, int32_t ncol, int32_t* id) {
+    if (!b || !data || !id || nlin <= 0 || ncol <= 0) { set_error("bad arguments"); return PPI_E_ARG; }
+    (void)hipSetDevice(b->device);
+    Dataset d;
+    d.nlin = nlin;
+    d.ncol = ncol;
+    d.ldx = ((int64_t)nlin + 2047) / 2048 * 2048;
+    std::vector<float> cm((size_t)d.ldx * ncol, 0.0f);
+    for (int32_t c = 0; c < ncol; ++c)
+        for (int32_t r = 0; r < nlin; ++r) cm[(size_t)c * d.ldx + r] = data[(size_t)r * ncol + c];
+    ppi_status s = dalloc(&d.dX, cm.size());
+    if (s) return s;
+    B_HIP(hipMemcpy(d.dX, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice));
+    b->datasets.push_back(d);
+    *id = (int32_t)b->datasets.size() - 1;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int32_t* phenotype, const float* ephemeral,
+                                    const int32_t* size, int32_t stride, int32_t n, int32_t* pop_id) {
+    if (!b || !b->configured) { set_error("configure the batch first"); return PPI_E_STATE; }
+    if (dataset_id < 0 || dataset_id >= (int32_t)b->datasets.size()) { set_error("unknown dataset"); return PPI_E_ARG; }
+    if (!phenotype || !ephemeral || !size || stride <= 0 || n <= 0 || !pop_id) { set_error("bad program arrays"); return PPI_E_ARG; }
+    Tape t;
+    std::string err;
+    int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (!compile_population_mt(b->symtab, phenotype, ephemeral, size, stride, n, b->datasets[dataset_id].ncol, false,
+                               threads, &t, &err)) {
+        set_error(err);
+        return PPI_E_PROGRAM;
+    }
+    if (t.stats.max_slots > 16) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
+    Pop p;
+    p.dataset = dataset_id;
+    p.n = n;
+    p.prog_base = total_programs(b);
+    p.total_size = t.stats.total_size;
+    const uint32_t off = (uint32_t)b->tape.code.size();
+    for (auto m : t.meta) { m.code_off += off; b->tape.meta.push_back(m); }
+    b->tape.code.insert(b->tape.code.end(), t.code.begin(), t.code.end());
+    b->tape.stats.total_size += t.stats.total_size;
+    b->tape.stats.total_code += t.stats.total_code;
+    b->tape.stats.max_slots = std::max(b->tape.stats.max_slots, t.stats.max_slots);
+    b->tape.stats.max_col = std::max(b->tape.stats.max_col, t.stats.max_col);
+    b->pops.push_back(p);
+    b->built = false;
+    *pop_id = (int32_t)b->pops.size() - 1;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_clear_populations(ppi_batch* b) {
+    if (!b) { set_error("batch is NULL"); return PPI_E_ARG; }
+    b->pops.clear();
+    b->tape = Tape{};
+    free_build(b);
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_build(ppi_batch* b) {
+    if (!b || !b->configured) { set_error("configure the batch first"); return PPI_E_STATE; }
+    if (b->pops.empty()) { set_error("add a population first"); return PPI_E_STATE; }
+    (void)hipSetDevice(b->device);
+    const double t0 = now_ms();
+    free_build(b);
+    ppi_status s;
+    const int32_t n = total_programs(b);
+    b->P = b->cfg.protected_ops != 0;
+    b->N = b->cfg.native_math != 0;
+    b->F = feature_slots(b->tape.stats.max_col);
+    b->S = b->tape.stats.max_slots <= 12 ? 12 : 16;
+
+    std::vector<Ins> code = b->tape.code;
+    code.insert(code.end(), kTapePad, Ins{H_END, kEmpty});
+    if ((s = dalloc(&b->dMeta, b->tape.meta.size()))) return s;
+    if ((s = dalloc(&b->dCode, code.size()))) return s;
+    B_HIP(hipMemcpy(b->dMeta, b->tape.meta.data(), b->tape.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice));
+    B_HIP(hipMemcpy(b->dCode, code.data(), code.size() * sizeof(Ins), hipMemcpyHostToDevice));
+    std::vector<int32_t> pair_of((size_t)n);
+    for (size_t k = 0; k < b->pops.size(); ++k)
+        std::fill(pair_of.begin() + b->pops[k].prog_base, pair_of.begin() + b->pops[k].prog_base + b->pops[k].n,
+                  (int32_t)k);
+    if ((s = dalloc(&b->dPairOf, (size_t)n))) return s;
+    B_HIP(hipMemcpy(b->dPairOf, pair_of.data(), pair_of.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if ((s = dalloc(&b->dBadLo, (size_t)n))) return s;
+    if ((s = dalloc(&b->dBadHi, (size_t)n))) return s;
+    if ((s = dalloc(&b->dRaw, (size_t)n))) return s;
+    if ((s = dalloc(&b->dVec, (size_t)n))) return s;
+    if (b->hRawCap < (size_t)n) {
+        if (b->hRaw) (void)hipHostFree(b->hRaw);
+        b->hRaw = nullptr;
+        B_HIP(hipHostMalloc((void**)&b->hRaw, (size_t)n * sizeof(float), hipHostMallocDefault));
+        b->hRawCap = (size_t)n;
+    }
+
+    // Candidates for rows per lane; the partial buffer is sized for the smallest R.
+    std::vector<int> cands;
+    if (b->cfg.rows_per_lane > 0) cands.push_back(b->cfg.rows_per_lane);
+    else cands = {2, 4, 8};
+    cands.erase(std::remove_if(cands.begin(), cands.end(),
+                               [&](int r) { return !batch_shape_supported(r, b->F, b->S); }),
+                cands.end());
+    if (cands.empty()) {
+        if (batch_shape_supported(2, 32, 16) && b->S == 16) { cands.push_back(2); b->F = 32; }
+        else if (batch_shape_supported(4, 0, 12)) { cands.push_back(4); b->F = 0; }
+        else { set_error("no batched kernel shape for these programs"); return PPI_E_PROGRAM; }
+    }
+    int64_t max_part = 0;
+    for (int r : cands) {
+        int64_t part = 0;
+        for (size_t k = 0; k < b->pops.size(); ++k) {
+            const Dataset& d = b->datasets[b->pops[k].dataset];
+            part += (int64_t)((d.nlin + 64 * r - 1) / (64 * r)) * b->pops[k].n;
+        }
+        max_part = std::max(max_part, part);
+    }
+    if ((s = dalloc(&b->dPartial, (size_t)max_part))) return s;
+
+    const double items_per_slot = 6.0;
+    std::memset(b->stats.tune_ms, 0, sizeof(b->stats.tune_ms));
+    b->stats.autotuned = 0;
+    if (cands.size() > 1) {
+        // Time each candidate on an evenly spaced sample of its own item list.
+        double best = 1e300;
+        int best_r = cands[0];
+        for (int r : cands) {
+            b->R = r;
+            const int occ = std::max(1, batch_occupancy(r, b->F, b->S, b->P, b->N));
+            b->grid = b->dev_cu * occ;
+            int64_t part;
+            plan_items(b, r, items_per_slot, &b->pairs, &b->items, &part);
+            const size_t want = std::min<size_t>(b->items.size(), (size_t)b->grid * 2);
+            const size_t stride = std::max<size_t>(1, b->items.size() / std::max<size_t>(1, want));
+            std::vector<WorkItem> sample;
+            double sample_cost = 0.0;
+            for (size_t i = 0; i < b->items.size() && sample.size() < want; i += stride) {
+                sample.push_back(b->items[i]);
+                const WorkItem& w = b->items[i];
+                const Pop& p = b->pops[w.pair];
+                double len = 0.0;
+                for (int32_t j = w.p0; j < w.p1; ++j) len += b->tape.meta[p.prog_base + j].ncode + 1.0;
+                sample_cost += len * 64.0 * r;
+            }
+            b->items = sample;
+            if ((s = upload_items(b))) return s;
+            if ((s = enqueue_run(b, (int32_t)sample.size(), false, false))) return s;  // warm-up
+            B_HIP(hipEventRecord(b->ev[2], b->stream));
+            if ((s = enqueue_run(b, (int32_t)sample.size(), false, false))) return s;
+            B_HIP(hipEventRecord(b->ev[3], b->stream));
+            B_HIP(hipEventSynchronize(b->ev[3]));
+            float ms = 0.0f;
+            B_HIP(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+            const double per_unit = ms / std::max(1.0, sample_cost);
+            b->stats.tune_ms[r == 2 ? 0 : (r == 4 ? 1 : 2)] = ms;
+            if (per_unit < best) { best = per_unit; best_r = r; }
+        }
+        b->R = best_r;
+        b->stats.autotuned = 1;
+    } else {
+        b->R = cands[0];
+    }
+    const int occ = std::max(1, batch_occupancy(b->R, b->F, b->S, b->P, b->N));
+    b->grid = b->dev_cu * occ;
+    int64_t part;
+    plan_items(b, b->R, items_per_slot, &b->pairs, &b->items, &part);
+    if ((s = upload_items(b))) return s;
+
+    // Capture the replay: resets, persistent kernel, epilogue, download.
+    B_HIP(hipStreamSynchronize(b->stream));
+    B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    s = enqueue_run(b, (int32_t)b->items.size(), true, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (s) return s;
+    if (e != hipSuccess) { set_error(std::string("capture: ") + hipGetErrorString(e)); return PPI_E_DEVICE; }
+    b->graph = g;
+    B_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->built = true;
+    b->stats.n_pairs = (int32_t)b->pops.size();
+    b->stats.n_items = (int32_t)b->items.size();
+    b->stats.grid = b->grid;
+    b->stats.rows_per_lane = b->R;
+    b->stats.feature_slots = b->F;
+    double w = 0.0;
+    for (auto& p : b->pops) w += (double)p.total_size * b->datasets[p.dataset].nlin;
+    b->stats.node_evals = w;
+    b->stats.build_ms = now_ms() - t0;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_launch(ppi_batch* b) {
+    if (!b || !b->built) { set_error("build the batch first"); return PPI_E_STATE; }
+    (void)hipSetDevice(b->device);
+    B_HIP(hipEventRecord(b->ev[0], b->stream));
+    B_HIP(hipGraphLaunch(b->exec, b->stream));
+    B_HIP(hipEventRecord(b->ev[1], b->stream));
+    b->launched = true;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_wait(ppi_batch* b) {
+    if (!b || !b->launched) { set_error("nothing launched"); return PPI_E_STATE; }
+    b->launched = false;
+    B_HIP(hipEventSynchronize(b->ev[1]));
+    float ms = 0.0f;
+    B_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->stats.graph_ms = ms;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_results(ppi_batch* b, int32_t pop_id, float alpha, float* vector, int32_t* index,
+                             int32_t* best_size) {
+    if (!b || !b->built) { set_error("build the batch first"); return PPI_E_STATE; }
+    if (pop_id < 0 || pop_id >= (int32_t)b->pops.size()) { set_error("unknown population"); return PPI_E_ARG; }
+    const Pop& p = b->pops[pop_id];
+    std::vector<float> vec((size_t)p.n);
+    for (int32_t i = 0; i < p.n; ++i) {
+        const float raw = b->hRaw[p.prog_base + i];
+        const ProgMeta& m = b->tape.meta[p.prog_base + i];
+        const float as = alpha * (float)m.size;
+        if (b->cfg.strategy == PPI_STRATEGY_PP || b->cfg.strategy == PPI_STRATEGY_PDP) vec[i] = raw + as;
+        else vec[i] = (raw == FLT_MAX || (m.flags & kEmpty)) ? raw : raw + as;
+    }
+    if (vector) std::memcpy(vector, vec.data(), vec.size() * sizeof(float));
+    if (index && best_size && *best_size > 0) {
+        int32_t pdp, dpg, bl;
+        pair_geometry(b, b->datasets[p.dataset].nlin, p.n, &pdp, &dpg, &bl);
+        if (b->cfg.strategy == PPI_STRATEGY_SEQ) {
+            if (*best_size > p.n) *best_size = p.n;
+            pick_n_best(*best_size, index, p.n, vec.data(), nullptr);
+        } else if (b->cfg.strategy == PPI_STRATEGY_FP) {
+            best_acc(vec.data(), p.n, bl, index, best_size);
+        } else {
+            best_acc(b->hRaw + p.prog_base, p.n, bl, index, best_size);
+        }
+    }
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
+    if (!b || !b->built || iters <= 0 || !ms_per_run) { set_error("bad arguments"); return PPI_E_ARG; }
+    (void)hipSetDevice(b->device);
+    B_HIP(hipEventRecord(b->ev[0], b->stream));
+    for (int32_t k = 0; k < iters; ++k) B_HIP(hipGraphLaunch(b->exec, b->stream));
+    B_HIP(hipEventRecord(b->ev[1], b->stream));
+    B_HIP(hipEventSynchronize(b->ev[1]));
+    float ms = 0.0f;
+    B_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    *ms_per_run = ms / iters;
+    b->stats.graph_ms = *ms_per_run;
+    // The persistent kernel alone, outside the graph, for the roofline figure.
+    double kms = 0.0;
+    for (int32_t k = 0; k < iters; ++k) {
+        const int32_t n = total_programs(b);
+        BatchArgs B = make_args(b, (int32_t)b->items.size());
+        B_HIP(launch_fill_u32(b->dBadLo, 0xffffffffu, n, b->stream));
+        B_HIP(launch_fill_u32(b->dBadHi, 0u, n, b->stream));
+        B_HIP(launch_fill_u32(b->dCounter, 0u, 1, b->stream));
+        B_HIP(hipEventRecord(b->ev[2], b->stream));
+        B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
+        B_HIP(hipEventRecord(b->ev[3], b->stream));
+        B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
+        B_HIP(hipEventSynchronize(b->ev[3]));
+        float t = 0.0f;
+        B_HIP(hipEventElapsedTime(&t, b->ev[2], b->ev[3]));
+        kms += t;
+    }
+    B_HIP(hipStreamSynchronize(b->stream));
+    b->stats.kernel_ms = kms / iters;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_get_stats(const ppi_batch* b, ppi_batch_stats* out) {
+    if (!b || !out) { set_error("bad arguments"); return PPI_E_ARG; }
+    *out = b->stats;
+    return PPI_OK;
+}
+
+}  // extern "C"

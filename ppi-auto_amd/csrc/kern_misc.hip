@@ -60,3 +60,11 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 }  // namespace ppi
+
+namespace ppi {
+hipError_t launch_batch_finalize(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, hipStream_t st) {
+    hipLaunchKernelGGL(dev::batch_finalize_kernel, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, st, B,
+                       pair_of, n_total);
+    return hipGetLastError();
+}
+}  // namespace ppi

@@ -46,7 +46,39 @@ struct EvalArgs {
     float* vec;           // per program: reported error
 };
 
+// Batched form: one (population, dataset) pair as the device sees it.
+struct PairDev {
+    const float* X;        // the pair's column-major dataset
+    int64_t ldx;
+    int32_t nlin, ncol;
+    int32_t prog_base;     // first program of the pair in the concatenated population
+    int32_t nprog;
+    int64_t partial_base;  // first partial of the pair: [row_block][program] block of nrb*nprog
+    int32_t nrb;           // row tiles of 64*R rows
+    int32_t pdp_lanes, dp_groups;
+    int32_t pad_;
+};
+
+// One work item: a row tile of a pair x a program range [p0, p1) of that pair.
+struct WorkItem {
+    int32_t pair;
+    int32_t row_tile;
+    int32_t p0, p1;
+};
+
+struct BatchArgs {
+    EvalArgs base;          // tape, meta (all pairs), result and work buffers; per-pair fields overwritten
+    const PairDev* pairs;
+    const WorkItem* items;
+    int32_t n_items;
+    uint32_t* counter;      // work-queue head, reset to 0 before every launch
+};
+
 bool shape_supported(int R, int F, int S);
+bool batch_shape_supported(int R, int F, int S);
+hipError_t launch_batch(const BatchArgs& B, int grid, int R, int F, int S, bool P, bool N, hipStream_t st);
+int batch_occupancy(int R, int F, int S, bool P, bool N);
+hipError_t launch_batch_finalize(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, hipStream_t st);
 bool pdp_shape_supported(int R, int F, int S);
 hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
 hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);

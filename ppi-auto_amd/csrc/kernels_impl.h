@@ -341,14 +341,16 @@ __device__ __forceinline__ void publish(const EvalArgs& A, int64_t slot, int p, 
 // dispatch path.  Programs end with H_END, where the wave reduces the program's
 // error over the tile and stores one partial.
 // ---------------------------------------------------------------------------
+// One FP work item: rows [rb*64R, rb*64R + 64R) of A's dataset x programs [c0, c1)
+// of A's population.  Called by every thread of the workgroup; `tile` is the
+// workgroup's LDS feature buffer.  The caller separates consecutive items with a
+// barrier (the tile is rewritten at the start).
 template <int R, int F, int S, bool P, bool N>
-__global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
+__device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, int c1,
+                                             float (*tile)[64][R]) {
     typedef Core<R, S, P, N> C;
-    constexpr int FL = F > 0 ? F : 1;
-    __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int rb = blockIdx.x;
     const int64_t row0 = (int64_t)rb * (64 * R);
 
     if constexpr (F > 0) {
@@ -363,8 +365,6 @@ __global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
     tile_rows<R>(A, row0, lane, row, y, valid);
     __syncthreads();
 
-    const int c0 = blockIdx.y * A.chunk;
-    const int c1 = min(c0 + A.chunk, A.nprog);
     const int per = (c1 - c0 + kWaves - 1) / kWaves;
     const int pa = c0 + wave * per;
     const int pb = min(pa + per, c1);
@@ -412,7 +412,71 @@ __global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
     }
 }
 
+template <int R, int F, int S, bool P, bool N>
+__global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
+    constexpr int FL = F > 0 ? F : 1;
+    __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
+    const int c0 = blockIdx.y * A.chunk;
+    fp_tile_body<R, F, S, P, N>(A, (int)blockIdx.x, c0, min(c0 + A.chunk, A.nprog), tile);
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent form: one launch over many (population, dataset) pairs.
+// A fixed grid (sized to the device's occupancy) pulls work items — (pair, row
+// tile, program range), cut so that every item costs about the same and ordered
+// longest first — from an atomic counter until the list is exhausted; every
+// workgroup leaves the loop when the counter passes the end.
+// ---------------------------------------------------------------------------
+template <int R, int F, int S, bool P, bool N>
+__global__ void __launch_bounds__(kBlock) batch_kernel(BatchArgs B) {
+    constexpr int FL = F > 0 ? F : 1;
+    __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
+    __shared__ int item_s;
+    for (;;) {
+        __syncthreads();  // previous item's readers are done with the tile and item_s
+        if (threadIdx.x == 0) item_s = (int)atomicAdd(B.counter, 1u);
+        __syncthreads();
+        const int it = item_s;
+        if (it >= B.n_items) break;
+        const WorkItem w = B.items[it];
+        const PairDev pd = B.pairs[w.pair];
+        EvalArgs A = B.base;
+        A.meta = B.base.meta + pd.prog_base;
+        A.X = pd.X;
+        A.ldx = pd.ldx;
+        A.nlin = pd.nlin;
+        A.ncol = pd.ncol;
+        A.ycol = pd.ncol - 1;
+        A.nprog = pd.nprog;
+        A.pdp_lanes = pd.pdp_lanes;
+        A.dp_groups = pd.dp_groups;
+        A.partial = B.base.partial + pd.partial_base;
+        A.bad_lo = B.base.bad_lo + pd.prog_base;
+        A.bad_hi = B.base.bad_hi + pd.prog_base;
+        fp_tile_body<R, F, S, P, N>(A, w.row_tile, w.p0, w.p1, tile);
+    }
+}
+
 #ifdef PPI_KERNELS_MISC
+// Per-program epilogue of the batched form: program g belongs to pair pair_of[g].
+__global__ void batch_finalize_kernel(BatchArgs B, const int32_t* pair_of, int32_t n_total) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_total) return;
+    const PairDev pd = B.pairs[pair_of[g]];
+    const int p = g - pd.prog_base;
+    EvalArgs A = B.base;
+    A.nlin = pd.nlin;
+    A.dp_groups = pd.dp_groups;
+    A.pdp_lanes = pd.pdp_lanes;
+    float total = 0.0f;
+    for (int rb = 0; rb < pd.nrb; ++rb) {
+        const float v = B.base.partial[pd.partial_base + (int64_t)rb * pd.nprog + p];
+        total = A.reducemax ? fmaxf(total, v) : total + v;
+    }
+    const ProgMeta m = B.base.meta[g];
+    semantics(A, total, m.flags, B.base.bad_lo[g], B.base.bad_hi[g], m.size, &B.base.raw[g], &B.base.vec[g]);
+}
+
 __global__ void finalize_kernel(EvalArgs A, int nrb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.nprog) return;
@@ -589,6 +653,26 @@ hipError_t launch_fp_t(const EvalArgs& A, bool P, bool N, hipStream_t st) {
     else if (N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, true>), grid, dim3(dev::kBlock), 0, st, A);
     else hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, false>), grid, dim3(dev::kBlock), 0, st, A);
     return hipGetLastError();
+}
+
+template <int R, int F, int S>
+hipError_t launch_batch_t(const BatchArgs& B, int grid, bool P, bool N, hipStream_t st) {
+    if (P && N) hipLaunchKernelGGL((dev::batch_kernel<R, F, S, true, true>), dim3(grid), dim3(dev::kBlock), 0, st, B);
+    else if (P) hipLaunchKernelGGL((dev::batch_kernel<R, F, S, true, false>), dim3(grid), dim3(dev::kBlock), 0, st, B);
+    else if (N) hipLaunchKernelGGL((dev::batch_kernel<R, F, S, false, true>), dim3(grid), dim3(dev::kBlock), 0, st, B);
+    else hipLaunchKernelGGL((dev::batch_kernel<R, F, S, false, false>), dim3(grid), dim3(dev::kBlock), 0, st, B);
+    return hipGetLastError();
+}
+
+template <int R, int F, int S>
+int batch_occupancy_t(bool P, bool N) {
+    int n = 0;
+    hipError_t e;
+    if (P && N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::batch_kernel<R, F, S, true, true>, dev::kBlock, 0);
+    else if (P) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::batch_kernel<R, F, S, true, false>, dev::kBlock, 0);
+    else if (N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::batch_kernel<R, F, S, false, true>, dev::kBlock, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::batch_kernel<R, F, S, false, false>, dev::kBlock, 0);
+    return e == hipSuccess ? n : 0;
 }
 
 template <int R, int F, int S>

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import math
 import os
 import sys
 from typing import List, Optional, Sequence, Tuple
@@ -67,11 +68,21 @@ class Stats(C.Structure):
 _lib = None
 _wl = None
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("graph_ms", C.c_double), ("kernel_ms", C.c_double), ("node_evals", C.c_double), ("build_ms", C.c_double),
+        ("n_pairs", C.c_int32), ("n_items", C.c_int32), ("grid", C.c_int32), ("rows_per_lane", C.c_int32),
+        ("feature_slots", C.c_int32), ("autotuned", C.c_int32), ("tune_ms", C.c_double * 4),
+    ]
+
+
 ENGINE_SYMBOLS = [
     "ppi_abi_version", "ppi_last_error", "ppi_device_count", "ppi_engine_create", "ppi_engine_destroy",
     "ppi_engine_configure", "ppi_engine_set_dataset", "ppi_engine_set_dataset_rowmajor", "ppi_engine_upload",
     "ppi_engine_launch", "ppi_engine_wait", "ppi_engine_evaluate", "ppi_engine_predict", "ppi_engine_get_stats",
-    "ppi_engine_time",
+    "ppi_engine_time", "ppi_batch_create", "ppi_batch_destroy", "ppi_batch_configure", "ppi_batch_add_dataset",
+    "ppi_batch_add_population", "ppi_batch_clear_populations", "ppi_batch_build", "ppi_batch_launch",
+    "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
 ]
 
 
@@ -100,8 +111,22 @@ def lib() -> C.CDLL:
         L.ppi_engine_predict.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32, P(C.c_float)]
         L.ppi_engine_get_stats.argtypes = [C.c_void_p, P(Stats)]
         L.ppi_engine_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+        L.ppi_batch_create.argtypes = [P(C.c_void_p), C.c_int32]
+        L.ppi_batch_destroy.argtypes = [C.c_void_p]
+        L.ppi_batch_destroy.restype = None
+        L.ppi_batch_configure.argtypes = [C.c_void_p, P(Config), P(Symbol), C.c_int32]
+        L.ppi_batch_add_dataset.argtypes = [C.c_void_p, P(C.c_float), C.c_int32, C.c_int32, P(C.c_int32)]
+        L.ppi_batch_add_population.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_float), P(C.c_int32),
+                                               C.c_int32, C.c_int32, P(C.c_int32)]
+        L.ppi_batch_clear_populations.argtypes = [C.c_void_p]
+        L.ppi_batch_build.argtypes = [C.c_void_p]
+        L.ppi_batch_launch.argtypes = [C.c_void_p]
+        L.ppi_batch_wait.argtypes = [C.c_void_p]
+        L.ppi_batch_results.argtypes = [C.c_void_p, C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
+        L.ppi_batch_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+        L.ppi_batch_get_stats.argtypes = [C.c_void_p, P(BatchStats)]
         for name in ENGINE_SYMBOLS:
-            if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy"):
+            if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy", "ppi_batch_destroy"):
                 getattr(L, name).restype = C.c_int32
         _lib = L
     return _lib
@@ -236,6 +261,100 @@ class Engine:
         ms = C.c_double(0.0)
         _check(lib().ppi_engine_time(self._h, iters, C.byref(ms)))
         return ms.value
+
+
+def make_config(strategy=SEQ, error_fn=EF_ABS, reducemax=False, protected=True, native=False, cost=None,
+                form=FORM_AUTO, max_local_size=0, compute_units=0, rows_per_lane=0, programs_per_tile=0):
+    cfg = Config()
+    cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
+    cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
+    keep = None
+    if cost is not None:
+        keep = np.ascontiguousarray(cost, dtype=np.float32)
+        cfg.cost_n = keep.shape[0]
+        cfg.cost = _ptr(keep, C.c_float)
+    cfg.max_local_size, cfg.compute_units = max_local_size, compute_units
+    cfg.rows_per_lane, cfg.programs_per_tile = rows_per_lane, programs_per_tile
+    return cfg, keep
+
+
+def _table_struct(table):
+    tab = (Symbol * max(1, len(table)))()
+    for k, (s, op, v) in enumerate(table):
+        tab[k].symbol, tab[k].op, tab[k].value = s, op, v
+    return tab
+
+
+class Batch:
+    """Many (population, dataset) pairs evaluated in one persistent, graph-captured launch."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().ppi_batch_create(C.byref(self._h), device))
+        self._keep = []
+        self.sizes = []
+
+    def close(self):
+        if self._h:
+            lib().ppi_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, table, **kw):
+        cfg, keep = make_config(**kw)
+        self._keep.append(keep)
+        _check(lib().ppi_batch_configure(self._h, C.byref(cfg), _table_struct(table), len(table)))
+        self.sizes = []
+
+    def add_dataset(self, data: np.ndarray) -> int:
+        d = np.ascontiguousarray(data, dtype=np.float32)
+        i = C.c_int32(-1)
+        _check(lib().ppi_batch_add_dataset(self._h, _ptr(d, C.c_float), d.shape[0], d.shape[1], C.byref(i)))
+        return i.value
+
+    def add_population(self, dataset_id: int, pop: "Population") -> int:
+        i = C.c_int32(-1)
+        _check(lib().ppi_batch_add_population(self._h, dataset_id, _ptr(pop.phenotype, C.c_int32),
+                                              _ptr(pop.ephemeral, C.c_float), _ptr(pop.size, C.c_int32),
+                                              pop.stride, pop.n, C.byref(i)))
+        self.sizes.append(pop.n)
+        return i.value
+
+    def clear_populations(self):
+        _check(lib().ppi_batch_clear_populations(self._h))
+        self.sizes = []
+
+    def build(self):
+        _check(lib().ppi_batch_build(self._h))
+
+    def run(self):
+        _check(lib().ppi_batch_launch(self._h))
+        _check(lib().ppi_batch_wait(self._h))
+
+    def results(self, pop_id: int, alpha: float = 0.0, best_size: int = 0):
+        n = self.sizes[pop_id]
+        vec = np.empty(n, dtype=np.float32)
+        idx = np.zeros(max(1, best_size), dtype=np.int32)
+        bs = C.c_int32(best_size)
+        _check(lib().ppi_batch_results(self._h, pop_id, alpha, _ptr(vec, C.c_float),
+                                       _ptr(idx, C.c_int32) if best_size else None,
+                                       C.byref(bs) if best_size else None))
+        return vec, (idx[:bs.value] if best_size else None)
+
+    def time(self, iters: int) -> float:
+        ms = C.c_double(0.0)
+        _check(lib().ppi_batch_time(self._h, iters, C.byref(ms)))
+        return ms.value
+
+    def stats(self) -> BatchStats:
+        s = BatchStats()
+        _check(lib().ppi_batch_get_stats(self._h, C.byref(s)))
+        return s
 
 
 # ---------------------------------------------------------------------------
@@ -393,3 +512,32 @@ def workload(name: str, n_programs: Optional[int] = None, nlin: Optional[int] = 
         data = dataset(DS_CLUSTERS, seed + 1, nlin or 150000, 4)
         return Workload(name, gr, pop, data, EF_COST_CLAMP, cost=COST4, reducemax=name.endswith("_max"))
     raise ValueError(f"unknown workload {name!r}")
+
+
+@dataclasses.dataclass
+class BatchedWorkload:
+    grammars: List[Grammar]
+    pops: List[Population]
+    datas: List[np.ndarray]
+    error_fn: int = EF_SQUARED
+
+    @property
+    def node_evals(self) -> float:
+        return float(sum(p.total_size * d.shape[0] for p, d in zip(self.pops, self.datas)))
+
+
+def batched_workload(n_pairs: int = 64, seed: int = 4, p_range=(500, 20000), len_range=(8, 256),
+                     rows_range=(1000, 1000000), feat_range=(2, 32)) -> BatchedWorkload:
+    """Uneven (population, dataset) pairs: population size and row count log-uniform,
+    exact program lengths uniform, 2..32 U[-1,1] features with a random polynomial target."""
+    rng = np.random.default_rng(seed)
+    grammars, pops, datas = [], [], []
+    for k in range(n_pairs):
+        P = int(round(math.exp(rng.uniform(math.log(p_range[0]), math.log(p_range[1])))))
+        nlin = int(round(math.exp(rng.uniform(math.log(rows_range[0]), math.log(rows_range[1])))))
+        nfeat = int(rng.integers(feat_range[0], feat_range[1] + 1))
+        gr = Grammar(nfeat)
+        pops.append(exact(gr, P, seed * 1000 + 2 * k, len_range[0], len_range[1]))
+        datas.append(dataset(DS_POLY_UNIFORM, seed * 1000 + 2 * k + 1, nlin, nfeat))
+        grammars.append(gr)
+    return BatchedWorkload(grammars, pops, datas)
