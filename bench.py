@@ -65,6 +65,35 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
             "form": st.form}
 
 
+def bench_batched(steps, warmup):
+    """64 uneven (population, dataset) pairs: one graph-captured persistent launch vs the plain
+    engine evaluating the pairs one after another (device time, programs resident)."""
+    w = ppiamd.batched_workload()
+    b = ppiamd.Batch(0)
+    b.configure(w.grammars[0].table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+    for p, d in zip(w.pops, w.datas):
+        b.add_population(b.add_dataset(d), p)
+    b.build()
+    for _ in range(warmup):
+        b.run()
+    ms = b.time(steps)
+    st = b.stats()
+    b.close()
+    eng = ppiamd.Engine(0)
+    plain_ms = 0.0
+    for gr, p, d in zip(w.grammars, w.pops, w.datas):
+        eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+        eng.set_dataset(d)
+        eng.upload(p)
+        eng.time(1)
+        plain_ms += eng.time(1)
+    eng.close()
+    return {"node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
+            "rows_per_lane": st.rows_per_lane, "items": st.n_items, "grid": st.grid,
+            "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs,
+            "programs": int(sum(p.n for p in w.pops))}
+
+
 def cpu_baseline(w, budget_s, threads):
     """Oracle (restated sequential path) on the first k programs x all rows."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -108,8 +137,8 @@ def main():
         log("timing PDP form")
         pdp = bench_engine(eng, w, ppiamd.FORM_PDP, max(1, a.steps // 5), 1)
         extra["pdp_gpops"] = pdp["node_evals"] / (pdp["ms_per_step"] * 1e-3) / 1e9
-        log("timing PP form (2,000-program sample)")
-        wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(2000), w.data, w.error_fn)
+        log("timing PP form (500-program sample)")
+        wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(500), w.data, w.error_fn)
         pp = bench_engine(eng, wpp, ppiamd.FORM_PP, 1, 0)
         extra["pp_gpops"] = pp["node_evals"] / (pp["ms_per_step"] * 1e-3) / 1e9
         log("timing classification (COST, 30,000 programs x 150,000 rows)")
@@ -117,6 +146,10 @@ def main():
         cl = bench_engine(eng, wc, ppiamd.FORM_FP, max(1, a.steps // 2), 1)
         extra["classification_gpops"] = cl["node_evals"] / (cl["ms_per_step"] * 1e-3) / 1e9
     eng.close()
+    batched = None
+    if not a.quick:
+        log("batched: 64 uneven pairs (populations 500-20,000, lengths 8-256, rows 1e3-1e6, 2-32 features)")
+        batched = bench_batched(max(1, a.steps // 5), 1)
 
     log("CPU baseline (oracle restatement of seq_interpret)")
     model, ncpu = host_info()
@@ -153,6 +186,18 @@ def main():
         "vs_cpu": round(value / (cpu_rate / 1e9), 2),
     }
     line.update({k: round(v, 3) for k, v in extra.items()})
+    if batched:
+        bg = batched["node_evals"] / (batched["graph_ms"] * 1e-3) / 1e9
+        pg = batched["node_evals"] / (batched["plain_ms"] * 1e-3) / 1e9
+        line["batched"] = {
+            "workload": f"{batched['pairs']} pairs, {batched['programs']} programs, "
+                        f"{batched['node_evals']:.3e} node-evaluations per step",
+            "gpops": round(bg, 3), "ms_per_step": round(batched["graph_ms"], 3),
+            "plain_gpops": round(pg, 3), "plain_ms": round(batched["plain_ms"], 3),
+            "vs_plain": round(bg / pg, 3), "vs_cpu": round(bg / (cpu_rate / 1e9), 2),
+            "kernel_frac_of_valu_peak": round(batched["node_evals"] / (batched["kernel_ms"] * 1e-3) / VALU_PEAK, 5),
+            "rows_per_lane": batched["rows_per_lane"], "items": batched["items"], "grid": batched["grid"],
+            "autotune_ms_R2_R4_R8": batched["tune_ms"]}
     print(json.dumps(line), flush=True)
 
 
