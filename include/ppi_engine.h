@@ -224,6 +224,70 @@ ppi_status ppi_batch_results(ppi_batch* b, int32_t pop_id, float alpha, float* v
 ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run);
 ppi_status ppi_batch_get_stats(const ppi_batch* b, ppi_batch_stats* out);
 
+/*
+ * Reverse-mode derivatives of the fitness with respect to the ephemeral constants.
+ * No counterpart in the reference (its T_CONST values are set by the genome,
+ * ppi.cc:124-137, and never refined); it serves constant refinement on the device
+ * while the grammar and the GE operators stay as they are.
+ *
+ * Shaped like an autograd function:
+ *   forward   fitness[i] of each program (sequential-path semantics, the engine's FP
+ *             form), and the programs are kept for backward;
+ *   backward  grad[i*stride + j] = grad_fitness[i] * d fitness[i] / d ephemeral[i*stride + j]
+ *             for every T_CONST position j (0 elsewhere), by a reverse sweep over the
+ *             program tape;
+ *   descend   `steps` normalised gradient steps with per-program backtracking, all on
+ *             the device; returns the refined constants and the plain evaluator's
+ *             fitness of the refined programs.
+ * Supported error functions: PPI_EF_ABS and PPI_EF_SQUARED with the sum reduction.
+ * A program whose fitness is a non-finite sentinel (some row's error is not finite,
+ * or size 0) gets gradient 0.  Constants under a comparison, a logical operator,
+ * ROUND/CEIL/FLOOR/STEP/SIGN or the condition of IF_THEN_ELSE have gradient 0.
+ */
+typedef struct ppi_adjoint ppi_adjoint;
+
+typedef enum ppi_partials {
+    PPI_PARTIALS_AUTO = 0,       /* pick by measurement */
+    PPI_PARTIALS_RECOMPUTE = 1,  /* 16 partials per row in registers; longer programs re-run the forward pass */
+    PPI_PARTIALS_STORE = 2       /* 32 partials per row in LDS; most programs need one forward pass */
+} ppi_partials;
+
+typedef struct ppi_adjoint_stats {
+    double compile_ms;       /* adjoint tape compile + upload (last backward/descend) */
+    double backward_ms;      /* last backward: adjoint kernel + epilogue, device time */
+    double kernel_ms;        /* adjoint kernel alone, device time (ppi_adjoint_time) */
+    double descend_ms;       /* last descend: all steps, device time */
+    double node_evals;       /* sum(size) * nlin */
+    double forward_ins;      /* executed forward tape instructions x rows, all jobs and segments */
+    double reverse_ins;      /* executed reverse instructions x rows */
+    int32_t n_jobs;
+    int32_t n_consts;        /* constants with a non-zero derivative path */
+    int32_t max_segments;
+    int32_t partials;        /* ppi_partials used last */
+    int32_t rows_per_lane;
+    int32_t ranges;          /* row ranges per job */
+    int32_t descend_steps;
+    int32_t pad_;
+    double tune_ms[2];       /* AUTO: sampled ms of RECOMPUTE and STORE */
+} ppi_adjoint_stats;
+
+ppi_status ppi_adjoint_create(ppi_adjoint** out, int32_t device);
+void ppi_adjoint_destroy(ppi_adjoint* a);
+/* Same configuration as an engine (strategy, form and rows_per_lane apply to forward). */
+ppi_status ppi_adjoint_configure(ppi_adjoint* a, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable);
+ppi_status ppi_adjoint_set_dataset_rowmajor(ppi_adjoint* a, const float* data, int32_t nlin, int32_t ncol);
+ppi_status ppi_adjoint_forward(ppi_adjoint* a, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                               int32_t stride, int32_t n_programs, float alpha, float* fitness);
+/* grad_fitness: n_programs upstream gradients, or NULL for all ones.  grad: n_programs*stride. */
+ppi_status ppi_adjoint_backward(ppi_adjoint* a, const float* grad_fitness, int32_t partials, float* grad);
+/* ephemeral_out: n_programs*stride (the forward's ephemerals with refined T_CONST values);
+   fitness_out: n_programs; either may be NULL.  The kept programs are updated. */
+ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_t partials, float* ephemeral_out,
+                               float* fitness_out);
+/* `iters` backward passes back to back (no host transfers); mean device ms per pass. */
+ppi_status ppi_adjoint_time(ppi_adjoint* a, int32_t iters, int32_t partials, double* ms_per_pass);
+ppi_status ppi_adjoint_get_stats(const ppi_adjoint* a, ppi_adjoint_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

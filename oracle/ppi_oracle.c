@@ -540,9 +540,12 @@ static dual dapply(int op, const dual* a, const dual* b, const dual* c, int K, i
 }
 
 /* Prefix program in double with K tangent components; constant j (in increasing
-   position order) seeds component j. */
+   position order) seeds component j.  f32v: every operator's value is replaced by
+   the sequential path's float result (apply1/apply2) while its tangent stays the
+   exact derivative at the float operands, i.e. the derivative of the computation
+   the float evaluators perform, free of float-vs-double forward differences. */
 static dual eval_dual(const lut* L, int prot, const int32_t* ph, const double* eph, int32_t size,
-                      const float* row, int32_t ncol, int K, dual* stack, const int32_t* const_slot) {
+                      const float* row, int32_t ncol, int K, dual* stack, const int32_t* const_slot, int f32v) {
     int top = -1;
     for (int32_t i = size - 1; i >= 0; --i) {
         const ppi_symbol* s = lut_find(L, ph[i]);
@@ -553,10 +556,14 @@ static dual eval_dual(const lut* L, int prot, const int32_t* ph, const double* e
             stack[top - 2] = dapply(op, &stack[top], &stack[top - 1], &stack[top - 2], K, prot);
             top -= 2;
         } else if (ar == 2) {
+            const double x = stack[top].v, y = stack[top - 1].v;
             stack[top - 1] = dapply(op, &stack[top], &stack[top - 1], NULL, K, prot);
+            if (f32v) stack[top - 1].v = (double)apply2(op, (float)x, (float)y, prot);
             --top;
         } else if (ar == 1) {
+            const double x = stack[top].v;
             stack[top] = dapply(op, &stack[top], NULL, NULL, K, prot);
+            if (f32v) stack[top].v = (double)apply1(op, (float)x, prot);
         } else {
             dual* t = &stack[++top];
             memset(t, 0, sizeof(*t));
@@ -598,7 +605,8 @@ static double ef_f64(const oracle_cfg* c, double X, double Y, double* dX) {
 static int grad_impl(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
                      const int32_t* phenotype, const double* eph64, const int32_t* size,
                      int32_t stride, int32_t nInd, const float* data, int32_t nlin, int32_t ncol,
-                     int32_t max_consts, double* fitness64, double* grad, int32_t* n_consts) {
+                     int32_t max_consts, double* fitness64, double* grad, int32_t* n_consts, double* gmag,
+                     int f32v) {
     if (!cfg || nlin <= 0 || ncol <= 0 || stride <= 0 || max_consts > MAXK) return -1;
     lut L;
     lut_init(&L, table, ntable);
@@ -619,13 +627,13 @@ static int grad_impl(const oracle_cfg* cfg, const ppi_symbol* table, int32_t nta
         }
         int K = nc < max_consts ? nc : max_consts;
         if (n_consts) n_consts[ind] = nc;
-        double g[MAXK] = {0};
+        double g[MAXK] = {0}, gm[MAXK] = {0};
         double acc = 0.0;
         int bad = (sz == 0);
         double best = -1.0;
         for (int32_t r = 0; r < nlin && !bad; ++r) {
             const float* row = data + (size_t)r * ncol;
-            dual v = eval_dual(&L, cfg->protected_ops, ph, eph, sz, row, ncol, K, stack, slot);
+            dual v = eval_dual(&L, cfg->protected_ops, ph, eph, sz, row, ncol, K, stack, slot, f32v);
             double dX;
             double e = ef_f64(cfg, v.v, (double)row[ncol - 1], &dX);
             if (isinf(e) || isnan(e)) { bad = 1; break; }
@@ -633,7 +641,7 @@ static int grad_impl(const oracle_cfg* cfg, const ppi_symbol* table, int32_t nta
                 if (e > best) { best = e; for (int k = 0; k < K; ++k) g[k] = dX * v.d[k]; }
             } else {
                 acc += e;
-                for (int k = 0; k < K; ++k) g[k] += dX * v.d[k];
+                for (int k = 0; k < K; ++k) { g[k] += dX * v.d[k]; gm[k] += fabs(dX * v.d[k]); }
             }
         }
         if (fitness64) fitness64[ind] = bad ? INFINITY : (cfg->reducemax ? best : acc / (double)nlin);
@@ -641,6 +649,9 @@ static int grad_impl(const oracle_cfg* cfg, const ppi_symbol* table, int32_t nta
             for (int k = 0; k < max_consts; ++k)
                 grad[(size_t)ind * max_consts + k] =
                     bad ? NAN : (k < K ? (cfg->reducemax ? g[k] : g[k] / (double)nlin) : 0.0);
+        if (gmag)
+            for (int k = 0; k < max_consts; ++k)
+                gmag[(size_t)ind * max_consts + k] = (bad || k >= K) ? 0.0 : gm[k] / (double)nlin;
         free(stack);
         free(slot);
     }
@@ -652,7 +663,7 @@ int oracle_fitness_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t n
                        int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
                        double* fitness64) {
     return grad_impl(cfg, table, ntable, phenotype, ephemeral64, size, stride, n_programs, data, nlin, ncol,
-                     0, fitness64, NULL, NULL);
+                     0, fitness64, NULL, NULL, NULL, 0);
 }
 
 int oracle_grad_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
@@ -660,7 +671,15 @@ int oracle_grad_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntab
                     int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
                     int32_t max_consts, double* fitness64, double* grad, int32_t* n_consts) {
     return grad_impl(cfg, table, ntable, phenotype, ephemeral64, size, stride, n_programs, data, nlin, ncol,
-                     max_consts, fitness64, grad, n_consts);
+                     max_consts, fitness64, grad, n_consts, NULL, 0);
+}
+
+int oracle_grad_mag_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                        const int32_t* phenotype, const double* ephemeral64, const int32_t* size,
+                        int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
+                        int32_t max_consts, double* grad, double* gmag, int32_t f32_values) {
+    return grad_impl(cfg, table, ntable, phenotype, ephemeral64, size, stride, n_programs, data, nlin, ncol,
+                     max_consts, NULL, grad, NULL, gmag, f32_values);
 }
 
 /* ------------------------------------------------------------------------- */

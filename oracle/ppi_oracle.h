@@ -74,6 +74,15 @@ int oracle_grad_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntab
                     const int32_t* phenotype, const double* ephemeral64, const int32_t* size,
                     int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
                     int32_t max_consts, double* fitness64, double* grad, int32_t* n_consts);
+/* Same gradient, plus gmag[i][k] = sum over rows of |EF'(pred) * d pred / d const_k| / nlin: the
+   magnitude of the terms the gradient sums, which bounds how far rounding of the terms can move it.
+   f32_values != 0: operator values are the sequential path's float results (the tangents stay
+   FP64 derivatives at those float operands), so the gradient is the exact derivative of the
+   float computation; 0: everything in double. */
+int oracle_grad_mag_f64(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable,
+                        const int32_t* phenotype, const double* ephemeral64, const int32_t* size,
+                        int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
+                        int32_t max_consts, double* grad, double* gmag, int32_t f32_values);
 
 /* The sequential path timed on `threads` host threads (threads <= 1: the reference's own
    single-threaded loop).  Returns elapsed seconds, or a negative value on error. */

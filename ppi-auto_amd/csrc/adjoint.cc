@@ -1,0 +1,544 @@
+// adjoint.cc — reverse-mode derivatives of the fitness with respect to the
+// ephemeral constants, and on-device constant refinement (C ABI in
+// include/ppi_engine.h, ppi_adjoint_*).
+//
+// forward() evaluates with an internal engine (the plain drop-in path) and
+// keeps the programs.  backward() compiles them into the adjoint tape (forward
+// tape + per-instruction partial records + reverse program, tape.h), then
+// launches one wave per (job, row range) unit.  Units are ordered heaviest job
+// first within each range; the number of ranges is chosen so that the units
+// outnumber the device's wave slots several times over.  The partial store —
+// VGPRs with forward recomputation per segment, or LDS with one forward pass
+// for most programs — is picked by timing both on a sample of the units.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/ppi_engine.h"
+#include "engine_internal.h"
+#include "kernels.h"
+#include "tape.h"
+
+using namespace ppi;
+
+namespace {
+
+#define A_HIP(call)                                                                            \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                      \
+            return PPI_E_DEVICE;                                                               \
+        }                                                                                      \
+    } while (0)
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device buffer that grows on demand.
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ppi_status reserve(size_t n) {
+        if (n <= cap && p) return PPI_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(n, 1);
+        if (hipMalloc((void**)&p, want * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            set_error("device allocation of " + std::to_string(want * sizeof(T)) + " bytes failed");
+            return PPI_E_NOMEM;
+        }
+        cap = want;
+        return PPI_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+template <typename T>
+ppi_status upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+    ppi_status s = b.reserve(v.size());
+    if (s) return s;
+    if (!v.empty()) A_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return PPI_OK;
+}
+
+}  // namespace
+
+struct ppi_adjoint {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int dev_cu = 0;
+    ppi_engine* eng = nullptr;
+    bool configured = false;
+    ppi_config cfg{};
+
+    // programs kept by forward()
+    std::vector<int32_t> ph, sz;
+    std::vector<float> eph;
+    int32_t stride = 0, n = 0;
+    float alpha = 0.0f;
+    bool have_programs = false;
+
+    // compiled adjoint tape for partial store `tape_ps` (-1: none); store picked by AUTO (-1: not yet)
+    int tape_ps = -1;
+    int tuned_ps = -1;
+    AdjTape T;
+    int R = 1, F = 12, S = 12;
+    int32_t nranges = 1;
+    int64_t range_rows = 0;
+    int grid = 0;
+    std::vector<int32_t> order;
+
+    DBuf<AdjJob> dJobs;
+    DBuf<AdjSeg> dSegs;
+    DBuf<RevIns> dRev;
+    DBuf<uint32_t> dRec;
+    DBuf<Ins> dCode;
+    DBuf<int32_t> dOrder, dSample;
+    DBuf<ProgMeta> dMeta;
+    DBuf<AdjConst> dConsts;
+    DBuf<uint32_t> dProgConsts;
+    DBuf<float> dGpart, dFpart, dFit, dGraw, dGscale, dGout, dCur, dCbest, dGbest, dFbest, dLam;
+    DBuf<uint32_t> dFbad, dBad;
+
+    ppi_adjoint_stats stats{};
+};
+
+namespace {
+
+bool differentiable(const ppi_config& c, std::string* why) {
+    if (c.error_fn != PPI_EF_ABS && c.error_fn != PPI_EF_SQUARED) {
+        *why = "the adjoint supports the ABS and SQUARED error functions";
+        return false;
+    }
+    if (c.reducemax) {
+        *why = "the adjoint supports the sum reduction only (REDUCEMAX is off)";
+        return false;
+    }
+    return true;
+}
+
+AdjArgs make_args(ppi_adjoint* a) {
+    const ppi_engine* E = a->eng;
+    AdjArgs A{};
+    A.jobs = a->dJobs.p;
+    A.segs = a->dSegs.p;
+    A.rev = a->dRev.p;
+    A.rec = a->dRec.p;
+    A.code = a->dCode.p;
+    A.order = a->dOrder.p;
+    A.njobs = (int32_t)a->T.jobs.size();
+    A.nranges = a->nranges;
+    A.range_rows = a->range_rows;
+    A.X = E->dX;
+    A.ldx = E->ldx;
+    A.nlin = E->nlin;
+    A.ncol = E->ncol;
+    A.ycol = E->ncol - 1;
+    A.error_fn = a->cfg.error_fn;
+    A.nprog = a->n;
+    A.nconst = (int64_t)a->T.consts.size();
+    A.gpart = a->dGpart.p;
+    A.fpart = a->dFpart.p;
+    A.fbad = a->dFbad.p;
+    return A;
+}
+
+AdjFinal make_final(ppi_adjoint* a, const float* gscale, float* gout) {
+    AdjFinal F{};
+    F.meta = a->dMeta.p;
+    F.consts = a->dConsts.p;
+    F.nprog = a->n;
+    F.nconst = (int64_t)a->T.consts.size();
+    F.nranges = a->nranges;
+    F.nlin = a->eng->nlin;
+    F.alpha = a->alpha;
+    F.fpart = a->dFpart.p;
+    F.fbad = a->dFbad.p;
+    F.gpart = a->dGpart.p;
+    F.fit = a->dFit.p;
+    F.bad = a->dBad.p;
+    F.graw = a->dGraw.p;
+    F.gscale = gscale;
+    F.gout = gout;
+    F.stride = a->stride;
+    return F;
+}
+
+// Compile the kept programs for partial store `ps` and upload the tape.
+ppi_status prepare(ppi_adjoint* a, int ps) {
+    if (a->tape_ps == ps) return PPI_OK;
+    const double t0 = now_ms();
+    std::string err;
+    const int32_t B = adjoint_store_partials(ps);
+    if (!compile_adjoint(a->eng->symtab, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->eng->ncol, B,
+                         &a->T, &err)) {
+        set_error(err);
+        a->tape_ps = -1;
+        return PPI_E_PROGRAM;
+    }
+    AdjTape& T = a->T;
+    for (const AdjConst& c : T.consts)
+        if (c.ins >= T.fwd.code.size()) {
+            set_error("program " + std::to_string(c.prog) + ": constant at " + std::to_string(c.pos) + " has no tape slot");
+            a->tape_ps = -1;
+            return PPI_E_PROGRAM;
+        }
+    T.fwd.code.insert(T.fwd.code.end(), kTapePad, Ins{H_END, kEmpty});
+    T.rec.push_back(0u);  // look-ahead of the last job's forward pass
+    a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
+    a->F = (T.fwd.stats.max_col < 12 && a->S == 12) ? 12 : 0;
+    a->R = 1;
+    if (!adjoint_shape_supported(a->R, a->F, a->S, ps)) {
+        set_error("no adjoint kernel shape for these programs");
+        return PPI_E_PROGRAM;
+    }
+    const size_t nj = T.jobs.size();
+    a->order.resize(nj);
+    std::iota(a->order.begin(), a->order.end(), 0);
+    std::stable_sort(a->order.begin(), a->order.end(),
+                     [&](int32_t x, int32_t y) { return T.jobs[(size_t)x].cost > T.jobs[(size_t)y].cost; });
+    // Row ranges: enough units to fill every wave slot several times over.
+    const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
+    const int occ = std::max(1, adjoint_occupancy(a->R, a->F, a->S, ps, P, N));
+    const int64_t slots = (int64_t)a->dev_cu * 4 * occ;
+    const int64_t nlin = a->eng->nlin;
+    const int64_t tile = 64 * a->R;
+    const int64_t max_ranges = std::max<int64_t>(1, nlin / (tile * 8));
+    int64_t want = (8 * slots + (int64_t)nj - 1) / std::max<int64_t>(1, (int64_t)nj);
+    want = std::max<int64_t>(1, std::min(want, max_ranges));
+    int64_t rr = (nlin + want - 1) / want;
+    rr = (rr + tile - 1) / tile * tile;
+    a->range_rows = rr;
+    a->nranges = (int32_t)((nlin + rr - 1) / rr);
+    const int64_t units = (int64_t)nj * a->nranges;
+    a->grid = (int)((units + 3) / 4);
+
+    ppi_status s;
+    hipStream_t st = a->stream;
+    if ((s = upload(a->dJobs, T.jobs, st)) || (s = upload(a->dSegs, T.segs, st)) || (s = upload(a->dRev, T.rev, st)) ||
+        (s = upload(a->dRec, T.rec, st)) || (s = upload(a->dCode, T.fwd.code, st)) ||
+        (s = upload(a->dOrder, a->order, st)) || (s = upload(a->dMeta, T.fwd.meta, st)) ||
+        (s = upload(a->dConsts, T.consts, st)) || (s = upload(a->dProgConsts, T.prog_consts, st)))
+        return s;
+    const size_t nc = T.consts.size();
+    if ((s = a->dGpart.reserve((size_t)a->nranges * nc)) || (s = a->dFpart.reserve((size_t)a->nranges * a->n)) ||
+        (s = a->dFbad.reserve((size_t)a->nranges * a->n)) || (s = a->dFit.reserve((size_t)a->n)) ||
+        (s = a->dBad.reserve((size_t)a->n)) || (s = a->dGraw.reserve(nc)) || (s = a->dCur.reserve(nc)) ||
+        (s = a->dCbest.reserve(nc)) || (s = a->dGbest.reserve(nc)) || (s = a->dFbest.reserve((size_t)a->n)) ||
+        (s = a->dLam.reserve((size_t)a->n)))
+        return s;
+    A_HIP(hipStreamSynchronize(st));
+    a->tape_ps = ps;
+    double fwd = 0.0, rev = 0.0;
+    for (const AdjJob& j : T.jobs) {
+        if (j.flags & kEmpty) continue;
+        for (uint32_t k = 0; k < j.nseg; ++k) {
+            const AdjSeg& sg = T.segs[j.seg_off + k];
+            fwd += sg.fwd_end;
+            rev += sg.rev_end - sg.rev_begin;
+        }
+    }
+    a->stats.forward_ins = fwd * (double)nlin;
+    a->stats.reverse_ins = rev * (double)nlin;
+    a->stats.n_jobs = (int32_t)nj;
+    a->stats.n_consts = (int32_t)nc;
+    a->stats.max_segments = T.max_seg;
+    a->stats.rows_per_lane = a->R;
+    a->stats.ranges = a->nranges;
+    a->stats.node_evals = (double)T.fwd.stats.total_size * (double)nlin;
+    a->stats.compile_ms = now_ms() - t0;
+    return PPI_OK;
+}
+
+ppi_status enqueue_adjoint(ppi_adjoint* a, const AdjArgs& A, int grid) {
+    const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
+    A_HIP(launch_adjoint(A, grid, a->R, a->F, a->S, a->tape_ps, P, N, a->stream));
+    return PPI_OK;
+}
+
+// Mean device ms of the adjoint kernel over an evenly spaced sample of the units
+// (one range of at most 4096 rows).
+ppi_status sample_ms(ppi_adjoint* a, double* ms) {
+    const size_t nj = a->T.jobs.size();
+    const size_t want = std::min<size_t>(nj, (size_t)a->dev_cu * 8);
+    std::vector<int32_t> pick(want);
+    for (size_t k = 0; k < want; ++k) pick[k] = a->order[k * nj / want];
+    ppi_status s = upload(a->dSample, pick, a->stream);
+    if (s) return s;
+    AdjArgs A = make_args(a);
+    A.order = a->dSample.p;
+    A.njobs = (int32_t)want;
+    A.nranges = 1;
+    A.range_rows = std::min<int64_t>(a->range_rows, 4096);
+    const int grid = (int)((want + 3) / 4);
+    float best = 0.0f;
+    for (int rep = 0; rep < 3; ++rep) {
+        A_HIP(hipEventRecord(a->ev[2], a->stream));
+        if ((s = enqueue_adjoint(a, A, grid))) return s;
+        A_HIP(hipEventRecord(a->ev[3], a->stream));
+        A_HIP(hipEventSynchronize(a->ev[3]));
+        float t = 0.0f;
+        A_HIP(hipEventElapsedTime(&t, a->ev[2], a->ev[3]));
+        if (rep == 1 || (rep == 2 && t < best)) best = t;
+    }
+    *ms = best;
+    return PPI_OK;
+}
+
+// Resolve PPI_PARTIALS_* to a partial store (0: VGPR + recompute, 1: LDS).
+ppi_status choose_store(ppi_adjoint* a, int32_t partials, int* ps) {
+    if (partials == PPI_PARTIALS_RECOMPUTE) { *ps = 0; return PPI_OK; }
+    if (partials == PPI_PARTIALS_STORE) { *ps = 1; return PPI_OK; }
+    if (partials != PPI_PARTIALS_AUTO) { set_error("bad partials mode"); return PPI_E_ARG; }
+    if (a->tuned_ps >= 0) { *ps = a->tuned_ps; return PPI_OK; }  // tuned for the kept programs
+    double t[2] = {0.0, 0.0};
+    for (int k = 0; k < 2; ++k) {
+        ppi_status s = prepare(a, k);
+        if (s) return s;
+        if ((s = sample_ms(a, &t[k]))) return s;
+        a->stats.tune_ms[k] = t[k];
+    }
+    *ps = t[1] < t[0] ? 1 : 0;
+    a->tuned_ps = *ps;
+    return PPI_OK;
+}
+
+ppi_status run_pass(ppi_adjoint* a, const float* gscale, float* gout, bool time_kernel) {
+    AdjArgs A = make_args(a);
+    ppi_status s;
+    if (time_kernel) A_HIP(hipEventRecord(a->ev[2], a->stream));
+    if ((s = enqueue_adjoint(a, A, a->grid))) return s;
+    if (time_kernel) A_HIP(hipEventRecord(a->ev[3], a->stream));
+    A_HIP(launch_adjoint_finalize(make_final(a, gscale, gout), a->stream));
+    return PPI_OK;
+}
+
+ppi_status check_ready(ppi_adjoint* a) {
+    if (!a) { set_error("adjoint is NULL"); return PPI_E_ARG; }
+    if (!a->configured) { set_error("configure first"); return PPI_E_STATE; }
+    if (!a->have_programs) { set_error("call forward first"); return PPI_E_STATE; }
+    (void)hipSetDevice(a->device);
+    return PPI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ppi_status ppi_adjoint_create(ppi_adjoint** out, int32_t device) {
+    if (!out) { set_error("out is NULL"); return PPI_E_ARG; }
+    *out = nullptr;
+    ppi_engine* E = nullptr;
+    ppi_status s = ppi_engine_create(&E, device);
+    if (s) return s;
+    ppi_adjoint* a = new ppi_adjoint();
+    a->eng = E;
+    a->device = E->device;
+    a->dev_cu = E->dev_cu;
+    hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&a->ev[k]);
+    if (e != hipSuccess) {
+        set_error(std::string("device init: ") + hipGetErrorString(e));
+        ppi_adjoint_destroy(a);
+        return PPI_E_DEVICE;
+    }
+    *out = a;
+    return PPI_OK;
+}
+
+void ppi_adjoint_destroy(ppi_adjoint* a) {
+    if (!a) return;
+    (void)hipSetDevice(a->device);
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    a->dJobs.release(); a->dSegs.release(); a->dRev.release(); a->dRec.release(); a->dCode.release();
+    a->dOrder.release(); a->dSample.release(); a->dMeta.release(); a->dConsts.release(); a->dProgConsts.release();
+    a->dGpart.release(); a->dFpart.release(); a->dFit.release(); a->dGraw.release(); a->dGscale.release();
+    a->dGout.release(); a->dCur.release(); a->dCbest.release(); a->dGbest.release(); a->dFbest.release();
+    a->dLam.release(); a->dFbad.release(); a->dBad.release();
+    for (auto& ev : a->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (a->stream) (void)hipStreamDestroy(a->stream);
+    if (a->eng) ppi_engine_destroy(a->eng);
+    delete a;
+}
+
+ppi_status ppi_adjoint_configure(ppi_adjoint* a, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable) {
+    if (!a || !cfg) { set_error("adjoint or config is NULL"); return PPI_E_ARG; }
+    std::string why;
+    if (!differentiable(*cfg, &why)) { set_error(why); return PPI_E_UNSUPPORTED; }
+    ppi_status s = ppi_engine_configure(a->eng, cfg, table, ntable);
+    if (s) return s;
+    a->cfg = *cfg;
+    a->cfg.cost = nullptr;
+    a->configured = true;
+    a->have_programs = false;
+    a->tape_ps = -1;
+    a->tuned_ps = -1;
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_set_dataset_rowmajor(ppi_adjoint* a, const float* data, int32_t nlin, int32_t ncol) {
+    if (!a) { set_error("adjoint is NULL"); return PPI_E_ARG; }
+    ppi_status s = ppi_engine_set_dataset_rowmajor(a->eng, data, nlin, ncol);
+    if (s) return s;
+    a->have_programs = false;
+    a->tape_ps = -1;
+    a->tuned_ps = -1;
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_forward(ppi_adjoint* a, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                               int32_t stride, int32_t n, float alpha, float* fitness) {
+    if (!a) { set_error("adjoint is NULL"); return PPI_E_ARG; }
+    if (!a->configured) { set_error("configure first"); return PPI_E_STATE; }
+    if (n <= 0 || stride <= 0 || !phenotype || !ephemeral || !size) { set_error("bad program arrays"); return PPI_E_ARG; }
+    std::vector<float> fit((size_t)n);
+    ppi_status s = ppi_engine_evaluate(a->eng, phenotype, ephemeral, size, stride, n, alpha, fit.data(), nullptr, nullptr);
+    if (s) return s;
+    a->ph.assign(phenotype, phenotype + (size_t)n * stride);
+    a->eph.assign(ephemeral, ephemeral + (size_t)n * stride);
+    a->sz.assign(size, size + n);
+    a->stride = stride;
+    a->n = n;
+    a->alpha = alpha;
+    a->have_programs = true;
+    a->tape_ps = -1;
+    a->tuned_ps = -1;
+    if (fitness) std::memcpy(fitness, fit.data(), (size_t)n * sizeof(float));
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_backward(ppi_adjoint* a, const float* grad_fitness, int32_t partials, float* grad) {
+    ppi_status s = check_ready(a);
+    if (s) return s;
+    if (!grad) { set_error("grad is NULL"); return PPI_E_ARG; }
+    int ps = 0;
+    if ((s = choose_store(a, partials, &ps))) return s;
+    if ((s = prepare(a, ps))) return s;
+    a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    const size_t total = (size_t)a->n * a->stride;
+    if ((s = a->dGout.reserve(total))) return s;
+    const float* gs = nullptr;
+    if (grad_fitness) {
+        if ((s = a->dGscale.reserve((size_t)a->n))) return s;
+        A_HIP(hipMemcpyAsync(a->dGscale.p, grad_fitness, (size_t)a->n * sizeof(float), hipMemcpyHostToDevice, a->stream));
+        gs = a->dGscale.p;
+    }
+    A_HIP(hipMemsetAsync(a->dGout.p, 0, total * sizeof(float), a->stream));
+    A_HIP(hipEventRecord(a->ev[0], a->stream));
+    if ((s = run_pass(a, gs, a->dGout.p, true))) return s;
+    A_HIP(hipEventRecord(a->ev[1], a->stream));
+    A_HIP(hipMemcpyAsync(grad, a->dGout.p, total * sizeof(float), hipMemcpyDeviceToHost, a->stream));
+    A_HIP(hipStreamSynchronize(a->stream));
+    float ms = 0.0f;
+    A_HIP(hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+    a->stats.backward_ms = ms;
+    A_HIP(hipEventElapsedTime(&ms, a->ev[2], a->ev[3]));
+    a->stats.kernel_ms = ms;
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_t partials, float* ephemeral_out,
+                               float* fitness_out) {
+    ppi_status s = check_ready(a);
+    if (s) return s;
+    if (steps < 0 || !(step > 0.0f) || !std::isfinite(step)) { set_error("need steps >= 0 and a finite step > 0"); return PPI_E_ARG; }
+    int ps = 0;
+    if ((s = choose_store(a, partials, &ps))) return s;
+    if ((s = prepare(a, ps))) return s;
+    a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    AdjTape& T = a->T;
+    const size_t nc = T.consts.size();
+    std::vector<float> cur(nc);
+    for (size_t k = 0; k < nc; ++k) cur[k] = a->eph[(size_t)T.consts[k].prog * a->stride + (size_t)T.consts[k].pos];
+    if (nc) A_HIP(hipMemcpyAsync(a->dCur.p, cur.data(), nc * sizeof(float), hipMemcpyHostToDevice, a->stream));
+    AdjDescend D{};
+    D.prog_consts = a->dProgConsts.p;
+    D.consts = a->dConsts.p;
+    D.nprog = a->n;
+    D.fit = a->dFit.p;
+    D.bad = a->dBad.p;
+    D.graw = a->dGraw.p;
+    D.cur = a->dCur.p;
+    D.cbest = a->dCbest.p;
+    D.gbest = a->dGbest.p;
+    D.fbest = a->dFbest.p;
+    D.lam = a->dLam.p;
+    D.code = a->dCode.p;
+    D.step0 = step;
+    A_HIP(hipEventRecord(a->ev[0], a->stream));
+    for (int32_t it = 0; it <= steps; ++it) {
+        if ((s = run_pass(a, nullptr, nullptr, false))) return s;
+        D.iter = it;
+        D.last = it == steps ? 1 : 0;
+        A_HIP(launch_adjoint_descend(D, a->stream));
+    }
+    A_HIP(hipEventRecord(a->ev[1], a->stream));
+    if (nc) A_HIP(hipMemcpyAsync(cur.data(), a->dCbest.p, nc * sizeof(float), hipMemcpyDeviceToHost, a->stream));
+    A_HIP(hipStreamSynchronize(a->stream));
+    float ms = 0.0f;
+    A_HIP(hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+    a->stats.descend_ms = ms;
+    a->stats.descend_steps = steps;
+    for (size_t k = 0; k < nc; ++k) a->eph[(size_t)T.consts[k].prog * a->stride + (size_t)T.consts[k].pos] = cur[k];
+    // The device tape now holds cbest, which matches the updated kept ephemerals.
+    if (fitness_out) {
+        if ((s = ppi_engine_evaluate(a->eng, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->alpha,
+                                     fitness_out, nullptr, nullptr)))
+            return s;
+    }
+    if (ephemeral_out) std::memcpy(ephemeral_out, a->eph.data(), a->eph.size() * sizeof(float));
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_time(ppi_adjoint* a, int32_t iters, int32_t partials, double* ms_per_pass) {
+    ppi_status s = check_ready(a);
+    if (s) return s;
+    if (iters <= 0 || !ms_per_pass) { set_error("bad arguments"); return PPI_E_ARG; }
+    int ps = 0;
+    if ((s = choose_store(a, partials, &ps))) return s;
+    if ((s = prepare(a, ps))) return s;
+    a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    double kern = 0.0;
+    A_HIP(hipEventRecord(a->ev[0], a->stream));
+    for (int32_t k = 0; k < iters; ++k) {
+        if ((s = run_pass(a, nullptr, nullptr, true))) return s;
+        A_HIP(hipEventSynchronize(a->ev[3]));
+        float t = 0.0f;
+        A_HIP(hipEventElapsedTime(&t, a->ev[2], a->ev[3]));
+        kern += t;
+    }
+    A_HIP(hipEventRecord(a->ev[1], a->stream));
+    A_HIP(hipEventSynchronize(a->ev[1]));
+    float ms = 0.0f;
+    A_HIP(hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+    *ms_per_pass = ms / iters;
+    a->stats.backward_ms = ms / iters;
+    a->stats.kernel_ms = kern / iters;
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_get_stats(const ppi_adjoint* a, ppi_adjoint_stats* out) {
+    if (!a || !out) { set_error("adjoint or out is NULL"); return PPI_E_ARG; }
+    *out = a->stats;
+    return PPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,502 @@
+// adjoint_impl.h — reverse-mode adjoint kernels for gfx950: d(fitness)/d(T_CONST).
+//
+// fitness = sum_rows EF(pred, y) / nlin + alpha*size (sequential.cc:170-189), so
+// d fitness / d c = sum_rows EF'(pred, y) * d pred / d c / nlin.  One wave owns
+// one (job, row range) unit and walks the range 64*R rows at a time:
+//
+//   for each segment s of the job:
+//       forward pass over the program's accumulator tape (up to the last
+//       instruction that records a partial of segment s), storing those partials;
+//       s == 0: seed the root adjoint with EF'(pred, y) (0 on rows past nlin)
+//               and add the row's error to the fitness sum;
+//       the segment's reverse instructions: adj[dst] = adj[src] * partial, or
+//       g[k] += adj[src] * partial for a constant k (a zero adjoint gives 0
+//       whatever the partial).
+//
+// The partial store is either VGPRs (16 per row: long programs re-run the
+// forward pass per segment — recomputation) or the wave's own LDS block (32 per
+// row: most programs need one forward pass — storage).  Adjoint slots and the
+// per-lane gradient accumulators live in VGPRs; wave-uniform slot indices use the
+// backend's indexed register moves.  Sums over a range are wave-reduced once per
+// unit; ranges are summed in a fixed order by adj_grad_kernel.
+#ifndef PPI_ADJOINT_IMPL_H
+#define PPI_ADJOINT_IMPL_H
+
+#include "kernels_impl.h"
+
+namespace ppi {
+namespace dev {
+
+constexpr float kLN10_F = 2.30258509299404568402f;
+constexpr float kLN2_F = 0.69314718055994530942f;
+
+// d GAMMA / da (interpreter_core:127-129 differentiated), in double.
+template <bool P>
+__device__ __noinline__ float dgamma(float a, float z) {
+    const double x = a, e = (double)kE_F, pi = (double)kPI_F;
+    const double in1 = x * sinh(1.0 / x), in2 = 2.0 * pi / x;
+    const bool p1 = P && in1 < 0.0, p2 = P && in2 < 0.0;
+    const double s1 = p1 ? 1.0 : sqrt(in1), s2 = p2 ? 1.0 : sqrt(in2);
+    const double b = (x / e) * s1;
+    const double ds1 = p1 ? 0.0 : 0.5 / s1 * (sinh(1.0 / x) - cosh(1.0 / x) / x);
+    const double ds2 = p2 ? 0.0 : 0.5 / s2 * (-2.0 * pi / (x * x));
+    const double db = s1 / e + (x / e) * ds1;
+    return (float)((double)z * (log(b) + x * db / b + ds2 / s2));
+}
+
+// d unary(u, a) / da, given z = unary(u, a).  Protected operators have derivative
+// 0 where the protection replaced the value (functions.h:115-193).
+template <bool P, bool N>
+__device__ __forceinline__ float dunary(uint32_t u, float a, float z) {
+    switch (u) {
+    case U_ABS: return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f);
+    case U_POW2: return 2.0f * a;
+    case U_POW3: return 3.0f * a * a;
+    case U_POW4: return 4.0f * a * a * a;
+    case U_POW5: return 5.0f * a * a * a * a;
+    case U_NEG: return -1.0f;
+    case U_EXP: return z;
+    case U_EXP10: return z * kLN10_F;
+    case U_EXP2: return z * kLN2_F;
+    case U_SIN:
+        if constexpr (N) return __cosf(a);
+        else return cosf(a);
+    case U_COS:
+        if constexpr (N) return -__sinf(a);
+        else return -sinf(a);
+    case U_LOGISTIC: return z * (1.0f - z);
+    case U_GAMMA: return dgamma<P>(a, z);
+    case U_GAUSS: return -2.0f * a * z;
+    case U_SQRT: return (P && a < 0.0f) ? 0.0f : 0.5f / z;
+    case U_LOG: return (P && a < 1.0f) ? 0.0f : 1.0f / a;
+    case U_LOG10: return (P && a < 1.0f) ? 0.0f : 1.0f / (a * kLN10_F);
+    case U_LOG2: return (P && a < 1.0f) ? 0.0f : 1.0f / (a * kLN2_F);
+    case U_TAN: return (P && (fabsf(a) == kPI_2_F || fabsf(a) == 3.0f * kPI_2_F)) ? 0.0f : 1.0f + z * z;
+    default: return 0.0f;  // NOT, ROUND, CEIL, FLOOR, STEP, SIGN: cut, never stored
+    }
+}
+
+// Partials of z = binary(op, a, b) with respect to a (the accumulator) and b.
+// MAX/MIN pass the derivative to the operand they return, the left one on ties
+// or when the right one is NaN (fmax/fmin semantics).
+template <bool P, bool N>
+__device__ __forceinline__ void dbinary(uint32_t op, float a, float b, float z, float& da, float& db) {
+    switch (op) {
+    case B_ADD: da = 1.0f; db = 1.0f; return;
+    case B_SUB: da = 1.0f; db = -1.0f; return;
+    case B_RSUB: da = -1.0f; db = 1.0f; return;
+    case B_MULT: da = b; db = a; return;
+    case B_MEAN: da = 0.5f; db = 0.5f; return;
+    case B_DIV: {
+        const bool zr = P && b == 0.0f;
+        da = zr ? 0.0f : 1.0f / b;
+        db = zr ? 0.0f : -z / b;
+        return;
+    }
+    case B_RDIV: {
+        const bool zr = P && a == 0.0f;
+        db = zr ? 0.0f : 1.0f / a;
+        da = zr ? 0.0f : -z / a;
+        return;
+    }
+    case B_MAX: da = (isnan(b) || a >= b) ? 1.0f : 0.0f; db = 1.0f - da; return;
+    case B_RMAX: db = (isnan(a) || b >= a) ? 1.0f : 0.0f; da = 1.0f - db; return;
+    case B_MIN: da = (isnan(b) || a <= b) ? 1.0f : 0.0f; db = 1.0f - da; return;
+    case B_RMIN: db = (isnan(a) || b <= a) ? 1.0f : 0.0f; da = 1.0f - db; return;
+    case B_MOD: da = 1.0f; db = -truncf(a / b); return;
+    case B_RMOD: db = 1.0f; da = -truncf(b / a); return;
+    case B_POW: da = b * powf(a, b - 1.0f); db = a > 0.0f ? z * logf(a) : 0.0f; return;
+    case B_RPOW: db = a * powf(b, a - 1.0f); da = b > 0.0f ? z * logf(b) : 0.0f; return;
+    default: da = 0.0f; db = 0.0f; return;
+    }
+}
+
+// Partial stores: slot s (< B) of row r.
+template <int R, int B>
+struct PStoreV {  // VGPRs
+    typedef typename VecT<B>::type V;
+    V v[R];
+    __device__ __forceinline__ void set(uint32_t s, int r, float x) { v[r][s] = x; }
+    __device__ __forceinline__ float get(uint32_t s, int r) const { return v[r][s]; }
+};
+template <int R>
+struct PStoreL {  // this lane's column of the wave's [B][R][64] LDS block
+    float* base;
+    __device__ __forceinline__ void set(uint32_t s, int r, float x) { base[(s * R + r) * 64] = x; }
+    __device__ __forceinline__ float get(uint32_t s, int r) const { return base[(s * R + r) * 64]; }
+};
+
+template <int PS> struct StoreCfg;
+template <> struct StoreCfg<0> { static constexpr int B = 16, LOGB = 4; };
+template <> struct StoreCfg<1> { static constexpr int B = 32, LOGB = 5; };
+
+__device__ __forceinline__ AdjJob load_job(const AdjJob* jobs, int i) {
+    ConstPtr<AdjJob> q = as_const(jobs) + i;
+    AdjJob j;
+    j.code_off = q->code_off;
+    j.ncode = q->ncode;
+    j.rec_off = q->rec_off;
+    j.seg_off = q->seg_off;
+    j.nseg = q->nseg;
+    j.prog = q->prog;
+    j.const_base = q->const_base;
+    j.nconst = q->nconst;
+    j.flags = q->flags;
+    j.cost = 0;
+    j.pad_[0] = j.pad_[1] = 0;
+    return j;
+}
+
+template <int R, int S, int PS, bool P, bool N>
+struct AdjCore {
+    typedef Core<R, S, P, N> C;
+    typedef typename C::VS VS;
+    static constexpr int B = StoreCfg<PS>::B;
+    static constexpr int LOGB = StoreCfg<PS>::LOGB;
+    static constexpr uint32_t kNone = 0xffffffffu;
+
+    // One instruction that stores partials: sa / sb = store slot of the partial
+    // with respect to the accumulator / the other operand (IF_THEN_ELSE: the
+    // then / else selector), kNone = not stored in this segment.
+    template <class FeatT, class PSt>
+    static __device__ __forceinline__ void step_rec(uint32_t c, uint32_t a, uint32_t sa, uint32_t sb,
+                                                    const FeatT& ft, float (&acc)[R], VS (&stk)[R], PSt& ps) {
+        const uint32_t h = c & 0xffu;
+        if (h == H_TERN) {
+            const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
+            const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
+                const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
+                const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
+                const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
+                const bool t = cd != 0.0f;
+                if (sa != kNone) ps.set(sa, r, t ? 1.0f : 0.0f);
+                if (sb != kNone) ps.set(sb, r, t ? 0.0f : 1.0f);
+                acc[r] = t ? th : el;
+            }
+        } else if (h >= H_BIN_C) {
+            const float cv = __uint_as_float(a);
+            float b[R];
+            uint32_t op;
+            if (h < H_BIN_A) {
+                op = h - H_BIN_C;
+#pragma unroll
+                for (int r = 0; r < R; ++r) b[r] = cv;
+            } else if (h < H_BIN_S) {
+                op = h - H_BIN_A;
+                ft.get(a, b);
+            } else {
+                op = h - H_BIN_S;
+                const uint32_t slot = (c >> 8) & 31u;
+#pragma unroll
+                for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
+            }
+            switch (op) {
+#define PPI_DBCASE(K)                                                                  \
+    case K:                                                                            \
+        _Pragma("unroll") for (int r = 0; r < R; ++r) {                                \
+            const float z = binary<P, N>(K, acc[r], b[r]);                             \
+            float da, db;                                                              \
+            dbinary<P, N>(K, acc[r], b[r], z, da, db);                                 \
+            if (sa != kNone) ps.set(sa, r, da);                                        \
+            if (sb != kNone) ps.set(sb, r, db);                                        \
+            acc[r] = z;                                                                \
+        }                                                                              \
+        break;
+                PPI_DBCASE(B_MULT) PPI_DBCASE(B_DIV) PPI_DBCASE(B_RDIV) PPI_DBCASE(B_MAX)
+                PPI_DBCASE(B_MIN) PPI_DBCASE(B_MOD) PPI_DBCASE(B_RMOD) PPI_DBCASE(B_POW)
+                PPI_DBCASE(B_RPOW) PPI_DBCASE(B_RMAX) PPI_DBCASE(B_RMIN)
+#undef PPI_DBCASE
+            default:  // operators whose stored partials are never requested
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = binary<P, N>(op, acc[r], b[r]);
+                break;
+            }
+        } else {
+            // H_UN (leaves never store partials)
+            switch (h - H_UN) {
+#define PPI_DUCASE(K)                                                                  \
+    case K:                                                                            \
+        _Pragma("unroll") for (int r = 0; r < R; ++r) {                                \
+            const float z = unary<P, N>(K, acc[r]);                                    \
+            ps.set(sa, r, dunary<P, N>(K, acc[r], z));                                 \
+            acc[r] = z;                                                                \
+        }                                                                              \
+        break;
+                PPI_DUCASE(U_ABS) PPI_DUCASE(U_POW2) PPI_DUCASE(U_POW3) PPI_DUCASE(U_POW4)
+                PPI_DUCASE(U_POW5) PPI_DUCASE(U_EXP) PPI_DUCASE(U_EXP10) PPI_DUCASE(U_EXP2)
+                PPI_DUCASE(U_SIN) PPI_DUCASE(U_COS) PPI_DUCASE(U_LOGISTIC) PPI_DUCASE(U_GAMMA)
+                PPI_DUCASE(U_GAUSS) PPI_DUCASE(U_SQRT) PPI_DUCASE(U_LOG) PPI_DUCASE(U_LOG10)
+                PPI_DUCASE(U_LOG2) PPI_DUCASE(U_TAN)
+#undef PPI_DUCASE
+            default:
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = unary<P, N>(h - H_UN, acc[r]);
+                break;
+            }
+        }
+    }
+
+    // Forward pass over instructions [0, n) storing the partials of segment seg.
+    // code[n] and rec[n] must be readable (look-ahead).
+    template <class FeatT, class PSt>
+    static __device__ __forceinline__ void forward(ConstPtr<Ins> code, ConstPtr<uint32_t> rec, uint32_t n,
+                                                   uint32_t seg, const FeatT& ft, float (&acc)[R], VS (&stk)[R],
+                                                   PSt& ps) {
+        uint32_t nc = code[0].code, na = code[0].arg, nr = rec[0];
+        for (uint32_t pc = 0; pc < n; ++pc) {
+            const uint32_t c = nc, a = na, rc = nr;
+            nc = code[pc + 1].code;
+            na = code[pc + 1].arg;
+            nr = rec[pc + 1];
+            const uint32_t ea = rc & 0xffffu, eb = rc >> 16;
+            const uint32_t sa = (ea != 0 && ((ea - 1) >> LOGB) == seg) ? ((ea - 1) & (B - 1)) : kNone;
+            const uint32_t sb = (eb != 0 && ((eb - 1) >> LOGB) == seg) ? ((eb - 1) & (B - 1)) : kNone;
+            if (sa == kNone && sb == kNone) C::step(c, a, ft, acc, stk);
+            else step_rec(c, a, sa, sb, ft, acc, stk, ps);
+        }
+    }
+};
+
+template <int R, int F>
+struct AdjFeat {
+    typedef typename std::conditional<(F > 0), VgprFeat<R, F>, GlobalFeat<R>>::type type;
+};
+
+template <int R, int F, int S, int PS, bool P, bool N>
+__global__ void __launch_bounds__(kBlock) adjoint_kernel(AdjArgs A) {
+    typedef AdjCore<R, S, PS, P, N> AC;
+    typedef typename AC::VS VS;
+    constexpr int B = AC::B;
+    typedef typename VecT<kAdjSlots>::type VA;
+    typedef typename VecT<kAdjMaxConsts>::type VG;
+    constexpr int LDS_WAVE = PS == 1 ? B * R * 64 : 1;
+    __shared__ float lds[kWaves][LDS_WAVE];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t u = (int64_t)blockIdx.x * kWaves + wave;
+    if (u >= (int64_t)A.njobs * A.nranges) return;  // whole wave: no block-level synchronisation below
+    const int rng = (int)(u / A.njobs);
+    const int ji = as_const(A.order)[u - (int64_t)rng * A.njobs];
+    const AdjJob J = load_job(A.jobs, ji);
+    const int64_t rbeg = (int64_t)rng * A.range_rows;
+    const int64_t rend = min((int64_t)A.nlin, rbeg + A.range_rows);
+
+    typename std::conditional<PS == 1, PStoreL<R>, PStoreV<R, B>>::type ps;
+    if constexpr (PS == 1) ps.base = &lds[wave][lane];
+
+    VG g = 0.0f;
+    float part = 0.0f;
+    bool bad = false;
+    if (!(J.flags & kEmpty)) {
+        ConstPtr<Ins> code = as_const(A.code) + J.code_off;
+        ConstPtr<uint32_t> rec = as_const(A.rec) + J.rec_off;
+        ConstPtr<RevIns> rv = as_const(A.rev);
+        for (int64_t row0 = rbeg; row0 < rend; row0 += 64 * R) {
+            int row[R];
+            float y[R];
+            bool valid[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                row[r] = (int)(row0 + r * 64 + lane);
+                valid[r] = row[r] < rend;
+                y[r] = A.X[(int64_t)A.ycol * A.ldx + row[r]];
+            }
+            typedef typename AdjFeat<R, F>::type FeatT;
+            typename VgprFeat<R, (F > 0 ? F : 1)>::VF xf[R];
+            if constexpr (F > 0) {
+#pragma unroll
+                for (int f = 0; f < F; ++f) {
+                    const bool have = f < A.ncol;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) xf[r][f] = have ? A.X[(int64_t)f * A.ldx + row[r]] : 0.0f;
+                }
+            }
+            FeatT ft = [&]() {
+                if constexpr (F > 0) return FeatT{xf};
+                else return FeatT{A.X, A.ldx, row};
+            }();
+            VA adj[R];
+            for (uint32_t s = 0; s < J.nseg; ++s) {
+                ConstPtr<AdjSeg> sq = as_const(A.segs) + (J.seg_off + s);
+                const uint32_t rb = sq->rev_begin, re = sq->rev_end, fe = sq->fwd_end;
+                float acc[R];
+                VS stk[R];
+                AC::forward(code, rec, fe, s, ft, acc, stk, ps);
+                if (s == 0) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        float seed = 0.0f;
+                        if (valid[r]) {
+                            const float d = acc[r] - y[r];
+                            const float e = A.error_fn == PPI_EF_SQUARED ? d * d : fabsf(d);
+                            if (!finite_f(e)) bad = true;
+                            else part += e;
+                            seed = A.error_fn == PPI_EF_SQUARED ? 2.0f * d : (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
+                        }
+                        adj[r][0] = seed;
+                    }
+                }
+                for (uint32_t i = rb; i < re; ++i) {
+                    const uint32_t w = rv[i].w, pb = rv[i].p;
+                    const uint32_t src = (w >> 4) & 15u, dst = (w >> 8) & 31u;
+                    // A zero adjoint stays zero whatever the partial (a NaN or inf
+                    // partial inside a subtree that MAX/MIN/IF_THEN_ELSE did not
+                    // select does not reach the constants): forward-mode semantics.
+                    float v[R];
+                    if (w & kRevStored) {
+                        const uint32_t sl = (w >> 16) & (uint32_t)(B - 1);
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const float ad = adj[r][src];
+                            v[r] = ad == 0.0f ? 0.0f : ad * ps.get(sl, r);
+                        }
+                    } else {
+                        const float pc = __uint_as_float(pb);
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const float ad = adj[r][src];
+                            v[r] = ad == 0.0f ? 0.0f : ad * pc;
+                        }
+                    }
+                    if ((w & 3u) == kRevAdj) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) adj[r][dst] = v[r];
+                    } else {
+                        float t = 0.0f;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) t += valid[r] ? v[r] : 0.0f;
+                        g[dst] += t;
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t k = 0; k < J.nconst; ++k) {
+        const float v = wave_sum(g[k]);
+        if (lane == 0) A.gpart[(int64_t)rng * A.nconst + J.const_base + k] = v;
+    }
+    if (J.flags & kAdjFitness) {
+        part = wave_sum(part);
+        const bool any_bad = __any(bad);
+        if (lane == 0) {
+            A.fpart[(int64_t)rng * A.nprog + J.prog] = part;
+            A.fbad[(int64_t)rng * A.nprog + J.prog] = any_bad ? 1u : 0u;
+        }
+    }
+}
+
+#ifdef PPI_ADJOINT_MISC
+// Fitness with the sequential path's semantics (sequential.cc:170-189), ranges summed in order.
+__global__ void adj_fit_kernel(AdjFinal F) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= F.nprog) return;
+    float total = 0.0f;
+    uint32_t b = 0;
+    for (int r = 0; r < F.nranges; ++r) {
+        total += F.fpart[(int64_t)r * F.nprog + p];
+        b |= F.fbad[(int64_t)r * F.nprog + p];
+    }
+    const ProgMeta m = F.meta[p];
+    const float fn = (float)F.nlin;
+    const float as = F.alpha * (float)m.size;
+    float fit;
+    if (m.flags & kEmpty) {
+        fit = FLT_MAX;
+        b = 1;
+    } else if (b) {
+        fit = FLT_MAX / fn + as;
+    } else if (!finite_f(total)) {
+        fit = FLT_MAX;
+        b = 1;
+    } else {
+        fit = total / fn + as;
+    }
+    F.fit[p] = fit;
+    F.bad[p] = b;
+}
+
+__global__ void adj_grad_kernel(AdjFinal F) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= F.nconst) return;
+    const AdjConst c = F.consts[k];
+    float s = 0.0f;
+    for (int r = 0; r < F.nranges; ++r) s += F.gpart[(int64_t)r * F.nconst + k];
+    const float v = F.bad[c.prog] ? 0.0f : s / (float)F.nlin;
+    F.graw[k] = v;
+    if (F.gout) F.gout[(int64_t)c.prog * F.stride + c.pos] = F.gscale ? v * F.gscale[c.prog] : v;
+}
+
+// One step of normalised gradient descent with backtracking, per program: an
+// evaluation that improves the best fitness is accepted (step length x 1.5),
+// otherwise the step from the best point is halved.  The next point is
+// cbest - lam * gbest / |gbest|; it is written into the tape for the next pass.
+__global__ void adj_descend_kernel(AdjDescend D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.nprog) return;
+    const uint32_t k0 = D.prog_consts[p], k1 = D.prog_consts[p + 1];
+    if (k0 == k1) return;
+    const float f = D.fit[p];
+    const bool b = D.bad[p] != 0;
+    if (D.iter == 0) {
+        D.lam[p] = D.step0;
+        D.fbest[p] = f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            D.cbest[k] = D.cur[k];
+            D.gbest[k] = b ? 0.0f : D.graw[k];
+        }
+    } else if (!b && f < D.fbest[p]) {
+        D.fbest[p] = f;
+        D.lam[p] *= 1.5f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            D.cbest[k] = D.cur[k];
+            D.gbest[k] = D.graw[k];
+        }
+    } else {
+        D.lam[p] *= 0.5f;
+    }
+    float nrm = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) nrm += D.gbest[k] * D.gbest[k];
+    nrm = sqrtf(nrm);
+    const bool move = !D.last && nrm > 0.0f && finite_f(nrm);
+    const float sc = move ? D.lam[p] / nrm : 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const float c = move ? D.cbest[k] - sc * D.gbest[k] : D.cbest[k];
+        D.cur[k] = c;
+        const uint32_t ins = D.consts[k].ins;
+        if (ins != 0xffffffffu) D.code[ins].arg = __float_as_uint(c);
+    }
+}
+#endif  // PPI_ADJOINT_MISC
+
+}  // namespace dev
+
+namespace launch {
+
+template <int R, int F, int S, int PS>
+hipError_t launch_adjoint_t(const AdjArgs& A, int grid, bool P, bool N, hipStream_t st) {
+    if (P && N) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, true, true>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, true, false>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, false, true>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, false, false>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    return hipGetLastError();
+}
+
+template <int R, int F, int S, int PS>
+int adjoint_occupancy_t(bool P, bool N) {
+    int n = 0;
+    hipError_t e;
+    if (P && N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, true, true>, dev::kBlock, 0);
+    else if (P) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, true, false>, dev::kBlock, 0);
+    else if (N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, false, true>, dev::kBlock, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, false, false>, dev::kBlock, 0);
+    return e == hipSuccess ? n : 0;
+}
+
+}  // namespace launch
+}  // namespace ppi
+
+#endif

@@ -74,6 +74,74 @@ struct BatchArgs {
     uint32_t* counter;      // work-queue head, reset to 0 before every launch
 };
 
+// Reverse-mode adjoint: one wave = one (job, row range) unit.
+struct AdjArgs {
+    const AdjJob* jobs;
+    const AdjSeg* segs;
+    const RevIns* rev;
+    const uint32_t* rec;
+    const Ins* code;
+    const int32_t* order;  // unit u -> job order[u % njobs] (heaviest first), range u / njobs
+    int32_t njobs;
+    int32_t nranges;
+    int64_t range_rows;    // rows per range, a multiple of 64*R
+    const float* X;        // column-major dataset, rows padded to ldx
+    int64_t ldx;
+    int32_t nlin, ncol, ycol;
+    int32_t error_fn;      // PPI_EF_ABS or PPI_EF_SQUARED
+    int32_t nprog;
+    int64_t nconst;
+    float* gpart;          // [range][nconst]: sum over the range's rows of dEF/dpred * dpred/dconst
+    float* fpart;          // [range][nprog]: sum of finite errors
+    uint32_t* fbad;        // [range][nprog]: 1 if a row's error is not finite
+};
+
+// Per-program and per-constant epilogue of the adjoint.
+struct AdjFinal {
+    const ProgMeta* meta;
+    const AdjConst* consts;
+    int32_t nprog;
+    int64_t nconst;
+    int32_t nranges;
+    int32_t nlin;
+    float alpha;
+    const float* fpart;
+    const uint32_t* fbad;
+    const float* gpart;
+    float* fit;            // [nprog] fitness, sequential-path semantics (sequential.cc:170-189)
+    uint32_t* bad;         // [nprog] 1: empty, or an error is not finite -> gradient 0
+    float* graw;           // [nconst] d fitness / d constant
+    const float* gscale;   // [nprog] upstream gradient, null = 1
+    float* gout;           // [nprog * stride] ephemeral layout, null = not written
+    int32_t stride;
+};
+
+// On-device constant refinement: one update per program and call.
+struct AdjDescend {
+    const uint32_t* prog_consts;  // nprog + 1 offsets into the constant table
+    const AdjConst* consts;
+    int32_t nprog;
+    const float* fit;
+    const uint32_t* bad;
+    const float* graw;
+    float* cur;    // [nconst] constants being evaluated
+    float* cbest;  // [nconst] best constants so far
+    float* gbest;  // [nconst] gradient at cbest
+    float* fbest;  // [nprog]
+    float* lam;    // [nprog] step length
+    Ins* code;     // forward tape: the constants' instruction arguments are rewritten
+    int32_t iter;
+    float step0;
+    int32_t last;  // 1: write back cbest and stop
+};
+
+bool adjoint_shape_supported(int R, int F, int S, int PS);
+int adjoint_store_partials(int PS);
+hipError_t launch_adjoint(const AdjArgs& A, int grid, int R, int F, int S, int PS, bool P, bool N, hipStream_t st);
+int adjoint_occupancy(int R, int F, int S, int PS, bool P, bool N);
+hipError_t launch_adjoint_finalize(const AdjFinal& F, hipStream_t st);
+hipError_t launch_adjoint_descend(const AdjDescend& D, hipStream_t st);
+
 bool shape_supported(int R, int F, int S);
 bool batch_shape_supported(int R, int F, int S);
 hipError_t launch_batch(const BatchArgs& B, int grid, int R, int F, int S, bool P, bool N, hipStream_t st);

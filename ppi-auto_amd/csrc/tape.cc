@@ -164,15 +164,30 @@ struct Compiler {
             vstack.push_back((int32_t)nodes.size() - 1);
         }
         if (vstack.empty()) return -1;
-        return emit(vstack.back(), out);
+        root_node = vstack.back();
+        return emit(root_node, out);
     }
 
     int pending_save = -1;
     int max_slot = 0;
+    int32_t root_node = -1;
+
+    // Derivative bookkeeping (compile_adjoint): the tape index of each operator
+    // node's instruction, and the operand side (0 = accumulator, 1 = folded or
+    // parked operand) on which each node enters its parent's instruction.
+    bool track = false;
+    std::vector<int32_t> ins_at;
+    std::vector<uint8_t> side;
 
     void put(Tape* out, uint32_t code, uint32_t arg, uint32_t cpos) {
         out->code.push_back(Ins{code, arg});
         if (want_const_pos) out->const_pos.push_back(cpos);
+    }
+    void mark(int32_t node, Tape* out) {
+        if (track) ins_at[node] = (int32_t)out->code.size() - 1;
+    }
+    void mark_side(int32_t node, uint8_t s) {
+        if (track) side[node] = s;
     }
 
     void emit_load(const Node& n, Tape* out) {
@@ -195,6 +210,10 @@ struct Compiler {
     int emit(int32_t root, Tape* out) {
         pending_save = -1;
         max_slot = 0;
+        if (track) {
+            ins_at.assign(nodes.size(), -1);
+            side.assign(nodes.size(), 0);
+        }
         frames.clear();
         frames.push_back(Frame{root, 0, 0, {0, 0, 0}});
         while (!frames.empty()) {
@@ -213,6 +232,8 @@ struct Compiler {
                     frames.push_back(Frame{c, b, 0, {0, 0, 0}});
                 } else {
                     put(out, H_UN + unop_of(n.op), 0, 0);
+                    mark(f.node, out);
+                    mark_side(n.ch[0], 0);
                     frames.pop_back();
                 }
                 continue;
@@ -224,6 +245,9 @@ struct Compiler {
                 if (lleaf && rleaf) {
                     emit_load(L, out);
                     emit_fold(binop_of(n.op, false), R, out);
+                    mark(f.node, out);
+                    mark_side(n.ch[0], 0);
+                    mark_side(n.ch[1], 1);
                     frames.pop_back();
                 } else if (lleaf || rleaf) {
                     if (f.phase == 0) {
@@ -233,6 +257,9 @@ struct Compiler {
                     } else {
                         // acc holds the computed child; fold the leaf on its side.
                         emit_fold(binop_of(n.op, /*rev=*/lleaf), lleaf ? L : R, out);
+                        mark(f.node, out);
+                        mark_side(n.ch[0], lleaf ? 1 : 0);
+                        mark_side(n.ch[1], lleaf ? 0 : 1);
                         frames.pop_back();
                     }
                 } else {
@@ -249,6 +276,9 @@ struct Compiler {
                     } else {
                         // r[base] holds the first-evaluated child, acc the second.
                         put(out, H_BIN_S + binop_of(n.op, /*rev=*/left_first) + ((uint32_t)f.base << 8), 0, 0);
+                        mark(f.node, out);
+                        mark_side(n.ch[0], left_first ? 1 : 0);
+                        mark_side(n.ch[1], left_first ? 0 : 1);
                         frames.pop_back();
                     }
                 }
@@ -283,6 +313,7 @@ struct Compiler {
                 uint32_t code = H_TERN | ((uint32_t)f.base << 8) | ((uint32_t)(f.base + 1) << 13) |
                                 (where[0] << 18) | (where[1] << 20) | (where[2] << 22);
                 put(out, code, 0, 0);
+                mark(f.node, out);
                 frames.pop_back();
             }
         }
@@ -397,6 +428,318 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
         out->stats.n_invalid += p.stats.n_invalid;
         out->stats.n_empty += p.stats.n_empty;
         out->stats.max_col = std::max(out->stats.max_col, p.stats.max_col);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint tape
+// ---------------------------------------------------------------------------
+namespace {
+
+// Partial of an operator instruction with respect to one operand: 0 = zero
+// everywhere it exists (the path is cut), 1 = the constant *v, 2 = stored by the
+// forward pass.  side: 0 = accumulator operand, 1 = the other operand; for
+// IF_THEN_ELSE the caller asks only about the then/else operands.
+int partial_kind(uint32_t code, int side, float* v) {
+    const uint32_t h = code & 0xffu;
+    if (h == H_TERN) return 2;
+    if (h >= H_UN && h < H_BIN_C) {
+        switch (h - H_UN) {
+        case U_NEG: *v = -1.0f; return 1;
+        case U_NOT: case U_ROUND: case U_CEIL: case U_FLOOR: case U_STEP: case U_SIGN: return 0;
+        default: return 2;
+        }
+    }
+    uint32_t op = h >= H_BIN_S ? h - H_BIN_S : (h >= H_BIN_A ? h - H_BIN_A : h - H_BIN_C);
+    switch (op) {
+    case B_AND: case B_OR: case B_XOR: case B_GT: case B_GE: case B_LT: case B_LE: case B_EQ: case B_NE:
+        return 0;
+    case B_ADD: *v = 1.0f; return 1;
+    case B_SUB: *v = side == 0 ? 1.0f : -1.0f; return 1;
+    case B_RSUB: *v = side == 0 ? -1.0f : 1.0f; return 1;
+    case B_MEAN: *v = 0.5f; return 1;
+    case B_MOD: if (side == 0) { *v = 1.0f; return 1; } return 2;
+    case B_RMOD: if (side == 1) { *v = 1.0f; return 1; } return 2;
+    default: return 2;
+    }
+}
+
+// Builds the jobs of one compiled program (Compiler state: nodes, ins_at, side, root_node).
+struct AdjBuilder {
+    const Compiler& c;
+    const Tape& t;
+    int32_t B;
+    AdjTape* out;
+    std::vector<uint8_t> live, act;
+    std::vector<int32_t> need, kidx;
+    struct Edge { int32_t parent, child; int kind; float v; };
+
+    AdjBuilder(const Compiler& cc, const Tape& tt, int32_t b, AdjTape* o) : c(cc), t(tt), B(b), out(o) {}
+
+    // Kind of the edge parent -> child (child index j in the tree).
+    int edge_kind(int32_t parent, int j, float* v) const {
+        const Node& n = c.nodes[parent];
+        const uint32_t code = t.code[(size_t)c.ins_at[parent]].code;
+        if (arity(n.op) == 3) return j == 0 ? 0 : 2;
+        return partial_kind(code, c.side[n.ch[j]], v);
+    }
+
+    // Live T_CONST leaves of the root's tree (uncut path), in phenotype order.
+    void live_consts(std::vector<int32_t>* consts) {
+        const size_t nn = c.nodes.size();
+        live.assign(nn, 0);
+        std::vector<int32_t> st{c.root_node};
+        live[(size_t)c.root_node] = 1;
+        while (!st.empty()) {
+            int32_t v = st.back();
+            st.pop_back();
+            const Node& n = c.nodes[(size_t)v];
+            if (n.op < 0) {
+                if (n.is_tconst) consts->push_back(v);
+                continue;
+            }
+            for (int j = 0; j < arity(n.op); ++j) {
+                float dummy;
+                if (edge_kind(v, j, &dummy) == 0) continue;
+                live[(size_t)n.ch[j]] = 1;
+                st.push_back(n.ch[j]);
+            }
+        }
+        std::sort(consts->begin(), consts->end(),
+                  [&](int32_t a, int32_t b) { return c.nodes[(size_t)a].pos < c.nodes[(size_t)b].pos; });
+    }
+
+    // Emits one job over the constants group[0..ng).
+    bool job(int32_t prog, const ProgMeta& m, const int32_t* group, int ng, bool first, std::string* err) {
+        const size_t nn = c.nodes.size();
+        act.assign(nn, 0);
+        kidx.assign(nn, -1);
+        for (int k = 0; k < ng; ++k) { act[(size_t)group[k]] = 1; kidx[(size_t)group[k]] = k; }
+        // Nodes are stored children-first, so one forward sweep propagates activity up.
+        for (size_t v = 0; v < nn; ++v) {
+            const Node& n = c.nodes[v];
+            if (n.op < 0 || !live[v]) continue;
+            for (int j = 0; j < arity(n.op); ++j) {
+                float dummy;
+                if (act[(size_t)n.ch[j]] && edge_kind((int32_t)v, j, &dummy) != 0) act[v] = 1;
+            }
+        }
+        need.assign(nn, 0);
+        for (size_t v = 0; v < nn; ++v) {
+            const Node& n = c.nodes[v];
+            if (!act[v] || n.op < 0) continue;
+            std::vector<int32_t> kids = active_kids((int32_t)v);
+            int nd = 1;
+            for (size_t j = 0; j < kids.size(); ++j) {
+                int cn = need[(size_t)kids[j]];
+                nd = std::max(nd, j + 1 < kids.size() ? 1 + cn : cn);
+            }
+            need[v] = nd;
+        }
+        AdjJob J{};
+        J.code_off = m.code_off;
+        J.ncode = m.ncode;
+        J.prog = prog;
+        J.const_base = (uint32_t)out->consts.size();
+        J.nconst = (uint32_t)ng;
+        J.flags = (m.flags & (kEmpty | kInvalid)) | (first ? kAdjFitness : 0u);
+        J.rec_off = (uint32_t)out->rec.size();
+        out->rec.resize(out->rec.size() + m.ncode, 0u);
+        const size_t rev0 = out->rev.size();
+        std::vector<int32_t> seg_of;     // per reverse instruction
+        std::vector<uint32_t> e_ins;     // per stored partial: instruction (relative)
+        int32_t cur_seg = 0;
+        int max_slot = 0;
+        auto put_rev = [&](uint32_t kind, int src, int dst, int kindp, float v, int32_t e) {
+            RevIns r{};
+            r.w = kind | ((uint32_t)src << 4) | ((uint32_t)dst << 8);
+            if (kindp == 2) {
+                r.w |= kRevStored | ((uint32_t)e << 16);
+                cur_seg = e / B;
+            } else {
+                std::memcpy(&r.p, &v, 4);
+            }
+            out->rev.push_back(r);
+            seg_of.push_back(cur_seg);
+        };
+        if (ng > 0) {
+            const int32_t root = c.root_node;
+            if (c.nodes[(size_t)root].op < 0) {
+                put_rev(kRevGrad, 0, kidx[(size_t)root], 1, 1.0f, 0);
+            } else {
+                if (need[(size_t)root] > kAdjSlots) {
+                    if (err) *err = "program " + std::to_string(prog) + " needs " + std::to_string(need[(size_t)root]) +
+                                    " adjoint slots (max " + std::to_string(kAdjSlots) + ")";
+                    return false;
+                }
+                // Preorder walk with an explicit stack of (node, slot, next child).
+                struct Fr { int32_t v; int slot; std::vector<int32_t> kids; size_t j; };
+                std::vector<Fr> st;
+                st.push_back(Fr{root, 0, active_kids(root), 0});
+                while (!st.empty()) {
+                    Fr& f = st.back();
+                    if (f.j == f.kids.size()) { st.pop_back(); continue; }
+                    const int32_t v = f.v, ch = f.kids[f.j];
+                    const bool last = f.j + 1 == f.kids.size();
+                    const int slot = f.slot;
+                    ++f.j;
+                    const Node& n = c.nodes[(size_t)v];
+                    int jj = 0;
+                    while (n.ch[jj] != ch) ++jj;
+                    float sv = 0.0f;
+                    const int kind = edge_kind(v, jj, &sv);
+                    int32_t e = -1;
+                    if (kind == 2) {
+                        e = (int32_t)e_ins.size();
+                        const uint32_t rel = (uint32_t)(c.ins_at[(size_t)v] - (int32_t)m.code_off);
+                        e_ins.push_back(rel);
+                        const bool hi = arity(n.op) == 3 ? (jj == 2) : (c.side[(size_t)ch] == 1);
+                        out->rec[J.rec_off + rel] |= (uint32_t)(e + 1) << (hi ? 16 : 0);
+                    }
+                    if (c.nodes[(size_t)ch].op < 0) {
+                        put_rev(kRevGrad, slot, kidx[(size_t)ch], kind, sv, e);
+                    } else {
+                        const int cs = last ? slot : slot + 1;
+                        max_slot = std::max(max_slot, cs + 1);
+                        put_rev(kRevAdj, slot, cs, kind, sv, e);
+                        st.push_back(Fr{ch, cs, active_kids(ch), 0});
+                    }
+                }
+            }
+        }
+        if (e_ins.size() >= 0xffffu) {
+            if (err) *err = "program " + std::to_string(prog) + ": too many stored partials";
+            return false;
+        }
+        const int32_t nst = (int32_t)e_ins.size();
+        const int32_t nseg = std::max<int32_t>(1, (nst + B - 1) / B);
+        J.seg_off = (uint32_t)out->segs.size();
+        J.nseg = (uint32_t)nseg;
+        size_t r = rev0;
+        for (int32_t s = 0; s < nseg; ++s) {
+            AdjSeg sg{};
+            sg.rev_begin = (uint32_t)r;
+            while (r < out->rev.size() && seg_of[r - rev0] == s) ++r;
+            sg.rev_end = (uint32_t)r;
+            if (s == 0) {
+                sg.fwd_end = m.ncode;
+            } else {
+                uint32_t hi = 0;
+                for (int32_t e = s * B; e < std::min(nst, (s + 1) * B); ++e) hi = std::max(hi, e_ins[(size_t)e] + 1);
+                sg.fwd_end = hi;
+            }
+            out->segs.push_back(sg);
+        }
+        J.cost = m.ncode * (uint32_t)nseg + (uint32_t)(out->rev.size() - rev0);
+        for (int k = 0; k < ng; ++k) {
+            AdjConst ac{};
+            ac.prog = prog;
+            ac.pos = c.nodes[(size_t)group[k]].pos;
+            ac.ins = 0xffffffffu;
+            for (uint32_t q = m.code_off; q < m.code_off + m.ncode; ++q)
+                if (t.const_pos[q] == (uint32_t)ac.pos + 1) { ac.ins = q; break; }
+            out->consts.push_back(ac);
+        }
+        out->jobs.push_back(J);
+        out->max_seg = std::max(out->max_seg, nseg);
+        out->max_adj_slots = std::max(out->max_adj_slots, max_slot);
+        out->stored_partials += nst;
+        out->total_rev += (int64_t)(out->rev.size() - rev0);
+        return true;
+    }
+
+    // Active children of v, ordered so that the one needing most adjoint slots
+    // comes last (it reuses v's slot).
+    std::vector<int32_t> active_kids(int32_t v) const {
+        const Node& n = c.nodes[(size_t)v];
+        std::vector<int32_t> k;
+        for (int j = 0; j < arity(n.op); ++j) {
+            float dummy;
+            if (act[(size_t)n.ch[j]] && edge_kind(v, j, &dummy) != 0) k.push_back(n.ch[j]);
+        }
+        std::stable_sort(k.begin(), k.end(), [&](int32_t a, int32_t b) {
+            return need[(size_t)a] < need[(size_t)b];
+        });
+        return k;
+    }
+};
+
+}  // namespace
+
+bool compile_adjoint(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                     int32_t stride, int32_t n, int32_t ncol, int32_t B, AdjTape* out, std::string* err) {
+    if (B <= 0 || (B & (B - 1))) {
+        if (err) *err = "partial store size must be a power of two";
+        return false;
+    }
+    *out = AdjTape{};
+    Tape& t = out->fwd;
+    Compiler c(st, ncol, true);
+    c.track = true;
+    AdjBuilder ab(c, t, B, out);
+    out->prog_jobs.push_back(0);
+    out->prog_consts.push_back(0);
+    std::vector<int32_t> consts;
+    for (int32_t i = 0; i < n; ++i) {
+        ProgMeta m{};
+        m.code_off = (uint32_t)t.code.size();
+        m.size = size[i];
+        const int32_t sz = size[i];
+        if (sz < 0 || sz > stride) {
+            if (err) *err = "program " + std::to_string(i) + ": size " + std::to_string(sz) + " outside [0, " +
+                            std::to_string(stride) + "]";
+            return false;
+        }
+        bool compiled = false;
+        if (sz == 0) {
+            m.flags = kEmpty;
+            t.stats.n_empty++;
+        } else {
+            int slots = c.compile(phenotype + (size_t)i * stride, ephemeral + (size_t)i * stride, sz, &t);
+            if (slots < 0) {
+                t.code.resize(m.code_off);
+                t.const_pos.resize(m.code_off);
+                t.code.push_back(Ins{H_LOAD_C, 0x7fc00000u});
+                t.const_pos.push_back(0);
+                m.flags = kInvalid;
+                t.stats.n_invalid++;
+                slots = 0;
+            } else {
+                compiled = true;
+            }
+            if (slots > kMaxSlots) {
+                if (err) *err = "program " + std::to_string(i) + " needs " + std::to_string(slots) +
+                                " register slots (max " + std::to_string(kMaxSlots) + ")";
+                return false;
+            }
+            m.flags |= (uint32_t)slots << 8;
+            t.stats.total_size += sz;
+            t.stats.max_slots = std::max(t.stats.max_slots, slots);
+        }
+        m.ncode = (uint32_t)t.code.size() - m.code_off;
+        t.code.push_back(Ins{H_END, m.flags & (kEmpty | kInvalid)});
+        t.const_pos.push_back(0);
+        t.stats.total_code += m.ncode;
+        t.stats.max_code = std::max<int32_t>(t.stats.max_code, (int32_t)m.ncode);
+        t.meta.push_back(m);
+        consts.clear();
+        if (compiled) ab.live_consts(&consts);
+        if (consts.empty()) {
+            if (!ab.job(i, m, nullptr, 0, true, err)) return false;
+        } else {
+            for (size_t g = 0; g < consts.size(); g += kAdjMaxConsts) {
+                int ng = (int)std::min<size_t>(kAdjMaxConsts, consts.size() - g);
+                if (!ab.job(i, m, consts.data() + g, ng, g == 0, err)) return false;
+            }
+        }
+        out->prog_jobs.push_back((uint32_t)out->jobs.size());
+        out->prog_consts.push_back((uint32_t)out->consts.size());
+    }
+    for (size_t k = 0; k < t.code.size(); ++k) {
+        uint32_t h = t.code[k].code & 0xffu;
+        if (h == H_LOAD_A || h == H_PUSH_A || (h >= H_BIN_A && h < H_BIN_S))
+            t.stats.max_col = std::max<int32_t>(t.stats.max_col, (int32_t)t.code[k].arg);
     }
     return true;
 }

@@ -115,6 +115,96 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
                            int threads, Tape* out, std::string* err);
 
+// ---------------------------------------------------------------------------
+// Reverse-mode adjoint tape: d(fitness)/d(ephemeral T_CONST values).
+//
+// Every T_CONST leaf is its own parameter and a tree node has one parent, so the
+// derivative of the program value with respect to a constant is the product of
+// the local partials on the constant's path to the root.  Paths through
+// operators whose derivative is zero everywhere it exists (logic, comparisons,
+// ROUND/CEIL/FLOOR/STEP/SIGN, the condition of IF_THEN_ELSE) are cut: those
+// constants have gradient 0 and are left out.
+//
+// The forward tape is the ordinary accumulator tape.  A parallel record word per
+// instruction names the partials that instruction must store while it executes
+// (derivative with respect to the accumulator operand, and with respect to the
+// other operand; for IF_THEN_ELSE the then- and else-selectors).  Partials that
+// are constants (ADD, SUB, MEAN, NEG, one side of MOD) are not stored.  Stored
+// partials are numbered e = 0, 1, ... in the order the reverse sweep reads them.
+//
+// The reverse sweep walks the active tree root-first.  Adjoints live in
+// register slots with stack discipline (a node's last child reuses its slot), and
+// a constant's contribution adj(parent) * partial is summed into the lane's
+// gradient accumulator k.  When a kernel keeps B partials per row, partials e
+// with e / B == s belong to segment s: the kernel re-runs the forward pass (up
+// to fwd_end) once per segment, keeping only that segment's partials, and then
+// runs the segment's reverse instructions.  Adjoint slots persist across
+// segments.  A program with more than kAdjMaxConsts live constants is split
+// into several jobs, each owning up to kAdjMaxConsts constants.
+// ---------------------------------------------------------------------------
+constexpr int kAdjMaxConsts = 16;  // gradient accumulators per lane
+constexpr int kAdjSlots = 12;      // adjoint register slots per row
+
+enum : uint32_t { kAdjFitness = 1u << 4 };  // AdjJob::flags: this job also accumulates the fitness
+
+struct AdjJob {
+    uint32_t code_off;    // forward tape of the program (as ProgMeta::code_off)
+    uint32_t ncode;
+    uint32_t rec_off;     // record words of this job: rec[rec_off + pc], pc < ncode
+    uint32_t seg_off;     // segments of this job: segs[seg_off + s], s < nseg
+    uint32_t nseg;
+    int32_t prog;
+    uint32_t const_base;  // gradient index of this job's constant 0
+    uint32_t nconst;
+    uint32_t flags;       // kEmpty / kInvalid of the program, kAdjFitness
+    uint32_t cost;        // ncode * nseg + reverse instructions (scheduling weight)
+    uint32_t pad_[2];
+};
+
+struct AdjSeg {
+    uint32_t rev_begin, rev_end;  // reverse instructions [rev_begin, rev_end) (absolute)
+    uint32_t fwd_end;             // forward instructions [0, fwd_end) needed by the segment
+    uint32_t pad_;
+};
+
+// Reverse instruction.  w: bits 0-1 kind (kRevAdj: adj[dst] = adj[src] * p;
+// kRevGrad: g[dst] += adj[src] * p), bit 2 = p is stored partial e (else the
+// float bits in `p`), bits 4-7 src slot, bits 8-12 dst slot / constant,
+// bits 16-31 e.
+struct RevIns {
+    uint32_t w;
+    uint32_t p;
+};
+enum : uint32_t { kRevAdj = 0u, kRevGrad = 1u, kRevStored = 4u };
+
+struct AdjConst {
+    int32_t prog;   // program
+    int32_t pos;    // phenotype position of the T_CONST
+    uint32_t ins;   // tape instruction whose argument holds the constant
+    uint32_t pad_;
+};
+
+struct AdjTape {
+    Tape fwd;                          // forward tape (identical to compile_population's), with const_pos
+    std::vector<AdjJob> jobs;          // program order; a program's jobs are consecutive
+    std::vector<uint32_t> rec;
+    std::vector<AdjSeg> segs;
+    std::vector<RevIns> rev;
+    std::vector<AdjConst> consts;      // gradient index -> constant
+    std::vector<uint32_t> prog_jobs;   // n + 1 offsets into jobs
+    std::vector<uint32_t> prog_consts; // n + 1 offsets into consts
+    int32_t max_seg = 0;               // most segments of one job
+    int32_t max_adj_slots = 0;
+    int64_t stored_partials = 0;       // sum over jobs of stored partials
+    int64_t total_rev = 0;
+};
+
+// Compiles programs [0, n) for the adjoint kernels, whose partial store holds B
+// (a power of two) partials per row.  Returns false with `err` set on a program
+// the forward compiler rejects or that needs more than kAdjSlots adjoint slots.
+bool compile_adjoint(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                     int32_t stride, int32_t n, int32_t ncol, int32_t B, AdjTape* out, std::string* err);
+
 }  // namespace ppi
 
 #endif

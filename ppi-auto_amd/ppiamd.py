@@ -76,6 +76,18 @@ class BatchStats(C.Structure):
     ]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [
+        ("compile_ms", C.c_double), ("backward_ms", C.c_double), ("kernel_ms", C.c_double),
+        ("descend_ms", C.c_double), ("node_evals", C.c_double), ("forward_ins", C.c_double),
+        ("reverse_ins", C.c_double), ("n_jobs", C.c_int32), ("n_consts", C.c_int32),
+        ("max_segments", C.c_int32), ("partials", C.c_int32), ("rows_per_lane", C.c_int32),
+        ("ranges", C.c_int32), ("descend_steps", C.c_int32), ("pad_", C.c_int32), ("tune_ms", C.c_double * 2),
+    ]
+
+
+PARTIALS_AUTO, PARTIALS_RECOMPUTE, PARTIALS_STORE = 0, 1, 2
+
 ENGINE_SYMBOLS = [
     "ppi_abi_version", "ppi_last_error", "ppi_device_count", "ppi_engine_create", "ppi_engine_destroy",
     "ppi_engine_configure", "ppi_engine_set_dataset", "ppi_engine_set_dataset_rowmajor", "ppi_engine_upload",
@@ -83,6 +95,9 @@ ENGINE_SYMBOLS = [
     "ppi_engine_time", "ppi_batch_create", "ppi_batch_destroy", "ppi_batch_configure", "ppi_batch_add_dataset",
     "ppi_batch_add_population", "ppi_batch_clear_populations", "ppi_batch_build", "ppi_batch_launch",
     "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
+    "ppi_adjoint_create", "ppi_adjoint_destroy", "ppi_adjoint_configure", "ppi_adjoint_set_dataset_rowmajor",
+    "ppi_adjoint_forward", "ppi_adjoint_backward", "ppi_adjoint_descend", "ppi_adjoint_time",
+    "ppi_adjoint_get_stats",
 ]
 
 
@@ -125,8 +140,20 @@ def lib() -> C.CDLL:
         L.ppi_batch_results.argtypes = [C.c_void_p, C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
         L.ppi_batch_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
         L.ppi_batch_get_stats.argtypes = [C.c_void_p, P(BatchStats)]
+        L.ppi_adjoint_create.argtypes = [P(C.c_void_p), C.c_int32]
+        L.ppi_adjoint_destroy.argtypes = [C.c_void_p]
+        L.ppi_adjoint_destroy.restype = None
+        L.ppi_adjoint_configure.argtypes = [C.c_void_p, P(Config), P(Symbol), C.c_int32]
+        L.ppi_adjoint_set_dataset_rowmajor.argtypes = [C.c_void_p, P(C.c_float), C.c_int32, C.c_int32]
+        L.ppi_adjoint_forward.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
+                                          C.c_int32, C.c_float, P(C.c_float)]
+        L.ppi_adjoint_backward.argtypes = [C.c_void_p, P(C.c_float), C.c_int32, P(C.c_float)]
+        L.ppi_adjoint_descend.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_int32, P(C.c_float), P(C.c_float)]
+        L.ppi_adjoint_time.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double)]
+        L.ppi_adjoint_get_stats.argtypes = [C.c_void_p, P(AdjointStats)]
         for name in ENGINE_SYMBOLS:
-            if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy", "ppi_batch_destroy"):
+            if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy", "ppi_batch_destroy",
+                            "ppi_adjoint_destroy"):
                 getattr(L, name).restype = C.c_int32
         _lib = L
     return _lib
@@ -357,6 +384,66 @@ class Batch:
         return s
 
 
+class Adjoint:
+    """d(fitness)/d(ephemeral constants): forward, backward and on-device descent."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().ppi_adjoint_create(C.byref(self._h), device))
+        self._keep = []
+        self.pop = None
+
+    def close(self):
+        if self._h:
+            lib().ppi_adjoint_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, table, **kw):
+        cfg, keep = make_config(**kw)
+        self._keep.append(keep)
+        _check(lib().ppi_adjoint_configure(self._h, C.byref(cfg), _table_struct(table), len(table)))
+
+    def set_dataset(self, data: np.ndarray) -> None:
+        d = np.ascontiguousarray(data, dtype=np.float32)
+        _check(lib().ppi_adjoint_set_dataset_rowmajor(self._h, _ptr(d, C.c_float), d.shape[0], d.shape[1]))
+
+    def forward(self, pop: "Population", alpha: float = 0.0) -> np.ndarray:
+        fit = np.empty(pop.n, dtype=np.float32)
+        _check(lib().ppi_adjoint_forward(self._h, _ptr(pop.phenotype, C.c_int32), _ptr(pop.ephemeral, C.c_float),
+                                         _ptr(pop.size, C.c_int32), pop.stride, pop.n, alpha, _ptr(fit, C.c_float)))
+        self.pop = pop
+        return fit
+
+    def backward(self, grad_fitness: Optional[np.ndarray] = None, partials: int = PARTIALS_AUTO) -> np.ndarray:
+        g = np.empty(self.pop.n * self.pop.stride, dtype=np.float32)
+        gf = None if grad_fitness is None else np.ascontiguousarray(grad_fitness, dtype=np.float32)
+        _check(lib().ppi_adjoint_backward(self._h, _ptr(gf, C.c_float) if gf is not None else None, partials,
+                                          _ptr(g, C.c_float)))
+        return g
+
+    def descend(self, steps: int, step: float = 0.1, partials: int = PARTIALS_AUTO):
+        eph = np.empty(self.pop.n * self.pop.stride, dtype=np.float32)
+        fit = np.empty(self.pop.n, dtype=np.float32)
+        _check(lib().ppi_adjoint_descend(self._h, steps, step, partials, _ptr(eph, C.c_float), _ptr(fit, C.c_float)))
+        return eph, fit
+
+    def time(self, iters: int, partials: int = PARTIALS_AUTO) -> float:
+        ms = C.c_double(0.0)
+        _check(lib().ppi_adjoint_time(self._h, iters, partials, C.byref(ms)))
+        return ms.value
+
+    def stats(self) -> AdjointStats:
+        s = AdjointStats()
+        _check(lib().ppi_adjoint_get_stats(self._h, C.byref(s)))
+        return s
+
+
 # ---------------------------------------------------------------------------
 # workloads
 # ---------------------------------------------------------------------------
@@ -495,6 +582,9 @@ def workload(name: str, n_programs: Optional[int] = None, nlin: Optional[int] = 
                      100,000 x 16 N(0,1), y = random cubic + noise, EF = (X-Y)^2
     classification   30,000 programs (depth <= 10, <= 128), 150,000 x 4 from 4 clusters,
                      EF = COST[(int)clamp(X,0,3)][(int)Y] with a 0/1 cost matrix
+    adjoint          30,000 programs (depth <= 10, <= 128 symbols, <= 8 T_CONST each, constants
+                     U[-5,5]), 1,000,000 x 8 N(0,1), y = exp(0.5*x0) + sin(2*x1) + x2*x3*x4,
+                     EF = (X-Y)^2
     """
     if name == "small":
         gr = Grammar(4)
@@ -511,6 +601,11 @@ def workload(name: str, n_programs: Optional[int] = None, nlin: Optional[int] = 
         pop = grow(gr, n_programs or 30000, seed, 10, 128)
         data = dataset(DS_CLUSTERS, seed + 1, nlin or 150000, 4)
         return Workload(name, gr, pop, data, EF_COST_CLAMP, cost=COST4, reducemax=name.endswith("_max"))
+    if name == "adjoint":
+        gr = Grammar(8)
+        pop = grow(gr, n_programs or 30000, seed, 10, 128, max_consts=8)
+        data = dataset(DS_FORMULA_ADJOINT, seed + 1, nlin or 1000000, 8)
+        return Workload(name, gr, pop, data, EF_SQUARED)
     raise ValueError(f"unknown workload {name!r}")
 
 

@@ -51,6 +51,10 @@ def lib():
                                       P(C.c_int32), C.c_int32, C.c_int32, P(C.c_float), C.c_int32, C.c_int32,
                                       C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32)]
         L.oracle_grad_f64.restype = C.c_int
+        L.oracle_grad_mag_f64.argtypes = [P(OracleCfg), P(ppiamd.Symbol), C.c_int32, P(C.c_int32), P(C.c_double),
+                                          P(C.c_int32), C.c_int32, C.c_int32, P(C.c_float), C.c_int32, C.c_int32,
+                                          C.c_int32, P(C.c_double), P(C.c_double), C.c_int32]
+        L.oracle_grad_mag_f64.restype = C.c_int
         L.oracle_time_seq.argtypes = [P(OracleCfg), P(ppiamd.Symbol), C.c_int32, P(C.c_int32), P(C.c_float),
                                       P(C.c_int32), C.c_int32, C.c_int32, P(C.c_float), C.c_int32, C.c_int32,
                                       C.c_float, P(C.c_float), C.c_int32]
@@ -161,6 +165,21 @@ def grad_f64(cfg, table, pop, data, max_consts=8, eph64=None):
                               _p(fit, C.c_double), _p(g, C.c_double), _p(nc, C.c_int32))
     assert r == 0
     return fit, g, nc
+
+
+def grad_mag_f64(cfg, table, pop, data, max_consts=16, eph64=None, f32_values=False):
+    """FP64 gradient and the L1 magnitude of its per-row terms (both / nlin), constants in position order.
+    f32_values: derivative of the float computation (float operator values, FP64 tangents)."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    e64 = np.ascontiguousarray(pop.ephemeral.astype(np.float64) if eph64 is None else eph64, dtype=np.float64)
+    g = np.zeros((pop.n, max_consts), dtype=np.float64)
+    m = np.zeros((pop.n, max_consts), dtype=np.float64)
+    r = lib().oracle_grad_mag_f64(C.byref(cfg), _table(table), len(table), _p(pop.phenotype, C.c_int32),
+                                  _p(e64, C.c_double), _p(pop.size, C.c_int32), pop.stride, pop.n,
+                                  _p(data, C.c_float), data.shape[0], data.shape[1], max_consts,
+                                  _p(g, C.c_double), _p(m, C.c_double), int(f32_values))
+    assert r == 0
+    return g, m
 
 
 def time_seq(cfg, table, pop, data, alpha=0.0, threads=1):
