@@ -266,7 +266,7 @@ struct AdjFeat {
 };
 
 template <int R, int F, int S, int PS, bool P, bool N>
-__global__ void __launch_bounds__(kBlock) adjoint_kernel(AdjArgs A) {
+__global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     typedef AdjCore<R, S, PS, P, N> AC;
     typedef typename AC::VS VS;
     constexpr int B = AC::B;

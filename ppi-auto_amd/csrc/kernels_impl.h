@@ -30,6 +30,18 @@ namespace dev {
 constexpr int kWaves = 4;              // waves per workgroup in FP / PDP forms
 constexpr int kBlock = 64 * kWaves;
 
+// Waves per SIMD the register allocator must leave room for (the second
+// __launch_bounds__ argument caps VGPRs at 512 / waves) in the FP and batched
+// forms.  Measured on gfx950 with the regression workload: 5 waves at 2 rows
+// per lane and 4 waves at 4 rows beat the unbounded allocation by 11% and 15%;
+// 1 row per lane (the 16-slot fallback) takes 4 to stay clear of spills; at 8
+// rows any bound spills, so none is set.  The PDP form keeps its rows'
+// features in VGPRs and is left unbounded.
+template <int R>
+struct MinWaves {
+    static constexpr int value = R == 2 ? 5 : (R <= 4 ? 4 : 1);
+};
+
 template <int K> struct VecT { typedef float type __attribute__((ext_vector_type(K))); };
 
 // The tape and program headers are read-only for the whole launch and every
@@ -413,7 +425,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
 }
 
 template <int R, int F, int S, bool P, bool N>
-__global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
+__global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(EvalArgs A) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     const int c0 = blockIdx.y * A.chunk;
@@ -428,7 +440,7 @@ __global__ void __launch_bounds__(kBlock) fp_tile_kernel(EvalArgs A) {
 // workgroup leaves the loop when the counter passes the end.
 // ---------------------------------------------------------------------------
 template <int R, int F, int S, bool P, bool N>
-__global__ void __launch_bounds__(kBlock) batch_kernel(BatchArgs B) {
+__global__ void __launch_bounds__(kBlock, MinWaves<R>::value) batch_kernel(BatchArgs B) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     __shared__ int item_s;
