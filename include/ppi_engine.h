@@ -200,7 +200,7 @@ typedef struct ppi_batch_stats {
     int32_t rows_per_lane;  /* chosen R */
     int32_t feature_slots;  /* F of the kernel shape (0 = attributes read from global memory) */
     int32_t autotuned;      /* 1 if R was picked by measurement */
-    double tune_ms[4];      /* sampled ms per candidate R = 2, 4, 8 (0 = not tried) */
+    double tune_ms[4];      /* sampled ms per 10^9 node-evaluations, candidates R = 2, 4, 8 (0 = not tried) */
 } ppi_batch_stats;
 
 ppi_status ppi_batch_create(ppi_batch** out, int32_t device);
@@ -267,7 +267,7 @@ typedef struct ppi_adjoint_stats {
     int32_t rows_per_lane;
     int32_t ranges;          /* row ranges per job */
     int32_t descend_steps;
-    int32_t pad_;
+    int32_t descend_reverted; /* programs whose refined constants did not lower the plain fitness (kept) */
     double tune_ms[2];       /* AUTO: sampled ms of RECOMPUTE and STORE */
 } ppi_adjoint_stats;
 
@@ -281,7 +281,9 @@ ppi_status ppi_adjoint_forward(ppi_adjoint* a, const int32_t* phenotype, const f
 /* grad_fitness: n_programs upstream gradients, or NULL for all ones.  grad: n_programs*stride. */
 ppi_status ppi_adjoint_backward(ppi_adjoint* a, const float* grad_fitness, int32_t partials, float* grad);
 /* ephemeral_out: n_programs*stride (the forward's ephemerals with refined T_CONST values);
-   fitness_out: n_programs; either may be NULL.  The kept programs are updated. */
+   fitness_out: n_programs, the plain evaluator's fitness of the returned programs; either may be
+   NULL.  A program whose refined constants do not lower that fitness keeps its original ones, so
+   no program comes out worse.  The kept programs are updated. */
 ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_t partials, float* ephemeral_out,
                                float* fitness_out);
 /* `iters` backward passes back to back (no host transfers); mean device ms per pass. */

@@ -464,6 +464,12 @@ ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_
     if ((s = choose_store(a, partials, &ps))) return s;
     if ((s = prepare(a, ps))) return s;
     a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    // Plain-evaluator fitness of the starting point, for the final no-regression check.
+    std::vector<float> f_start((size_t)a->n), f_end((size_t)a->n);
+    if ((s = ppi_engine_evaluate(a->eng, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->alpha,
+                                 f_start.data(), nullptr, nullptr)))
+        return s;
+    const std::vector<float> eph_start = a->eph;
     AdjTape& T = a->T;
     const size_t nc = T.consts.size();
     std::vector<float> cur(nc);
@@ -498,12 +504,23 @@ ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_
     a->stats.descend_ms = ms;
     a->stats.descend_steps = steps;
     for (size_t k = 0; k < nc; ++k) a->eph[(size_t)T.consts[k].prog * a->stride + (size_t)T.consts[k].pos] = cur[k];
-    // The device tape now holds cbest, which matches the updated kept ephemerals.
-    if (fitness_out) {
-        if ((s = ppi_engine_evaluate(a->eng, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->alpha,
-                                     fitness_out, nullptr, nullptr)))
-            return s;
+    // The device tape now holds cbest.  Acceptance above used the adjoint pass's own
+    // fitness, whose row sums run in another order than the plain evaluator's: a
+    // program "improved" by less than that rounding keeps its original constants.
+    if ((s = ppi_engine_evaluate(a->eng, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->alpha,
+                                 f_end.data(), nullptr, nullptr)))
+        return s;
+    int32_t reverted = 0;
+    for (int32_t p = 0; p < a->n; ++p) {
+        if (!(f_end[(size_t)p] > f_start[(size_t)p])) continue;
+        const size_t base = (size_t)p * a->stride;
+        std::memcpy(&a->eph[base], &eph_start[base], (size_t)a->stride * sizeof(float));
+        f_end[(size_t)p] = f_start[(size_t)p];
+        ++reverted;
     }
+    if (reverted) a->tape_ps = -1;  // the device tape holds cbest for the reverted programs: recompile on next use
+    a->stats.descend_reverted = reverted;
+    if (fitness_out) std::memcpy(fitness_out, f_end.data(), f_end.size() * sizeof(float));
     if (ephemeral_out) std::memcpy(ephemeral_out, a->eph.data(), a->eph.size() * sizeof(float));
     return PPI_OK;
 }

@@ -483,7 +483,7 @@ ppi_status ppi_batch_build(ppi_batch* b) {
             float ms = 0.0f;
             B_HIP(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
             const double per_unit = ms / std::max(1.0, sample_cost);
-            b->stats.tune_ms[r == 2 ? 0 : (r == 4 ? 1 : 2)] = ms;
+            b->stats.tune_ms[r == 2 ? 0 : (r == 4 ? 1 : 2)] = per_unit * 1e9;
             if (per_unit < best) { best = per_unit; best_r = r; }
         }
         b->R = best_r;

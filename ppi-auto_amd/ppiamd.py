@@ -82,7 +82,8 @@ class AdjointStats(C.Structure):
         ("descend_ms", C.c_double), ("node_evals", C.c_double), ("forward_ins", C.c_double),
         ("reverse_ins", C.c_double), ("n_jobs", C.c_int32), ("n_consts", C.c_int32),
         ("max_segments", C.c_int32), ("partials", C.c_int32), ("rows_per_lane", C.c_int32),
-        ("ranges", C.c_int32), ("descend_steps", C.c_int32), ("pad_", C.c_int32), ("tune_ms", C.c_double * 2),
+        ("ranges", C.c_int32), ("descend_steps", C.c_int32), ("descend_reverted", C.c_int32),
+        ("tune_ms", C.c_double * 2),
     ]
 
 
