@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     __shared__ float lds[kWaves][LDS_WAVE];
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = wave_index();
     const int64_t u = (int64_t)blockIdx.x * kWaves + wave;
     if (u >= (int64_t)A.njobs * A.nranges) return;  // whole wave: no block-level synchronisation below
     const int rng = (int)(u / A.njobs);
