@@ -146,17 +146,22 @@ def cpu_baseline(w, budget_s, threads):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fp_tile_regression_traffic.md")
+VALU_PROFILE = os.path.join(ROOT, "profiles", "fp_tile_regression_valu_counters.md")
+
+
+def profile_block(path):
+    """The JSON block of a committed rocprofv3 summary, or None."""
+    try:
+        text = open(path).read()
+        return json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+    except (OSError, IndexError, ValueError):
+        return None
 
 
 def measured_traffic():
-    """HBM bytes per launch of the headline kernel, from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), or None."""
-    try:
-        text = open(TRAFFIC_PROFILE).read()
-        block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block)
-    except (OSError, IndexError, ValueError):
-        return None
+    """HBM bytes per launch of the headline kernel (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the
+    copy-kernel calibration), from the committed counter summary, or None."""
+    return profile_block(TRAFFIC_PROFILE)
 
 
 def calibrate_copy():
@@ -220,6 +225,7 @@ def main():
         value = fp["node_evals"] / (fp["ms_per_step"] * 1e-3) / 1e9
         achieved = fp["node_evals"] / (fp["main_kernel_ms"] * 1e-3)
         traffic = measured_traffic()
+        valu = profile_block(VALU_PROFILE)
         line.update({
             "value": round(value, 3),
             "ms_per_step": round(fp["ms_per_step"], 4),
@@ -233,7 +239,12 @@ def main():
                          "frac": round(achieved / VALU_PEAK, 5),
                          "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
                          "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                         "algorithmic_bytes": traffic["algorithmic_bytes_per_launch"] if traffic else None},
+                         "algorithmic_bytes": traffic["algorithmic_bytes_per_launch"] if traffic else None,
+                         "valu_lane_ops_per_node": valu["valu_lane_ops_per_node"] if valu else None,
+                         "valu_issue_frac": valu["valu_issue_frac"] if valu else None,
+                         "note": "frac counts 1 lane-op per node-evaluation; the exact double-rounded "
+                                 "transcendentals take valu_lane_ops_per_node on this workload, and rocprofv3 "
+                                 "SQ counters put VALU issue at valu_issue_frac of the SIMDs' capacity"},
         })
         if extras:
             log("timing PDP form")
