@@ -132,6 +132,7 @@ typedef struct ppi_stats {
     int32_t rows_per_lane;
     int32_t programs_per_tile;
     int32_t max_slots;      /* register stack slots used by the deepest program */
+    int32_t derived_columns;/* unary-of-input columns precomputed for the FP form (this dataset) */
 } ppi_stats;
 
 typedef struct ppi_engine ppi_engine;

@@ -131,6 +131,8 @@ static void reference_geometry(ppi_engine* E) {
     if (E->nprog > 0) E->best_local = std::max(1, std::min(mls2, (int)std::ceil(E->nprog / (float)cu)));
 }
 
+static void reset_derived(ppi_engine* E);
+
 static bool plan(ppi_engine* E, std::string* why) {
     const CompileStats& cs = E->tape.stats;
     int need_f = cs.max_col + 1;
@@ -234,7 +236,7 @@ void ppi_engine_destroy(ppi_engine* E) {
     (void)hipSetDevice(E->device);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void* dev[] = {E->dX, E->dMeta, E->dCode, E->dPartial, E->dBadLo, E->dBadHi, E->dRaw, E->dVec,
-                   E->dOrder, E->dCost, E->dPred};
+                   E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     if (E->hRaw) (void)hipHostFree(E->hRaw);
@@ -282,7 +284,63 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
     E->symtab.build(table, ntable);
     E->configured = true;
     E->nprog = 0;  // programs compile against the table: re-upload after reconfiguring
+    reset_derived(E);  // derived values depend on PROTECTED / NATIVE
     reference_geometry(E);
+    return PPI_OK;
+}
+
+// Derived columns (unary operators over input columns, tape.h): at most 256, and
+// at most 2 GiB of dataset memory with them.
+static void reset_derived(ppi_engine* E) {
+    int64_t cap = 0;
+    if (E->ldx > 0) {
+        const int64_t by_mem = ((int64_t)1 << 31) / (E->ldx * (int64_t)sizeof(float)) - E->ncol;
+        cap = std::max<int64_t>(0, std::min<int64_t>(256, by_mem));
+    }
+    E->derived.reset(E->ncol, (int32_t)cap);
+    E->derived_ready = 0;
+}
+
+// Computes the derived columns the last compile added, growing the dataset buffer
+// (keeping the columns already there) when needed.
+static ppi_status stage_derived(ppi_engine* E) {
+    const int32_t total = (int32_t)E->derived.cols.size();
+    const int32_t fresh = total - E->derived_ready;
+    if (fresh <= 0) return PPI_OK;
+    const size_t need = (size_t)(E->ncol + total) * (size_t)E->ldx;
+    if (need > E->cap_X) {
+        const int32_t room = std::min<int32_t>(E->derived.cap, total + 32);
+        const size_t want = (size_t)(E->ncol + room) * (size_t)E->ldx;
+        float* nx = nullptr;
+        if (hipMalloc((void**)&nx, want * sizeof(float)) != hipSuccess) {
+            set_error("device allocation of derived columns failed");
+            return PPI_E_NOMEM;
+        }
+        const size_t keep = (size_t)(E->ncol + E->derived_ready) * (size_t)E->ldx;
+        PPI_HIP(hipMemcpyAsync(nx, E->dX, keep * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+        PPI_HIP(hipStreamSynchronize(E->stream));
+        (void)hipFree(E->dX);
+        E->dX = nx;
+        E->cap_X = want;
+    }
+    std::vector<uint32_t> ops((size_t)total);
+    std::vector<int32_t> srcs((size_t)total);
+    for (int32_t j = 0; j < total; ++j) {
+        ops[(size_t)j] = E->derived.cols[(size_t)j].op;
+        srcs[(size_t)j] = E->derived.cols[(size_t)j].src;
+    }
+    size_t cap = E->cap_der;
+    ppi_status s = ensure(&E->dDerOps, &cap, (size_t)total);
+    if (s) return s;
+    cap = E->cap_der;
+    if ((s = ensure(&E->dDerSrc, &cap, (size_t)total))) return s;
+    E->cap_der = cap;
+    PPI_HIP(hipMemcpyAsync(E->dDerOps, ops.data(), ops.size() * sizeof(uint32_t), hipMemcpyHostToDevice, E->stream));
+    PPI_HIP(hipMemcpyAsync(E->dDerSrc, srcs.data(), srcs.size() * sizeof(int32_t), hipMemcpyHostToDevice, E->stream));
+    PPI_HIP(launch_derive(E->dX, E->ldx, E->dDerOps + E->derived_ready, E->dDerSrc + E->derived_ready,
+                          E->ncol + E->derived_ready, fresh, E->cfg.protected_ops != 0, E->cfg.native_math != 0,
+                          E->stream));
+    E->derived_ready = total;
     return PPI_OK;
 }
 
@@ -304,6 +362,7 @@ static ppi_status stage_dataset(ppi_engine* E, int32_t nlin, int32_t ncol, const
     E->ncol = ncol;
     E->ldx = ldx;
     E->nprog = 0;  // attribute bounds depend on ncol: re-upload programs
+    reset_derived(E);
     reference_geometry(E);
     return PPI_OK;
 }
@@ -328,7 +387,11 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
     double t0 = now_ms();
     std::string err;
     int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (!compile_population_mt(E->symtab, phenotype, ephemeral, size, stride, n, E->ncol, false, threads, &E->tape, &err)) {
+    // The FP form reads unary-of-input nodes from precomputed derived columns.
+    const bool fp_form = E->cfg.form == PPI_FORM_AUTO || E->cfg.form == PPI_FORM_FP;
+    DerivedColumns* der = (fp_form && E->derived.cap > 0) ? &E->derived : nullptr;
+    if (!compile_population_mt(E->symtab, phenotype, ephemeral, size, stride, n, E->ncol, false, threads, &E->tape, &err,
+                               der)) {
         set_error(err);
         return PPI_E_PROGRAM;
     }
@@ -352,6 +415,7 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
         if ((s = ensure(&E->dPartial, &cap, (size_t)E->nrb * n))) return s;
         E->cap_partial = cap;
     }
+    if (der && (s = stage_derived(E))) return s;
     PPI_HIP(hipMemcpyAsync(E->dMeta, E->tape.meta.data(), E->tape.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice, E->stream));
     PPI_HIP(hipMemcpyAsync(E->dCode, E->tape.code.data(), E->tape.code.size() * sizeof(Ins), hipMemcpyHostToDevice, E->stream));
     if (E->form == PPI_FORM_PP) {
@@ -534,6 +598,7 @@ ppi_status ppi_engine_get_stats(const ppi_engine* E, ppi_stats* out) {
     out->form = E->form;
     out->rows_per_lane = E->R;
     out->programs_per_tile = E->chunk;
+    out->derived_columns = E->derived_ready;
     return PPI_OK;
 }
 

@@ -30,11 +30,16 @@ struct ppi_engine {
     std::vector<float> cost;
     ppi::SymbolTable symtab;
 
-    // dataset, column-major with rows padded to ldx
+    // dataset, column-major with rows padded to ldx; derived columns follow the ncol input columns
     int32_t nlin = 0, ncol = 0;
     int64_t ldx = 0;
     float* dX = nullptr;
     size_t cap_X = 0;
+    ppi::DerivedColumns derived;
+    int32_t derived_ready = 0;  // derived columns already computed on the device
+    uint32_t* dDerOps = nullptr;
+    int32_t* dDerSrc = nullptr;
+    size_t cap_der = 0;
 
     // resident programs
     ppi::Tape tape;

@@ -54,6 +54,17 @@ hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_derive(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first,
+                         int32_t count, bool P, bool N, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    dim3 grid((unsigned)((ldx + 255) / 256), (unsigned)count);
+    if (P && N) hipLaunchKernelGGL((dev::derive_kernel<true, true>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
+    else if (P) hipLaunchKernelGGL((dev::derive_kernel<true, false>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
+    else if (N) hipLaunchKernelGGL((dev::derive_kernel<false, true>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
+    else hipLaunchKernelGGL((dev::derive_kernel<false, false>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(dev::fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);

@@ -89,19 +89,30 @@ struct GlobalFeat {  // straight from the column-major dataset (more than 32 col
         for (int r = 0; r < R; ++r) b[r] = X[(int64_t)col * ldx + row[r]];
     }
 };
-template <int R>
-struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP form)
+template <int R, int F>
+struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP form).  The tile holds the
+                  // input columns (every input column a program reads is < F); columns >= ncol are
+                  // derived columns and come from the dataset through the caches.
     const float* base;  // &tile[0][lane][0]
+    const float* __restrict__ X;
+    int64_t ldx;
+    uint32_t ncol;
+    const int (&row)[R];
     __device__ __forceinline__ void get(uint32_t col, float (&b)[R]) const {
-        const float* q = base + col * (64 * R);
+        if (col < ncol) {
+            const float* q = base + col * (64 * R);
 #pragma unroll
-        for (int r = 0; r < R; ++r) b[r] = q[r];
+            for (int r = 0; r < R; ++r) b[r] = q[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) b[r] = X[(int64_t)col * ldx + row[r]];
+        }
     }
 };
 
 template <int R, int F>
 __device__ __forceinline__ auto make_feat(const EvalArgs& A, const float* lds_lane, const int (&row)[R]) {
-    if constexpr (F > 0) return LdsFeat<R>{lds_lane};
+    if constexpr (F > 0) return LdsFeat<R, F>{lds_lane, A.X, A.ldx, (uint32_t)A.ncol, row};
     else return GlobalFeat<R>{A.X, A.ldx, row};
 }
 
@@ -394,7 +405,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
     const int64_t gbeg = first.code_off;
     const int64_t gend = (int64_t)last.code_off + last.ncode + 1;  // through the last H_END
 
-    typedef typename std::conditional<(F > 0), LdsFeat<R>, GlobalFeat<R>>::type FeatT;
+    typedef typename std::conditional<(F > 0), LdsFeat<R, F>, GlobalFeat<R>>::type FeatT;
     FeatT ft = make_feat<R, F>(A, &tile[0][lane][0], row);
 
     // Instruction window: lane L holds tape entry blk + L of the current 64-entry
@@ -650,6 +661,16 @@ __global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out)
 }
 
 #ifdef PPI_KERNELS_MISC
+// Derived columns: X[(first + j) * ldx + row] = unary(op_j, X[src_j * ldx + row]) for
+// every row of the padded column, with the interpreters' own operator code.
+template <bool P, bool N>
+__global__ void derive_kernel(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= ldx) return;
+    const int j = blockIdx.y;
+    X[(int64_t)(first + j) * ldx + row] = unary<P, N>(ops[j], X[(int64_t)srcs[j] * ldx + row]);
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

@@ -101,6 +101,8 @@ struct Compiler {
     const SymbolTable& st;
     int32_t ncol;
     bool want_const_pos;
+    DerivedColumns* derived = nullptr;
+    int64_t derived_uses = 0;
     std::vector<Node> nodes;
     std::vector<int32_t> vstack;
     struct Frame { int32_t node, base, phase, o[3]; };
@@ -141,6 +143,20 @@ struct Compiler {
                 }
             } else {
                 if ((int)vstack.size() < ar) return -1;
+                if (ar == 1 && derived) {
+                    // Unary operator over an input column: becomes a leaf reading the
+                    // derived column that holds op(x[col]).
+                    Node& leaf = nodes[(size_t)vstack.back()];
+                    if (leaf.op < 0 && leaf.leaf_attr && (int32_t)leaf.arg < ncol) {
+                        const int32_t dcol = derived->lookup_or_add(unop_of(op), (int32_t)leaf.arg);
+                        if (dcol >= 0) {
+                            leaf.arg = (uint32_t)dcol;
+                            leaf.pos = i;
+                            ++derived_uses;
+                            continue;
+                        }
+                    }
+                }
                 n.op = op;
                 for (int k = 0; k < ar; ++k) { n.ch[k] = vstack.back(); vstack.pop_back(); }
                 // Sethi-Ullman register need.
@@ -330,8 +346,9 @@ void put_end(Tape* out, bool want_const_pos, uint32_t flags) {
 
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                        Tape* out, std::string* err) {
+                        Tape* out, std::string* err, DerivedColumns* derived) {
     Compiler c(st, ncol, want_const_pos);
+    c.derived = derived;
     out->meta.reserve(out->meta.size() + (size_t)n);
     for (int32_t i = 0; i < n; ++i) {
         ProgMeta m{};
@@ -378,21 +395,22 @@ bool compile_population(const SymbolTable& st, const int32_t* phenotype, const f
     }
     for (size_t k = 0; k < out->code.size(); ++k) {
         uint32_t h = out->code[k].code & 0xffu;
-        if (h == H_LOAD_A || h == H_PUSH_A || (h >= H_BIN_A && h < H_BIN_S))
+        if ((h == H_LOAD_A || h == H_PUSH_A || (h >= H_BIN_A && h < H_BIN_S)) && (int32_t)out->code[k].arg < ncol)
             out->stats.max_col = std::max<int32_t>(out->stats.max_col, (int32_t)out->code[k].arg);
     }
+    out->stats.derived_uses += c.derived_uses;
     return true;
 }
 
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                           int threads, Tape* out, std::string* err) {
+                           int threads, Tape* out, std::string* err, DerivedColumns* derived) {
     out->meta.clear();
     out->code.clear();
     out->const_pos.clear();
     out->stats = CompileStats{};
     if (threads <= 1 || n < 4096)
-        return compile_population(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err);
+        return compile_population(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived);
     std::vector<Tape> parts((size_t)threads);
     std::vector<std::string> errs((size_t)threads);
     std::vector<char> ok((size_t)threads, 1);
@@ -403,7 +421,7 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
         if (lo >= hi) continue;
         pool.emplace_back([&, t, lo, hi] {
             ok[t] = compile_population(st, phenotype + (size_t)lo * stride, ephemeral + (size_t)lo * stride,
-                                       size + lo, stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t]);
+                                       size + lo, stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t], derived);
         });
     }
     for (auto& th : pool) th.join();
@@ -428,6 +446,7 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
         out->stats.n_invalid += p.stats.n_invalid;
         out->stats.n_empty += p.stats.n_empty;
         out->stats.max_col = std::max(out->stats.max_col, p.stats.max_col);
+        out->stats.derived_uses += p.stats.derived_uses;
     }
     return true;
 }

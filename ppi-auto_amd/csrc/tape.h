@@ -19,7 +19,9 @@
 #define PPI_TAPE_H
 
 #include <cstdint>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ppi_engine.h"
@@ -93,7 +95,40 @@ struct CompileStats {
     int32_t max_code = 0;
     int32_t n_invalid = 0;
     int32_t n_empty = 0;
-    int32_t max_col = -1;       // largest attribute column referenced
+    int32_t max_col = -1;       // largest input column referenced (derived columns excluded)
+    int64_t derived_uses = 0;   // unary-of-column nodes read from derived columns
+};
+
+// Unary operators applied directly to input columns, precomputed once per
+// dataset: the compiler turns a unary node over an attribute leaf into a leaf
+// that reads derived column base + j, which holds op(x[src]) for every row
+// (computed on the device by the same operator code, so the values are
+// bit-identical to evaluating the node).  Shared by the compiler's threads.
+struct DerivedColumns {
+    int32_t base = 0;  // first derived column (the dataset's column count)
+    int32_t cap = 0;   // most derived columns (0: off)
+    struct Col { uint32_t op; int32_t src; };
+    std::vector<Col> cols;                          // column base + j = op(column src), src < base
+    std::unordered_map<uint64_t, int32_t> index;    // (op, src) -> column
+    std::mutex mu;
+    void reset(int32_t b, int32_t c) {
+        base = b;
+        cap = c;
+        cols.clear();
+        index.clear();
+    }
+    // Column holding op(x[src]), or -1 when the table is full.
+    int32_t lookup_or_add(uint32_t op, int32_t src) {
+        const uint64_t key = ((uint64_t)op << 32) | (uint32_t)src;
+        std::lock_guard<std::mutex> g(mu);
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        if ((int32_t)cols.size() >= cap) return -1;
+        const int32_t col = base + (int32_t)cols.size();
+        cols.push_back(Col{op, src});
+        index.emplace(key, col);
+        return col;
+    }
 };
 
 struct Tape {
@@ -104,16 +139,17 @@ struct Tape {
 };
 
 // Compiles programs [0, n) into `out` (appending).  ncol bounds attribute columns
-// (an out-of-range column evaluates as NaN).  Returns false with `err` set if a
-// program needs more than kMaxSlots register slots.
+// (an out-of-range column evaluates as NaN).  With `derived`, unary operators
+// over attribute leaves read derived columns (see DerivedColumns).  Returns false
+// with `err` set if a program needs more than kMaxSlots register slots.
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                        Tape* out, std::string* err);
+                        Tape* out, std::string* err, DerivedColumns* derived = nullptr);
 
 // Multi-threaded front end: splits the population over `threads` workers.
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                           int threads, Tape* out, std::string* err);
+                           int threads, Tape* out, std::string* err, DerivedColumns* derived = nullptr);
 
 // ---------------------------------------------------------------------------
 // Reverse-mode adjoint tape: d(fitness)/d(ephemeral T_CONST values).

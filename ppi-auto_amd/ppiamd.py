@@ -61,7 +61,7 @@ class Stats(C.Structure):
         ("upload_ms", C.c_double), ("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
         ("download_ms", C.c_double), ("node_evals", C.c_double), ("executed_nodes", C.c_double),
         ("form", C.c_int32), ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32),
-        ("max_slots", C.c_int32),
+        ("max_slots", C.c_int32), ("derived_columns", C.c_int32),
     ]
 
 

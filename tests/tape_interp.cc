@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../ppi-auto_amd/csrc/tape.h"
 
@@ -84,14 +85,29 @@ static float bin(uint32_t op, float a, float b, int P) {
     }
 }
 
-extern "C" int tape_eval(const ppi_symbol* table, int ntable, const int32_t* ph, const float* eph, const int32_t* size,
-                         int stride, int n, const float* data, int nlin, int ncol, int P, float* out,
-                         int32_t* slots, int32_t* ncode) {
+// derive != 0: compile with derived columns (unary operators over input columns read
+// precomputed columns), computed here on the host from the same operator code;
+// *derived_out = number of derived columns (may be NULL).
+extern "C" int tape_eval_ex(const ppi_symbol* table, int ntable, const int32_t* ph, const float* eph, const int32_t* size,
+                            int stride, int n, const float* data, int nlin, int ncol, int P, float* out,
+                            int32_t* slots, int32_t* ncode, int derive, int32_t* derived_out) {
     SymbolTable st;
     st.build(table, ntable);
     Tape t;
     std::string err;
-    if (!compile_population(st, ph, eph, size, stride, n, ncol, false, &t, &err)) return -1;
+    DerivedColumns dc;
+    dc.reset(ncol, derive ? 256 : 0);
+    if (!compile_population(st, ph, eph, size, stride, n, ncol, false, &t, &err, derive ? &dc : nullptr)) return -1;
+    if (derived_out) *derived_out = (int32_t)dc.cols.size();
+    const int wide = ncol + (int)dc.cols.size();
+    std::vector<float> ext((size_t)nlin * wide);
+    for (int r = 0; r < nlin; ++r) {
+        for (int c = 0; c < ncol; ++c) ext[(size_t)r * wide + c] = data[(size_t)r * ncol + c];
+        for (size_t j = 0; j < dc.cols.size(); ++j)
+            ext[(size_t)r * wide + ncol + j] = un(dc.cols[j].op, data[(size_t)r * ncol + dc.cols[j].src], P);
+    }
+    data = ext.data();
+    ncol = wide;
     for (int i = 0; i < n; ++i) {
         const ProgMeta& m = t.meta[i];
         slots[i] = (int32_t)((m.flags >> 8) & 0xff);
@@ -123,4 +139,10 @@ extern "C" int tape_eval(const ppi_symbol* table, int ntable, const int32_t* ph,
         }
     }
     return 0;
+}
+
+extern "C" int tape_eval(const ppi_symbol* table, int ntable, const int32_t* ph, const float* eph, const int32_t* size,
+                         int stride, int n, const float* data, int nlin, int ncol, int P, float* out,
+                         int32_t* slots, int32_t* ncode) {
+    return tape_eval_ex(table, ntable, ph, eph, size, stride, n, data, nlin, ncol, P, out, slots, ncode, 0, nullptr);
 }

@@ -25,6 +25,8 @@ def tape_lib(tmp_path_factory):
     L.tape_eval.argtypes = [P(ppiamd.Symbol), C.c_int, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int, C.c_int,
                             P(C.c_float), C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_int32), P(C.c_int32)]
     L.tape_eval.restype = C.c_int
+    L.tape_eval_ex.argtypes = L.tape_eval.argtypes + [C.c_int, P(C.c_int32)]
+    L.tape_eval_ex.restype = C.c_int
     return L
 
 
@@ -105,3 +107,28 @@ def test_deep_comb_program(tape_lib):
     ov = oracle_values(b.table, pop, data, True)
     assert same_bits(tv, ov).all()
     assert slots[0] <= 1 and ncode[0] == n_ops + 1
+
+
+@pytest.mark.parametrize("prot", [True, False])
+def test_derived_columns_match_prefix_stack(tape_lib, prot):
+    """Unary operators over input columns read precomputed derived columns: same values, bit for bit."""
+    gr = full_grammar(5)
+    pop = ppiamd.grow(gr, 300, 23, 8, 160, p_leaf=0.35)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 4, 64, 5) * np.float32(4.0)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    out = np.zeros((pop.n, data.shape[0]), dtype=np.float32)
+    slots = np.zeros(pop.n, dtype=np.int32)
+    ncode = np.zeros(pop.n, dtype=np.int32)
+    nder = C.c_int32(0)
+    p = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    r = tape_lib.tape_eval_ex(oracle._table(gr.table), len(gr.table), p(pop.phenotype, C.c_int32),
+                              p(pop.ephemeral, C.c_float), p(pop.size, C.c_int32), pop.stride, pop.n,
+                              p(data, C.c_float), data.shape[0], data.shape[1], int(prot), p(out, C.c_float),
+                              p(slots, C.c_int32), p(ncode, C.c_int32), 1, C.byref(nder))
+    assert r == 0
+    assert nder.value > 20, "the population must exercise derived columns"
+    _, _, ncode_plain = tape_values(tape_lib, gr.table, pop, data, prot)
+    assert ncode.sum() < ncode_plain.sum()  # unary-of-column nodes cost no instruction
+    ov = oracle_values(gr.table, pop, data, prot)
+    ok = same_bits(out, ov)
+    assert ok.all(), f"{(~ok).sum()} values differ, programs {np.unique(np.nonzero(~ok)[0])[:10]}"
