@@ -202,6 +202,7 @@ typedef struct ppi_batch_stats {
     int32_t feature_slots;  /* F of the kernel shape (0 = attributes read from global memory) */
     int32_t autotuned;      /* 1 if R was picked by measurement */
     double tune_ms[4];      /* sampled ms per 10^9 node-evaluations, candidates R = 2, 4, 8 (0 = not tried) */
+    int32_t derived_columns; /* derived columns (unary operators over input columns) computed, all datasets */
 } ppi_batch_stats;
 
 ppi_status ppi_batch_create(ppi_batch** out, int32_t device);

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -42,10 +43,18 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A staged dataset: column-major input columns, then the derived columns
+// (unary operators over input columns, tape.h) its populations compiled to.
 struct Dataset {
     int32_t nlin = 0, ncol = 0;
     int64_t ldx = 0;
     float* dX = nullptr;
+    size_t cap_X = 0;  // floats allocated at dX
+    std::unique_ptr<DerivedColumns> derived;
+    int32_t derived_ready = 0;
+    uint32_t* dDerOps = nullptr;
+    int32_t* dDerSrc = nullptr;
+    size_t cap_der = 0;
 };
 
 struct Pop {
@@ -294,8 +303,11 @@ void ppi_batch_destroy(ppi_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     free_build(b);
     if (b->dCounter) (void)hipFree(b->dCounter);
-    for (auto& d : b->datasets)
+    for (auto& d : b->datasets) {
         if (d.dX) (void)hipFree(d.dX);
+        if (d.dDerOps) (void)hipFree(d.dDerOps);
+        if (d.dDerSrc) (void)hipFree(d.dDerSrc);
+    }
     if (b->dCost) (void)hipFree(b->dCost);
     if (b->hRaw) (void)hipHostFree(b->hRaw);
     for (auto& ev : b->ev)
@@ -325,6 +337,11 @@ ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_sy
     }
     b->symtab.build(table, ntable);
     b->configured = true;
+    // Derived columns depend on PROTECTED / NATIVE: recompute them on demand.
+    for (auto& d : b->datasets) {
+        d.derived->reset(d.ncol, derived_capacity(d.ldx, d.ncol));
+        d.derived_ready = 0;
+    }
     b->pops.clear();
     b->tape = Tape{};
     free_build(b);
@@ -343,8 +360,16 @@ ppi_status ppi_batch_add_dataset(ppi_batch* b, const float* data, int32_t nlin, 
         for (int32_t r = 0; r < nlin; ++r) cm[(size_t)c * d.ldx + r] = data[(size_t)r * ncol + c];
     ppi_status s = dalloc(&d.dX, cm.size());
     if (s) return s;
-    B_HIP(hipMemcpy(d.dX, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice));
-    b->datasets.push_back(d);
+    d.cap_X = cm.size();
+    d.derived.reset(new DerivedColumns());
+    d.derived->reset(ncol, derived_capacity(d.ldx, ncol));
+    hipError_t e = hipMemcpy(d.dX, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d.dX);
+        set_error(std::string("dataset upload: ") + hipGetErrorString(e));
+        return PPI_E_DEVICE;
+    }
+    b->datasets.push_back(std::move(d));
     *id = (int32_t)b->datasets.size() - 1;
     return PPI_OK;
 }
@@ -354,15 +379,27 @@ ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int3
     if (!b || !b->configured) { set_error("configure the batch first"); return PPI_E_STATE; }
     if (dataset_id < 0 || dataset_id >= (int32_t)b->datasets.size()) { set_error("unknown dataset"); return PPI_E_ARG; }
     if (!phenotype || !ephemeral || !size || stride <= 0 || n <= 0 || !pop_id) { set_error("bad program arrays"); return PPI_E_ARG; }
+    (void)hipSetDevice(b->device);
+    Dataset& d = b->datasets[dataset_id];
     Tape t;
     std::string err;
     int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (!compile_population_mt(b->symtab, phenotype, ephemeral, size, stride, n, b->datasets[dataset_id].ncol, false,
-                               threads, &t, &err)) {
+    if (!compile_population_mt(b->symtab, phenotype, ephemeral, size, stride, n, d.ncol, false, threads, &t, &err,
+                               d.derived.get())) {
         set_error(err);
         return PPI_E_PROGRAM;
     }
     if (t.stats.max_slots > 16) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
+    // New derived columns are computed now; the dataset buffer may move, which
+    // the next build picks up (a built graph is dropped below).
+    if (d.derived_ready < (int32_t)d.derived->cols.size()) {
+        (void)hipStreamSynchronize(b->stream);
+        free_build(b);
+        ppi_status s = stage_derived_columns(&d.dX, &d.cap_X, d.ldx, d.ncol, *d.derived, &d.derived_ready,
+                                             &d.dDerOps, &d.dDerSrc, &d.cap_der, b->cfg.protected_ops != 0,
+                                             b->cfg.native_math != 0, b->stream);
+        if (s) return s;
+    }
     Pop p;
     p.dataset = dataset_id;
     p.n = n;
@@ -516,6 +553,8 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     double w = 0.0;
     for (auto& p : b->pops) w += (double)p.total_size * b->datasets[p.dataset].nlin;
     b->stats.node_evals = w;
+    b->stats.derived_columns = 0;
+    for (auto& d : b->datasets) b->stats.derived_columns += d.derived_ready;
     b->stats.build_ms = now_ms() - t0;
     return PPI_OK;
 }

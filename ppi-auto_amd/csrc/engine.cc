@@ -36,6 +36,67 @@ void set_error(const std::string& s) { g_last_error = s; }
         }                                                                                      \
     } while (0)
 
+int32_t derived_capacity(int64_t ldx, int32_t ncol) {
+    if (ldx <= 0) return 0;
+    const int64_t by_mem = ((int64_t)1 << 31) / (ldx * (int64_t)sizeof(float)) - ncol;
+    return (int32_t)std::max<int64_t>(0, std::min<int64_t>(256, by_mem));
+}
+
+ppi_status stage_derived_columns(float** X, size_t* cap_floats, int64_t ldx, int32_t ncol, DerivedColumns& table,
+                                 int32_t* ready, uint32_t** dOps, int32_t** dSrc, size_t* cap_list, bool P, bool N,
+                                 hipStream_t st) {
+    const int32_t total = (int32_t)table.cols.size();
+    const int32_t fresh = total - *ready;
+    if (fresh <= 0) return PPI_OK;
+    const size_t need = (size_t)(ncol + total) * (size_t)ldx;
+    if (need > *cap_floats) {
+        const int32_t room = std::min<int32_t>(table.cap, total + 32);
+        const size_t want = (size_t)(ncol + room) * (size_t)ldx;
+        float* nx = nullptr;
+        if (hipMalloc((void**)&nx, want * sizeof(float)) != hipSuccess) {
+            set_error("device allocation of derived columns failed");
+            return PPI_E_NOMEM;
+        }
+        const size_t keep = (size_t)(ncol + *ready) * (size_t)ldx;
+        hipError_t e = hipMemcpyAsync(nx, *X, keep * sizeof(float), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            (void)hipFree(nx);
+            set_error(std::string("derived columns: ") + hipGetErrorString(e));
+            return PPI_E_DEVICE;
+        }
+        (void)hipFree(*X);
+        *X = nx;
+        *cap_floats = want;
+    }
+    std::vector<uint32_t> ops((size_t)total);
+    std::vector<int32_t> srcs((size_t)total);
+    for (int32_t j = 0; j < total; ++j) {
+        ops[(size_t)j] = table.cols[(size_t)j].op;
+        srcs[(size_t)j] = table.cols[(size_t)j].src;
+    }
+    if ((size_t)total > *cap_list) {
+        if (*dOps) (void)hipFree(*dOps);
+        if (*dSrc) (void)hipFree(*dSrc);
+        *dOps = nullptr;
+        *dSrc = nullptr;
+        *cap_list = 0;
+        const size_t want = (size_t)table.cap;
+        if (hipMalloc((void**)dOps, want * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc((void**)dSrc, want * sizeof(int32_t)) != hipSuccess) {
+            set_error("device allocation of the derived column list failed");
+            return PPI_E_NOMEM;
+        }
+        *cap_list = want;
+    }
+    PPI_HIP(hipMemcpyAsync(*dOps, ops.data(), ops.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    PPI_HIP(hipMemcpyAsync(*dSrc, srcs.data(), srcs.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    PPI_HIP(launch_derive(*X, ldx, *dOps + *ready, *dSrc + *ready, ncol + *ready, fresh, P, N, st));
+    PPI_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
+    *ready = total;
+    return PPI_OK;
+}
+
 template <typename T>
 static ppi_status ensure(T** p, size_t* cap, size_t n) {
     if (n <= *cap && *p) return PPI_OK;
@@ -292,56 +353,15 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
 // Derived columns (unary operators over input columns, tape.h): at most 256, and
 // at most 2 GiB of dataset memory with them.
 static void reset_derived(ppi_engine* E) {
-    int64_t cap = 0;
-    if (E->ldx > 0) {
-        const int64_t by_mem = ((int64_t)1 << 31) / (E->ldx * (int64_t)sizeof(float)) - E->ncol;
-        cap = std::max<int64_t>(0, std::min<int64_t>(256, by_mem));
-    }
-    E->derived.reset(E->ncol, (int32_t)cap);
+    E->derived.reset(E->ncol, derived_capacity(E->ldx, E->ncol));
     E->derived_ready = 0;
 }
 
-// Computes the derived columns the last compile added, growing the dataset buffer
-// (keeping the columns already there) when needed.
+// Computes the derived columns the last compile added.
 static ppi_status stage_derived(ppi_engine* E) {
-    const int32_t total = (int32_t)E->derived.cols.size();
-    const int32_t fresh = total - E->derived_ready;
-    if (fresh <= 0) return PPI_OK;
-    const size_t need = (size_t)(E->ncol + total) * (size_t)E->ldx;
-    if (need > E->cap_X) {
-        const int32_t room = std::min<int32_t>(E->derived.cap, total + 32);
-        const size_t want = (size_t)(E->ncol + room) * (size_t)E->ldx;
-        float* nx = nullptr;
-        if (hipMalloc((void**)&nx, want * sizeof(float)) != hipSuccess) {
-            set_error("device allocation of derived columns failed");
-            return PPI_E_NOMEM;
-        }
-        const size_t keep = (size_t)(E->ncol + E->derived_ready) * (size_t)E->ldx;
-        PPI_HIP(hipMemcpyAsync(nx, E->dX, keep * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
-        PPI_HIP(hipStreamSynchronize(E->stream));
-        (void)hipFree(E->dX);
-        E->dX = nx;
-        E->cap_X = want;
-    }
-    std::vector<uint32_t> ops((size_t)total);
-    std::vector<int32_t> srcs((size_t)total);
-    for (int32_t j = 0; j < total; ++j) {
-        ops[(size_t)j] = E->derived.cols[(size_t)j].op;
-        srcs[(size_t)j] = E->derived.cols[(size_t)j].src;
-    }
-    size_t cap = E->cap_der;
-    ppi_status s = ensure(&E->dDerOps, &cap, (size_t)total);
-    if (s) return s;
-    cap = E->cap_der;
-    if ((s = ensure(&E->dDerSrc, &cap, (size_t)total))) return s;
-    E->cap_der = cap;
-    PPI_HIP(hipMemcpyAsync(E->dDerOps, ops.data(), ops.size() * sizeof(uint32_t), hipMemcpyHostToDevice, E->stream));
-    PPI_HIP(hipMemcpyAsync(E->dDerSrc, srcs.data(), srcs.size() * sizeof(int32_t), hipMemcpyHostToDevice, E->stream));
-    PPI_HIP(launch_derive(E->dX, E->ldx, E->dDerOps + E->derived_ready, E->dDerSrc + E->derived_ready,
-                          E->ncol + E->derived_ready, fresh, E->cfg.protected_ops != 0, E->cfg.native_math != 0,
-                          E->stream));
-    E->derived_ready = total;
-    return PPI_OK;
+    return stage_derived_columns(&E->dX, &E->cap_X, E->ldx, E->ncol, E->derived, &E->derived_ready, &E->dDerOps,
+                                 &E->dDerSrc, &E->cap_der, E->cfg.protected_ops != 0, E->cfg.native_math != 0,
+                                 E->stream);
 }
 
 static ppi_status stage_dataset(ppi_engine* E, int32_t nlin, int32_t ncol, const float* const* rows, const float* block) {

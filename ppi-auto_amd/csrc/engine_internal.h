@@ -12,6 +12,13 @@
 
 namespace ppi {
 void set_error(const std::string& s);
+// Derived columns (tape.h): the most a dataset of this shape may hold (256, and 2 GiB with its input columns).
+int32_t derived_capacity(int64_t ldx, int32_t ncol);
+// Grows X (keeping its ncol + *ready columns) to hold every column of `table` and computes the
+// columns added since *ready on stream `st`.
+ppi_status stage_derived_columns(float** X, size_t* cap_floats, int64_t ldx, int32_t ncol, DerivedColumns& table,
+                                 int32_t* ready, uint32_t** dOps, int32_t** dSrc, size_t* cap_list, bool P, bool N,
+                                 hipStream_t st);
 void pick_n_best(int n, int32_t* best, int num, const float* errors, const int32_t* indices);
 void best_acc(const float* vec, int P, int local2, int32_t* index, int32_t* best_size);
 }  // namespace ppi

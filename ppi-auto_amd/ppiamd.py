@@ -73,6 +73,7 @@ class BatchStats(C.Structure):
         ("graph_ms", C.c_double), ("kernel_ms", C.c_double), ("node_evals", C.c_double), ("build_ms", C.c_double),
         ("n_pairs", C.c_int32), ("n_items", C.c_int32), ("grid", C.c_int32), ("rows_per_lane", C.c_int32),
         ("feature_slots", C.c_int32), ("autotuned", C.c_int32), ("tune_ms", C.c_double * 4),
+        ("derived_columns", C.c_int32),
     ]
 
 
