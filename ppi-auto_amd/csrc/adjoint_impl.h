@@ -274,6 +274,8 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     typedef typename VecT<kAdjMaxConsts>::type VG;
     constexpr int LDS_WAVE = PS == 1 ? B * R * 64 : 1;
     __shared__ float lds[kWaves][LDS_WAVE];
+    stage_math<N>();
+    __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const int wave = wave_index();

@@ -4,9 +4,12 @@
 // sequential.cc includes functions.h and interpreter_core in a C++ unit whose
 // sin/cos/exp/log/pow/fmod/sinh/tan/exp10/exp2/log10/log2 calls resolve to the
 // double routines, so every transcendental is evaluated in double and rounded
-// to float when it is stored on the float stack.  Here the same happens with the
-// ROCm device library's double routines.  Basic arithmetic is float with no FMA
-// contraction (the file is compiled with -ffp-contract=off).
+// to float when it is stored on the float stack.  Here exp, log, sin and cos
+// are the correctly rounding float routines of exact_math.h (identical results
+// to rounding the ROCm device library's double routines, at a fraction of the
+// cost); the other transcendentals call the device library's double routines
+// and round.  Basic arithmetic is float with no FMA contraction (the file is
+// compiled with -ffp-contract=off).
 //
 // NATIVE mode mirrors the reference's -DNATIVE OpenCL build (functions.h:19-57,
 // native_* built-ins): fast float intrinsics, not bit-compatible with the
@@ -16,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "exact_math.h"
 #include "tape.h"
 
 namespace ppi {
@@ -30,7 +34,7 @@ __device__ __forceinline__ float b2f(bool b) { return b ? 1.0f : 0.0f; }
 
 template <bool N> __device__ __forceinline__ float f_exp(float x) {
     if constexpr (N) return __expf(x);
-    else return (float)exp((double)x);
+    else return exact_exp(x);
 }
 template <bool N> __device__ __forceinline__ float f_exp10(float x) {
     if constexpr (N) return __exp10f(x);
@@ -40,24 +44,17 @@ template <bool N> __device__ __forceinline__ float f_exp2(float x) {
     if constexpr (N) return exp2f(x);
     else return (float)exp2((double)x);
 }
-// The double-precision trigonometric routines carry a large-argument
-// reduction.  Inlined into the interpreters' unrolled per-row loops, their
-// registers add up across rows: the FP kernel at 4 rows per lane needs 175
-// VGPRs with them inline and 135 with them out of line, i.e. one more wave
-// per SIMD.  Each use then costs a call; measured on the 30,000 x 100,000
-// regression workload, out of line is 22% faster (40% with the occupancy
-// bound of the kernels).
-__device__ __noinline__ float sin_of_double(float x) { return (float)sin((double)x); }
-__device__ __noinline__ float cos_of_double(float x) { return (float)cos((double)x); }
-__device__ __noinline__ float tan_of_double(float x) { return (float)tan((double)x); }
-
+// The device library's double trigonometric routines carry a large-argument
+// reduction; inlined into the interpreters' unrolled per-row loops their
+// registers add up across rows, so they are only ever called out of line
+// (exact_math.h): as the rare fallback of exact_sin / exact_cos, and for tan.
 template <bool N> __device__ __forceinline__ float f_sin(float x) {
     if constexpr (N) return __sinf(x);
-    else return sin_of_double(x);
+    else return exact_sin(x);
 }
 template <bool N> __device__ __forceinline__ float f_cos(float x) {
     if constexpr (N) return __cosf(x);
-    else return cos_of_double(x);
+    else return exact_cos(x);
 }
 // functions.h:115-129 (SQRT): float(sqrt(double(x))) == sqrtf(x) exactly.
 template <bool P, bool N> __device__ __forceinline__ float f_sqrt(float x) {
@@ -68,7 +65,7 @@ template <bool P, bool N> __device__ __forceinline__ float f_sqrt(float x) {
 template <bool P, bool N> __device__ __forceinline__ float f_log(float x) {
     if (P && x < 1.0f) return 1.0f;
     if constexpr (N) return __logf(x);
-    else return (float)log((double)x);
+    else return exact_log(x);
 }
 template <bool P, bool N> __device__ __forceinline__ float f_log10(float x) {
     if (P && x < 1.0f) return 1.0f;
@@ -135,6 +132,33 @@ template <bool P, bool N> __device__ __forceinline__ float unary(uint32_t u, flo
     case U_LOG2: return f_log2<P, N>(a);
     default: return f_tan<P, N>(a);
     }
+}
+
+// A unary operator over the R rows of one lane.  exp, log, sin and cos take the
+// row-batched exact routines (their table loads issue together, one fallback
+// branch for all rows); the others go row by row.
+template <int R, bool P, bool N> __device__ __forceinline__ void unary_rows(uint32_t u, float (&v)[R]) {
+    if constexpr (!N) {
+        if (u == U_EXP) return exact_rows<kFnExp, R>(v);
+        if (u == U_SIN) return exact_rows<kFnSin, R>(v);
+        if (u == U_COS) return exact_rows<kFnCos, R>(v);
+        if (u == U_LOG) {
+            // functions.h:131-147: x < 1 gives 1 under PROTECTED; such rows evaluate log(1).
+            bool prot[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                prot[r] = P && v[r] < 1.0f;
+                if (prot[r]) v[r] = 1.0f;
+            }
+            exact_rows<kFnLog, R>(v);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (prot[r]) v[r] = 1.0f;
+            return;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = unary<P, N>(u, v[r]);
 }
 
 // Binary operators; `a` is the accumulator, `b` the folded or parked operand.

@@ -169,7 +169,7 @@ struct Core {
             switch (h - H_UN) {
 #define PPI_UCASE(K)                                                           \
     case K:                                                                    \
-        _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = unary<P, N>(K, acc[r]); \
+        unary_rows<R, P, N>(K, acc);                                           \
         break;
                 PPI_UCASE(U_NOT) PPI_UCASE(U_ABS) PPI_UCASE(U_POW2) PPI_UCASE(U_POW3)
                 PPI_UCASE(U_POW4) PPI_UCASE(U_POW5) PPI_UCASE(U_NEG) PPI_UCASE(U_ROUND)
@@ -228,6 +228,15 @@ struct Core {
 // would be treated as divergent (vector loads of the scalar tape, exec-mask
 // loops).  Measured: +7.7% on the FP form.
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// Copies the exact routines' tables into the workgroup's LDS (exact_math.h).
+// Called by every thread at kernel entry, before any thread leaves; a barrier
+// must follow before the first operator is evaluated.  NATIVE kernels use the
+// hardware approximations and need no tables.
+template <bool N>
+__device__ __forceinline__ void stage_math() {
+    if constexpr (!N) stage_exact_tables();
+}
 
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -446,6 +455,7 @@ template <int R, int F, int S, bool P, bool N>
 __global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(EvalArgs A) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
+    stage_math<N>();  // fp_tile_body's barrier follows
     const int c0 = blockIdx.y * A.chunk;
     fp_tile_body<R, F, S, P, N>(A, (int)blockIdx.x, c0, min(c0 + A.chunk, A.nprog), tile);
 }
@@ -462,6 +472,7 @@ __global__ void __launch_bounds__(kBlock, MinWaves<R>::value) batch_kernel(Batch
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     __shared__ int item_s;
+    stage_math<N>();  // the loop's first barrier follows
     for (;;) {
         __syncthreads();  // previous item's readers are done with the tile and item_s
         if (threadIdx.x == 0) item_s = (int)atomicAdd(B.counter, 1u);
@@ -528,6 +539,8 @@ __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
     typedef Core<R, S, P, N> C;
     __shared__ float red[kWaves];
     __shared__ uint32_t rlo[kWaves], rhi[kWaves];
+    stage_math<N>();
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = wave_index();
     const int p = blockIdx.x;
@@ -576,6 +589,8 @@ __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
 template <bool P, bool N>
 __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
     __shared__ float stk[kMaxSlots][256];
+    stage_math<N>();
+    __syncthreads();
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= A.nprog) return;
     const int p = A.order ? A.order[t] : t;
@@ -639,6 +654,8 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
 template <int F, int S, bool P, bool N>
 __global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out) {
     typedef Core<1, S, P, N> C;
+    stage_math<N>();
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t row0 = ((int64_t)blockIdx.x * kWaves + wave_index()) * 64;
     int row[1];
@@ -665,6 +682,8 @@ __global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out)
 // every row of the padded column, with the interpreters' own operator code.
 template <bool P, bool N>
 __global__ void derive_kernel(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first) {
+    stage_math<N>();
+    __syncthreads();
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= ldx) return;
     const int j = blockIdx.y;
