@@ -50,8 +50,11 @@ __device__ __noinline__ float tan_of_double(float x) { return (float)tan((double
 
 constexpr uint32_t kGuard = 64;  // double ulps kept between y and a float rounding boundary
 
-// The tables live in LDS (3.5 KB per workgroup): a table read is a ds_read with
-// LDS latency instead of a vector-memory load.  Every kernel that evaluates the
+// The tables live in LDS (4 KB per workgroup): a table read is a ds_read with
+// LDS latency instead of a vector-memory load.  The polynomial coefficients are
+// scalar constants; a gfx9 VALU instruction takes at most one scalar operand,
+// so each polynomial's first step is a multiply and an add rather than an fma
+// with two constant operands (which would need a v_mov pair).  Every kernel that evaluates the
 // exact routines stages them once at entry — stage_exact_tables() by all
 // threads of the workgroup, then a barrier — before any thread reads them.
 __shared__ ExactTables lds_exact_tables;
@@ -84,7 +87,7 @@ __device__ __forceinline__ float exact_exp_fast(float x, bool& ok) {
     double r = __builtin_fma(-k, kExpL1, xd);
     r = __builtin_fma(-k, kExpL2, r);
     const double t = lds_exact_tables.exp2j[ki & 63];
-    double q = __builtin_fma(r, kE5, kE4);
+    double q = r * kE5 + kE4;  // mul + add: one scalar operand per instruction
     q = __builtin_fma(r, q, kE3);
     q = __builtin_fma(r, q, kE2);
     const double p = __builtin_fma(r * r, q, r);
@@ -109,7 +112,7 @@ __device__ __forceinline__ float exact_log_fast(float x, bool& ok) {
     const uint32_t i = (tmp >> 16) & 127u;
     const float z = __uint_as_float(ix - (tmp & 0xff800000u));
     const double r = __builtin_fma((double)z, lds_exact_tables.log[i][0], -1.0);
-    double q = __builtin_fma(r, kL8, kL7);
+    double q = r * kL8 + kL7;
     q = __builtin_fma(r, q, kL6);
     q = __builtin_fma(r, q, kL5);
     q = __builtin_fma(r, q, kL4);
@@ -126,7 +129,8 @@ __device__ __forceinline__ float exact_log_fast(float x, bool& ok) {
 }
 
 // sin and cos.  a = |x| = k pi/64 + r, |r| <= pi/128 (+ rounding); with S_j = sin(j pi/64),
-// sin(a) = S_j cos r + S_{j+32} sin r (j = k mod 128) and cos(a) = sin(a + pi/2) takes j + 32.
+// sin(a) = S_j cos r + S_{j+32} sin r (j = k mod 128) and cos(a) = sin(a + pi/2) takes j + 32
+// (the table runs to j = 191, so both entries sit at fixed offsets from one index).
 // r: the first fma is exact (a has at most 24 significant bits, k < 2^28.4 on the domain), the
 // second rounds once, and the tail of pi/64 beyond kTrigP1 + kTrigP2 times k is below 2^-82;
 // a float below 2^24 never comes closer than 2^-26.8 to a multiple of pi (checked by enumerating
@@ -142,11 +146,11 @@ __device__ __forceinline__ float exact_sincos_fast(float x, bool& ok) {
     const int ki = (int)k;
     double r = __builtin_fma(-k, kTrigP1, a);
     r = __builtin_fma(-k, kTrigP2, r);
-    const int j = COS ? ki + 32 : ki;
-    const double S = lds_exact_tables.sinpi64[j & 127], C = lds_exact_tables.sinpi64[(j + 32) & 127];
+    const double* T = lds_exact_tables.sinpi64 + (ki & 127) + (COS ? 32 : 0);
+    const double S = T[0], C = T[32];
     const double r2 = r * r;
-    const double s = __builtin_fma(r2 * r, __builtin_fma(r2, __builtin_fma(r2, kS7, kS5), kS3), r);
-    const double cm1 = r2 * __builtin_fma(r2, __builtin_fma(r2, kC6, kC4), kC2);
+    const double s = __builtin_fma(r2 * r, __builtin_fma(r2, r2 * kS7 + kS5, kS3), r);
+    const double cm1 = r2 * __builtin_fma(r2, r2 * kC6 + kC4, kC2);
     const double y = __builtin_fma(S, cm1, __builtin_fma(C, s, S));
     float res = (float)y;
     if (!COS) res = __uint_as_float(__float_as_uint(res) ^ (__float_as_uint(x) & 0x80000000u));  // sin is odd

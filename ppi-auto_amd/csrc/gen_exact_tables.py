@@ -82,8 +82,9 @@ def main() -> None:
     w(f"constexpr double kTrigInvP = {hexd(64 / mp.pi)};  // 64/pi")
     w(f"constexpr double kTrigP1 = {hexd(p1)};  // pi/64 rounded")
     w(f"constexpr double kTrigP2 = {hexd(p2)};  // pi/64 - kTrigP1 rounded")
-    tables.append(("sin(j pi/64)", "double sinpi64[128];",
-                   [", ".join(hexd(sin_pi64(i)) for i in range(j, j + 4)) for j in range(0, 128, 4)]))
+    tables.append(("sin(j pi/64), j < 192: sin and cos read entries j and j + 32 (cos: j + 32 and j + 64) "
+                   "of one index j < 128", "double sinpi64[192];",
+                   [", ".join(hexd(sin_pi64(i)) for i in range(j, j + 4)) for j in range(0, 192, 4)]))
 
     # One struct, so that kernels can stage all tables into LDS with one copy.
     w("")
