@@ -254,7 +254,7 @@ struct AdjCore {
             const uint32_t ea = rc & 0xffffu, eb = rc >> 16;
             const uint32_t sa = (ea != 0 && ((ea - 1) >> LOGB) == seg) ? ((ea - 1) & (B - 1)) : kNone;
             const uint32_t sb = (eb != 0 && ((eb - 1) >> LOGB) == seg) ? ((eb - 1) & (B - 1)) : kNone;
-            if (sa == kNone && sb == kNone) C::step(c, a, ft, acc, stk);
+            if (sa == kNone && sb == kNone) C::step(c, a, ft, acc, stk, typename C::NoEnd{});
             else step_rec(c, a, sa, sb, ft, acc, stk, ps);
         }
     }

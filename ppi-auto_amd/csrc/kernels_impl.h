@@ -93,16 +93,20 @@ template <int R, int F>
 struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP form).  The tile holds the
                   // input columns (every input column a program reads is < F); columns >= ncol are
                   // derived columns and come from the dataset through the caches.
-    const float* base;  // &tile[0][lane][0]
+    typedef typename VecT<R>::type VR;
+    typedef __attribute__((address_space(3))) const VR LdsVR;
+    LdsVR* base;  // &tile[0][lane][0]: a lane's R rows of column c are one vector at base[c * 64]
     const float* __restrict__ X;
     int64_t ldx;
     uint32_t ncol;
     const int (&row)[R];
     __device__ __forceinline__ void get(uint32_t col, float (&b)[R]) const {
         if (col < ncol) {
-            const float* q = base + col * (64 * R);
+            // One typed LDS vector load (ds_read_b128 at R = 4); kept apart from the global
+            // branch so the two cannot be merged into a flat load.
+            const VR v = base[col * 64];
 #pragma unroll
-            for (int r = 0; r < R; ++r) b[r] = q[r];
+            for (int r = 0; r < R; ++r) b[r] = v[r];
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) b[r] = X[(int64_t)col * ldx + row[r]];
@@ -112,7 +116,8 @@ struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP for
 
 template <int R, int F>
 __device__ __forceinline__ auto make_feat(const EvalArgs& A, const float* lds_lane, const int (&row)[R]) {
-    if constexpr (F > 0) return LdsFeat<R, F>{lds_lane, A.X, A.ldx, (uint32_t)A.ncol, row};
+    if constexpr (F > 0)
+        return LdsFeat<R, F>{(typename LdsFeat<R, F>::LdsVR*)lds_lane, A.X, A.ldx, (uint32_t)A.ncol, row};
     else return GlobalFeat<R>{A.X, A.ldx, row};
 }
 
@@ -129,44 +134,52 @@ struct Core {
         for (int r = 0; r < R; ++r) stk[r][slot] = acc[r];
     }
 
-    // Executes instruction (c, a); H_END is handled by the caller.
-    template <class FeatT>
+    // Executes instruction (c, a); H_END calls on_end().  The handler's category
+    // (bits 5-7) and operator (bits 0-4) select the code through a short
+    // wave-uniform branch tree: binary operators first, the six arithmetic
+    // operators (+, -, reversed -, *, /, reversed /) ahead of the rest, then the
+    // unary operators, then loads, pushes, IF_THEN_ELSE and the program end.
+    template <class FeatT, class EndF>
     static __device__ __forceinline__ void step(uint32_t c, uint32_t a, const FeatT& ft, float (&acc)[R],
-                                                VS (&stk)[R]) {
+                                                VS (&stk)[R], EndF&& on_end) {
         const uint32_t h = c & 0xffu;
+        const uint32_t op = h & 31u;
         const float cv = __uint_as_float(a);
         if (h >= H_BIN_C) {
             float b[R];
-            uint32_t op;
-            if (h < H_BIN_A) {
-                op = h - H_BIN_C;
-#pragma unroll
-                for (int r = 0; r < R; ++r) b[r] = cv;
-            } else if (h < H_BIN_S) {
-                op = h - H_BIN_A;
-                ft.get(a, b);
-            } else {
-                op = h - H_BIN_S;
+            if (h >= H_BIN_S) {
                 const uint32_t slot = (c >> 8) & 31u;
 #pragma unroll
                 for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
+            } else if (h >= H_BIN_A) {
+                ft.get(a, b);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) b[r] = cv;
             }
-            switch (op) {
 #define PPI_BCASE(K)                                                           \
     case K:                                                                    \
         _Pragma("unroll") for (int r = 0; r < R; ++r) acc[r] = binary<P, N>(K, acc[r], b[r]); \
         break;
-                PPI_BCASE(B_ADD) PPI_BCASE(B_SUB) PPI_BCASE(B_RSUB) PPI_BCASE(B_MULT)
-                PPI_BCASE(B_DIV) PPI_BCASE(B_RDIV) PPI_BCASE(B_MEAN) PPI_BCASE(B_MAX)
-                PPI_BCASE(B_MIN) PPI_BCASE(B_MOD) PPI_BCASE(B_RMOD) PPI_BCASE(B_POW)
-                PPI_BCASE(B_RPOW) PPI_BCASE(B_AND) PPI_BCASE(B_OR) PPI_BCASE(B_XOR)
-                PPI_BCASE(B_GT) PPI_BCASE(B_GE) PPI_BCASE(B_LT) PPI_BCASE(B_LE)
-                PPI_BCASE(B_EQ) PPI_BCASE(B_NE) PPI_BCASE(B_RMAX) PPI_BCASE(B_RMIN)
-#undef PPI_BCASE
-            default: break;
+            if (__builtin_expect(op <= B_RDIV, 1)) {
+                switch (op) {
+                    PPI_BCASE(B_ADD) PPI_BCASE(B_SUB) PPI_BCASE(B_RSUB) PPI_BCASE(B_MULT)
+                    PPI_BCASE(B_DIV) PPI_BCASE(B_RDIV)
+                default: break;
+                }
+            } else {
+                switch (op) {
+                    PPI_BCASE(B_MEAN) PPI_BCASE(B_MAX)
+                    PPI_BCASE(B_MIN) PPI_BCASE(B_MOD) PPI_BCASE(B_RMOD) PPI_BCASE(B_POW)
+                    PPI_BCASE(B_RPOW) PPI_BCASE(B_AND) PPI_BCASE(B_OR) PPI_BCASE(B_XOR)
+                    PPI_BCASE(B_GT) PPI_BCASE(B_GE) PPI_BCASE(B_LT) PPI_BCASE(B_LE)
+                    PPI_BCASE(B_EQ) PPI_BCASE(B_NE) PPI_BCASE(B_RMAX) PPI_BCASE(B_RMIN)
+                default: break;
+                }
             }
+#undef PPI_BCASE
         } else if (h >= H_UN) {
-            switch (h - H_UN) {
+            switch (op) {
 #define PPI_UCASE(K)                                                           \
     case K:                                                                    \
         unary_rows<R, P, N>(K, acc);                                           \
@@ -181,32 +194,48 @@ struct Core {
 #undef PPI_UCASE
             default: break;
             }
-        } else if (h == H_LOAD_C) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] = cv;
-        } else if (h == H_LOAD_A) {
-            ft.get(a, acc);
-        } else if (h == H_PUSH_C) {
-            push(stk, acc, (c >> 8) & 31u);
-#pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] = cv;
-        } else if (h == H_PUSH_A) {
-            push(stk, acc, (c >> 8) & 31u);
-            ft.get(a, acc);
         } else {
-            // H_TERN: IF_THEN_ELSE (interpreter_core:1-4): (bool)cond ? then : else.
-            const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
-            const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
+            switch (h) {
+            case H_LOAD_C:
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
-                const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
-                const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
-                const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
-                acc[r] = (cd != 0.0f) ? th : el;
+                for (int r = 0; r < R; ++r) acc[r] = cv;
+                break;
+            case H_LOAD_A:
+                ft.get(a, acc);
+                break;
+            case H_PUSH_C:
+                push(stk, acc, (c >> 8) & 31u);
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = cv;
+                break;
+            case H_PUSH_A:
+                push(stk, acc, (c >> 8) & 31u);
+                ft.get(a, acc);
+                break;
+            case H_END:
+                on_end(a);  // H_END carries the program's flags in its argument
+                break;
+            default: {
+                // H_TERN: IF_THEN_ELSE (interpreter_core:1-4): (bool)cond ? then : else.
+                const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
+                const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
+                    const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
+                    const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
+                    const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
+                    acc[r] = (cd != 0.0f) ? th : el;
+                }
+                break;
+            }
             }
         }
     }
+
+    struct NoEnd {
+        __device__ __forceinline__ void operator()(uint32_t) const {}
+    };
 
     // One program from scalar-loaded instructions (one instruction of look-ahead).
     template <class FeatT>
@@ -217,7 +246,7 @@ struct Core {
             const uint32_t c = nc, a = na;
             nc = code[pc + 1].code;  // code[code_off + ncode] is the program's H_END
             na = code[pc + 1].arg;
-            step(c, a, ft, acc, stk);
+            step(c, a, ft, acc, stk, NoEnd{});  // the loop stops before the program's H_END
         }
     }
 };
@@ -443,7 +472,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
                 ++p;
                 continue;
             }
-            C::step(c, a, ft, acc, stk);
+            C::step(c, a, ft, acc, stk, typename C::NoEnd{});
         }
         k = 0;
         wc = nc;

@@ -52,7 +52,8 @@ enum UnOp : uint32_t {
     U_COUNT
 };
 
-// Handlers (bits 0-7 of Ins::code).
+// Handlers (bits 0-7 of Ins::code): bits 5-7 are the category, bits 0-4 the
+// operator, so a kernel decodes both with a shift and a mask.
 enum Handler : uint32_t {
     H_LOAD_C = 0,   // acc = c
     H_LOAD_A = 1,   // acc = x[col]
@@ -60,12 +61,13 @@ enum Handler : uint32_t {
     H_PUSH_A = 3,   // r[slot] = acc; acc = x[col]
     H_TERN = 4,     // acc = select(cond, then, else) over {acc, r[slot], r[slot2]}
     H_END = 5,      // end of program: code[code_off + ncode] of every program
-    H_UN = 8,       // + UnOp:        acc = f(acc)
-    H_BIN_C = 40,   // + BinOp:       acc = f(acc, c)
-    H_BIN_A = 72,   // + BinOp:       acc = f(acc, x[col])
-    H_BIN_S = 104,  // + BinOp:       acc = f(acc, r[slot])
-    H_LIMIT = 136
+    H_UN = 32,      // + UnOp:        acc = f(acc)
+    H_BIN_C = 64,   // + BinOp:       acc = f(acc, c)
+    H_BIN_A = 96,   // + BinOp:       acc = f(acc, x[col])
+    H_BIN_S = 128,  // + BinOp:       acc = f(acc, r[slot])
+    H_LIMIT = 160
 };
+static_assert(U_COUNT <= 32 && B_COUNT <= 32, "operators fit the 5-bit operator field");
 
 constexpr int kMaxSlots = 16;
 
