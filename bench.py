@@ -52,7 +52,10 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
                   protected=w.protected, cost=w.cost, form=form, rows_per_lane=rows_per_lane,
                   programs_per_tile=programs_per_tile)
     eng.set_dataset(w.data)
+    t_up = time.perf_counter()
     eng.upload(w.pop)
+    upload_s = time.perf_counter() - t_up
+    st0 = eng.stats()
     if warmup > 0:
         eng.time(warmup)
     t0 = time.perf_counter()
@@ -62,7 +65,7 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
     return {"ms_per_step": ms, "wall_s": wall, "main_kernel_ms": st.main_kernel_ms,
             "node_evals": st.node_evals, "executed_nodes": st.executed_nodes,
             "rows_per_lane": st.rows_per_lane, "programs_per_tile": st.programs_per_tile,
-            "form": st.form}
+            "form": st.form, "derived_columns": st0.derived_columns, "upload_ms": round(upload_s * 1e3, 2)}
 
 
 def bench_batched(steps, warmup):
@@ -233,7 +236,13 @@ def main():
                                    "features, EF=(X-Y)^2, sum, PROTECTED, sequential-path semantics",
                        "form": "FP", "rows_per_lane": fp["rows_per_lane"],
                        "programs_per_tile": fp["programs_per_tile"],
-                       "node_evals_per_step": fp["node_evals"]},
+                       "node_evals_per_step": fp["node_evals"],
+                       "executed_tape_ins_x_rows_per_step": fp["executed_nodes"],
+                       "derived_columns": fp["derived_columns"],
+                       "upload_ms": fp["upload_ms"],
+                       "upload_note": "one-time per population, outside the timed steps: tape compile, H2D "
+                                      "copy, and the derived columns (chains of unary operators over input "
+                                      "columns, cached per dataset) it adds"},
             "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4), "peak": VALU_PEAK / 1e12,
                          "unit": "Tnode/s (1 node-evaluation = 1 FP32 VALU lane-op)",
                          "frac": round(achieved / VALU_PEAK, 5),

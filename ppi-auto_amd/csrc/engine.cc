@@ -39,7 +39,7 @@ void set_error(const std::string& s) { g_last_error = s; }
 int32_t derived_capacity(int64_t ldx, int32_t ncol) {
     if (ldx <= 0) return 0;
     const int64_t by_mem = ((int64_t)1 << 31) / (ldx * (int64_t)sizeof(float)) - ncol;
-    return (int32_t)std::max<int64_t>(0, std::min<int64_t>(256, by_mem));
+    return (int32_t)std::max<int64_t>(0, std::min<int64_t>(4096, by_mem));
 }
 
 ppi_status stage_derived_columns(float** X, size_t* cap_floats, int64_t ldx, int32_t ncol, DerivedColumns& table,
@@ -350,8 +350,8 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
     return PPI_OK;
 }
 
-// Derived columns (unary operators over input columns, tape.h): at most 256, and
-// at most 2 GiB of dataset memory with them.
+// Derived columns (chains of unary operators over input columns, tape.h): at most
+// 4096, and at most 2 GiB of dataset memory with them.
 static void reset_derived(ppi_engine* E) {
     E->derived.reset(E->ncol, derived_capacity(E->ldx, E->ncol));
     E->derived_ready = 0;

@@ -57,11 +57,11 @@ hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float
 hipError_t launch_derive(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first,
                          int32_t count, bool P, bool N, hipStream_t st) {
     if (count <= 0) return hipSuccess;
-    dim3 grid((unsigned)((ldx + 255) / 256), (unsigned)count);
-    if (P && N) hipLaunchKernelGGL((dev::derive_kernel<true, true>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
-    else if (P) hipLaunchKernelGGL((dev::derive_kernel<true, false>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
-    else if (N) hipLaunchKernelGGL((dev::derive_kernel<false, true>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
-    else hipLaunchKernelGGL((dev::derive_kernel<false, false>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first);
+    dim3 grid((unsigned)((ldx + 255) / 256));
+    if (P && N) hipLaunchKernelGGL((dev::derive_kernel<true, true>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first, count);
+    else if (P) hipLaunchKernelGGL((dev::derive_kernel<true, false>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first, count);
+    else if (N) hipLaunchKernelGGL((dev::derive_kernel<false, true>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first, count);
+    else hipLaunchKernelGGL((dev::derive_kernel<false, false>), grid, dim3(256), 0, st, X, ldx, ops, srcs, first, count);
     return hipGetLastError();
 }
 

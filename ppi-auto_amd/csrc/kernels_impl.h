@@ -708,15 +708,18 @@ __global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out)
 
 #ifdef PPI_KERNELS_MISC
 // Derived columns: X[(first + j) * ldx + row] = unary(op_j, X[src_j * ldx + row]) for
-// every row of the padded column, with the interpreters' own operator code.
+// j = 0 .. count-1 and every row of the padded column, with the interpreters' own
+// operator code.  A thread owns one row and computes the columns in order, so a
+// column whose source is an earlier derived column reads its finished value.
 template <bool P, bool N>
-__global__ void derive_kernel(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first) {
+__global__ void derive_kernel(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first,
+                              int32_t count) {
     stage_math<N>();
     __syncthreads();
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= ldx) return;
-    const int j = blockIdx.y;
-    X[(int64_t)(first + j) * ldx + row] = unary<P, N>(ops[j], X[(int64_t)srcs[j] * ldx + row]);
+    for (int32_t j = 0; j < count; ++j)
+        X[(int64_t)(first + j) * ldx + row] = unary<P, N>(ops[j], X[(int64_t)srcs[j] * ldx + row]);
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {

@@ -144,10 +144,11 @@ struct Compiler {
             } else {
                 if ((int)vstack.size() < ar) return -1;
                 if (ar == 1 && derived) {
-                    // Unary operator over an input column: becomes a leaf reading the
-                    // derived column that holds op(x[col]).
+                    // Unary operator over a column (an input column or a derived one):
+                    // becomes a leaf reading the derived column that holds op(x[col]),
+                    // so chains of unary operators over an input column fold too.
                     Node& leaf = nodes[(size_t)vstack.back()];
-                    if (leaf.op < 0 && leaf.leaf_attr && (int32_t)leaf.arg < ncol) {
+                    if (leaf.op < 0 && leaf.leaf_attr) {
                         const int32_t dcol = derived->lookup_or_add(unop_of(op), (int32_t)leaf.arg);
                         if (dcol >= 0) {
                             leaf.arg = (uint32_t)dcol;

@@ -101,16 +101,19 @@ struct CompileStats {
     int64_t derived_uses = 0;   // unary-of-column nodes read from derived columns
 };
 
-// Unary operators applied directly to input columns, precomputed once per
-// dataset: the compiler turns a unary node over an attribute leaf into a leaf
-// that reads derived column base + j, which holds op(x[src]) for every row
-// (computed on the device by the same operator code, so the values are
-// bit-identical to evaluating the node).  Shared by the compiler's threads.
+// Chains of unary operators over input columns, precomputed once per dataset:
+// the compiler turns a unary node over an attribute leaf into a leaf that reads
+// derived column base + j, which holds op(x[src]) for every row, and a unary
+// node over such a leaf into another derived column (sin(cos(x3)) reads one
+// column).  The values are computed on the device by the same operator code, in
+// column order (a column's source is an input column or an earlier derived
+// column), so they are bit-identical to evaluating the nodes.  Shared by the
+// compiler's threads.
 struct DerivedColumns {
     int32_t base = 0;  // first derived column (the dataset's column count)
     int32_t cap = 0;   // most derived columns (0: off)
     struct Col { uint32_t op; int32_t src; };
-    std::vector<Col> cols;                          // column base + j = op(column src), src < base
+    std::vector<Col> cols;                          // column base + j = op(column src), src < base + j
     std::unordered_map<uint64_t, int32_t> index;    // (op, src) -> column
     std::mutex mu;
     void reset(int32_t b, int32_t c) {

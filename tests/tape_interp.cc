@@ -85,9 +85,10 @@ static float bin(uint32_t op, float a, float b, int P) {
     }
 }
 
-// derive != 0: compile with derived columns (unary operators over input columns read
-// precomputed columns), computed here on the host from the same operator code;
-// *derived_out = number of derived columns (may be NULL).
+// derive != 0: compile with derived columns (chains of unary operators over input
+// columns read precomputed columns), computed here on the host from the same
+// operator code, in column order; derived_out (may be NULL) receives the number of
+// derived columns and the number of them whose source is another derived column.
 extern "C" int tape_eval_ex(const ppi_symbol* table, int ntable, const int32_t* ph, const float* eph, const int32_t* size,
                             int stride, int n, const float* data, int nlin, int ncol, int P, float* out,
                             int32_t* slots, int32_t* ncode, int derive, int32_t* derived_out) {
@@ -96,15 +97,19 @@ extern "C" int tape_eval_ex(const ppi_symbol* table, int ntable, const int32_t* 
     Tape t;
     std::string err;
     DerivedColumns dc;
-    dc.reset(ncol, derive ? 256 : 0);
+    dc.reset(ncol, derive ? 4096 : 0);
     if (!compile_population(st, ph, eph, size, stride, n, ncol, false, &t, &err, derive ? &dc : nullptr)) return -1;
-    if (derived_out) *derived_out = (int32_t)dc.cols.size();
+    if (derived_out) {
+        derived_out[0] = (int32_t)dc.cols.size();
+        derived_out[1] = 0;
+        for (const auto& c : dc.cols) derived_out[1] += c.src >= ncol ? 1 : 0;
+    }
     const int wide = ncol + (int)dc.cols.size();
     std::vector<float> ext((size_t)nlin * wide);
     for (int r = 0; r < nlin; ++r) {
         for (int c = 0; c < ncol; ++c) ext[(size_t)r * wide + c] = data[(size_t)r * ncol + c];
-        for (size_t j = 0; j < dc.cols.size(); ++j)
-            ext[(size_t)r * wide + ncol + j] = un(dc.cols[j].op, data[(size_t)r * ncol + dc.cols[j].src], P);
+        for (size_t j = 0; j < dc.cols.size(); ++j)  // in order: a chain's source column comes first
+            ext[(size_t)r * wide + ncol + j] = un(dc.cols[j].op, ext[(size_t)r * wide + dc.cols[j].src], P);
     }
     data = ext.data();
     ncol = wide;

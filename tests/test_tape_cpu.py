@@ -111,7 +111,7 @@ def test_deep_comb_program(tape_lib):
 
 @pytest.mark.parametrize("prot", [True, False])
 def test_derived_columns_match_prefix_stack(tape_lib, prot):
-    """Unary operators over input columns read precomputed derived columns: same values, bit for bit."""
+    """Chains of unary operators over input columns read precomputed derived columns: same values, bit for bit."""
     gr = full_grammar(5)
     pop = ppiamd.grow(gr, 300, 23, 8, 160, p_leaf=0.35)
     data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 4, 64, 5) * np.float32(4.0)
@@ -119,14 +119,15 @@ def test_derived_columns_match_prefix_stack(tape_lib, prot):
     out = np.zeros((pop.n, data.shape[0]), dtype=np.float32)
     slots = np.zeros(pop.n, dtype=np.int32)
     ncode = np.zeros(pop.n, dtype=np.int32)
-    nder = C.c_int32(0)
+    nder = (C.c_int32 * 2)()
     p = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
     r = tape_lib.tape_eval_ex(oracle._table(gr.table), len(gr.table), p(pop.phenotype, C.c_int32),
                               p(pop.ephemeral, C.c_float), p(pop.size, C.c_int32), pop.stride, pop.n,
                               p(data, C.c_float), data.shape[0], data.shape[1], int(prot), p(out, C.c_float),
-                              p(slots, C.c_int32), p(ncode, C.c_int32), 1, C.byref(nder))
+                              p(slots, C.c_int32), p(ncode, C.c_int32), 1, nder)
     assert r == 0
-    assert nder.value > 20, "the population must exercise derived columns"
+    assert nder[0] > 20, "the population must exercise derived columns"
+    assert nder[1] > 5, "the population must exercise chained derived columns"
     _, _, ncode_plain = tape_values(tape_lib, gr.table, pop, data, prot)
     assert ncode.sum() < ncode_plain.sum()  # unary-of-column nodes cost no instruction
     ov = oracle_values(gr.table, pop, data, prot)
