@@ -267,15 +267,47 @@ __device__ __forceinline__ void stage_math() {
     if constexpr (!N) stage_exact_tables();
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// Cross-lane reductions with DPP (data-parallel primitives: a VALU operand read
+// from another lane of the same row, or broadcast from the last lane of a row):
+// within each row of 16 lanes, swaps of adjacent lanes, adjacent pairs and
+// mirrored halves and rows leave the row's total in every lane; two row
+// broadcasts then add row 0 into row 1, row 2 into row 3 and rows 0-1 into rows
+// 2-3, so lane 63 holds the wave's total.  Six VALU instructions and no LDS round
+// trip; the summation order is fixed, so the result is deterministic.  Lanes a
+// broadcast does not write keep `neutral`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float v, float neutral) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(neutral), __float_as_int(v), CTRL, ROW_MASK,
+                                                      0xf, false));
 }
+enum : int {
+    kDppSwap1 = 0xB1,      // quad_perm [1,0,3,2]
+    kDppSwap2 = 0x4E,      // quad_perm [2,3,0,1]
+    kDppHalfMirror = 0x141,
+    kDppMirror = 0x140,
+    kDppBcast15 = 0x142,   // lane 15 of each row to the next row
+    kDppBcast31 = 0x143,   // lane 31 to rows 2 and 3
+};
+// The wave's sum, in every lane.
+__device__ __forceinline__ float wave_sum(float v) {
+    v = v + dpp_f<kDppSwap1, 0xf>(v, 0.0f);
+    v = v + dpp_f<kDppSwap2, 0xf>(v, 0.0f);
+    v = v + dpp_f<kDppHalfMirror, 0xf>(v, 0.0f);
+    v = v + dpp_f<kDppMirror, 0xf>(v, 0.0f);
+    v = v + dpp_f<kDppBcast15, 0xa>(v, 0.0f);
+    v = v + dpp_f<kDppBcast31, 0xc>(v, 0.0f);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+// The wave's maximum (fmaxf), in every lane.
 __device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
+    const float ninf = -__builtin_inff();
+    v = fmaxf(v, dpp_f<kDppSwap1, 0xf>(v, ninf));
+    v = fmaxf(v, dpp_f<kDppSwap2, 0xf>(v, ninf));
+    v = fmaxf(v, dpp_f<kDppHalfMirror, 0xf>(v, ninf));
+    v = fmaxf(v, dpp_f<kDppMirror, 0xf>(v, ninf));
+    v = fmaxf(v, dpp_f<kDppBcast15, 0xa>(v, ninf));
+    v = fmaxf(v, dpp_f<kDppBcast31, 0xc>(v, ninf));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
 #pragma unroll
