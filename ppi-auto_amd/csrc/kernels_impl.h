@@ -139,21 +139,27 @@ struct Core {
     // wave-uniform branch tree: binary operators first, the six arithmetic
     // operators (+, -, reversed -, *, /, reversed /) ahead of the rest, then the
     // unary operators, then loads, pushes, IF_THEN_ELSE and the program end.
+    // The operator is taken as h - (category base) inside each branch: decoding
+    // it once up front as h & 31 made the adjoint kernel demote its feature
+    // vector to scratch memory (17% slower there).
     template <class FeatT, class EndF>
     static __device__ __forceinline__ void step(uint32_t c, uint32_t a, const FeatT& ft, float (&acc)[R],
                                                 VS (&stk)[R], EndF&& on_end) {
         const uint32_t h = c & 0xffu;
-        const uint32_t op = h & 31u;
         const float cv = __uint_as_float(a);
         if (h >= H_BIN_C) {
             float b[R];
+            uint32_t op;
             if (h >= H_BIN_S) {
+                op = h - H_BIN_S;
                 const uint32_t slot = (c >> 8) & 31u;
 #pragma unroll
                 for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
             } else if (h >= H_BIN_A) {
+                op = h - H_BIN_A;
                 ft.get(a, b);
             } else {
+                op = h - H_BIN_C;
 #pragma unroll
                 for (int r = 0; r < R; ++r) b[r] = cv;
             }
@@ -179,7 +185,7 @@ struct Core {
             }
 #undef PPI_BCASE
         } else if (h >= H_UN) {
-            switch (op) {
+            switch (h - H_UN) {
 #define PPI_UCASE(K)                                                           \
     case K:                                                                    \
         unary_rows<R, P, N>(K, acc);                                           \
