@@ -28,6 +28,7 @@ namespace ppi {
 namespace dev {
 
 constexpr int kWaves = 4;              // waves per workgroup in FP / PDP forms
+constexpr uint32_t kXcds = 8;          // MI355X: 8 accelerator complex dies, workgroups dealt round robin
 constexpr int kBlock = 64 * kWaves;
 
 // Waves per SIMD the register allocator must leave room for (the second
@@ -523,8 +524,18 @@ __global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(Eva
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     stage_math<N>();  // fp_tile_body's barrier follows
-    const int c0 = blockIdx.y * A.chunk;
-    fp_tile_body<R, F, S, P, N>(A, (int)blockIdx.x, c0, min(c0 + A.chunk, A.nprog), tile);
+    // XCD-aware order.  The hardware hands workgroup i to XCD i mod kXcds; this
+    // maps XCD x's workgroups onto a contiguous run of (row tile, chunk) items,
+    // chunk fastest, so the chunks of one row tile run at about the same time on
+    // one XCD and share its L2 (the tile's derived columns are read by every
+    // chunk).  A bijection on [0, tiles * chunks) for any count.
+    const uint32_t nch = (uint32_t)((A.nprog + A.chunk - 1) / A.chunk);
+    const uint32_t total = gridDim.x, i = blockIdx.x;
+    const uint32_t x = i % kXcds, k = i / kXcds, q = total / kXcds, rem = total % kXcds;
+    const uint32_t item = x * q + min(x, rem) + k;
+    const int rb = (int)(item / nch);
+    const int c0 = (int)(item - (uint32_t)rb * nch) * A.chunk;
+    fp_tile_body<R, F, S, P, N>(A, rb, c0, min(c0 + A.chunk, A.nprog), tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -776,7 +787,8 @@ namespace launch {
 template <int R, int F, int S>
 hipError_t launch_fp_t(const EvalArgs& A, bool P, bool N, hipStream_t st) {
     const int64_t rows_per_block = fp_rows_per_block(R);
-    dim3 grid((unsigned)((A.nlin + rows_per_block - 1) / rows_per_block), (unsigned)((A.nprog + A.chunk - 1) / A.chunk));
+    const int64_t tiles = (A.nlin + rows_per_block - 1) / rows_per_block, chunks = (A.nprog + A.chunk - 1) / A.chunk;
+    dim3 grid((unsigned)(tiles * chunks));  // fp_tile_kernel maps the 1-D index onto (tile, chunk)
     if (P && N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, true>), grid, dim3(dev::kBlock), 0, st, A);
     else if (P) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, false>), grid, dim3(dev::kBlock), 0, st, A);
     else if (N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, true>), grid, dim3(dev::kBlock), 0, st, A);
