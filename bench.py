@@ -251,9 +251,10 @@ def main():
                          "algorithmic_bytes": traffic["algorithmic_bytes_per_launch"] if traffic else None,
                          "valu_lane_ops_per_node": valu["valu_lane_ops_per_node"] if valu else None,
                          "valu_issue_frac": valu["valu_issue_frac"] if valu else None,
-                         "note": "frac counts 1 lane-op per node-evaluation; the exact double-rounded "
-                                 "transcendentals take valu_lane_ops_per_node on this workload, and rocprofv3 "
-                                 "SQ counters put VALU issue at valu_issue_frac of the SIMDs' capacity"},
+                         "note": "frac counts one FP32 VALU lane-op per node-evaluation; on this workload the "
+                                 "exact (double-rounded) transcendentals and the per-instruction epilogues take "
+                                 "valu_lane_ops_per_node lane-ops per node, and rocprofv3 SQ counters put VALU "
+                                 "issue at valu_issue_frac of the SIMDs' capacity (the rest is scalar dispatch)"},
         })
         if extras:
             log("timing PDP form")
