@@ -99,7 +99,9 @@ typedef enum ppi_form {
     PPI_FORM_AUTO = 0,  /* autotuned */
     PPI_FORM_PP = 1,    /* one lane per program, rows sequential */
     PPI_FORM_FP = 2,    /* row tiles x program chunks, rows across lanes, programs wave-uniform */
-    PPI_FORM_PDP = 3    /* one workgroup per program, lanes over rows */
+    PPI_FORM_PDP = 3,   /* one workgroup per program, lanes over rows */
+    PPI_FORM_ORDERED = 4 /* FP row tiles write per-row errors; each program's errors are then summed in row
+                            order, so sums are bit-identical to the sequential loop (SEQ and PP semantics only) */
 } ppi_form;
 
 typedef struct ppi_config {
@@ -119,6 +121,8 @@ typedef struct ppi_config {
     /* Tuning; 0 = autotune. */
     int32_t rows_per_lane;  /* 1, 2, 4 or 8 */
     int32_t programs_per_tile;
+    int32_t ordered_pass_rows; /* PPI_FORM_ORDERED: rows per pass of stored row errors (0: min(16 GiB, half the
+                                  free device memory) of errors) */
 } ppi_config;
 
 typedef struct ppi_stats {

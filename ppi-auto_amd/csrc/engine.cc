@@ -297,7 +297,7 @@ void ppi_engine_destroy(ppi_engine* E) {
     (void)hipSetDevice(E->device);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void* dev[] = {E->dX, E->dMeta, E->dCode, E->dPartial, E->dBadLo, E->dBadHi, E->dRaw, E->dVec,
-                   E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc};
+                   E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     if (E->hRaw) (void)hipHostFree(E->hRaw);
@@ -311,7 +311,12 @@ void ppi_engine_destroy(ppi_engine* E) {
 ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable) {
     if (!E || !cfg) { set_error("engine or config is NULL"); return PPI_E_ARG; }
     if (cfg->strategy < PPI_STRATEGY_SEQ || cfg->strategy > PPI_STRATEGY_PDP) { set_error("bad strategy"); return PPI_E_ARG; }
-    if (cfg->form < PPI_FORM_AUTO || cfg->form > PPI_FORM_PDP) { set_error("bad form"); return PPI_E_ARG; }
+    if (cfg->form < PPI_FORM_AUTO || cfg->form > PPI_FORM_ORDERED) { set_error("bad form"); return PPI_E_ARG; }
+    if (cfg->form == PPI_FORM_ORDERED && cfg->strategy != PPI_STRATEGY_SEQ && cfg->strategy != PPI_STRATEGY_PP) {
+        set_error("the ordered form serves the row-order strategies (SEQ, PP) only");
+        return PPI_E_ARG;
+    }
+    if (cfg->ordered_pass_rows < 0) { set_error("bad ordered_pass_rows"); return PPI_E_ARG; }
     if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
     bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
     if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs cost_n > 0 and cost"); return PPI_E_ARG; }
@@ -407,8 +412,8 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
     double t0 = now_ms();
     std::string err;
     int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    // The FP form reads unary-of-input nodes from precomputed derived columns.
-    const bool fp_form = E->cfg.form == PPI_FORM_AUTO || E->cfg.form == PPI_FORM_FP;
+    // The FP and ordered forms read unary-of-input chains from precomputed derived columns.
+    const bool fp_form = E->cfg.form == PPI_FORM_AUTO || E->cfg.form == PPI_FORM_FP || E->cfg.form == PPI_FORM_ORDERED;
     DerivedColumns* der = (fp_form && E->derived.cap > 0) ? &E->derived : nullptr;
     if (!compile_population_mt(E->symtab, phenotype, ephemeral, size, stride, n, E->ncol, false, threads, &E->tape, &err,
                                der)) {
@@ -434,6 +439,22 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
         cap = E->cap_partial;
         if ((s = ensure(&E->dPartial, &cap, (size_t)E->nrb * n))) return s;
         E->cap_partial = cap;
+    }
+    if (E->form == PPI_FORM_ORDERED) {
+        // Row errors of one pass: nrb_pass row tiles x n programs x 64*R rows; by default as many
+        // rows as fit min(16 GiB, half the free device memory), so most populations take one pass.
+        const int64_t tile = fp_rows_per_block(E->R);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)4 << 30;
+        const int64_t budget = std::min<int64_t>((int64_t)16 << 30, (int64_t)(free_b / 2));
+        int64_t per = E->cfg.ordered_pass_rows > 0 ? (E->cfg.ordered_pass_rows + tile - 1) / tile
+                                                   : budget / ((int64_t)n * tile * (int64_t)sizeof(float));
+        E->nrb_pass = (int)std::max<int64_t>(1, std::min<int64_t>(per, E->nrb));
+        cap = E->cap_rowerr;
+        if ((s = ensure(&E->dRowErr, &cap, (size_t)E->nrb_pass * n * tile))) return s;
+        E->cap_rowerr = cap;
+        cap = E->cap_osum; if ((s = ensure(&E->dOSum, &cap, (size_t)n))) return s;
+        cap = E->cap_osum; if ((s = ensure(&E->dOStop, &cap, (size_t)n))) return s; E->cap_osum = cap;
     }
     if (der && (s = stage_derived(E))) return s;
     PPI_HIP(hipMemcpyAsync(E->dMeta, E->tape.meta.data(), E->tape.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice, E->stream));
@@ -482,6 +503,11 @@ static EvalArgs make_args(ppi_engine* E, float alpha) {
     A.bad_hi = E->dBadHi;
     A.raw = E->dRaw;
     A.vec = E->dVec;
+    A.rb0 = 0;
+    A.nrb_pass = E->nrb;
+    A.rowerr = E->dRowErr;
+    A.osum = E->dOSum;
+    A.ostop = E->dOStop;
     return A;
 }
 
@@ -496,6 +522,17 @@ static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
         PPI_HIP(launch_fp(A, E->R, E->F, E->S, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
         PPI_HIP(launch_finalize(A, E->nrb, E->stream));
+    } else if (E->form == PPI_FORM_ORDERED) {
+        // Passes of nrb_pass row tiles: per-row errors, then each program's row-order sum continues.
+        PPI_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(E->dOSum), 0u, E->nprog, E->stream));  // 0.0f
+        PPI_HIP(launch_fill_u32(E->dOStop, 0u, E->nprog, E->stream));
+        for (int rb0 = 0; rb0 < E->nrb; rb0 += E->nrb_pass) {
+            A.rb0 = rb0;
+            A.nrb_pass = std::min(E->nrb_pass, E->nrb - rb0);
+            PPI_HIP(launch_fp(A, E->R, E->F, E->S, P, N, E->stream, true));
+            PPI_HIP(launch_ordered_sum(A, fp_rows_per_block(E->R), rb0 + A.nrb_pass >= E->nrb, E->stream));
+        }
+        if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
     } else if (E->form == PPI_FORM_PDP) {
         PPI_HIP(launch_pdp(A, E->R, E->F, E->S, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));

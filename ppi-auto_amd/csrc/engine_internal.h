@@ -74,6 +74,13 @@ struct ppi_engine {
     float* hRaw = nullptr;
     float* hVec = nullptr;
     size_t cap_hres = 0;
+    // ordered form: one pass of per-row errors, running sums and stop flags
+    float* dRowErr = nullptr;
+    size_t cap_rowerr = 0;
+    float* dOSum = nullptr;
+    uint32_t* dOStop = nullptr;
+    size_t cap_osum = 0;
+    int nrb_pass = 1;
 
     // reference geometry (accelerator.cc:272-355) and the launch plan
     int dp_local = 1, dp_groups = 1, pdp_local = 1, best_local = 1;

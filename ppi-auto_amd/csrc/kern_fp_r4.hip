@@ -2,11 +2,11 @@
 #include "kernels_impl.h"
 
 namespace ppi {
-hipError_t launch_fp_r4(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st) {
-    if (F == 8 && S == 12) return launch::launch_fp_t<4, 8, 12>(A, P, N, st);
-    if (F == 16 && S == 12) return launch::launch_fp_t<4, 16, 12>(A, P, N, st);
-    if (F == 32 && S == 12) return launch::launch_fp_t<4, 32, 12>(A, P, N, st);
-    if (F == 0 && S == 12) return launch::launch_fp_t<4, 0, 12>(A, P, N, st);
+hipError_t launch_fp_r4(const EvalArgs& A, int F, int S, bool P, bool N, bool rows, hipStream_t st) {
+    if (F == 8 && S == 12) return launch::launch_fp_t<4, 8, 12>(A, P, N, rows, st);
+    if (F == 16 && S == 12) return launch::launch_fp_t<4, 16, 12>(A, P, N, rows, st);
+    if (F == 32 && S == 12) return launch::launch_fp_t<4, 32, 12>(A, P, N, rows, st);
+    if (F == 0 && S == 12) return launch::launch_fp_t<4, 0, 12>(A, P, N, rows, st);
     return hipErrorInvalidValue;
 }
 }  // namespace ppi

@@ -3,10 +3,10 @@
 #include "kernels_impl.h"
 
 namespace ppi {
-hipError_t launch_fp_r1(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
-hipError_t launch_fp_r2(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
-hipError_t launch_fp_r4(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
-hipError_t launch_fp_r8(const EvalArgs& A, int F, int S, bool P, bool N, hipStream_t st);
+hipError_t launch_fp_r1(const EvalArgs& A, int F, int S, bool P, bool N, bool rows, hipStream_t st);
+hipError_t launch_fp_r2(const EvalArgs& A, int F, int S, bool P, bool N, bool rows, hipStream_t st);
+hipError_t launch_fp_r4(const EvalArgs& A, int F, int S, bool P, bool N, bool rows, hipStream_t st);
+hipError_t launch_fp_r8(const EvalArgs& A, int F, int S, bool P, bool N, bool rows, hipStream_t st);
 
 bool shape_supported(int R, int F, int S) {
     return (R == 1 && F == 32 && S == 16) ||
@@ -24,14 +24,20 @@ bool shape_supported(int R, int F, int S) {
            (R == 8 && F == 16 && S == 12);
 }
 
-hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st) {
+hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st, bool rows) {
     switch (R) {
-    case 1: return launch_fp_r1(A, F, S, P, N, st);
-    case 2: return launch_fp_r2(A, F, S, P, N, st);
-    case 4: return launch_fp_r4(A, F, S, P, N, st);
-    case 8: return launch_fp_r8(A, F, S, P, N, st);
+    case 1: return launch_fp_r1(A, F, S, P, N, rows, st);
+    case 2: return launch_fp_r2(A, F, S, P, N, rows, st);
+    case 4: return launch_fp_r4(A, F, S, P, N, rows, st);
+    case 8: return launch_fp_r8(A, F, S, P, N, rows, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_ordered_sum(const EvalArgs& A, int rows_per_block, bool last, hipStream_t st) {
+    hipLaunchKernelGGL(dev::ordered_sum_kernel, dim3((unsigned)((A.nprog + 255) / 256)), dim3(256), 0, st, A,
+                       rows_per_block, last ? 1 : 0);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st) {

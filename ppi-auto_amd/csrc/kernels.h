@@ -44,6 +44,12 @@ struct EvalArgs {
     uint32_t* bad_hi;     // per program: max bad-row key
     float* raw;           // per program: kernel-side error (before alpha*size for PP/PDP)
     float* vec;           // per program: reported error
+    // Ordered form: the rows run in passes of nrb_pass row blocks starting at row block rb0
+    // (rb0 also offsets the FP form's row blocks; 0 outside the ordered form).
+    int32_t rb0, nrb_pass;
+    float* rowerr;        // [row block - rb0][nprog][64*R]: per-row errors of one pass
+    float* osum;          // [nprog] running sum (REDUCEMAX: maximum) in row order
+    uint32_t* ostop;      // [nprog] 1 once a row's error was not finite
 };
 
 // Batched form: one (population, dataset) pair as the device sees it.
@@ -148,7 +154,10 @@ hipError_t launch_batch(const BatchArgs& B, int grid, int R, int F, int S, bool 
 int batch_occupancy(int R, int F, int S, bool P, bool N);
 hipError_t launch_batch_finalize(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, hipStream_t st);
 bool pdp_shape_supported(int R, int F, int S);
-hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
+// rows: the ordered form's FP pass (per-row errors to A.rowerr instead of partials).
+hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st, bool rows = false);
+// Ordered form: continues every program's row-order sum over the pass's rows; last: apply the epilogue.
+hipError_t launch_ordered_sum(const EvalArgs& A, int rows_per_block, bool last, hipStream_t st);
 hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
 hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st);
 hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st);

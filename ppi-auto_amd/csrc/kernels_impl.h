@@ -448,11 +448,21 @@ __device__ __forceinline__ void publish(const EvalArgs& A, int64_t slot, int p, 
 // dispatch path.  Programs end with H_END, where the wave reduces the program's
 // error over the tile and stores one partial.
 // ---------------------------------------------------------------------------
+// Ordered form: the R row errors of one lane for program p, in row order within the
+// pass buffer (row block rb - rb0, program p, row r*64 + lane of the tile).
+template <int R>
+__device__ __forceinline__ void store_row_errors(const EvalArgs& A, int rb, int p, int lane, const float (&acc)[R],
+                                                 const float (&y)[R], const bool (&valid)[R]) {
+    float* dst = A.rowerr + ((int64_t)(rb - A.rb0) * A.nprog + p) * (64 * R) + lane;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[r * 64] = valid[r] ? error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n) : 0.0f;
+}
+
 // One FP work item: rows [rb*64R, rb*64R + 64R) of A's dataset x programs [c0, c1)
 // of A's population.  Called by every thread of the workgroup; `tile` is the
 // workgroup's LDS feature buffer.  The caller separates consecutive items with a
 // barrier (the tile is rewritten at the start).
-template <int R, int F, int S, bool P, bool N>
+template <int R, int F, int S, bool P, bool N, bool ROWS = false>
 __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, int c1,
                                              float (*tile)[64][R]) {
     typedef Core<R, S, P, N> C;
@@ -504,10 +514,14 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
             const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)wa, k);
             if ((c & 0xffu) == H_END) {
                 // H_END carries the program's flags in its argument.
-                float part = 0.0f;
-                uint32_t kmin = 0xffffffffu, kmax = 0u;
-                if (!(a & kEmpty)) accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
-                publish(A, (int64_t)rb * A.nprog + p, p, lane, part, kmin, kmax);
+                if constexpr (ROWS) {
+                    if (!(a & kEmpty)) store_row_errors<R>(A, rb, p, lane, acc, y, valid);
+                } else {
+                    float part = 0.0f;
+                    uint32_t kmin = 0xffffffffu, kmax = 0u;
+                    if (!(a & kEmpty)) accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
+                    publish(A, (int64_t)rb * A.nprog + p, p, lane, part, kmin, kmax);
+                }
                 ++p;
                 continue;
             }
@@ -519,7 +533,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
     }
 }
 
-template <int R, int F, int S, bool P, bool N>
+template <int R, int F, int S, bool P, bool N, bool ROWS>
 __global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(EvalArgs A) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
@@ -533,9 +547,9 @@ __global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(Eva
     const uint32_t total = gridDim.x, i = blockIdx.x;
     const uint32_t x = i % kXcds, k = i / kXcds, q = total / kXcds, rem = total % kXcds;
     const uint32_t item = x * q + min(x, rem) + k;
-    const int rb = (int)(item / nch);
-    const int c0 = (int)(item - (uint32_t)rb * nch) * A.chunk;
-    fp_tile_body<R, F, S, P, N>(A, rb, c0, min(c0 + A.chunk, A.nprog), tile);
+    const int rbl = (int)(item / nch);
+    const int c0 = (int)(item - (uint32_t)rbl * nch) * A.chunk;
+    fp_tile_body<R, F, S, P, N, ROWS>(A, A.rb0 + rbl, c0, min(c0 + A.chunk, A.nprog), tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -606,6 +620,61 @@ __global__ void finalize_kernel(EvalArgs A, int nrb) {
     }
     const ProgMeta m = A.meta[p];
     semantics(A, total, m.flags, A.bad_lo[p], A.bad_hi[p], m.size, &A.raw[p], &A.vec[p]);
+}
+
+// Ordered form: one thread per program continues its sum (REDUCEMAX: maximum) over the
+// pass's rows in row order, exactly as the sequential loop does (sequential.cc:169-179:
+// float accumulation row by row, stop at the first non-finite error), and the last pass
+// applies the strategy's epilogue.  A program's row errors are contiguous within each
+// row tile; they are read 64 at a time as float4, the next batch in flight while the
+// current one is summed, so the serial sum does not wait on memory latency per load.
+__device__ __forceinline__ void ordered_take(float e, float fn, bool rmax, float& part, uint32_t& stop) {
+    const bool fin = finite_f(e);
+    const bool take = !stop && fin;
+    const float cand = rmax ? ((e * fn > part) ? e * fn : part) : part + e;
+    part = take ? cand : part;
+    stop = (stop || !fin) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) ordered_sum_kernel(EvalArgs A, int rows_per_block, int last) {
+    constexpr int V = 16;  // float4 per batch
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.nprog) return;
+    const ProgMeta m = A.meta[p];
+    float part = A.osum[p];
+    uint32_t stop = A.ostop[p];
+    const float fn = (float)A.nlin;
+    const bool rmax = A.reducemax != 0;
+    if (!(m.flags & kEmpty)) {
+        for (int b = 0; b < A.nrb_pass && !stop; ++b) {
+            const int64_t row0 = (int64_t)(A.rb0 + b) * rows_per_block;
+            const int nrows = (int)min<int64_t>(rows_per_block, (int64_t)A.nlin - row0);
+            const int nv = (nrows + 3) >> 2;
+            const float4* src = reinterpret_cast<const float4*>(A.rowerr + ((int64_t)b * A.nprog + p) * rows_per_block);
+            float4 cur[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) cur[k] = k < nv ? src[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int base = 0; base < nv && !stop; base += V) {
+                float4 nxt[V];
+#pragma unroll
+                for (int k = 0; k < V; ++k)
+                    nxt[k] = base + V + k < nv ? src[base + V + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const int j = (base + k) * 4;
+                    if (j + 0 < nrows) ordered_take(cur[k].x, fn, rmax, part, stop);
+                    if (j + 1 < nrows) ordered_take(cur[k].y, fn, rmax, part, stop);
+                    if (j + 2 < nrows) ordered_take(cur[k].z, fn, rmax, part, stop);
+                    if (j + 3 < nrows) ordered_take(cur[k].w, fn, rmax, part, stop);
+                }
+#pragma unroll
+                for (int k = 0; k < V; ++k) cur[k] = nxt[k];
+            }
+        }
+    }
+    A.osum[p] = part;
+    A.ostop[p] = stop;
+    if (last) semantics(A, part, m.flags, stop ? 0u : 0xffffffffu, 0u, m.size, &A.raw[p], &A.vec[p]);
 }
 #endif  // PPI_KERNELS_MISC
 
@@ -784,16 +853,23 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
 // ---------------------------------------------------------------------------
 namespace launch {
 
-template <int R, int F, int S>
-hipError_t launch_fp_t(const EvalArgs& A, bool P, bool N, hipStream_t st) {
+template <int R, int F, int S, bool ROWS>
+hipError_t launch_fp_rt(const EvalArgs& A, bool P, bool N, hipStream_t st) {
     const int64_t rows_per_block = fp_rows_per_block(R);
-    const int64_t tiles = (A.nlin + rows_per_block - 1) / rows_per_block, chunks = (A.nprog + A.chunk - 1) / A.chunk;
+    // All row tiles, or the ordered form's pass of nrb_pass tiles from rb0.
+    const int64_t tiles = ROWS ? A.nrb_pass : (A.nlin + rows_per_block - 1) / rows_per_block;
+    const int64_t chunks = (A.nprog + A.chunk - 1) / A.chunk;
     dim3 grid((unsigned)(tiles * chunks));  // fp_tile_kernel maps the 1-D index onto (tile, chunk)
-    if (P && N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, true>), grid, dim3(dev::kBlock), 0, st, A);
-    else if (P) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, false>), grid, dim3(dev::kBlock), 0, st, A);
-    else if (N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, true>), grid, dim3(dev::kBlock), 0, st, A);
-    else hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, false>), grid, dim3(dev::kBlock), 0, st, A);
+    if (P && N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, true, ROWS>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, true, false, ROWS>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, true, ROWS>), grid, dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::fp_tile_kernel<R, F, S, false, false, ROWS>), grid, dim3(dev::kBlock), 0, st, A);
     return hipGetLastError();
+}
+
+template <int R, int F, int S>
+hipError_t launch_fp_t(const EvalArgs& A, bool P, bool N, bool rows, hipStream_t st) {
+    return rows ? launch_fp_rt<R, F, S, true>(A, P, N, st) : launch_fp_rt<R, F, S, false>(A, P, N, st);
 }
 
 template <int R, int F, int S>

@@ -30,7 +30,7 @@ OK, E_ARG, E_STATE, E_PROGRAM, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_NODEVICE = ra
 SEQ, PP, FP, PDP = 0, 1, 2, 3
 STRATEGIES = {"SEQ": SEQ, "PP": PP, "FP": FP, "DP": FP, "PDP": PDP}
 # ppi_form
-FORM_AUTO, FORM_PP, FORM_FP, FORM_PDP = 0, 1, 2, 3
+FORM_AUTO, FORM_PP, FORM_FP, FORM_PDP, FORM_ORDERED = 0, 1, 2, 3, 4
 # ppi_error_fn
 EF_ABS, EF_SQUARED, EF_NEQ, EF_COST, EF_COST_CLAMP = range(5)
 
@@ -52,7 +52,7 @@ class Config(C.Structure):
         ("strategy", C.c_int32), ("form", C.c_int32), ("error_fn", C.c_int32), ("reducemax", C.c_int32),
         ("protected_ops", C.c_int32), ("native_math", C.c_int32), ("cost_n", C.c_int32),
         ("cost", C.POINTER(C.c_float)), ("max_local_size", C.c_int32), ("compute_units", C.c_int32),
-        ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32),
+        ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32), ("ordered_pass_rows", C.c_int32),
     ]
 
 
@@ -223,7 +223,8 @@ class Engine:
     def configure(self, table: Sequence[Tuple[int, int, float]], strategy: int = SEQ, error_fn: int = EF_ABS,
                   reducemax: bool = False, protected: bool = True, native: bool = False,
                   cost: Optional[np.ndarray] = None, form: int = FORM_AUTO, max_local_size: int = 0,
-                  compute_units: int = 0, rows_per_lane: int = 0, programs_per_tile: int = 0) -> None:
+                  compute_units: int = 0, rows_per_lane: int = 0, programs_per_tile: int = 0,
+                  ordered_pass_rows: int = 0) -> None:
         cfg = Config()
         cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
         cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
@@ -233,6 +234,7 @@ class Engine:
             cfg.cost = _ptr(self._cost, C.c_float)
         cfg.max_local_size, cfg.compute_units = max_local_size, compute_units
         cfg.rows_per_lane, cfg.programs_per_tile = rows_per_lane, programs_per_tile
+        cfg.ordered_pass_rows = ordered_pass_rows
         tab = (Symbol * max(1, len(table)))()
         for k, (s, op, v) in enumerate(table):
             tab[k].symbol, tab[k].op, tab[k].value = s, op, v

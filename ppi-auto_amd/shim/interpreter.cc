@@ -103,7 +103,7 @@ struct Options {
 
 // The options acc_interpret_init reads (accelerator.cc:565-571); everything else
 // in argv belongs to other modules and is ignored, as CmdLine::Parser does.  A
-// "--" stops the scan.  Two engine extras: -acc-form PP|FP|PDP and -acc-rows N.
+// "--" stops the scan.  Two engine extras: -acc-form PP|FP|PDP|ORDERED and -acc-rows N.
 Options parse_options(int argc, char** argv) {
     Options o;
     for (int i = 1; i < argc; ++i) {
@@ -140,7 +140,8 @@ Options parse_options(int argc, char** argv) {
             o.transpose = true;  // the engine always stores the dataset column-major
         } else if (a == "-acc-form") {
             if (!value(&v)) break;
-            o.form = v == "PP" ? PPI_FORM_PP : v == "FP" ? PPI_FORM_FP : v == "PDP" ? PPI_FORM_PDP : PPI_FORM_AUTO;
+            o.form = v == "PP" ? PPI_FORM_PP : v == "FP" ? PPI_FORM_FP : v == "PDP" ? PPI_FORM_PDP
+                   : v == "ORDERED" ? PPI_FORM_ORDERED : PPI_FORM_AUTO;
         } else if (a == "-acc-rows") {
             if (!value(&v)) break;
             o.rows_per_lane = std::atoi(v.c_str());
