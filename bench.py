@@ -260,6 +260,11 @@ def main():
             log("timing PDP form")
             pdp = bench_engine(eng, w, ppiamd.FORM_PDP, max(1, a.steps // 5), 1)
             line["pdp_gpops"] = round(pdp["node_evals"] / (pdp["ms_per_step"] * 1e-3) / 1e9, 3)
+            log("timing ordered form (sums bit-identical to the sequential path)")
+            od = bench_engine(eng, w, ppiamd.FORM_ORDERED, max(1, a.steps // 2), 1)
+            line["ordered_gpops"] = round(od["node_evals"] / (od["ms_per_step"] * 1e-3) / 1e9, 3)
+            line["ordered_note"] = ("FP row tiles store per-row errors and each program's errors are summed in row "
+                                    "order: SEQ/PP results bit-identical to the sequential path")
             log("timing PP form (500-program sample)")
             wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(500), w.data, w.error_fn)
             pp = bench_engine(eng, wpp, ppiamd.FORM_PP, 1, 0)
