@@ -36,13 +36,23 @@ template <bool N> __device__ __forceinline__ float f_exp(float x) {
     if constexpr (N) return __expf(x);
     else return exact_exp(x);
 }
+// The remaining double routines (exp10, exp2, log10, log2, pow, and GAMMA's sinh and
+// pow) are called out of line: inlined into every row of the interpreters' unrolled
+// handlers, their register needs set the whole kernel's register count, although
+// common grammars rarely or never use them.
+__device__ __noinline__ float exp10_of_double(float x) { return (float)exp10((double)x); }
+__device__ __noinline__ float exp2_of_double(float x) { return (float)exp2((double)x); }
+__device__ __noinline__ float log10_of_double(float x) { return (float)log10((double)x); }
+__device__ __noinline__ float log2_of_double(float x) { return (float)log2((double)x); }
+__device__ __noinline__ float pow_of_double(float x, float y) { return (float)pow((double)x, (double)y); }
+
 template <bool N> __device__ __forceinline__ float f_exp10(float x) {
     if constexpr (N) return __exp10f(x);
-    else return (float)exp10((double)x);
+    else return exp10_of_double(x);
 }
 template <bool N> __device__ __forceinline__ float f_exp2(float x) {
     if constexpr (N) return exp2f(x);
-    else return (float)exp2((double)x);
+    else return exp2_of_double(x);
 }
 // The device library's double trigonometric routines carry a large-argument
 // reduction; inlined into the interpreters' unrolled per-row loops their
@@ -70,12 +80,12 @@ template <bool P, bool N> __device__ __forceinline__ float f_log(float x) {
 template <bool P, bool N> __device__ __forceinline__ float f_log10(float x) {
     if (P && x < 1.0f) return 1.0f;
     if constexpr (N) return __log10f(x);
-    else return (float)log10((double)x);
+    else return log10_of_double(x);
 }
 template <bool P, bool N> __device__ __forceinline__ float f_log2(float x) {
     if (P && x < 1.0f) return 1.0f;
     if constexpr (N) return __log2f(x);
-    else return (float)log2((double)x);
+    else return log2_of_double(x);
 }
 // functions.h:99-113 (DIV).
 template <bool P, bool N> __device__ __forceinline__ float f_div(float x, float y) {
@@ -91,7 +101,17 @@ template <bool P, bool N> __device__ __forceinline__ float f_tan(float x) {
 }
 template <bool N> __device__ __forceinline__ float f_pow(float x, float y) {
     if constexpr (N) return __powf(x, y);
-    else return (float)pow((double)x, (double)y);
+    else return pow_of_double(x, y);
+}
+
+// GAMMA (interpreter_core:127-129): sinh and pow are double routines in the reference,
+// and the last product is formed in double before it is rounded.  Out of line, like the
+// other rare double routines.
+template <bool P> __device__ __noinline__ float gamma_of_double(float a) {
+    const float s1 = f_sqrt<P, false>((float)((double)a * sinh((double)(1.0f / a))));
+    const float base = (a / kE_F) * s1;
+    const float s2 = f_sqrt<P, false>(2.0f * kPI_F / a);
+    return (float)(pow((double)base, (double)a) * (double)s2);
 }
 
 // interpreter_core:1-151, unary operators.
@@ -115,16 +135,15 @@ template <bool P, bool N> __device__ __forceinline__ float unary(uint32_t u, flo
     case U_STEP: return b2f(a >= 0.0f);
     case U_SIGN: return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f);
     case U_LOGISTIC: return 1.0f / (1.0f + f_exp<N>(-a));
-    case U_GAMMA: {
-        // interpreter_core:127-129: sinh and pow are double routines in the reference.
-        float s1;
-        if constexpr (N) s1 = f_sqrt<P, N>(a * sinhf(1.0f / a));
-        else s1 = f_sqrt<P, N>((float)((double)a * sinh((double)(1.0f / a))));
-        float base = (a / kE_F) * s1;
-        float s2 = f_sqrt<P, N>(2.0f * kPI_F / a);
-        if constexpr (N) return __powf(base, a) * s2;
-        else return (float)(pow((double)base, (double)a) * (double)s2);
-    }
+    case U_GAMMA:
+        if constexpr (N) {
+            const float s1 = f_sqrt<P, N>(a * sinhf(1.0f / a));
+            const float base = (a / kE_F) * s1;
+            const float s2 = f_sqrt<P, N>(2.0f * kPI_F / a);
+            return __powf(base, a) * s2;
+        } else {
+            return gamma_of_double<P>(a);
+        }
     case U_GAUSS: return f_exp<N>(-a * a);
     case U_SQRT: return f_sqrt<P, N>(a);
     case U_LOG: return f_log<P, N>(a);

@@ -33,14 +33,15 @@ constexpr int kBlock = 64 * kWaves;
 
 // Waves per SIMD the register allocator must leave room for (the second
 // __launch_bounds__ argument caps VGPRs at 512 / waves) in the FP and batched
-// forms.  Measured on gfx950 with the regression workload: 5 waves at 2 rows
-// per lane and 4 waves at 4 rows beat the unbounded allocation by 11% and 15%;
-// 1 row per lane (the 16-slot fallback) takes 4 to stay clear of spills; at 8
-// rows any bound spills, so none is set.  The PDP form keeps its rows'
-// features in VGPRs and is left unbounded.
+// forms.  Measured on gfx950 with the regression workload: 5 waves at 2 and at
+// 4 rows per lane (96 VGPRs; at 4 rows a few VGPRs spill, still 4.5% faster
+// than 4 waves without spills: more waves hide more of the scalar dispatch);
+// 1 row per lane (the 16-slot fallback) takes 4; at 8 rows any bound spills,
+// so none is set.  The PDP form keeps its rows' features in VGPRs and is left
+// unbounded.
 template <int R>
 struct MinWaves {
-    static constexpr int value = R == 2 ? 5 : (R <= 4 ? 4 : 1);
+    static constexpr int value = R == 2 || R == 4 ? 5 : (R == 1 ? 4 : 1);
 };
 
 template <int K> struct VecT { typedef float type __attribute__((ext_vector_type(K))); };
