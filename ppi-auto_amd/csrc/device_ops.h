@@ -45,6 +45,7 @@ __device__ __noinline__ float exp2_of_double(float x) { return (float)exp2((doub
 __device__ __noinline__ float log10_of_double(float x) { return (float)log10((double)x); }
 __device__ __noinline__ float log2_of_double(float x) { return (float)log2((double)x); }
 __device__ __noinline__ float pow_of_double(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __noinline__ float fmod_float(float x, float y) { return fmodf(x, y); }
 
 template <bool N> __device__ __forceinline__ float f_exp10(float x) {
     if constexpr (N) return __exp10f(x);
@@ -195,8 +196,8 @@ template <bool P, bool N> __device__ __forceinline__ float binary(uint32_t op, f
     case B_RMAX: return fmaxf(b, a);
     case B_MIN: return fminf(a, b);
     case B_RMIN: return fminf(b, a);
-    case B_MOD: return fmodf(a, b);
-    case B_RMOD: return fmodf(b, a);
+    case B_MOD: return fmod_float(a, b);
+    case B_RMOD: return fmod_float(b, a);
     case B_POW: return f_pow<N>(a, b);
     case B_RPOW: return f_pow<N>(b, a);
     case B_AND: return b2f((a != 0.0f) && (b != 0.0f));
