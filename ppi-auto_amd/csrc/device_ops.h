@@ -70,7 +70,7 @@ template <bool N> __device__ __forceinline__ float f_cos(float x) {
 // functions.h:115-129 (SQRT): float(sqrt(double(x))) == sqrtf(x) exactly.
 template <bool P, bool N> __device__ __forceinline__ float f_sqrt(float x) {
     if (P && x < 0.0f) return 1.0f;
-    return sqrtf(x);
+    return exact_sqrt(x);
 }
 // functions.h:131-177 (LOG, LOG10, LOG2).
 template <bool P, bool N> __device__ __forceinline__ float f_log(float x) {
@@ -162,6 +162,15 @@ template <int R, bool P, bool N> __device__ __forceinline__ void unary_rows(uint
         if (u == U_EXP) return exact_rows<kFnExp, R>(v);
         if (u == U_SIN) return exact_rows<kFnSin, R>(v);
         if (u == U_COS) return exact_rows<kFnCos, R>(v);
+        if (u == U_SQRT) {
+            // functions.h:115-129: x < 0 gives 1 under PROTECTED, which is sqrt(1).
+            if constexpr (P) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (v[r] < 0.0f) v[r] = 1.0f;
+            }
+            return exact_rows<kFnSqrt, R>(v);
+        }
         if (u == U_LOG) {
             // functions.h:131-147: x < 1 gives 1 under PROTECTED; such rows evaluate log(1).
             bool prot[R];

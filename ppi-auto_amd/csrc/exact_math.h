@@ -47,6 +47,7 @@ __device__ __noinline__ float log_of_double(float x) { return (float)log((double
 __device__ __noinline__ float sin_of_double(float x) { return (float)sin((double)x); }
 __device__ __noinline__ float cos_of_double(float x) { return (float)cos((double)x); }
 __device__ __noinline__ float tan_of_double(float x) { return (float)tan((double)x); }
+__device__ __noinline__ float sqrt_float(float x) { return sqrtf(x); }
 
 constexpr uint32_t kGuard = 64;  // double ulps kept between y and a float rounding boundary
 
@@ -61,8 +62,8 @@ __shared__ ExactTables lds_exact_tables;
 
 __device__ __forceinline__ void stage_exact_tables() {
     constexpr int kWords = (int)(sizeof(ExactTables) / sizeof(double));
-    const double* src = &kExactTables.exp2j[0];
-    double* dst = &lds_exact_tables.exp2j[0];
+    const double* src = reinterpret_cast<const double*>(&kExactTables);
+    double* dst = reinterpret_cast<double*>(&lds_exact_tables);
     for (int i = (int)threadIdx.x; i < kWords; i += (int)blockDim.x) dst[i] = src[i];
 }
 
@@ -158,13 +159,30 @@ __device__ __forceinline__ float exact_sincos_fast(float x, bool& ok) {
     return res;
 }
 
+// sqrt (functions.h:115-129: float(sqrt(double(x))) == sqrtf(x), the correctly rounded float
+// square root).  v_sqrt_f32 is within 1 ulp; the correctly rounded result is that value or a
+// neighbour, chosen by the signs of the exact residuals x - s'*s of the two neighbours (one fma
+// each) — the compiler's own sqrtf expansion, without its scaling of inputs below 2^-96 (whose
+// residuals could leave the normal range) and its handling of 0, inf and NaN, which stay with
+// the fallback.  Domain: 2^-96 <= x < inf.
+__device__ __forceinline__ float exact_sqrt_fast(float x, bool& ok) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u), sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, x), rp = __builtin_fmaf(-sp, s, x);
+    float res = rm <= 0.0f ? sm : s;
+    res = rp > 0.0f ? sp : res;
+    ok = x >= 0x1.0p-96f && x < __builtin_inff();
+    return res;
+}
+
 // Function ids of the wrappers below.
-enum : int { kFnExp = 0, kFnLog = 1, kFnSin = 2, kFnCos = 3 };
+enum : int { kFnExp = 0, kFnLog = 1, kFnSin = 2, kFnCos = 3, kFnSqrt = 4 };
 
 template <int FN>
 __device__ __forceinline__ float exact_fast(float x, bool& ok) {
     if constexpr (FN == kFnExp) return exact_exp_fast(x, ok);
     else if constexpr (FN == kFnLog) return exact_log_fast(x, ok);
+    else if constexpr (FN == kFnSqrt) return exact_sqrt_fast(x, ok);
     else return exact_sincos_fast<FN == kFnCos>(x, ok);
 }
 template <int FN>
@@ -172,6 +190,7 @@ __device__ __forceinline__ float exact_fallback(float x) {
     if constexpr (FN == kFnExp) return exp_of_double(x);
     else if constexpr (FN == kFnLog) return log_of_double(x);
     else if constexpr (FN == kFnSin) return sin_of_double(x);
+    else if constexpr (FN == kFnSqrt) return sqrt_float(x);
     else return cos_of_double(x);
 }
 
@@ -187,6 +206,7 @@ __device__ __forceinline__ float exact_exp(float x) { return exact_one<kFnExp>(x
 __device__ __forceinline__ float exact_log(float x) { return exact_one<kFnLog>(x); }
 __device__ __forceinline__ float exact_sin(float x) { return exact_one<kFnSin>(x); }
 __device__ __forceinline__ float exact_cos(float x) { return exact_one<kFnCos>(x); }
+__device__ __forceinline__ float exact_sqrt(float x) { return exact_one<kFnSqrt>(x); }
 
 // R values of one lane (the interpreters' rows): every fast path runs first, so
 // the R table loads are in flight together, and a single rarely taken branch

@@ -40,7 +40,7 @@ def main() -> None:
     w(f"constexpr double kExpInvL = {hexd(64 / ln2)};  // 64/ln2")
     w(f"constexpr double kExpL1 = {hexd(l1)};  // ln2/64 rounded")
     w(f"constexpr double kExpL2 = {hexd(l2)};  // ln2/64 - kExpL1 rounded")
-    tables = []  # (comment, member declaration, rows of initialiser text)
+    tables = []  # (comment, member declaration, rows of initialiser text); the sine table goes first (below)
     tables.append(("2^(j/64)", "double exp2j[64];",
                    [", ".join(hexd(mp.power(2, mp.mpf(i) / 64)) for i in range(j, j + 4)) for j in range(0, 64, 4)]))
 
@@ -82,9 +82,10 @@ def main() -> None:
     w(f"constexpr double kTrigInvP = {hexd(64 / mp.pi)};  // 64/pi")
     w(f"constexpr double kTrigP1 = {hexd(p1)};  // pi/64 rounded")
     w(f"constexpr double kTrigP2 = {hexd(p2)};  // pi/64 - kTrigP1 rounded")
-    tables.append(("sin(j pi/64), j < 192: sin and cos read entries j and j + 32 (cos: j + 32 and j + 64) "
-                   "of one index j < 128", "double sinpi64[192];",
-                   [", ".join(hexd(sin_pi64(i)) for i in range(j, j + 4)) for j in range(0, 192, 4)]))
+    # First in the struct (LDS offset 0): both of its reads then fit the 8-bit offsets of one ds_read2_b64.
+    tables.insert(0, ("sin(j pi/64), j < 192: sin and cos read entries j and j + 32 (cos: j + 32 and j + 64) "
+                      "of one index j < 128", "double sinpi64[192];",
+                      [", ".join(hexd(sin_pi64(i)) for i in range(j, j + 4)) for j in range(0, 192, 4)]))
 
     # One struct, so that kernels can stage all tables into LDS with one copy.
     w("")

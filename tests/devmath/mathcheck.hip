@@ -17,13 +17,15 @@ template <int FN> __device__ __forceinline__ float fast(float x) {
     if constexpr (FN == 0) return exact_exp(x);
     else if constexpr (FN == 1) return exact_log(x);
     else if constexpr (FN == 2) return exact_sin(x);
-    else return exact_cos(x);
+    else if constexpr (FN == 3) return exact_cos(x);
+    else return exact_sqrt(x);
 }
 template <int FN> __device__ __forceinline__ float ref(float x) {
     if constexpr (FN == 0) return exp_of_double(x);
     else if constexpr (FN == 1) return log_of_double(x);
     else if constexpr (FN == 2) return sin_of_double(x);
-    else return cos_of_double(x);
+    else if constexpr (FN == 3) return cos_of_double(x);
+    else return sqrtf(x);
 }
 
 // Grid-stride sweep over bit patterns [start, start + count); every thread exits
@@ -49,7 +51,7 @@ __global__ void sweep(uint64_t start, uint64_t count, unsigned long long* bad, u
 
 }  // namespace
 
-// fn: 0 exp, 1 log, 2 sin, 3 cos.  Sweeps all 2^32 inputs; returns 0 on success
+// fn: 0 exp, 1 log, 2 sin, 3 cos, 4 sqrt (against sqrtf).  Sweeps all 2^32 inputs; returns 0 on success
 // (mismatch count in *mismatches, up to 64 mismatching inputs in samples[]), or
 // a HIP error code.
 extern "C" int ppi_mathcheck(int fn, unsigned long long* mismatches, uint32_t* samples, uint32_t* nsamples,
@@ -74,6 +76,7 @@ extern "C" int ppi_mathcheck(int fn, unsigned long long* mismatches, uint32_t* s
         case 1: hipLaunchKernelGGL(sweep<1>, grid, block, 0, 0, 0ull, total, d_bad, d_ns, d_s); break;
         case 2: hipLaunchKernelGGL(sweep<2>, grid, block, 0, 0, 0ull, total, d_bad, d_ns, d_s); break;
         case 3: hipLaunchKernelGGL(sweep<3>, grid, block, 0, 0, 0ull, total, d_bad, d_ns, d_s); break;
+        case 4: hipLaunchKernelGGL(sweep<4>, grid, block, 0, 0, 0ull, total, d_bad, d_ns, d_s); break;
         default: e = hipErrorInvalidValue;
         }
         if (!e) e = hipGetLastError();

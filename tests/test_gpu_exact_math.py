@@ -3,7 +3,8 @@
 Each routine (ppi-auto_amd/csrc/exact_math.h) is compared with the device library's
 double routine rounded to float — the engine's previous implementation of the
 sequential path's (float) f((double) x) (functions.h, interpreter_core:103-117,
-140-142) — on every one of the 2^32 float bit patterns, NaNs compared as NaN.
+140-142) — and sqrt with the library's correctly rounded sqrtf, on every one of the
+2^32 float bit patterns, NaNs compared as NaN.
 Identical results on all inputs make the two implementations interchangeable;
 their agreement with the oracle (glibc's double routines, rounded) is covered by
 the parity tests.
@@ -31,7 +32,7 @@ def mathcheck():
     return lib
 
 
-@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "log"), (2, "sin"), (3, "cos")])
+@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "log"), (2, "sin"), (3, "cos"), (4, "sqrt")])
 def test_all_float_inputs(mathcheck, fn, name):
     bad = C.c_ulonglong(0)
     samples = (C.c_uint32 * 64)()

@@ -1,4 +1,4 @@
-"""Device exp, log, sin and cos against the sequential path's glibc results, bit for bit.
+"""Device exp, log, sin, cos and sqrt against the sequential path's glibc results, bit for bit.
 
 The sequential path evaluates these operators as (float) f((double) x) with glibc's
 double routines (sequential.cc:117-209 with functions.h:131-147 and
@@ -42,9 +42,9 @@ def eng():
 
 
 @pytest.mark.parametrize("prot", [True, False])
-@pytest.mark.parametrize("op", ["exp", "log", "sin", "cos"])
+@pytest.mark.parametrize("op", ["exp", "log", "sin", "cos", "sqrt"])
 def test_unary_matches_glibc(eng, op, prot):
-    x = inputs(seed=2 * ["exp", "log", "sin", "cos"].index(op) + int(prot))
+    x = inputs(seed=2 * ["exp", "log", "sin", "cos", "sqrt"].index(op) + int(prot))
     data = np.stack([x, np.zeros_like(x)], axis=1).astype(np.float32)
     b = Builder(1)
     print(f"{op} protected={prot}: {x.size} inputs", flush=True)
