@@ -90,10 +90,17 @@ struct Node {
     int32_t ch[3];
     uint32_t arg;     // leaf: float bits or column
     int8_t leaf_attr; // leaf: 1 = attribute column, 0 = constant
+    int8_t chain_ops; // leaf: operators of the chain in `chain`
     int32_t pos;      // phenotype position (for T_CONST bookkeeping)
     bool is_tconst;
     int16_t regs;
+    uint64_t chain;   // leaf: chain key of an input or derived column (DerivedColumns), 0 = none
 };
+
+// Chain key of unary operator u applied to the chain `key` of `ops` operators.
+inline uint64_t chain_extend(uint64_t key, int ops, uint32_t u) {
+    return key | ((uint64_t)(u + 1) << (16 + 5 * ops));
+}
 
 uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 
@@ -101,7 +108,7 @@ struct Compiler {
     const SymbolTable& st;
     int32_t ncol;
     bool want_const_pos;
-    DerivedColumns* derived = nullptr;
+    const DerivedColumns* derived = nullptr;
     int64_t derived_uses = 0;
     std::vector<Node> nodes;
     std::vector<int32_t> vstack;
@@ -121,6 +128,8 @@ struct Compiler {
             Node n{};
             n.pos = i;
             n.regs = 0;
+            n.chain = 0;
+            n.chain_ops = 0;
             if (ar == 0) {
                 n.op = -1;
                 n.leaf_attr = 0;
@@ -133,6 +142,7 @@ struct Compiler {
                     if (c > -1.0f && c < (float)ncol) {
                         n.leaf_attr = 1;
                         n.arg = (uint32_t)(int32_t)c;
+                        if (ncol < 65535) n.chain = (uint64_t)n.arg + 1;
                     } else {
                         n.arg = fbits(__builtin_nanf(""));
                     }
@@ -145,13 +155,16 @@ struct Compiler {
                 if ((int)vstack.size() < ar) return -1;
                 if (ar == 1 && derived) {
                     // Unary operator over a column (an input column or a derived one):
-                    // becomes a leaf reading the derived column that holds op(x[col]),
-                    // so chains of unary operators over an input column fold too.
+                    // becomes a leaf reading the derived column of the extended chain,
+                    // if the plan gave it one.
                     Node& leaf = nodes[(size_t)vstack.back()];
-                    if (leaf.op < 0 && leaf.leaf_attr) {
-                        const int32_t dcol = derived->lookup_or_add(unop_of(op), (int32_t)leaf.arg);
+                    if (leaf.op < 0 && leaf.leaf_attr && leaf.chain && leaf.chain_ops < kChainMaxOps) {
+                        const uint64_t key = chain_extend(leaf.chain, leaf.chain_ops, unop_of(op));
+                        const int32_t dcol = derived->find(key);
                         if (dcol >= 0) {
                             leaf.arg = (uint32_t)dcol;
+                            leaf.chain = key;
+                            leaf.chain_ops++;
                             leaf.pos = i;
                             ++derived_uses;
                             continue;
@@ -345,9 +358,118 @@ void put_end(Tape* out, bool want_const_pos, uint32_t flags) {
 
 }  // namespace
 
+namespace {
+
+// Chains of unary operators over input columns in programs [lo, hi), counted into `counts`.
+void count_chains(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                  int32_t stride, int32_t lo, int32_t hi, int32_t ncol,
+                  std::unordered_map<uint64_t, uint32_t>* counts) {
+    struct Item { uint64_t key; int ops; };
+    std::vector<Item> stk;
+    for (int32_t p = lo; p < hi; ++p) {
+        const int32_t* ph = phenotype + (size_t)p * stride;
+        const float* eph = ephemeral + (size_t)p * stride;
+        stk.clear();
+        for (int32_t i = size[p] - 1; i >= 0; --i) {
+            const ppi_symbol* s = st.find(ph[i]);
+            const int32_t op = s ? s->op : PPI_OP_UNKNOWN;
+            const int ar = arity(op);
+            if (ar == 0) {
+                const float c = eph[i];
+                const bool col = op == PPI_OP_ATTRIBUTE && c > -1.0f && c < (float)ncol && ncol < 65535;
+                stk.push_back(Item{col ? (uint64_t)(int32_t)c + 1 : 0, 0});
+            } else if ((int)stk.size() < ar) {
+                break;  // underflow: the compiler rejects the program
+            } else if (ar == 1) {
+                Item it = stk.back();
+                stk.pop_back();
+                if (it.key && it.ops < kChainMaxOps) {
+                    it.key = chain_extend(it.key, it.ops, unop_of(op));
+                    it.ops++;
+                    ++(*counts)[it.key];
+                } else {
+                    it.key = 0;
+                }
+                stk.push_back(it);
+            } else {
+                stk.resize(stk.size() - (size_t)ar);
+                stk.push_back(Item{0, 0});
+            }
+        }
+    }
+}
+
+int chain_ops_of(uint64_t key) {
+    int ops = 0;
+    while (ops < kChainMaxOps && ((key >> (16 + 5 * ops)) & 31u)) ++ops;
+    return ops;
+}
+
+bool compile_range(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                   int32_t stride, int32_t n, int32_t ncol, bool want_const_pos, Tape* out, std::string* err,
+                   const DerivedColumns* derived);
+
+}  // namespace
+
+void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                  int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived) {
+    if (!derived || derived->cap <= 0 || (int32_t)derived->cols.size() >= derived->cap) return;
+    threads = std::max(1, std::min(threads, (n + 1023) / 1024));
+    std::vector<std::unordered_map<uint64_t, uint32_t>> counts((size_t)threads);
+    if (threads == 1) {
+        count_chains(st, phenotype, ephemeral, size, stride, 0, n, ncol, &counts[0]);
+    } else {
+        std::vector<std::thread> pool;
+        const int32_t chunk = (n + threads - 1) / threads;
+        for (int t = 0; t < threads; ++t) {
+            const int32_t lo = t * chunk, hi = std::min(n, lo + chunk);
+            if (lo < hi)
+                pool.emplace_back([&, t, lo, hi] {
+                    count_chains(st, phenotype, ephemeral, size, stride, lo, hi, ncol, &counts[(size_t)t]);
+                });
+        }
+        for (auto& th : pool) th.join();
+        for (int t = 1; t < threads; ++t)
+            for (const auto& kv : counts[(size_t)t]) counts[0][kv.first] += kv.second;
+    }
+    struct Cand { uint64_t key; uint32_t count; int ops; };
+    std::vector<Cand> cand;
+    cand.reserve(counts[0].size());
+    for (const auto& kv : counts[0])
+        if (!derived->index.count(kv.first)) cand.push_back(Cand{kv.first, kv.second, chain_ops_of(kv.first)});
+    std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) {
+        if (a.count != b.count) return a.count > b.count;
+        if (a.ops != b.ops) return a.ops < b.ops;
+        return a.key < b.key;
+    });
+    for (const Cand& c : cand) {
+        if ((int32_t)derived->cols.size() >= derived->cap) break;
+        const int last = c.ops - 1;
+        const uint32_t u = (uint32_t)((c.key >> (16 + 5 * last)) & 31u) - 1u;
+        int32_t src;
+        if (last == 0) {
+            src = (int32_t)(c.key & 0xffffu) - 1;
+        } else {
+            const uint64_t prefix = c.key & ~((uint64_t)31 << (16 + 5 * last));
+            src = derived->find(prefix);
+            if (src < 0) continue;  // the prefix got no column
+        }
+        derived->index.emplace(c.key, derived->base + (int32_t)derived->cols.size());
+        derived->cols.push_back(DerivedColumns::Col{u, src});
+    }
+}
+
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
                         Tape* out, std::string* err, DerivedColumns* derived) {
+    if (derived) plan_derived(st, phenotype, ephemeral, size, stride, n, ncol, 1, derived);
+    return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived);
+}
+
+namespace {
+bool compile_range(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                   int32_t stride, int32_t n, int32_t ncol, bool want_const_pos, Tape* out, std::string* err,
+                   const DerivedColumns* derived) {
     Compiler c(st, ncol, want_const_pos);
     c.derived = derived;
     out->meta.reserve(out->meta.size() + (size_t)n);
@@ -402,6 +524,7 @@ bool compile_population(const SymbolTable& st, const int32_t* phenotype, const f
     out->stats.derived_uses += c.derived_uses;
     return true;
 }
+}  // namespace
 
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
@@ -410,8 +533,9 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
     out->code.clear();
     out->const_pos.clear();
     out->stats = CompileStats{};
+    if (derived) plan_derived(st, phenotype, ephemeral, size, stride, n, ncol, threads, derived);
     if (threads <= 1 || n < 4096)
-        return compile_population(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived);
+        return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived);
     std::vector<Tape> parts((size_t)threads);
     std::vector<std::string> errs((size_t)threads);
     std::vector<char> ok((size_t)threads, 1);
@@ -421,8 +545,8 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
         int32_t lo = t * chunk, hi = std::min(n, lo + chunk);
         if (lo >= hi) continue;
         pool.emplace_back([&, t, lo, hi] {
-            ok[t] = compile_population(st, phenotype + (size_t)lo * stride, ephemeral + (size_t)lo * stride,
-                                       size + lo, stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t], derived);
+            ok[t] = compile_range(st, phenotype + (size_t)lo * stride, ephemeral + (size_t)lo * stride, size + lo,
+                                  stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t], derived);
         });
     }
     for (auto& th : pool) th.join();

@@ -19,7 +19,6 @@
 #define PPI_TAPE_H
 
 #include <cstdint>
-#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -107,34 +106,39 @@ struct CompileStats {
 // node over such a leaf into another derived column (sin(cos(x3)) reads one
 // column).  The values are computed on the device by the same operator code, in
 // column order (a column's source is an input column or an earlier derived
-// column), so they are bit-identical to evaluating the nodes.  Shared by the
-// compiler's threads.
+// column), so they are bit-identical to evaluating the nodes.
+//
+// A chain is named by its key: bits 0-15 = input column + 1, then 5 bits per
+// operator, innermost first ((unary op + 1) << (16 + 5 i)), at most
+// kChainMaxOps operators.  plan_derived() counts the chains of a population in
+// parallel and gives columns to the most frequent chains not yet in the table
+// (ties: fewer operators, then smaller key; a chain's prefix always ranks
+// first), so the table is deterministic and, while compiling, read-only.
+constexpr int kChainMaxOps = 9;
+
 struct DerivedColumns {
     int32_t base = 0;  // first derived column (the dataset's column count)
     int32_t cap = 0;   // most derived columns (0: off)
     struct Col { uint32_t op; int32_t src; };
     std::vector<Col> cols;                          // column base + j = op(column src), src < base + j
-    std::unordered_map<uint64_t, int32_t> index;    // (op, src) -> column
-    std::mutex mu;
+    std::unordered_map<uint64_t, int32_t> index;    // chain key -> column
     void reset(int32_t b, int32_t c) {
         base = b;
         cap = c;
         cols.clear();
         index.clear();
     }
-    // Column holding op(x[src]), or -1 when the table is full.
-    int32_t lookup_or_add(uint32_t op, int32_t src) {
-        const uint64_t key = ((uint64_t)op << 32) | (uint32_t)src;
-        std::lock_guard<std::mutex> g(mu);
+    // Column holding the chain, or -1.
+    int32_t find(uint64_t key) const {
         auto it = index.find(key);
-        if (it != index.end()) return it->second;
-        if ((int32_t)cols.size() >= cap) return -1;
-        const int32_t col = base + (int32_t)cols.size();
-        cols.push_back(Col{op, src});
-        index.emplace(key, col);
-        return col;
+        return it == index.end() ? -1 : it->second;
     }
 };
+
+// Adds columns for the unary chains of programs [0, n) to `derived` (see above);
+// `threads` workers count the chains.
+void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
+                  int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived);
 
 struct Tape {
     std::vector<ProgMeta> meta;
@@ -144,14 +148,15 @@ struct Tape {
 };
 
 // Compiles programs [0, n) into `out` (appending).  ncol bounds attribute columns
-// (an out-of-range column evaluates as NaN).  With `derived`, unary operators
-// over attribute leaves read derived columns (see DerivedColumns).  Returns false
-// with `err` set if a program needs more than kMaxSlots register slots.
+// (an out-of-range column evaluates as NaN).  With `derived`, the population's
+// unary chains are planned first (plan_derived) and read from derived columns.
+// Returns false with `err` set if a program needs more than kMaxSlots register
+// slots.
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
                         Tape* out, std::string* err, DerivedColumns* derived = nullptr);
 
-// Multi-threaded front end: splits the population over `threads` workers.
+// Multi-threaded front end: plans the derived columns and compiles with `threads` workers.
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
                            int threads, Tape* out, std::string* err, DerivedColumns* derived = nullptr);
