@@ -68,6 +68,28 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
             "form": st.form, "derived_columns": st0.derived_columns, "upload_ms": round(upload_s * 1e3, 2)}
 
 
+def bench_generations(eng, w, gens):
+    """What a GE loop calling the drop-in sees per generation: a new population each time
+    (alternating between two seeded populations of the regression workload), compiled,
+    uploaded, evaluated and its error vector and best index copied back (ppi.cc:604-634).
+    Wall-clock time on the host, FP form; derived columns stay cached with the dataset."""
+    pops = [w.pop, ppiamd.workload("regression", seed=2).pop]
+    eng.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=w.protected,
+                  form=ppiamd.FORM_FP)
+    eng.set_dataset(w.data)
+    for p in pops:
+        eng.evaluate(p, alpha=1e-6, best_size=1)
+    up = ker = 0.0
+    t0 = time.perf_counter()
+    for g in range(gens):
+        eng.evaluate(pops[g % 2], alpha=1e-6, best_size=1)
+        st = eng.stats()
+        up += st.upload_ms
+        ker += st.kernel_ms
+    wall = (time.perf_counter() - t0) * 1e3 / gens
+    return {"ms_per_generation": round(wall, 3), "upload_ms": round(up / gens, 3), "device_ms": round(ker / gens, 3)}
+
+
 def bench_batched(steps, warmup):
     """64 uneven (population, dataset) pairs: one graph-captured persistent launch vs the plain
     engine evaluating the pairs one after another (device time, programs resident)."""
@@ -269,6 +291,13 @@ def main():
             wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(500), w.data, w.error_fn)
             pp = bench_engine(eng, wpp, ppiamd.FORM_PP, 1, 0)
             line["pp_gpops"] = round(pp["node_evals"] / (pp["ms_per_step"] * 1e-3) / 1e9, 3)
+            log("timing whole generations (compile + upload + evaluate + results)")
+            gen = bench_generations(eng, w, 10)
+            gen["gpops_end_to_end"] = round(fp["node_evals"] / (gen["ms_per_generation"] * 1e-3) / 1e9, 3)
+            gen["host_threads"] = min(16, os.cpu_count() or 1)
+            gen["note"] = ("a new 30,000-program population per generation through upload/launch/wait: host "
+                           "compile of the tape + H2D + kernels + D2H of the error vector and the best index")
+            line["generation"] = gen
             log("timing classification (COST, 30,000 programs x 150,000 rows)")
             wc = ppiamd.workload("classification")
             cl = bench_engine(eng, wc, ppiamd.FORM_FP, max(1, a.steps // 2), 1)

@@ -37,11 +37,17 @@ constexpr int kBlock = 64 * kWaves;
 // 4 rows per lane (96 VGPRs; at 4 rows a few VGPRs spill, still 4.5% faster
 // than 4 waves without spills: more waves hide more of the scalar dispatch);
 // 1 row per lane (the 16-slot fallback) takes 4; at 8 rows any bound spills,
-// so none is set.  The PDP form keeps its rows' features in VGPRs and is left
-// unbounded.
-template <int R>
+// so none is set.  A workgroup is one wave per SIMD, so the waves per SIMD are
+// also the workgroups per CU that fit its 160 KB of LDS (feature tile + exact-
+// math tables): the bound never asks for more, since spilling for waves that
+// LDS cannot hold only costs (32 feature slots at 4 rows: 4 waves).  The PDP
+// form keeps its rows' features in VGPRs and is left unbounded.
+template <int R, int F>
 struct MinWaves {
-    static constexpr int value = R == 2 || R == 4 ? 5 : (R == 1 ? 4 : 1);
+    static constexpr int pref = R == 2 || R == 4 ? 5 : (R == 1 ? 4 : 1);
+    static constexpr int lds = (F > 0 ? F : 1) * 64 * R * 4 + (int)sizeof(ExactTables) + 64;
+    static constexpr int fit = (160 * 1024) / lds;
+    static constexpr int value = pref < fit ? pref : (fit > 0 ? fit : 1);
 };
 
 template <int K> struct VecT { typedef float type __attribute__((ext_vector_type(K))); };
@@ -535,7 +541,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
 }
 
 template <int R, int F, int S, bool P, bool N, bool ROWS>
-__global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(EvalArgs A) {
+__global__ void __launch_bounds__(kBlock, (MinWaves<R, F>::value)) fp_tile_kernel(EvalArgs A) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     stage_math<N>();  // fp_tile_body's barrier follows
@@ -561,7 +567,7 @@ __global__ void __launch_bounds__(kBlock, MinWaves<R>::value) fp_tile_kernel(Eva
 // workgroup leaves the loop when the counter passes the end.
 // ---------------------------------------------------------------------------
 template <int R, int F, int S, bool P, bool N>
-__global__ void __launch_bounds__(kBlock, MinWaves<R>::value) batch_kernel(BatchArgs B) {
+__global__ void __launch_bounds__(kBlock, (MinWaves<R, F>::value)) batch_kernel(BatchArgs B) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     __shared__ int item_s;
