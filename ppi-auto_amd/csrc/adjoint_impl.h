@@ -150,7 +150,7 @@ __device__ __forceinline__ AdjJob load_job(const AdjJob* jobs, int i) {
 template <int R, int S, int PS, bool P, bool N>
 struct AdjCore {
     typedef Core<R, S, P, N> C;
-    typedef typename C::VS VS;
+    typedef typename C::Stk Stk;
     static constexpr int B = StoreCfg<PS>::B;
     static constexpr int LOGB = StoreCfg<PS>::LOGB;
     static constexpr uint32_t kNone = 0xffffffffu;
@@ -160,14 +160,14 @@ struct AdjCore {
     // then / else selector), kNone = not stored in this segment.
     template <class FeatT, class PSt>
     static __device__ __forceinline__ void step_rec(uint32_t c, uint32_t a, uint32_t sa, uint32_t sb,
-                                                    const FeatT& ft, float (&acc)[R], VS (&stk)[R], PSt& ps) {
+                                                    const FeatT& ft, float (&acc)[R], Stk& stk, PSt& ps) {
         const uint32_t h = c & 0xffu;
         if (h == H_TERN) {
             const uint32_t s1 = (c >> 8) & 31u, s2 = (c >> 13) & 31u;
             const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
+                const float v0 = acc[r], v1 = stk.get(r, s1), v2 = stk.get(r, s2);
                 const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
                 const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
                 const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
@@ -191,7 +191,7 @@ struct AdjCore {
                 op = h - H_BIN_S;
                 const uint32_t slot = (c >> 8) & 31u;
 #pragma unroll
-                for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
+                for (int r = 0; r < R; ++r) b[r] = stk.get(r, slot);
             }
             switch (op) {
 #define PPI_DBCASE(K)                                                                  \
@@ -243,7 +243,7 @@ struct AdjCore {
     // code[n] and rec[n] must be readable (look-ahead).
     template <class FeatT, class PSt>
     static __device__ __forceinline__ void forward(ConstPtr<Ins> code, ConstPtr<uint32_t> rec, uint32_t n,
-                                                   uint32_t seg, const FeatT& ft, float (&acc)[R], VS (&stk)[R],
+                                                   uint32_t seg, const FeatT& ft, float (&acc)[R], Stk& stk,
                                                    PSt& ps) {
         uint32_t nc = code[0].code, na = code[0].arg, nr = rec[0];
         for (uint32_t pc = 0; pc < n; ++pc) {
@@ -268,7 +268,7 @@ struct AdjFeat {
 template <int R, int F, int S, int PS, bool P, bool N>
 __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     typedef AdjCore<R, S, PS, P, N> AC;
-    typedef typename AC::VS VS;
+    typedef typename AC::Stk Stk;
     constexpr int B = AC::B;
     typedef typename VecT<kAdjSlots>::type VA;
     typedef typename VecT<kAdjMaxConsts>::type VG;
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
                 ConstPtr<AdjSeg> sq = as_const(A.segs) + (J.seg_off + s);
                 const uint32_t rb = sq->rev_begin, re = sq->rev_end, fe = sq->fwd_end;
                 float acc[R];
-                VS stk[R];
+                Stk stk;
                 AC::forward(code, rec, fe, s, ft, acc, stk, ps);
                 if (s == 0) {
 #pragma unroll

@@ -113,7 +113,7 @@ struct ppi_batch {
 
     std::vector<PairDev> pairs;
     std::vector<WorkItem> items;
-    int R = 4, F = 32, S = 12, grid = 0;
+    int R = 4, F = 32, S = 9, grid = 0;
     bool P = false, N = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -389,7 +389,7 @@ ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int3
         set_error(err);
         return PPI_E_PROGRAM;
     }
-    if (t.stats.max_slots > 16) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
+    if (t.stats.max_slots > kMaxSlots) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
     // New derived columns are computed now; the dataset buffer may move, which
     // the next build picks up (a built graph is dropped below).
     if (d.derived_ready < (int32_t)d.derived->cols.size()) {
@@ -437,7 +437,8 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     b->P = b->cfg.protected_ops != 0;
     b->N = b->cfg.native_math != 0;
     b->F = feature_slots(b->tape.stats.max_col);
-    b->S = b->tape.stats.max_slots <= 12 ? 12 : 16;
+    const int ms = b->tape.stats.max_slots;
+    b->S = ms <= 4 ? 4 : (ms <= 9 ? 9 : 16);
 
     std::vector<Ins> code = b->tape.code;
     code.insert(code.end(), kTapePad, Ins{H_END, kEmpty});
@@ -469,9 +470,17 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     cands.erase(std::remove_if(cands.begin(), cands.end(),
                                [&](int r) { return !batch_shape_supported(r, b->F, b->S); }),
                 cands.end());
+    if (cands.empty() && b->S == 4) {  // the 4-slot stack exists at 4 and 8 rows only
+        b->S = 9;
+        if (b->cfg.rows_per_lane > 0) cands.push_back(b->cfg.rows_per_lane);
+        else cands = {2, 4, 8};
+        cands.erase(std::remove_if(cands.begin(), cands.end(),
+                                   [&](int r) { return !batch_shape_supported(r, b->F, b->S); }),
+                    cands.end());
+    }
     if (cands.empty()) {
-        if (batch_shape_supported(2, 32, 16) && b->S == 16) { cands.push_back(2); b->F = 32; }
-        else if (batch_shape_supported(4, 0, 12)) { cands.push_back(4); b->F = 0; }
+        // Only the 16-slot programs reach here; they run at 2 rows with 32 feature slots.
+        if (b->S == 16 && b->F > 0 && b->F <= 32 && batch_shape_supported(2, 32, 16)) { cands.push_back(2); b->F = 32; }
         else { set_error("no batched kernel shape for these programs"); return PPI_E_PROGRAM; }
     }
     int64_t max_part = 0;

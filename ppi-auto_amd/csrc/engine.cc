@@ -198,8 +198,8 @@ static bool plan(ppi_engine* E, std::string* why) {
     const CompileStats& cs = E->tape.stats;
     int need_f = cs.max_col + 1;
     int F = need_f <= 8 ? 8 : (need_f <= 16 ? 16 : (need_f <= 32 ? 32 : 0));
-    int S = cs.max_slots <= 12 ? 12 : 16;
-    if (cs.max_slots > 16) {
+    int S = cs.max_slots <= 9 ? 9 : 16;
+    if (cs.max_slots > kMaxSlots) {
         *why = "program needs more than 16 register slots";
         return false;
     }
@@ -223,6 +223,8 @@ static bool plan(ppi_engine* E, std::string* why) {
         if (!found && shape_supported(1, 0, 16)) { R = 1; F = 0; S = 16; found = true; }
         if (!found) { *why = "no kernel shape"; return false; }
     }
+    // The 4-slot stack where the shape has one: 20 VGPRs fewer at 4 rows, 40 at 8.
+    if (S == 9 && cs.max_slots <= 4 && shape_supported(R, F, 4)) S = 4;
     E->form = form;
     E->R = R;
     E->F = F;
@@ -630,7 +632,7 @@ ppi_status ppi_engine_predict(ppi_engine* E, const int32_t* phenotype, const flo
         set_error(err);
         return PPI_E_PROGRAM;
     }
-    if (t.stats.max_slots > 16) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
+    if (t.stats.max_slots > kMaxSlots) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
     t.code.insert(t.code.end(), kTapePad, Ins{H_END, kEmpty});
     ppi_status s;
     size_t cap;

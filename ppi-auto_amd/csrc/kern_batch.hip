@@ -2,7 +2,8 @@
 #include "kernels_impl.h"
 
 namespace ppi {
-#define PPI_BATCH_SHAPES(X) X(2, 32, 12) X(4, 32, 12) X(8, 32, 12) X(4, 16, 12) X(8, 16, 12) X(2, 32, 16) X(4, 0, 12)
+#define PPI_BATCH_SHAPES(X) X(2, 32, 9) X(4, 32, 9) X(8, 32, 9) X(4, 16, 9) X(8, 16, 9) X(2, 32, 16) X(4, 0, 9) \
+    X(4, 32, 4) X(8, 32, 4) X(4, 16, 4) X(8, 16, 4) X(4, 0, 4)
 
 bool batch_shape_supported(int R, int F, int S) {
 #define PPI_X(r, f, s) if (R == r && F == f && S == s) return true;

@@ -11,17 +11,23 @@ hipError_t launch_fp_r8(const EvalArgs& A, int F, int S, bool P, bool N, bool ro
 bool shape_supported(int R, int F, int S) {
     return (R == 1 && F == 32 && S == 16) ||
            (R == 1 && F == 0 && S == 16) ||
-           (R == 2 && F == 8 && S == 12) ||
-           (R == 2 && F == 16 && S == 12) ||
-           (R == 2 && F == 32 && S == 12) ||
-           (R == 2 && F == 0 && S == 12) ||
+           (R == 2 && F == 8 && S == 9) ||
+           (R == 2 && F == 16 && S == 9) ||
+           (R == 2 && F == 32 && S == 9) ||
+           (R == 2 && F == 0 && S == 9) ||
            (R == 2 && F == 32 && S == 16) ||
-           (R == 4 && F == 8 && S == 12) ||
-           (R == 4 && F == 16 && S == 12) ||
-           (R == 4 && F == 32 && S == 12) ||
-           (R == 4 && F == 0 && S == 12) ||
-           (R == 8 && F == 8 && S == 12) ||
-           (R == 8 && F == 16 && S == 12);
+           (R == 4 && F == 8 && S == 9) ||
+           (R == 4 && F == 16 && S == 9) ||
+           (R == 4 && F == 32 && S == 9) ||
+           (R == 4 && F == 0 && S == 9) ||
+           (R == 8 && F == 8 && S == 9) ||
+           (R == 8 && F == 16 && S == 9) ||
+           (R == 4 && F == 8 && S == 4) ||
+           (R == 4 && F == 16 && S == 4) ||
+           (R == 4 && F == 32 && S == 4) ||
+           (R == 4 && F == 0 && S == 4) ||
+           (R == 8 && F == 8 && S == 4) ||
+           (R == 8 && F == 16 && S == 4);
 }
 
 hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st, bool rows) {

@@ -5,10 +5,10 @@ namespace ppi {
 hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st) {
     if (R == 1 && F == 32 && S == 16) return launch::launch_pdp_t<1, 32, 16>(A, P, N, st);
     if (R == 1 && F == 0 && S == 16) return launch::launch_pdp_t<1, 0, 16>(A, P, N, st);
-    if (R == 2 && F == 8 && S == 12) return launch::launch_pdp_t<2, 8, 12>(A, P, N, st);
-    if (R == 2 && F == 16 && S == 12) return launch::launch_pdp_t<2, 16, 12>(A, P, N, st);
-    if (R == 2 && F == 32 && S == 12) return launch::launch_pdp_t<2, 32, 12>(A, P, N, st);
-    if (R == 2 && F == 0 && S == 12) return launch::launch_pdp_t<2, 0, 12>(A, P, N, st);
+    if (R == 2 && F == 8 && S == 9) return launch::launch_pdp_t<2, 8, 9>(A, P, N, st);
+    if (R == 2 && F == 16 && S == 9) return launch::launch_pdp_t<2, 16, 9>(A, P, N, st);
+    if (R == 2 && F == 32 && S == 9) return launch::launch_pdp_t<2, 32, 9>(A, P, N, st);
+    if (R == 2 && F == 0 && S == 9) return launch::launch_pdp_t<2, 0, 9>(A, P, N, st);
     if (R == 2 && F == 32 && S == 16) return launch::launch_pdp_t<2, 32, 16>(A, P, N, st);
     return hipErrorInvalidValue;
 }
@@ -16,10 +16,10 @@ hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hi
 bool pdp_shape_supported(int R, int F, int S) {
     return (R == 1 && F == 32 && S == 16) ||
            (R == 1 && F == 0 && S == 16) ||
-           (R == 2 && F == 8 && S == 12) ||
-           (R == 2 && F == 16 && S == 12) ||
-           (R == 2 && F == 32 && S == 12) ||
-           (R == 2 && F == 0 && S == 12) ||
+           (R == 2 && F == 8 && S == 9) ||
+           (R == 2 && F == 16 && S == 9) ||
+           (R == 2 && F == 32 && S == 9) ||
+           (R == 2 && F == 0 && S == 9) ||
            (R == 2 && F == 32 && S == 16);
 }
 }  // namespace ppi

@@ -33,18 +33,20 @@ constexpr int kBlock = 64 * kWaves;
 
 // Waves per SIMD the register allocator must leave room for (the second
 // __launch_bounds__ argument caps VGPRs at 512 / waves) in the FP and batched
-// forms.  Measured on gfx950 with the regression workload: 5 waves at 2 and at
-// 4 rows per lane (96 VGPRs; at 4 rows a few VGPRs spill, still 4.5% faster
-// than 4 waves without spills: more waves hide more of the scalar dispatch);
-// 1 row per lane (the 16-slot fallback) takes 4; at 8 rows any bound spills,
-// so none is set.  A workgroup is one wave per SIMD, so the waves per SIMD are
-// also the workgroups per CU that fit its 160 KB of LDS (feature tile + exact-
-// math tables): the bound never asks for more, since spilling for waves that
-// LDS cannot hold only costs (32 feature slots at 4 rows: 4 waves).  The PDP
-// form keeps its rows' features in VGPRs and is left unbounded.
-template <int R, int F>
+// forms.  Measured on gfx950 with the regression workload (ms per launch):
+// 4 rows per lane with the 4-slot stack 6 waves (80 VGPRs, no spills; 6.8%
+// faster than 5 waves, and 7 waves spill), with the 9-slot stack 5 (96 VGPRs;
+// 10% faster than 4 waves without the bound); 2 rows per lane 5 (76 VGPRs: 6
+// waves run); 1 row per lane (the 16-slot fallback) takes 4; at 8 rows any
+// bound spills, so none is set.  More waves hide more of the scalar dispatch.
+// A workgroup is one wave per SIMD, so the waves per SIMD are also the
+// workgroups per CU that fit its 160 KB of LDS (feature tile + exact-math
+// tables): the bound never asks for more, since spilling for waves that LDS
+// cannot hold only costs (32 feature slots at 4 rows: 4 waves).  The PDP form
+// keeps its rows' features in VGPRs and is left unbounded.
+template <int R, int F, int S>
 struct MinWaves {
-    static constexpr int pref = R == 2 || R == 4 ? 5 : (R == 1 ? 4 : 1);
+    static constexpr int pref = R == 4 ? (S <= 4 ? 6 : 5) : (R == 2 ? 5 : (R == 1 ? 4 : 1));
     static constexpr int lds = (F > 0 ? F : 1) * 64 * R * 4 + (int)sizeof(ExactTables) + 64;
     static constexpr int fit = (160 * 1024) / lds;
     static constexpr int value = pref < fit ? pref : (fit > 0 ? fit : 1);
@@ -129,17 +131,37 @@ __device__ __forceinline__ auto make_feat(const EvalArgs& A, const float* lds_la
     else return GlobalFeat<R>{A.X, A.ldx, row};
 }
 
+// Register stack of S slots for each of a lane's R rows.  The slot index is
+// wave-uniform.  With at least 9 elements per register tuple the backend
+// indexes it directly (s_set_gpr_idx_on + one v_mov per row) instead of
+// expanding into compare/select chains, so stacks of 9 or more slots keep one
+// tuple per row, and smaller stacks keep all rows in one tuple of S*R elements
+// (row r of slot k at k*R + r: one index register per access, the row as a
+// constant offset).  Slots cost S*R VGPRs, so the smallest stack that holds
+// the population's programs frees registers for occupancy.
+template <int R, int S, bool Joint = (S < 9)>
+struct RegStack {
+    static_assert(S >= 9, "per-row register stacks need >= 9 slots for indexed moves");
+    typename VecT<S>::type v[R];
+    __device__ __forceinline__ float get(int r, uint32_t slot) const { return v[r][slot]; }
+    __device__ __forceinline__ void set(int r, uint32_t slot, float x) { v[r][slot] = x; }
+};
+
+template <int R, int S>
+struct RegStack<R, S, true> {
+    static_assert(S * R >= 9, "joint register stacks need >= 9 elements for indexed moves");
+    typename VecT<S * R>::type v;
+    __device__ __forceinline__ float get(int r, uint32_t slot) const { return v[slot * R + r]; }
+    __device__ __forceinline__ void set(int r, uint32_t slot, float x) { v[slot * R + r] = x; }
+};
+
 template <int R, int S, bool P, bool N>
 struct Core {
-    typedef typename VecT<S>::type VS;
+    typedef RegStack<R, S> Stk;
 
-    // The slot index is wave-uniform.  With at least 9 elements per vector the
-    // backend indexes the register tuple directly (s_set_gpr_idx_on + one v_mov
-    // per row) instead of expanding into compare/select chains.
-    static_assert(S >= 9, "register stack vectors need >= 9 elements for indexed moves");
-    static __device__ __forceinline__ void push(VS (&stk)[R], const float (&acc)[R], uint32_t slot) {
+    static __device__ __forceinline__ void push(Stk& stk, const float (&acc)[R], uint32_t slot) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) stk[r][slot] = acc[r];
+        for (int r = 0; r < R; ++r) stk.set(r, slot, acc[r]);
     }
 
     // Executes instruction (c, a); H_END calls on_end().  The handler's category
@@ -152,7 +174,7 @@ struct Core {
     // vector to scratch memory (17% slower there).
     template <class FeatT, class EndF>
     static __device__ __forceinline__ void step(uint32_t c, uint32_t a, const FeatT& ft, float (&acc)[R],
-                                                VS (&stk)[R], EndF&& on_end) {
+                                                Stk& stk, EndF&& on_end) {
         const uint32_t h = c & 0xffu;
         const float cv = __uint_as_float(a);
         if (h >= H_BIN_C) {
@@ -162,7 +184,7 @@ struct Core {
                 op = h - H_BIN_S;
                 const uint32_t slot = (c >> 8) & 31u;
 #pragma unroll
-                for (int r = 0; r < R; ++r) b[r] = stk[r][slot];
+                for (int r = 0; r < R; ++r) b[r] = stk.get(r, slot);
             } else if (h >= H_BIN_A) {
                 op = h - H_BIN_A;
                 ft.get(a, b);
@@ -235,7 +257,7 @@ struct Core {
                 const uint32_t wc = (c >> 18) & 3u, wt = (c >> 20) & 3u, we = (c >> 22) & 3u;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const float v0 = acc[r], v1 = stk[r][s1], v2 = stk[r][s2];
+                    const float v0 = acc[r], v1 = stk.get(r, s1), v2 = stk.get(r, s2);
                     const float cd = wc == 0 ? v0 : (wc == 1 ? v1 : v2);
                     const float th = wt == 0 ? v0 : (wt == 1 ? v1 : v2);
                     const float el = we == 0 ? v0 : (we == 1 ? v1 : v2);
@@ -254,7 +276,7 @@ struct Core {
     // One program from scalar-loaded instructions (one instruction of look-ahead).
     template <class FeatT>
     static __device__ __forceinline__ void run(ConstPtr<Ins> code, uint32_t n, const FeatT& ft, float (&acc)[R],
-                                               VS (&stk)[R]) {
+                                               Stk& stk) {
         uint32_t nc = code[0].code, na = code[0].arg;
         for (uint32_t pc = 0; pc < n; ++pc) {
             const uint32_t c = nc, a = na;
@@ -512,7 +534,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
     int k = (int)(gbeg - blk);
     int p = pa;
     float acc[R];
-    typename C::VS stk[R];
+    typename C::Stk stk;
     for (; blk < gend; blk += 64) {
         const uint32_t nc = code[blk + 64 + lane].code, na = code[blk + 64 + lane].arg;
         const int kend = (int)min<int64_t>(64, gend - blk);
@@ -541,7 +563,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
 }
 
 template <int R, int F, int S, bool P, bool N, bool ROWS>
-__global__ void __launch_bounds__(kBlock, (MinWaves<R, F>::value)) fp_tile_kernel(EvalArgs A) {
+__global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S>::value)) fp_tile_kernel(EvalArgs A) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     stage_math<N>();  // fp_tile_body's barrier follows
@@ -567,7 +589,7 @@ __global__ void __launch_bounds__(kBlock, (MinWaves<R, F>::value)) fp_tile_kerne
 // workgroup leaves the loop when the counter passes the end.
 // ---------------------------------------------------------------------------
 template <int R, int F, int S, bool P, bool N>
-__global__ void __launch_bounds__(kBlock, (MinWaves<R, F>::value)) batch_kernel(BatchArgs B) {
+__global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S>::value)) batch_kernel(BatchArgs B) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     __shared__ int item_s;
@@ -709,7 +731,7 @@ __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
             bool valid[R];
             tile_rows<R>(A, row0, lane, row, y, valid);
             float acc[R];
-            typename C::VS stk[R];
+            typename C::Stk stk;
             if constexpr (F > 0) {
                 typename VgprFeat<R, F>::VF xf[R];
                 tile_features<R, F>(A, row, xf);
@@ -818,7 +840,7 @@ __global__ void __launch_bounds__(kBlock) predict_kernel(EvalArgs A, float* out)
     tile_rows<1>(A, row0, lane, row, y, valid);
     const ProgMeta m = load_meta(A.meta, 0);
     float acc[1] = {__builtin_nanf("")};
-    typename C::VS stk[1];
+    typename C::Stk stk;
     if (!(m.flags & kEmpty)) {
         if constexpr (F > 0) {
             typename VgprFeat<1, F>::VF xf[1];
