@@ -65,7 +65,8 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
     return {"ms_per_step": ms, "wall_s": wall, "main_kernel_ms": st.main_kernel_ms,
             "node_evals": st.node_evals, "executed_nodes": st.executed_nodes,
             "rows_per_lane": st.rows_per_lane, "programs_per_tile": st.programs_per_tile,
-            "form": st.form, "derived_columns": st0.derived_columns, "upload_ms": round(upload_s * 1e3, 2)}
+            "form": st.form, "derived_columns": st0.derived_columns, "folded_subtrees": st0.folded_subtrees,
+            "folded_nodes": st0.folded_nodes, "upload_ms": round(upload_s * 1e3, 2)}
 
 
 def bench_generations(eng, w, gens):
@@ -261,10 +262,12 @@ def main():
                        "node_evals_per_step": fp["node_evals"],
                        "executed_tape_ins_x_rows_per_step": fp["executed_nodes"],
                        "derived_columns": fp["derived_columns"],
+                       "folded_subtrees": fp["folded_subtrees"], "folded_nodes": fp["folded_nodes"],
                        "upload_ms": fp["upload_ms"],
-                       "upload_note": "one-time per population, outside the timed steps: tape compile, H2D "
-                                      "copy, and the derived columns (chains of unary operators over input "
-                                      "columns, cached per dataset) it adds"},
+                       "upload_note": "one-time per population, outside the timed steps: tape compile, constant "
+                                      "subtrees evaluated once on the device, H2D copy, and the derived columns "
+                                      "(chains of unary operators over input columns, cached per dataset) it "
+                                      "adds; the generation leg times all of it per generation"},
             "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4), "peak": VALU_PEAK / 1e12,
                          "unit": "Tnode/s (1 node-evaluation = 1 FP32 VALU lane-op)",
                          "frac": round(achieved / VALU_PEAK, 5),

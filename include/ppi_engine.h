@@ -123,6 +123,8 @@ typedef struct ppi_config {
     int32_t programs_per_tile;
     int32_t ordered_pass_rows; /* PPI_FORM_ORDERED: rows per pass of stored row errors (0: min(16 GiB, half the
                                   free device memory) of errors) */
+    int32_t no_constant_folding; /* 1: evaluate constant subtrees on every row instead of once per upload
+                                    (results are the same bits either way) */
 } ppi_config;
 
 typedef struct ppi_stats {
@@ -137,6 +139,8 @@ typedef struct ppi_stats {
     int32_t programs_per_tile;
     int32_t max_slots;      /* register stack slots used by the deepest program */
     int32_t derived_columns;/* unary-of-input columns precomputed for the FP form (this dataset) */
+    int32_t folded_subtrees;/* constant subtrees of the last program set evaluated once at upload */
+    int32_t folded_nodes;   /* operator nodes inside them */
 } ppi_stats;
 
 typedef struct ppi_engine ppi_engine;

@@ -83,6 +83,7 @@ struct ppi_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    FoldBuffers folds;  // fold programs of the population being added
     int dev_cu = 0, dev_max_wg = 0;
     size_t dev_lds = 0;
 
@@ -309,6 +310,7 @@ void ppi_batch_destroy(ppi_batch* b) {
         if (d.dDerSrc) (void)hipFree(d.dDerSrc);
     }
     if (b->dCost) (void)hipFree(b->dCost);
+    b->folds.release();
     if (b->hRaw) (void)hipHostFree(b->hRaw);
     for (auto& ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -385,10 +387,12 @@ ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int3
     std::string err;
     int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
     if (!compile_population_mt(b->symtab, phenotype, ephemeral, size, stride, n, d.ncol, false, threads, &t, &err,
-                               d.derived.get())) {
+                               d.derived.get(), b->cfg.no_constant_folding == 0)) {
         set_error(err);
         return PPI_E_PROGRAM;
     }
+    if (ppi_status fs = evaluate_folds(&t, &b->folds, b->cfg.protected_ops != 0, b->cfg.native_math != 0, b->stream))
+        return fs;
     if (t.stats.max_slots > kMaxSlots) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
     // New derived columns are computed now; the dataset buffer may move, which
     // the next build picks up (a built graph is dropped below).

@@ -135,6 +135,34 @@ static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+void FoldBuffers::release() {
+    if (meta) (void)hipFree(meta);
+    if (code) (void)hipFree(code);
+    if (out) (void)hipFree(out);
+    meta = nullptr;
+    code = nullptr;
+    out = nullptr;
+    cap_meta = cap_code = cap_out = 0;
+}
+
+ppi_status evaluate_folds(Tape* t, FoldBuffers* B, bool P, bool N, hipStream_t st) {
+    const FoldSet& F = t->folds;
+    const size_t n = F.meta.size();
+    if (n == 0) return PPI_OK;
+    ppi_status s;
+    if ((s = ensure(&B->meta, &B->cap_meta, n))) return s;
+    if ((s = ensure(&B->code, &B->cap_code, F.code.size()))) return s;
+    if ((s = ensure(&B->out, &B->cap_out, n))) return s;
+    B->host.resize(n);
+    PPI_HIP(hipMemcpyAsync(B->meta, F.meta.data(), n * sizeof(ProgMeta), hipMemcpyHostToDevice, st));
+    PPI_HIP(hipMemcpyAsync(B->code, F.code.data(), F.code.size() * sizeof(Ins), hipMemcpyHostToDevice, st));
+    PPI_HIP(launch_fold(B->meta, B->code, (int32_t)n, B->out, P, N, st));
+    PPI_HIP(hipMemcpyAsync(B->host.data(), B->out, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    PPI_HIP(hipStreamSynchronize(st));
+    apply_folds(t, B->host.data());
+    return PPI_OK;
+}
+
 // util::PickNBest (Util.h:158-173): k-th best = k-th smallest error, ties to the
 // larger index; `indices` maps positions to program indices when given.
 void pick_n_best(int n, int32_t* best, int num, const float* errors, const int32_t* indices) {
@@ -302,6 +330,7 @@ void ppi_engine_destroy(ppi_engine* E) {
                    E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop};
     for (void* p : dev)
         if (p) (void)hipFree(p);
+    E->folds.release();
     if (E->hRaw) (void)hipHostFree(E->hRaw);
     if (E->hVec) (void)hipHostFree(E->hVec);
     for (auto& ev : E->ev)
@@ -418,10 +447,12 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
     const bool fp_form = E->cfg.form == PPI_FORM_AUTO || E->cfg.form == PPI_FORM_FP || E->cfg.form == PPI_FORM_ORDERED;
     DerivedColumns* der = (fp_form && E->derived.cap > 0) ? &E->derived : nullptr;
     if (!compile_population_mt(E->symtab, phenotype, ephemeral, size, stride, n, E->ncol, false, threads, &E->tape, &err,
-                               der)) {
+                               der, E->cfg.no_constant_folding == 0)) {
         set_error(err);
         return PPI_E_PROGRAM;
     }
+    ppi_status fs = evaluate_folds(&E->tape, &E->folds, E->cfg.protected_ops != 0, E->cfg.native_math != 0, E->stream);
+    if (fs) return fs;
     E->tape.code.insert(E->tape.code.end(), kTapePad, Ins{H_END, kEmpty});  // window / look-ahead padding
     E->nprog = n;
     reference_geometry(E);
@@ -477,6 +508,8 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
     E->stats.node_evals = (double)E->tape.stats.total_size * E->nlin;
     E->stats.executed_nodes = (double)E->tape.stats.total_code * E->nlin;
     E->stats.max_slots = E->tape.stats.max_slots;
+    E->stats.folded_subtrees = (int32_t)E->tape.folds.meta.size();
+    E->stats.folded_nodes = (int32_t)E->tape.stats.folded_nodes;
     return PPI_OK;
 }
 

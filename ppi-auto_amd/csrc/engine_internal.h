@@ -19,6 +19,20 @@ int32_t derived_capacity(int64_t ldx, int32_t ncol);
 ppi_status stage_derived_columns(float** X, size_t* cap_floats, int64_t ldx, int32_t ncol, DerivedColumns& table,
                                  int32_t* ready, uint32_t** dOps, int32_t** dSrc, size_t* cap_list, bool P, bool N,
                                  hipStream_t st);
+// Device buffers of a tape's fold programs (tape.h: FoldSet), kept across uploads.
+struct FoldBuffers {
+    ProgMeta* meta = nullptr;
+    size_t cap_meta = 0;
+    Ins* code = nullptr;
+    size_t cap_code = 0;
+    float* out = nullptr;
+    size_t cap_out = 0;
+    std::vector<float> host;
+    void release();
+};
+// Evaluates the tape's folds on stream `st` with the interpreters' operator code
+// (P = PROTECTED, N = NATIVE) and writes their values into the tape (apply_folds).
+ppi_status evaluate_folds(Tape* t, FoldBuffers* B, bool P, bool N, hipStream_t st);
 void pick_n_best(int n, int32_t* best, int num, const float* errors, const int32_t* indices);
 void best_acc(const float* vec, int P, int local2, int32_t* index, int32_t* best_size);
 }  // namespace ppi
@@ -81,6 +95,8 @@ struct ppi_engine {
     uint32_t* dOStop = nullptr;
     size_t cap_osum = 0;
     int nrb_pass = 1;
+    // fold programs of the last upload (constant subtrees, evaluated once)
+    ppi::FoldBuffers folds;
 
     // reference geometry (accelerator.cc:272-355) and the launch plan
     int dp_local = 1, dp_groups = 1, pdp_local = 1, best_local = 1;

@@ -66,6 +66,16 @@ hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_fold(const ProgMeta* meta, const Ins* code, int32_t n, float* out, bool P, bool N, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid((unsigned)((n + dev::kWaves - 1) / dev::kWaves));
+    if (P && N) hipLaunchKernelGGL((dev::fold_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, meta, code, n, out);
+    else if (P) hipLaunchKernelGGL((dev::fold_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, meta, code, n, out);
+    else if (N) hipLaunchKernelGGL((dev::fold_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, meta, code, n, out);
+    else hipLaunchKernelGGL((dev::fold_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, meta, code, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_derive(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first,
                          int32_t count, bool P, bool N, hipStream_t st) {
     if (count <= 0) return hipSuccess;

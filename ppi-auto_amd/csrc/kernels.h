@@ -163,6 +163,8 @@ hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st);
 hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st);
 hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+// Folded constant subtrees: out[j] = value of fold program j (meta[j], code).
+hipError_t launch_fold(const ProgMeta* meta, const Ins* code, int32_t n, float* out, bool P, bool N, hipStream_t st);
 // Derived columns first .. first+count-1 of X: column first+j = unary(ops[j], column srcs[j]).
 hipError_t launch_derive(float* X, int64_t ldx, const uint32_t* ops, const int32_t* srcs, int32_t first,
                          int32_t count, bool P, bool N, hipStream_t st);

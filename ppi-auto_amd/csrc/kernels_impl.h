@@ -869,6 +869,28 @@ __global__ void derive_kernel(float* X, int64_t ldx, const uint32_t* ops, const 
         X[(int64_t)(first + j) * ldx + row] = unary<P, N>(ops[j], X[(int64_t)srcs[j] * ldx + row]);
 }
 
+// Folded constant subtrees (FoldSet): one wave per fold program, run by the
+// interpreters' own step code (wave-uniform, like a program of the FP form), so
+// a fold's value has the bits the unfolded program would compute on any row.
+// Fold programs read no attribute; the accessor below is never called.
+struct NoFeat {
+    __device__ __forceinline__ void get(uint32_t, float (&b)[1]) const { b[0] = __builtin_nanf(""); }
+};
+
+template <bool P, bool N>
+__global__ void __launch_bounds__(kBlock) fold_kernel(const ProgMeta* meta, const Ins* code, int32_t n, float* out) {
+    typedef Core<1, kMaxSlots, P, N> C;
+    stage_math<N>();
+    __syncthreads();
+    const int64_t f = (int64_t)blockIdx.x * kWaves + wave_index();
+    if (f >= n) return;  // whole wave
+    const ProgMeta m = load_meta(meta, (int32_t)f);
+    float acc[1] = {__builtin_nanf("")};
+    typename C::Stk stk;
+    C::run(as_const(code) + m.code_off, m.ncode, NoFeat{}, acc, stk);
+    if ((threadIdx.x & 63) == 0) out[f] = acc[0];
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

@@ -95,6 +95,7 @@ struct Node {
     bool is_tconst;
     int16_t regs;
     uint64_t chain;   // leaf: chain key of an input or derived column (DerivedColumns), 0 = none
+    int32_t fold;     // leaf: 1 + index of the operator node of the constant subtree it stands for, 0 = none
 };
 
 // Chain key of unary operator u applied to the chain `key` of `ops` operators.
@@ -110,6 +111,11 @@ struct Compiler {
     bool want_const_pos;
     const DerivedColumns* derived = nullptr;
     int64_t derived_uses = 0;
+    bool fold_consts = false;  // constant subtrees go to Tape::folds
+    bool fold_too_deep = false;
+    int64_t folded_nodes = 0;
+    std::vector<std::pair<int32_t, uint32_t>> fold_pending;  // (subtree root, placeholder instruction)
+    Tape fold_scratch;
     std::vector<Node> nodes;
     std::vector<int32_t> vstack;
     struct Frame { int32_t node, base, phase, o[3]; };
@@ -173,6 +179,18 @@ struct Compiler {
                 }
                 n.op = op;
                 for (int k = 0; k < ar; ++k) { n.ch[k] = vstack.back(); vstack.pop_back(); }
+                bool constant = fold_consts;
+                for (int k = 0; k < ar && constant; ++k) {
+                    const Node& c = nodes[(size_t)n.ch[k]];
+                    constant = c.op < 0 && !c.leaf_attr;
+                }
+                if (constant) {
+                    // Every operand is a constant: the operator node keeps its
+                    // children (a folded child contributes its whole subtree) and a
+                    // constant leaf standing for its value takes its place.
+                    for (int k = 0; k < ar; ++k)
+                        if (nodes[(size_t)n.ch[k]].fold) n.ch[k] = nodes[(size_t)n.ch[k]].fold - 1;
+                }
                 // Sethi-Ullman register need.
                 if (ar == 1) {
                     n.regs = nodes[n.ch[0]].regs;
@@ -189,13 +207,51 @@ struct Compiler {
                     std::sort(r, r + 3, [](int16_t x, int16_t y) { return x > y; });
                     n.regs = (int16_t)std::max<int>(r[0], std::max<int>(1 + r[1], 2 + r[2]));
                 }
+                if (constant) {
+                    nodes.push_back(n);
+                    Node leaf{};
+                    leaf.op = -1;
+                    leaf.pos = i;
+                    leaf.fold = (int32_t)nodes.size();
+                    nodes.push_back(leaf);
+                    vstack.push_back((int32_t)nodes.size() - 1);
+                    continue;
+                }
             }
             nodes.push_back(n);
             vstack.push_back((int32_t)nodes.size() - 1);
         }
         if (vstack.empty()) return -1;
         root_node = vstack.back();
-        return emit(root_node, out);
+        fold_pending.clear();
+        const int slots = emit(root_node, out);
+        if (!fold_pending.empty()) emit_folds(out);
+        return slots;
+    }
+
+    // The constant subtrees the program's tape refers to, each compiled into a
+    // program of its own in out->folds.
+    void emit_folds(Tape* out) {
+        FoldSet& F = out->folds;
+        const auto pending = fold_pending;
+        for (const auto& fp : pending) {
+            fold_scratch.code.clear();
+            const int slots = emit(fp.first, &fold_scratch);
+            if (slots > kMaxSlots) fold_too_deep = true;
+            ProgMeta m{};
+            m.code_off = (uint32_t)F.code.size();
+            m.ncode = (uint32_t)fold_scratch.code.size();
+            m.flags = (uint32_t)slots << 8;
+            for (const Ins& in : fold_scratch.code) {
+                const uint32_t h = in.code & 0xffu;
+                folded_nodes += (h >= H_UN || h == H_TERN) ? 1 : 0;
+            }
+            F.code.insert(F.code.end(), fold_scratch.code.begin(), fold_scratch.code.end());
+            F.code.push_back(Ins{H_END, 0});
+            F.meta.push_back(m);
+            F.at.push_back(fp.second);
+        }
+        fold_pending.clear();
     }
 
     int pending_save = -1;
@@ -220,21 +276,29 @@ struct Compiler {
         if (track) side[node] = s;
     }
 
+    // A leaf's argument; a folded subtree's leaf gets a placeholder that
+    // apply_folds() replaces with the subtree's value.
+    uint32_t leaf_arg(const Node& n, Tape* out) {
+        if (!n.fold) return n.arg;
+        fold_pending.emplace_back(n.fold - 1, (uint32_t)out->code.size());
+        return 0;
+    }
+
     void emit_load(const Node& n, Tape* out) {
         uint32_t cpos = n.is_tconst ? (uint32_t)n.pos + 1 : 0;
         if (pending_save >= 0) {
             uint32_t h = n.leaf_attr ? H_PUSH_A : H_PUSH_C;
-            put(out, h | ((uint32_t)pending_save << 8), n.arg, cpos);
+            put(out, h | ((uint32_t)pending_save << 8), leaf_arg(n, out), cpos);
             max_slot = std::max(max_slot, pending_save + 1);
             pending_save = -1;
         } else {
-            put(out, n.leaf_attr ? H_LOAD_A : H_LOAD_C, n.arg, cpos);
+            put(out, n.leaf_attr ? H_LOAD_A : H_LOAD_C, leaf_arg(n, out), cpos);
         }
     }
 
     void emit_fold(uint32_t bop, const Node& leaf, Tape* out) {
         uint32_t cpos = leaf.is_tconst ? (uint32_t)leaf.pos + 1 : 0;
-        put(out, (leaf.leaf_attr ? H_BIN_A : H_BIN_C) + bop, leaf.arg, cpos);
+        put(out, (leaf.leaf_attr ? H_BIN_A : H_BIN_C) + bop, leaf_arg(leaf, out), cpos);
     }
 
     int emit(int32_t root, Tape* out) {
@@ -407,9 +471,13 @@ int chain_ops_of(uint64_t key) {
 
 bool compile_range(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                    int32_t stride, int32_t n, int32_t ncol, bool want_const_pos, Tape* out, std::string* err,
-                   const DerivedColumns* derived);
+                   const DerivedColumns* derived, bool fold);
 
 }  // namespace
+
+void apply_folds(Tape* t, const float* value) {
+    for (size_t j = 0; j < t->folds.at.size(); ++j) t->code[t->folds.at[j]].arg = fbits(value[j]);
+}
 
 void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                   int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived) {
@@ -461,17 +529,18 @@ void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* 
 
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                        Tape* out, std::string* err, DerivedColumns* derived) {
+                        Tape* out, std::string* err, DerivedColumns* derived, bool fold) {
     if (derived) plan_derived(st, phenotype, ephemeral, size, stride, n, ncol, 1, derived);
-    return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived);
+    return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived, fold);
 }
 
 namespace {
 bool compile_range(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                    int32_t stride, int32_t n, int32_t ncol, bool want_const_pos, Tape* out, std::string* err,
-                   const DerivedColumns* derived) {
+                   const DerivedColumns* derived, bool fold) {
     Compiler c(st, ncol, want_const_pos);
     c.derived = derived;
+    c.fold_consts = fold && !want_const_pos;
     out->meta.reserve(out->meta.size() + (size_t)n);
     for (int32_t i = 0; i < n; ++i) {
         ProgMeta m{};
@@ -502,9 +571,9 @@ bool compile_range(const SymbolTable& st, const int32_t* phenotype, const float*
             slots = 0;
             out->stats.n_invalid++;
         }
-        if (slots > kMaxSlots) {
-            if (err) *err = "program " + std::to_string(i) + " needs " + std::to_string(slots) +
-                            " register slots (max " + std::to_string(kMaxSlots) + ")";
+        if (slots > kMaxSlots || c.fold_too_deep) {
+            if (err) *err = "program " + std::to_string(i) + " needs more than " + std::to_string(kMaxSlots) +
+                            " register slots";
             return false;
         }
         m.ncode = (uint32_t)out->code.size() - m.code_off;
@@ -522,20 +591,23 @@ bool compile_range(const SymbolTable& st, const int32_t* phenotype, const float*
             out->stats.max_col = std::max<int32_t>(out->stats.max_col, (int32_t)out->code[k].arg);
     }
     out->stats.derived_uses += c.derived_uses;
+    out->stats.folded_nodes += c.folded_nodes;
     return true;
 }
 }  // namespace
 
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                           int threads, Tape* out, std::string* err, DerivedColumns* derived) {
+                           int threads, Tape* out, std::string* err, DerivedColumns* derived, bool fold) {
     out->meta.clear();
     out->code.clear();
     out->const_pos.clear();
     out->stats = CompileStats{};
+    out->folds = FoldSet{};
     if (derived) plan_derived(st, phenotype, ephemeral, size, stride, n, ncol, threads, derived);
     if (threads <= 1 || n < 4096)
-        return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived);
+        return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived,
+                             fold);
     std::vector<Tape> parts((size_t)threads);
     std::vector<std::string> errs((size_t)threads);
     std::vector<char> ok((size_t)threads, 1);
@@ -546,7 +618,7 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
         if (lo >= hi) continue;
         pool.emplace_back([&, t, lo, hi] {
             ok[t] = compile_range(st, phenotype + (size_t)lo * stride, ephemeral + (size_t)lo * stride, size + lo,
-                                  stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t], derived);
+                                  stride, hi - lo, ncol, want_const_pos, &parts[t], &errs[t], derived, fold);
         });
     }
     for (auto& th : pool) th.join();
@@ -561,6 +633,10 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
     out->meta.reserve((size_t)n);
     for (auto& p : parts) {
         uint32_t base = (uint32_t)out->code.size();
+        const uint32_t fbase = (uint32_t)out->folds.code.size();
+        for (auto m : p.folds.meta) { m.code_off += fbase; out->folds.meta.push_back(m); }
+        out->folds.code.insert(out->folds.code.end(), p.folds.code.begin(), p.folds.code.end());
+        for (uint32_t a : p.folds.at) out->folds.at.push_back(a + base);
         for (auto m : p.meta) { m.code_off += base; out->meta.push_back(m); }
         out->code.insert(out->code.end(), p.code.begin(), p.code.end());
         if (want_const_pos) out->const_pos.insert(out->const_pos.end(), p.const_pos.begin(), p.const_pos.end());
@@ -572,6 +648,7 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
         out->stats.n_empty += p.stats.n_empty;
         out->stats.max_col = std::max(out->stats.max_col, p.stats.max_col);
         out->stats.derived_uses += p.stats.derived_uses;
+        out->stats.folded_nodes += p.stats.folded_nodes;
     }
     return true;
 }

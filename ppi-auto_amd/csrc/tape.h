@@ -98,6 +98,7 @@ struct CompileStats {
     int32_t n_empty = 0;
     int32_t max_col = -1;       // largest input column referenced (derived columns excluded)
     int64_t derived_uses = 0;   // unary-of-column nodes read from derived columns
+    int64_t folded_nodes = 0;   // operator nodes inside folded constant subtrees
 };
 
 // Chains of unary operators over input columns, precomputed once per dataset:
@@ -140,26 +141,46 @@ struct DerivedColumns {
 void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                   int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived);
 
+// Constant subtrees (every leaf a constant, at least one operator) folded out of
+// the programs: each is a program of its own in `meta`/`code` (same format, ending
+// in H_END), and the value it evaluates to is the argument of instruction at[j]
+// of the main tape, which the compiler leaves as a placeholder.  The folds are
+// evaluated once per population by the engine's own interpreter code on the
+// device (fold_kernel), so the folded tape computes the same bits as the
+// unfolded one; apply_folds() writes the values in.
+struct FoldSet {
+    std::vector<ProgMeta> meta;
+    std::vector<Ins> code;
+    std::vector<uint32_t> at;
+};
+
 struct Tape {
     std::vector<ProgMeta> meta;
     std::vector<Ins> code;
     std::vector<uint32_t> const_pos;  // per instruction: phenotype position + 1 of a T_CONST operand, else 0
     CompileStats stats;
+    FoldSet folds;
 };
+
+// Writes the values of the tape's folds (value[j] of fold j) into their
+// placeholder instructions.
+void apply_folds(Tape* t, const float* value);
 
 // Compiles programs [0, n) into `out` (appending).  ncol bounds attribute columns
 // (an out-of-range column evaluates as NaN).  With `derived`, the population's
 // unary chains are planned first (plan_derived) and read from derived columns.
-// Returns false with `err` set if a program needs more than kMaxSlots register
-// slots.
+// With `fold`, constant subtrees go to out->folds (not with want_const_pos: the
+// adjoint needs every T_CONST in the tape).  Returns false with `err` set if a
+// program needs more than kMaxSlots register slots.
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                        Tape* out, std::string* err, DerivedColumns* derived = nullptr);
+                        Tape* out, std::string* err, DerivedColumns* derived = nullptr, bool fold = false);
 
 // Multi-threaded front end: plans the derived columns and compiles with `threads` workers.
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
-                           int threads, Tape* out, std::string* err, DerivedColumns* derived = nullptr);
+                           int threads, Tape* out, std::string* err, DerivedColumns* derived = nullptr,
+                           bool fold = false);
 
 // ---------------------------------------------------------------------------
 // Reverse-mode adjoint tape: d(fitness)/d(ephemeral T_CONST values).

@@ -53,6 +53,7 @@ class Config(C.Structure):
         ("protected_ops", C.c_int32), ("native_math", C.c_int32), ("cost_n", C.c_int32),
         ("cost", C.POINTER(C.c_float)), ("max_local_size", C.c_int32), ("compute_units", C.c_int32),
         ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32), ("ordered_pass_rows", C.c_int32),
+        ("no_constant_folding", C.c_int32),
     ]
 
 
@@ -61,7 +62,8 @@ class Stats(C.Structure):
         ("upload_ms", C.c_double), ("kernel_ms", C.c_double), ("main_kernel_ms", C.c_double),
         ("download_ms", C.c_double), ("node_evals", C.c_double), ("executed_nodes", C.c_double),
         ("form", C.c_int32), ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32),
-        ("max_slots", C.c_int32), ("derived_columns", C.c_int32),
+        ("max_slots", C.c_int32), ("derived_columns", C.c_int32), ("folded_subtrees", C.c_int32),
+        ("folded_nodes", C.c_int32),
     ]
 
 
@@ -224,7 +226,7 @@ class Engine:
                   reducemax: bool = False, protected: bool = True, native: bool = False,
                   cost: Optional[np.ndarray] = None, form: int = FORM_AUTO, max_local_size: int = 0,
                   compute_units: int = 0, rows_per_lane: int = 0, programs_per_tile: int = 0,
-                  ordered_pass_rows: int = 0) -> None:
+                  ordered_pass_rows: int = 0, fold_constants: bool = True) -> None:
         cfg = Config()
         cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
         cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
@@ -235,6 +237,7 @@ class Engine:
         cfg.max_local_size, cfg.compute_units = max_local_size, compute_units
         cfg.rows_per_lane, cfg.programs_per_tile = rows_per_lane, programs_per_tile
         cfg.ordered_pass_rows = ordered_pass_rows
+        cfg.no_constant_folding = 0 if fold_constants else 1
         tab = (Symbol * max(1, len(table)))()
         for k, (s, op, v) in enumerate(table):
             tab[k].symbol, tab[k].op, tab[k].value = s, op, v
@@ -295,7 +298,8 @@ class Engine:
 
 
 def make_config(strategy=SEQ, error_fn=EF_ABS, reducemax=False, protected=True, native=False, cost=None,
-                form=FORM_AUTO, max_local_size=0, compute_units=0, rows_per_lane=0, programs_per_tile=0):
+                form=FORM_AUTO, max_local_size=0, compute_units=0, rows_per_lane=0, programs_per_tile=0,
+                fold_constants=True):
     cfg = Config()
     cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
     cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
@@ -306,6 +310,7 @@ def make_config(strategy=SEQ, error_fn=EF_ABS, reducemax=False, protected=True, 
         cfg.cost = _ptr(keep, C.c_float)
     cfg.max_local_size, cfg.compute_units = max_local_size, compute_units
     cfg.rows_per_lane, cfg.programs_per_tile = rows_per_lane, programs_per_tile
+    cfg.no_constant_folding = 0 if fold_constants else 1
     return cfg, keep
 
 

@@ -85,10 +85,38 @@ static float bin(uint32_t op, float a, float b, int P) {
     }
 }
 
-// derive != 0: compile with derived columns (chains of unary operators over input
+// One program of the tape on one row (row may be NULL for programs without attributes).
+static float run_program(const std::vector<Ins>& code, const ProgMeta& m, const float* row, int P) {
+    float acc = NAN, reg[kMaxSlots];
+    for (int k = 0; k < kMaxSlots; ++k) reg[k] = NAN;
+    for (uint32_t pc = 0; pc < m.ncode; ++pc) {
+        const Ins in = code[m.code_off + pc];
+        const uint32_t h = in.code & 0xffu, s1 = (in.code >> 8) & 31u, s2 = (in.code >> 13) & 31u;
+        float cv;
+        std::memcpy(&cv, &in.arg, 4);
+        auto term = [&](bool attr) { return attr ? row[in.arg] : cv; };
+        if (h >= H_BIN_S) acc = bin(h - H_BIN_S, acc, reg[s1], P);
+        else if (h >= H_BIN_A) acc = bin(h - H_BIN_A, acc, term(true), P);
+        else if (h >= H_BIN_C) acc = bin(h - H_BIN_C, acc, term(false), P);
+        else if (h >= H_UN) acc = un(h - H_UN, acc, P);
+        else if (h == H_LOAD_C || h == H_LOAD_A) acc = term(h == H_LOAD_A);
+        else if (h == H_PUSH_C || h == H_PUSH_A) { reg[s1] = acc; acc = term(h == H_PUSH_A); }
+        else {
+            const float v[3] = {acc, reg[s1], reg[s2]};
+            const float c = v[(in.code >> 18) & 3u], th = v[(in.code >> 20) & 3u], el = v[(in.code >> 22) & 3u];
+            acc = (c != 0.0f) ? th : el;
+        }
+    }
+    return acc;
+}
+
+// derive bit 0: compile with derived columns (chains of unary operators over input
 // columns read precomputed columns), computed here on the host from the same
-// operator code, in column order; derived_out (may be NULL) receives the number of
-// derived columns and the number of them whose source is another derived column.
+// operator code, in column order.  derive bit 1: fold constant subtrees; the fold
+// programs run here first and their values are written into the tape, as the
+// engine does on the device.  derived_out (may be NULL) receives the number of
+// derived columns, the number of them whose source is another derived column, the
+// number of folded subtrees and the operator nodes inside them.
 extern "C" int tape_eval_ex(const ppi_symbol* table, int ntable, const int32_t* ph, const float* eph, const int32_t* size,
                             int stride, int n, const float* data, int nlin, int ncol, int P, float* out,
                             int32_t* slots, int32_t* ncode, int derive, int32_t* derived_out) {
@@ -97,12 +125,21 @@ extern "C" int tape_eval_ex(const ppi_symbol* table, int ntable, const int32_t* 
     Tape t;
     std::string err;
     DerivedColumns dc;
-    dc.reset(ncol, derive ? 4096 : 0);
-    if (!compile_population(st, ph, eph, size, stride, n, ncol, false, &t, &err, derive ? &dc : nullptr)) return -1;
+    dc.reset(ncol, (derive & 1) ? 4096 : 0);
+    if (!compile_population(st, ph, eph, size, stride, n, ncol, false, &t, &err, (derive & 1) ? &dc : nullptr,
+                            (derive & 2) != 0))
+        return -1;
+    if (!t.folds.meta.empty()) {
+        std::vector<float> value(t.folds.meta.size());
+        for (size_t j = 0; j < value.size(); ++j) value[j] = run_program(t.folds.code, t.folds.meta[j], nullptr, P);
+        apply_folds(&t, value.data());
+    }
     if (derived_out) {
         derived_out[0] = (int32_t)dc.cols.size();
         derived_out[1] = 0;
         for (const auto& c : dc.cols) derived_out[1] += c.src >= ncol ? 1 : 0;
+        derived_out[2] = (int32_t)t.folds.meta.size();
+        derived_out[3] = (int32_t)t.stats.folded_nodes;
     }
     const int wide = ncol + (int)dc.cols.size();
     std::vector<float> ext((size_t)nlin * wide);
@@ -119,28 +156,7 @@ extern "C" int tape_eval_ex(const ppi_symbol* table, int ntable, const int32_t* 
         ncode[i] = (int32_t)m.ncode;
         for (int r = 0; r < nlin; ++r) {
             const float* row = data + (size_t)r * ncol;
-            float acc = NAN, reg[kMaxSlots];
-            for (int k = 0; k < kMaxSlots; ++k) reg[k] = NAN;
-            if (m.flags & kEmpty) { out[(size_t)i * nlin + r] = NAN; continue; }
-            for (uint32_t pc = 0; pc < m.ncode; ++pc) {
-                const Ins in = t.code[m.code_off + pc];
-                const uint32_t h = in.code & 0xffu, s1 = (in.code >> 8) & 31u, s2 = (in.code >> 13) & 31u;
-                float cv;
-                std::memcpy(&cv, &in.arg, 4);
-                auto term = [&](bool attr) { return attr ? row[in.arg] : cv; };
-                if (h >= H_BIN_S) acc = bin(h - H_BIN_S, acc, reg[s1], P);
-                else if (h >= H_BIN_A) acc = bin(h - H_BIN_A, acc, term(true), P);
-                else if (h >= H_BIN_C) acc = bin(h - H_BIN_C, acc, term(false), P);
-                else if (h >= H_UN) acc = un(h - H_UN, acc, P);
-                else if (h == H_LOAD_C || h == H_LOAD_A) acc = term(h == H_LOAD_A);
-                else if (h == H_PUSH_C || h == H_PUSH_A) { reg[s1] = acc; acc = term(h == H_PUSH_A); }
-                else {
-                    const float v[3] = {acc, reg[s1], reg[s2]};
-                    const float c = v[(in.code >> 18) & 3u], th = v[(in.code >> 20) & 3u], el = v[(in.code >> 22) & 3u];
-                    acc = (c != 0.0f) ? th : el;
-                }
-            }
-            out[(size_t)i * nlin + r] = acc;
+            out[(size_t)i * nlin + r] = (m.flags & kEmpty) ? NAN : run_program(t.code, m, row, P);
         }
     }
     return 0;

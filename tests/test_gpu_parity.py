@@ -151,3 +151,26 @@ def test_prediction_mode(eng):
     cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS)
     ref, _ = oracle.interpret(cfg, b.table, pop, data, ppp_mode=1, prediction_mode=1)
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("native", [False, True])
+@pytest.mark.parametrize("prot", [True, False])
+def test_constant_folding_same_bits(eng, prot, native):
+    """Constant subtrees evaluated once at upload by fold_kernel give the same bits as the same
+    programs evaluated with every subtree on every row (folding switched off), in both math modes."""
+    gr = full_grammar(6)
+    pop = ppiamd.grow(gr, 2000, 31, 8, 128, p_leaf=0.3)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 13, 5000, 6) * np.float32(3.0)
+    print(f"case folding protected={prot} native={native}", flush=True)
+    res = {}
+    for fold in (True, False):
+        eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_ABS, protected=prot, native=native,
+                      form=ppiamd.FORM_FP, fold_constants=fold)
+        eng.set_dataset(data)
+        vec, _ = eng.evaluate(pop, alpha=ALPHA, best_size=1)
+        st = eng.stats()
+        res[fold] = (vec.copy(), st.folded_subtrees, st.executed_nodes)
+    assert res[True][1] > 100 and res[False][1] == 0
+    assert res[True][2] < res[False][2]
+    same = res[True][0].view(np.uint32) == res[False][0].view(np.uint32)
+    assert same.all(), f"{(~same).sum()} programs differ, e.g. {np.nonzero(~same)[0][:5]}"

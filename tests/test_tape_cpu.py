@@ -43,6 +43,21 @@ def tape_values(L, table, pop, data, prot):
     return out, slots, ncode
 
 
+def tape_values_ex(L, table, pop, data, prot, flags):
+    """tape_eval_ex: flags bit 0 = derived columns, bit 1 = folded constant subtrees."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    out = np.zeros((pop.n, data.shape[0]), dtype=np.float32)
+    slots = np.zeros(pop.n, dtype=np.int32)
+    ncode = np.zeros(pop.n, dtype=np.int32)
+    info = (C.c_int32 * 4)()
+    p = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    r = L.tape_eval_ex(oracle._table(table), len(table), p(pop.phenotype, C.c_int32), p(pop.ephemeral, C.c_float),
+                       p(pop.size, C.c_int32), pop.stride, pop.n, p(data, C.c_float), data.shape[0], data.shape[1],
+                       int(prot), p(out, C.c_float), p(slots, C.c_int32), p(ncode, C.c_int32), flags, info)
+    assert r == 0
+    return out, slots, ncode, list(info)
+
+
 def oracle_values(table, pop, data, prot):
     cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS, protected=prot)
     out = np.zeros((pop.n, data.shape[0]), dtype=np.float32)
@@ -119,7 +134,7 @@ def test_derived_columns_match_prefix_stack(tape_lib, prot):
     out = np.zeros((pop.n, data.shape[0]), dtype=np.float32)
     slots = np.zeros(pop.n, dtype=np.int32)
     ncode = np.zeros(pop.n, dtype=np.int32)
-    nder = (C.c_int32 * 2)()
+    nder = (C.c_int32 * 4)()
     p = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
     r = tape_lib.tape_eval_ex(oracle._table(gr.table), len(gr.table), p(pop.phenotype, C.c_int32),
                               p(pop.ephemeral, C.c_float), p(pop.size, C.c_int32), pop.stride, pop.n,
@@ -133,3 +148,42 @@ def test_derived_columns_match_prefix_stack(tape_lib, prot):
     ov = oracle_values(gr.table, pop, data, prot)
     ok = same_bits(out, ov)
     assert ok.all(), f"{(~ok).sum()} values differ, programs {np.unique(np.nonzero(~ok)[0])[:10]}"
+
+
+@pytest.mark.parametrize("prot", [True, False])
+def test_folded_constants_match_prefix_stack(tape_lib, prot):
+    """Constant subtrees compiled into fold programs, evaluated once and written into the tape's
+    placeholders (alone and with derived columns): the oracle's values, bit for bit."""
+    gr = full_grammar(5)
+    pop = ppiamd.grow(gr, 400, 29, 8, 160, p_leaf=0.3)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 4, 48, 5) * np.float32(4.0)
+    ov = oracle_values(gr.table, pop, data, prot)
+    _, _, ncode_plain = tape_values(tape_lib, gr.table, pop, data, prot)
+    for flags in (2, 3):
+        tv, slots, ncode, info = tape_values_ex(tape_lib, gr.table, pop, data, prot, flags)
+        ok = same_bits(tv, ov)
+        assert ok.all(), f"flags {flags}: {(~ok).sum()} values differ, programs {np.unique(np.nonzero(~ok)[0])[:10]}"
+        assert info[2] > 50, "the population must exercise folded subtrees"
+        assert info[3] >= info[2]
+        assert ncode.sum() < ncode_plain.sum()
+
+
+def test_folded_edge_programs(tape_lib):
+    """Whole-constant programs, nested and ternary constant subtrees, protected operators on constants,
+    constants mixed with attributes, and folds feeding register-stack operands."""
+    b = Builder(4)
+    progs = [["+", 2.0, 3.0], ["sin", "+", 1.0, 2.0], ["ite", 1.0, 2.0, 3.0], ["ite", 0.0, 2.0, "x1"],
+             ["*", "x0", "log", "+", 2.0, 3.0], ["log", -2.0], ["/", 1.0, 0.0], ["sqrt", -4.0],
+             ["-", "exp", "cos", 0.5, "x2"], ["+", "*", "x0", "x1", "*", "sin", 1.5, "exp", 0.25],
+             ["/", "*", "+", "x0", "x1", "-", 2.0, 3.0, "+", "*", 4.0, 5.0, "sin", "x1"],
+             ["ite", ">", 1.0, 2.0, "x0", "sin", "*", 3.0, 0.5], ["exp", "exp", "exp", "exp", 3.0],
+             ["+", "x0", 1.0], ["neg", "x3"], [2.5], ["mod", 7.0, 0.0], ["pow", -8.0, 0.5]]
+    pop = b.population(progs)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 5, 40, 4)
+    for prot in (True, False):
+        ov = oracle_values(b.table, pop, data, prot)
+        tv, slots, ncode, info = tape_values_ex(tape_lib, b.table, pop, data, prot, 2)
+        ok = same_bits(tv, ov)
+        assert ok.all(), f"protected={prot}: programs {np.unique(np.nonzero(~ok)[0])}"
+        assert info[2] >= 13
+        assert (ncode[[0, 1, 2, 5, 6, 7, 12, 16, 17]] == 1).all()  # one load of the folded value
