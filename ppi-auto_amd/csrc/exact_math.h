@@ -58,7 +58,10 @@ constexpr uint32_t kGuard = 64;  // double ulps kept between y and a float round
 // with two constant operands (which would need a v_mov pair).  Every kernel that evaluates the
 // exact routines stages them once at entry — stage_exact_tables() by all
 // threads of the workgroup, then a barrier — before any thread reads them.
-__shared__ ExactTables lds_exact_tables;
+// 64-byte alignment puts the tables at LDS address 0, ahead of the kernels' feature tiles: every
+// table read then takes its base from the instruction's offset field (the sine pair's ds_read2_b64
+// has only 8-bit offsets), one VALU add fewer per sin, cos and log row.
+__shared__ __attribute__((aligned(64))) ExactTables lds_exact_tables;
 
 __device__ __forceinline__ void stage_exact_tables() {
     constexpr int kWords = (int)(sizeof(ExactTables) / sizeof(double));
@@ -102,10 +105,11 @@ __device__ __forceinline__ float exact_exp_fast(float x, bool& ok) {
 
 // log.  x = 2^k z with z in [0.69921875, 1.3984375); the bucket i of z gives invc ~ 1/c and
 // logc = -log(invc); log(x) = k ln2 + logc + log1p(r), r = z invc - 1 (one rounding: the product
-// of a 24-bit z and a double is exact inside the fma).  |r| <= 2^-7; truncating log1p after r^8
-// leaves r^8/9 < 2^-59 relative.  The two buckets around z = 1 use invc = 1, so log(x) ~ r there
-// carries no cancellation; elsewhere |k ln2 + logc| and |log1p r| are at most 2x |log x|, so the
-// four roundings (table, two fmas, sum) stay below 2^-50.5 relative.  Domain: positive normal x.
+// of a 24-bit z and a double is exact inside the fma).  |r| <= 2^-7; truncating log1p after r^7
+// leaves r^7/8 < 2^-52 relative.  The two buckets around z = 1 use invc = 1, so log(x) ~ r there
+// carries no cancellation; elsewhere |k ln2 + logc| and |log1p r| are at most 2x |log x|, so with
+// the four roundings (table, two fmas, sum) the error stays below 2^-50 relative.  Domain:
+// positive normal x.
 __device__ __forceinline__ float exact_log_fast(float x, bool& ok) {
     const uint32_t ix = __float_as_uint(x);
     const uint32_t tmp = ix - 0x3f330000u;
@@ -113,8 +117,7 @@ __device__ __forceinline__ float exact_log_fast(float x, bool& ok) {
     const uint32_t i = (tmp >> 16) & 127u;
     const float z = __uint_as_float(ix - (tmp & 0xff800000u));
     const double r = __builtin_fma((double)z, lds_exact_tables.log[i][0], -1.0);
-    double q = r * kL8 + kL7;
-    q = __builtin_fma(r, q, kL6);
+    double q = r * kL7 + kL6;
     q = __builtin_fma(r, q, kL5);
     q = __builtin_fma(r, q, kL4);
     q = __builtin_fma(r, q, kL3);

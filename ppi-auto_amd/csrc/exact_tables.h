@@ -162,7 +162,7 @@ __device__ const ExactTables kExactTables = {
     },
 };
 
-// Taylor coefficients: exp(r) - 1 to r^5; log1p(r) to r^8; sin(r) to r^7; cos(r) - 1 to r^6.
+// Taylor coefficients: exp(r) - 1 to r^5; log1p(r) to r^7; sin(r) to r^7; cos(r) - 1 to r^6.
 constexpr double kE2 = 0x1.0000000000000p-1;
 constexpr double kE3 = 0x1.5555555555555p-3;
 constexpr double kE4 = 0x1.5555555555555p-5;
@@ -173,7 +173,6 @@ constexpr double kL4 = -0x1.0000000000000p-2;
 constexpr double kL5 = 0x1.999999999999ap-3;
 constexpr double kL6 = -0x1.5555555555555p-3;
 constexpr double kL7 = 0x1.2492492492492p-3;
-constexpr double kL8 = -0x1.0000000000000p-3;
 constexpr double kS3 = -0x1.5555555555555p-3;
 constexpr double kS5 = 0x1.1111111111111p-7;
 constexpr double kS7 = -0x1.a01a01a01a01ap-13;

@@ -104,10 +104,10 @@ def main() -> None:
 
     # Taylor coefficients.
     fac = mp.factorial
-    w("// Taylor coefficients: exp(r) - 1 to r^5; log1p(r) to r^8; sin(r) to r^7; cos(r) - 1 to r^6.")
+    w("// Taylor coefficients: exp(r) - 1 to r^5; log1p(r) to r^7; sin(r) to r^7; cos(r) - 1 to r^6.")
     for n in range(2, 6):
         w(f"constexpr double kE{n} = {hexd(1 / fac(n))};")
-    for n in range(2, 9):
+    for n in range(2, 8):
         w(f"constexpr double kL{n} = {hexd(mp.mpf((-1) ** (n + 1)) / n)};")
     for n in (3, 5, 7):
         w(f"constexpr double kS{n} = {hexd(mp.mpf((-1) ** ((n - 1) // 2)) / fac(n))};")
