@@ -78,10 +78,10 @@ __device__ __forceinline__ bool rounds_safely(double y) {
     return lo - (0x10000000u - kGuard) > 2u * kGuard;
 }
 
-// exp.  x = k ln2/64 + r with |r| <= ln2/128 (+ rounding); exp(x) = 2^(k>>6) T[k&63] (1 + p(r)).
-// Error: r is exact to 2^-61 absolute (the first fma is exact, the second rounds once, the tail of
-// ln2/64 beyond kExpL1 + kExpL2 times |k| < 2^14 is below 2^-99); truncating exp(r) - 1 after r^5
-// leaves r^6/720 < 2^-54.7; T[j] and the last fma each add 2^-53.  Total < 2^-51.5 relative.
+// exp.  x = k ln2/128 + r with |r| <= ln2/256 (+ rounding); exp(x) = 2^(k>>7) T[k&127] (1 + p(r)).
+// Error: r is exact to 2^-62 absolute (the first fma is exact, the second rounds once, the tail of
+// ln2/128 beyond kExpL1 + kExpL2 times |k| < 2^15 is below 2^-99); truncating exp(r) - 1 after r^4
+// leaves r^5/120 < 2^-49.6; T[j] and the last fma each add 2^-53.  Total < 2^-49.3 relative.
 // Domain: -87 < x < 128 (float results in the normal range or overflowing to inf); x >= 128 gives
 // +inf and x <= -104 gives +0 directly (exp(-104) < 2^-150, half the smallest denormal).
 __device__ __forceinline__ float exact_exp_fast(float x, bool& ok) {
@@ -90,12 +90,11 @@ __device__ __forceinline__ float exact_exp_fast(float x, bool& ok) {
     const int ki = (int)k;
     double r = __builtin_fma(-k, kExpL1, xd);
     r = __builtin_fma(-k, kExpL2, r);
-    const double t = lds_exact_tables.exp2j[ki & 63];
-    double q = r * kE5 + kE4;  // mul + add: one scalar operand per instruction
-    q = __builtin_fma(r, q, kE3);
+    const double t = lds_exact_tables.exp2j[ki & 127];
+    double q = r * kE4 + kE3;  // mul + add: one scalar operand per instruction
     q = __builtin_fma(r, q, kE2);
     const double p = __builtin_fma(r * r, q, r);
-    const double y = __builtin_ldexp(__builtin_fma(t, p, t), ki >> 6);
+    const double y = __builtin_ldexp(__builtin_fma(t, p, t), ki >> 7);
     float res = (float)y;
     if (x >= 128.0f) res = __builtin_inff();
     if (x <= -104.0f) res = 0.0f;
@@ -132,16 +131,16 @@ __device__ __forceinline__ float exact_log_fast(float x, bool& ok) {
     return res;
 }
 
-// sin and cos.  a = |x| = k pi/64 + r, |r| <= pi/128 (+ rounding); with S_j = sin(j pi/64),
-// sin(a) = S_j cos r + S_{j+32} sin r (j = k mod 128) and cos(a) = sin(a + pi/2) takes j + 32
-// (the table runs to j = 191, so both entries sit at fixed offsets from one index).
-// r: the first fma is exact (a has at most 24 significant bits, k < 2^28.4 on the domain), the
-// second rounds once, and the tail of pi/64 beyond kTrigP1 + kTrigP2 times k is below 2^-82;
-// a float below 2^24 never comes closer than 2^-26.8 to a multiple of pi (checked by enumerating
-// the multiples), so where the result is ~ +-r (S_j = 0) r is relative-exact to 2^-53.
-// Elsewhere |result| >= sin(pi/128) and the terms are at most 2x the result.  Truncation: sin r
-// after r^7 leaves r^9/9! < 2^-61 relative, cos r - 1 after r^6 leaves r^8/8! < 2^-58.  Total
-// below 2^-50.  Domain: |x| < 2^24.
+// sin and cos.  a = |x| = k pi/128 + r, |r| <= pi/256 (+ rounding); with S_j = sin(j pi/128),
+// sin(a) = S_j cos r + S_{j+64} sin r (j = k mod 256) and cos(a) = sin(a + pi/2) takes j + 64
+// (the table runs to j = 383, so both entries sit at fixed offsets from one index).
+// r: the first fma is exact (a has at most 24 significant bits, k < 2^29.4 on the domain, and
+// a - k kTrigP1 needs at most 52 bits), the second rounds once, and the tail of pi/128 beyond
+// kTrigP1 + kTrigP2 times k is below 2^-82; a float below 2^24 never comes closer than 2^-26.8 to
+// a multiple of pi (checked by enumerating the multiples), so where the result is ~ +-r (S_j = 0)
+// r is relative-exact to 2^-53.  Elsewhere |result| >= sin(pi/256) and the terms are at most 2x
+// the result.  Truncation: sin r after r^5 leaves r^7/7! < 2^-50.4 relative, cos r - 1 after r^6
+// leaves r^8/8! < 2^-60.  Total below 2^-49.  Domain: |x| < 2^24.
 template <bool COS>
 __device__ __forceinline__ float exact_sincos_fast(float x, bool& ok) {
     const float ax = __builtin_fabsf(x);
@@ -150,10 +149,10 @@ __device__ __forceinline__ float exact_sincos_fast(float x, bool& ok) {
     const int ki = (int)k;
     double r = __builtin_fma(-k, kTrigP1, a);
     r = __builtin_fma(-k, kTrigP2, r);
-    const double* T = lds_exact_tables.sinpi64 + (ki & 127) + (COS ? 32 : 0);
-    const double S = T[0], C = T[32];
+    const double* T = lds_exact_tables.sinpi128 + (ki & 255) + (COS ? 64 : 0);
+    const double S = T[0], C = T[64];
     const double r2 = r * r;
-    const double s = __builtin_fma(r2 * r, __builtin_fma(r2, r2 * kS7 + kS5, kS3), r);
+    const double s = __builtin_fma(r2 * r, r2 * kS5 + kS3, r);
     const double cm1 = r2 * __builtin_fma(r2, r2 * kC6 + kC4, kC2);
     const double y = __builtin_fma(S, cm1, __builtin_fma(C, s, S));
     float res = (float)y;

@@ -6,93 +6,157 @@
 namespace ppi {
 namespace dev {
 
-// exp(x) = 2^(k >> 6) * 2^((k & 63)/64) * exp(r),  k = rint(x * 64/ln2),  r = x - k*(L1 + L2)
-constexpr double kExpInvL = 0x1.71547652b82fep+6;  // 64/ln2
-constexpr double kExpL1 = 0x1.62e42fefa39efp-7;  // ln2/64 rounded
-constexpr double kExpL2 = 0x1.abc9e3b39803fp-62;  // ln2/64 - kExpL1 rounded
+// exp(x) = 2^(k >> 7) * 2^((k & 127)/128) * exp(r),  k = rint(x * 128/ln2),  r = x - k*(L1 + L2)
+constexpr double kExpInvL = 0x1.71547652b82fep+7;  // 128/ln2
+constexpr double kExpL1 = 0x1.62e42fefa39efp-8;  // ln2/128 rounded
+constexpr double kExpL2 = 0x1.abc9e3b39803fp-63;  // ln2/128 - kExpL1 rounded
 // log(x) = k*ln2 - log(invc_i) + log1p(z*invc_i - 1), x = 2^k z, z in [0.69921875, 1.3984375);
 // bucket i = bits 16..22 of (bits(x) - 0x3f330000); invc_i ~ 1 / (bucket centre), 1 exactly for the
 // two buckets around z = 1 (no cancellation between the table term and the polynomial there).
 constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
 constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
-// sin(a) = S_j cos(r) + S_{j+32} sin(r),  a = |x| = k*pi/64 + r,  j = k mod 128,  S_j = sin(j pi/64)
-constexpr double kTrigInvP = 0x1.45f306dc9c883p+4;  // 64/pi
-constexpr double kTrigP1 = 0x1.921fb54442d18p-5;  // pi/64 rounded
-constexpr double kTrigP2 = 0x1.1a62633145c07p-59;  // pi/64 - kTrigP1 rounded
+// sin(a) = S_j cos(r) + S_{j+64} sin(r),  a = |x| = k*pi/128 + r,  j = k mod 256,  S_j = sin(j pi/128)
+constexpr double kTrigInvP = 0x1.45f306dc9c883p+5;  // 128/pi
+constexpr double kTrigP1 = 0x1.921fb54442d18p-6;  // pi/128 rounded
+constexpr double kTrigP2 = 0x1.1a62633145c07p-60;  // pi/128 - kTrigP1 rounded
 
 struct ExactTables {
-    double sinpi64[192];  // sin(j pi/64), j < 192: sin and cos read entries j and j + 32 (cos: j + 32 and j + 64) of one index j < 128
-    double exp2j[64];  // 2^(j/64)
+    double sinpi128[384];  // sin(j pi/128), j < 384: sin and cos read entries j and j + 64 (cos: j + 64 and j + 128) of one index j < 256
+    double exp2j[128];  // 2^(j/128)
     double log[128][2];  // {invc_i, -log(invc_i)}
 };
 __device__ const ExactTables kExactTables = {
     {
-        0x0.0p+0, 0x1.91f65f10dd814p-5, 0x1.917a6bc29b42cp-4, 0x1.2c8106e8e613ap-3,
-        0x1.8f8b83c69a60bp-3, 0x1.f19f97b215f1bp-3, 0x1.294062ed59f06p-2, 0x1.58f9a75ab1fddp-2,
-        0x1.87de2a6aea963p-2, 0x1.b5d1009e15cc0p-2, 0x1.e2b5d3806f63bp-2, 0x1.073879922ffeep-1,
-        0x1.1c73b39ae68c8p-1, 0x1.30ff7fce17035p-1, 0x1.44cf325091dd6p-1, 0x1.57d69348ceca0p-1,
-        0x1.6a09e667f3bcdp-1, 0x1.7b5df226aafafp-1, 0x1.8bc806b151741p-1, 0x1.9b3e047f38741p-1,
-        0x1.a9b66290ea1a3p-1, 0x1.b728345196e3ep-1, 0x1.c38b2f180bdb1p-1, 0x1.ced7af43cc773p-1,
-        0x1.d906bcf328d46p-1, 0x1.e212104f686e5p-1, 0x1.e9f4156c62ddap-1, 0x1.f0a7efb9230d7p-1,
-        0x1.f6297cff75cb0p-1, 0x1.fa7557f08a517p-1, 0x1.fd88da3d12526p-1, 0x1.ff621e3796d7ep-1,
-        0x1.0000000000000p+0, 0x1.ff621e3796d7ep-1, 0x1.fd88da3d12526p-1, 0x1.fa7557f08a517p-1,
-        0x1.f6297cff75cb0p-1, 0x1.f0a7efb9230d7p-1, 0x1.e9f4156c62ddap-1, 0x1.e212104f686e5p-1,
-        0x1.d906bcf328d46p-1, 0x1.ced7af43cc773p-1, 0x1.c38b2f180bdb1p-1, 0x1.b728345196e3ep-1,
-        0x1.a9b66290ea1a3p-1, 0x1.9b3e047f38741p-1, 0x1.8bc806b151741p-1, 0x1.7b5df226aafafp-1,
-        0x1.6a09e667f3bcdp-1, 0x1.57d69348ceca0p-1, 0x1.44cf325091dd6p-1, 0x1.30ff7fce17035p-1,
-        0x1.1c73b39ae68c8p-1, 0x1.073879922ffeep-1, 0x1.e2b5d3806f63bp-2, 0x1.b5d1009e15cc0p-2,
-        0x1.87de2a6aea963p-2, 0x1.58f9a75ab1fddp-2, 0x1.294062ed59f06p-2, 0x1.f19f97b215f1bp-3,
-        0x1.8f8b83c69a60bp-3, 0x1.2c8106e8e613ap-3, 0x1.917a6bc29b42cp-4, 0x1.91f65f10dd814p-5,
-        0x0.0p+0, -0x1.91f65f10dd814p-5, -0x1.917a6bc29b42cp-4, -0x1.2c8106e8e613ap-3,
-        -0x1.8f8b83c69a60bp-3, -0x1.f19f97b215f1bp-3, -0x1.294062ed59f06p-2, -0x1.58f9a75ab1fddp-2,
-        -0x1.87de2a6aea963p-2, -0x1.b5d1009e15cc0p-2, -0x1.e2b5d3806f63bp-2, -0x1.073879922ffeep-1,
-        -0x1.1c73b39ae68c8p-1, -0x1.30ff7fce17035p-1, -0x1.44cf325091dd6p-1, -0x1.57d69348ceca0p-1,
-        -0x1.6a09e667f3bcdp-1, -0x1.7b5df226aafafp-1, -0x1.8bc806b151741p-1, -0x1.9b3e047f38741p-1,
-        -0x1.a9b66290ea1a3p-1, -0x1.b728345196e3ep-1, -0x1.c38b2f180bdb1p-1, -0x1.ced7af43cc773p-1,
-        -0x1.d906bcf328d46p-1, -0x1.e212104f686e5p-1, -0x1.e9f4156c62ddap-1, -0x1.f0a7efb9230d7p-1,
-        -0x1.f6297cff75cb0p-1, -0x1.fa7557f08a517p-1, -0x1.fd88da3d12526p-1, -0x1.ff621e3796d7ep-1,
-        -0x1.0000000000000p+0, -0x1.ff621e3796d7ep-1, -0x1.fd88da3d12526p-1, -0x1.fa7557f08a517p-1,
-        -0x1.f6297cff75cb0p-1, -0x1.f0a7efb9230d7p-1, -0x1.e9f4156c62ddap-1, -0x1.e212104f686e5p-1,
-        -0x1.d906bcf328d46p-1, -0x1.ced7af43cc773p-1, -0x1.c38b2f180bdb1p-1, -0x1.b728345196e3ep-1,
-        -0x1.a9b66290ea1a3p-1, -0x1.9b3e047f38741p-1, -0x1.8bc806b151741p-1, -0x1.7b5df226aafafp-1,
-        -0x1.6a09e667f3bcdp-1, -0x1.57d69348ceca0p-1, -0x1.44cf325091dd6p-1, -0x1.30ff7fce17035p-1,
-        -0x1.1c73b39ae68c8p-1, -0x1.073879922ffeep-1, -0x1.e2b5d3806f63bp-2, -0x1.b5d1009e15cc0p-2,
-        -0x1.87de2a6aea963p-2, -0x1.58f9a75ab1fddp-2, -0x1.294062ed59f06p-2, -0x1.f19f97b215f1bp-3,
-        -0x1.8f8b83c69a60bp-3, -0x1.2c8106e8e613ap-3, -0x1.917a6bc29b42cp-4, -0x1.91f65f10dd814p-5,
-        0x0.0p+0, 0x1.91f65f10dd814p-5, 0x1.917a6bc29b42cp-4, 0x1.2c8106e8e613ap-3,
-        0x1.8f8b83c69a60bp-3, 0x1.f19f97b215f1bp-3, 0x1.294062ed59f06p-2, 0x1.58f9a75ab1fddp-2,
-        0x1.87de2a6aea963p-2, 0x1.b5d1009e15cc0p-2, 0x1.e2b5d3806f63bp-2, 0x1.073879922ffeep-1,
-        0x1.1c73b39ae68c8p-1, 0x1.30ff7fce17035p-1, 0x1.44cf325091dd6p-1, 0x1.57d69348ceca0p-1,
-        0x1.6a09e667f3bcdp-1, 0x1.7b5df226aafafp-1, 0x1.8bc806b151741p-1, 0x1.9b3e047f38741p-1,
-        0x1.a9b66290ea1a3p-1, 0x1.b728345196e3ep-1, 0x1.c38b2f180bdb1p-1, 0x1.ced7af43cc773p-1,
-        0x1.d906bcf328d46p-1, 0x1.e212104f686e5p-1, 0x1.e9f4156c62ddap-1, 0x1.f0a7efb9230d7p-1,
-        0x1.f6297cff75cb0p-1, 0x1.fa7557f08a517p-1, 0x1.fd88da3d12526p-1, 0x1.ff621e3796d7ep-1,
-        0x1.0000000000000p+0, 0x1.ff621e3796d7ep-1, 0x1.fd88da3d12526p-1, 0x1.fa7557f08a517p-1,
-        0x1.f6297cff75cb0p-1, 0x1.f0a7efb9230d7p-1, 0x1.e9f4156c62ddap-1, 0x1.e212104f686e5p-1,
-        0x1.d906bcf328d46p-1, 0x1.ced7af43cc773p-1, 0x1.c38b2f180bdb1p-1, 0x1.b728345196e3ep-1,
-        0x1.a9b66290ea1a3p-1, 0x1.9b3e047f38741p-1, 0x1.8bc806b151741p-1, 0x1.7b5df226aafafp-1,
-        0x1.6a09e667f3bcdp-1, 0x1.57d69348ceca0p-1, 0x1.44cf325091dd6p-1, 0x1.30ff7fce17035p-1,
-        0x1.1c73b39ae68c8p-1, 0x1.073879922ffeep-1, 0x1.e2b5d3806f63bp-2, 0x1.b5d1009e15cc0p-2,
-        0x1.87de2a6aea963p-2, 0x1.58f9a75ab1fddp-2, 0x1.294062ed59f06p-2, 0x1.f19f97b215f1bp-3,
-        0x1.8f8b83c69a60bp-3, 0x1.2c8106e8e613ap-3, 0x1.917a6bc29b42cp-4, 0x1.91f65f10dd814p-5,
+        0x0.0p+0, 0x1.92155f7a3667ep-6, 0x1.91f65f10dd814p-5, 0x1.2d52092ce19f6p-4,
+        0x1.917a6bc29b42cp-4, 0x1.f564e56a9730ep-4, 0x1.2c8106e8e613ap-3, 0x1.5e214448b3fc6p-3,
+        0x1.8f8b83c69a60bp-3, 0x1.c0b826a7e4f63p-3, 0x1.f19f97b215f1bp-3, 0x1.111d262b1f677p-2,
+        0x1.294062ed59f06p-2, 0x1.4135c94176601p-2, 0x1.58f9a75ab1fddp-2, 0x1.7088530fa459fp-2,
+        0x1.87de2a6aea963p-2, 0x1.9ef7943a8ed8ap-2, 0x1.b5d1009e15cc0p-2, 0x1.cc66e9931c45ep-2,
+        0x1.e2b5d3806f63bp-2, 0x1.f8ba4dbf89abap-2, 0x1.073879922ffeep-1, 0x1.11eb3541b4b23p-1,
+        0x1.1c73b39ae68c8p-1, 0x1.26d054cdd12dfp-1, 0x1.30ff7fce17035p-1, 0x1.3affa292050b9p-1,
+        0x1.44cf325091dd6p-1, 0x1.4e6cabbe3e5e9p-1, 0x1.57d69348ceca0p-1, 0x1.610b7551d2cdfp-1,
+        0x1.6a09e667f3bcdp-1, 0x1.72d0837efff96p-1, 0x1.7b5df226aafafp-1, 0x1.83b0e0bff976ep-1,
+        0x1.8bc806b151741p-1, 0x1.93a22499263fbp-1, 0x1.9b3e047f38741p-1, 0x1.a29a7a0462782p-1,
+        0x1.a9b66290ea1a3p-1, 0x1.b090a58150200p-1, 0x1.b728345196e3ep-1, 0x1.bd7c0ac6f952ap-1,
+        0x1.c38b2f180bdb1p-1, 0x1.c954b213411f5p-1, 0x1.ced7af43cc773p-1, 0x1.d4134d14dc93ap-1,
+        0x1.d906bcf328d46p-1, 0x1.ddb13b6ccc23cp-1, 0x1.e212104f686e5p-1, 0x1.e6288ec48e112p-1,
+        0x1.e9f4156c62ddap-1, 0x1.ed740e7684963p-1, 0x1.f0a7efb9230d7p-1, 0x1.f38f3ac64e589p-1,
+        0x1.f6297cff75cb0p-1, 0x1.f8764fa714ba9p-1, 0x1.fa7557f08a517p-1, 0x1.fc26470e19fd3p-1,
+        0x1.fd88da3d12526p-1, 0x1.fe9cdad01883ap-1, 0x1.ff621e3796d7ep-1, 0x1.ffd886084cd0dp-1,
+        0x1.0000000000000p+0, 0x1.ffd886084cd0dp-1, 0x1.ff621e3796d7ep-1, 0x1.fe9cdad01883ap-1,
+        0x1.fd88da3d12526p-1, 0x1.fc26470e19fd3p-1, 0x1.fa7557f08a517p-1, 0x1.f8764fa714ba9p-1,
+        0x1.f6297cff75cb0p-1, 0x1.f38f3ac64e589p-1, 0x1.f0a7efb9230d7p-1, 0x1.ed740e7684963p-1,
+        0x1.e9f4156c62ddap-1, 0x1.e6288ec48e112p-1, 0x1.e212104f686e5p-1, 0x1.ddb13b6ccc23cp-1,
+        0x1.d906bcf328d46p-1, 0x1.d4134d14dc93ap-1, 0x1.ced7af43cc773p-1, 0x1.c954b213411f5p-1,
+        0x1.c38b2f180bdb1p-1, 0x1.bd7c0ac6f952ap-1, 0x1.b728345196e3ep-1, 0x1.b090a58150200p-1,
+        0x1.a9b66290ea1a3p-1, 0x1.a29a7a0462782p-1, 0x1.9b3e047f38741p-1, 0x1.93a22499263fbp-1,
+        0x1.8bc806b151741p-1, 0x1.83b0e0bff976ep-1, 0x1.7b5df226aafafp-1, 0x1.72d0837efff96p-1,
+        0x1.6a09e667f3bcdp-1, 0x1.610b7551d2cdfp-1, 0x1.57d69348ceca0p-1, 0x1.4e6cabbe3e5e9p-1,
+        0x1.44cf325091dd6p-1, 0x1.3affa292050b9p-1, 0x1.30ff7fce17035p-1, 0x1.26d054cdd12dfp-1,
+        0x1.1c73b39ae68c8p-1, 0x1.11eb3541b4b23p-1, 0x1.073879922ffeep-1, 0x1.f8ba4dbf89abap-2,
+        0x1.e2b5d3806f63bp-2, 0x1.cc66e9931c45ep-2, 0x1.b5d1009e15cc0p-2, 0x1.9ef7943a8ed8ap-2,
+        0x1.87de2a6aea963p-2, 0x1.7088530fa459fp-2, 0x1.58f9a75ab1fddp-2, 0x1.4135c94176601p-2,
+        0x1.294062ed59f06p-2, 0x1.111d262b1f677p-2, 0x1.f19f97b215f1bp-3, 0x1.c0b826a7e4f63p-3,
+        0x1.8f8b83c69a60bp-3, 0x1.5e214448b3fc6p-3, 0x1.2c8106e8e613ap-3, 0x1.f564e56a9730ep-4,
+        0x1.917a6bc29b42cp-4, 0x1.2d52092ce19f6p-4, 0x1.91f65f10dd814p-5, 0x1.92155f7a3667ep-6,
+        0x0.0p+0, -0x1.92155f7a3667ep-6, -0x1.91f65f10dd814p-5, -0x1.2d52092ce19f6p-4,
+        -0x1.917a6bc29b42cp-4, -0x1.f564e56a9730ep-4, -0x1.2c8106e8e613ap-3, -0x1.5e214448b3fc6p-3,
+        -0x1.8f8b83c69a60bp-3, -0x1.c0b826a7e4f63p-3, -0x1.f19f97b215f1bp-3, -0x1.111d262b1f677p-2,
+        -0x1.294062ed59f06p-2, -0x1.4135c94176601p-2, -0x1.58f9a75ab1fddp-2, -0x1.7088530fa459fp-2,
+        -0x1.87de2a6aea963p-2, -0x1.9ef7943a8ed8ap-2, -0x1.b5d1009e15cc0p-2, -0x1.cc66e9931c45ep-2,
+        -0x1.e2b5d3806f63bp-2, -0x1.f8ba4dbf89abap-2, -0x1.073879922ffeep-1, -0x1.11eb3541b4b23p-1,
+        -0x1.1c73b39ae68c8p-1, -0x1.26d054cdd12dfp-1, -0x1.30ff7fce17035p-1, -0x1.3affa292050b9p-1,
+        -0x1.44cf325091dd6p-1, -0x1.4e6cabbe3e5e9p-1, -0x1.57d69348ceca0p-1, -0x1.610b7551d2cdfp-1,
+        -0x1.6a09e667f3bcdp-1, -0x1.72d0837efff96p-1, -0x1.7b5df226aafafp-1, -0x1.83b0e0bff976ep-1,
+        -0x1.8bc806b151741p-1, -0x1.93a22499263fbp-1, -0x1.9b3e047f38741p-1, -0x1.a29a7a0462782p-1,
+        -0x1.a9b66290ea1a3p-1, -0x1.b090a58150200p-1, -0x1.b728345196e3ep-1, -0x1.bd7c0ac6f952ap-1,
+        -0x1.c38b2f180bdb1p-1, -0x1.c954b213411f5p-1, -0x1.ced7af43cc773p-1, -0x1.d4134d14dc93ap-1,
+        -0x1.d906bcf328d46p-1, -0x1.ddb13b6ccc23cp-1, -0x1.e212104f686e5p-1, -0x1.e6288ec48e112p-1,
+        -0x1.e9f4156c62ddap-1, -0x1.ed740e7684963p-1, -0x1.f0a7efb9230d7p-1, -0x1.f38f3ac64e589p-1,
+        -0x1.f6297cff75cb0p-1, -0x1.f8764fa714ba9p-1, -0x1.fa7557f08a517p-1, -0x1.fc26470e19fd3p-1,
+        -0x1.fd88da3d12526p-1, -0x1.fe9cdad01883ap-1, -0x1.ff621e3796d7ep-1, -0x1.ffd886084cd0dp-1,
+        -0x1.0000000000000p+0, -0x1.ffd886084cd0dp-1, -0x1.ff621e3796d7ep-1, -0x1.fe9cdad01883ap-1,
+        -0x1.fd88da3d12526p-1, -0x1.fc26470e19fd3p-1, -0x1.fa7557f08a517p-1, -0x1.f8764fa714ba9p-1,
+        -0x1.f6297cff75cb0p-1, -0x1.f38f3ac64e589p-1, -0x1.f0a7efb9230d7p-1, -0x1.ed740e7684963p-1,
+        -0x1.e9f4156c62ddap-1, -0x1.e6288ec48e112p-1, -0x1.e212104f686e5p-1, -0x1.ddb13b6ccc23cp-1,
+        -0x1.d906bcf328d46p-1, -0x1.d4134d14dc93ap-1, -0x1.ced7af43cc773p-1, -0x1.c954b213411f5p-1,
+        -0x1.c38b2f180bdb1p-1, -0x1.bd7c0ac6f952ap-1, -0x1.b728345196e3ep-1, -0x1.b090a58150200p-1,
+        -0x1.a9b66290ea1a3p-1, -0x1.a29a7a0462782p-1, -0x1.9b3e047f38741p-1, -0x1.93a22499263fbp-1,
+        -0x1.8bc806b151741p-1, -0x1.83b0e0bff976ep-1, -0x1.7b5df226aafafp-1, -0x1.72d0837efff96p-1,
+        -0x1.6a09e667f3bcdp-1, -0x1.610b7551d2cdfp-1, -0x1.57d69348ceca0p-1, -0x1.4e6cabbe3e5e9p-1,
+        -0x1.44cf325091dd6p-1, -0x1.3affa292050b9p-1, -0x1.30ff7fce17035p-1, -0x1.26d054cdd12dfp-1,
+        -0x1.1c73b39ae68c8p-1, -0x1.11eb3541b4b23p-1, -0x1.073879922ffeep-1, -0x1.f8ba4dbf89abap-2,
+        -0x1.e2b5d3806f63bp-2, -0x1.cc66e9931c45ep-2, -0x1.b5d1009e15cc0p-2, -0x1.9ef7943a8ed8ap-2,
+        -0x1.87de2a6aea963p-2, -0x1.7088530fa459fp-2, -0x1.58f9a75ab1fddp-2, -0x1.4135c94176601p-2,
+        -0x1.294062ed59f06p-2, -0x1.111d262b1f677p-2, -0x1.f19f97b215f1bp-3, -0x1.c0b826a7e4f63p-3,
+        -0x1.8f8b83c69a60bp-3, -0x1.5e214448b3fc6p-3, -0x1.2c8106e8e613ap-3, -0x1.f564e56a9730ep-4,
+        -0x1.917a6bc29b42cp-4, -0x1.2d52092ce19f6p-4, -0x1.91f65f10dd814p-5, -0x1.92155f7a3667ep-6,
+        0x0.0p+0, 0x1.92155f7a3667ep-6, 0x1.91f65f10dd814p-5, 0x1.2d52092ce19f6p-4,
+        0x1.917a6bc29b42cp-4, 0x1.f564e56a9730ep-4, 0x1.2c8106e8e613ap-3, 0x1.5e214448b3fc6p-3,
+        0x1.8f8b83c69a60bp-3, 0x1.c0b826a7e4f63p-3, 0x1.f19f97b215f1bp-3, 0x1.111d262b1f677p-2,
+        0x1.294062ed59f06p-2, 0x1.4135c94176601p-2, 0x1.58f9a75ab1fddp-2, 0x1.7088530fa459fp-2,
+        0x1.87de2a6aea963p-2, 0x1.9ef7943a8ed8ap-2, 0x1.b5d1009e15cc0p-2, 0x1.cc66e9931c45ep-2,
+        0x1.e2b5d3806f63bp-2, 0x1.f8ba4dbf89abap-2, 0x1.073879922ffeep-1, 0x1.11eb3541b4b23p-1,
+        0x1.1c73b39ae68c8p-1, 0x1.26d054cdd12dfp-1, 0x1.30ff7fce17035p-1, 0x1.3affa292050b9p-1,
+        0x1.44cf325091dd6p-1, 0x1.4e6cabbe3e5e9p-1, 0x1.57d69348ceca0p-1, 0x1.610b7551d2cdfp-1,
+        0x1.6a09e667f3bcdp-1, 0x1.72d0837efff96p-1, 0x1.7b5df226aafafp-1, 0x1.83b0e0bff976ep-1,
+        0x1.8bc806b151741p-1, 0x1.93a22499263fbp-1, 0x1.9b3e047f38741p-1, 0x1.a29a7a0462782p-1,
+        0x1.a9b66290ea1a3p-1, 0x1.b090a58150200p-1, 0x1.b728345196e3ep-1, 0x1.bd7c0ac6f952ap-1,
+        0x1.c38b2f180bdb1p-1, 0x1.c954b213411f5p-1, 0x1.ced7af43cc773p-1, 0x1.d4134d14dc93ap-1,
+        0x1.d906bcf328d46p-1, 0x1.ddb13b6ccc23cp-1, 0x1.e212104f686e5p-1, 0x1.e6288ec48e112p-1,
+        0x1.e9f4156c62ddap-1, 0x1.ed740e7684963p-1, 0x1.f0a7efb9230d7p-1, 0x1.f38f3ac64e589p-1,
+        0x1.f6297cff75cb0p-1, 0x1.f8764fa714ba9p-1, 0x1.fa7557f08a517p-1, 0x1.fc26470e19fd3p-1,
+        0x1.fd88da3d12526p-1, 0x1.fe9cdad01883ap-1, 0x1.ff621e3796d7ep-1, 0x1.ffd886084cd0dp-1,
+        0x1.0000000000000p+0, 0x1.ffd886084cd0dp-1, 0x1.ff621e3796d7ep-1, 0x1.fe9cdad01883ap-1,
+        0x1.fd88da3d12526p-1, 0x1.fc26470e19fd3p-1, 0x1.fa7557f08a517p-1, 0x1.f8764fa714ba9p-1,
+        0x1.f6297cff75cb0p-1, 0x1.f38f3ac64e589p-1, 0x1.f0a7efb9230d7p-1, 0x1.ed740e7684963p-1,
+        0x1.e9f4156c62ddap-1, 0x1.e6288ec48e112p-1, 0x1.e212104f686e5p-1, 0x1.ddb13b6ccc23cp-1,
+        0x1.d906bcf328d46p-1, 0x1.d4134d14dc93ap-1, 0x1.ced7af43cc773p-1, 0x1.c954b213411f5p-1,
+        0x1.c38b2f180bdb1p-1, 0x1.bd7c0ac6f952ap-1, 0x1.b728345196e3ep-1, 0x1.b090a58150200p-1,
+        0x1.a9b66290ea1a3p-1, 0x1.a29a7a0462782p-1, 0x1.9b3e047f38741p-1, 0x1.93a22499263fbp-1,
+        0x1.8bc806b151741p-1, 0x1.83b0e0bff976ep-1, 0x1.7b5df226aafafp-1, 0x1.72d0837efff96p-1,
+        0x1.6a09e667f3bcdp-1, 0x1.610b7551d2cdfp-1, 0x1.57d69348ceca0p-1, 0x1.4e6cabbe3e5e9p-1,
+        0x1.44cf325091dd6p-1, 0x1.3affa292050b9p-1, 0x1.30ff7fce17035p-1, 0x1.26d054cdd12dfp-1,
+        0x1.1c73b39ae68c8p-1, 0x1.11eb3541b4b23p-1, 0x1.073879922ffeep-1, 0x1.f8ba4dbf89abap-2,
+        0x1.e2b5d3806f63bp-2, 0x1.cc66e9931c45ep-2, 0x1.b5d1009e15cc0p-2, 0x1.9ef7943a8ed8ap-2,
+        0x1.87de2a6aea963p-2, 0x1.7088530fa459fp-2, 0x1.58f9a75ab1fddp-2, 0x1.4135c94176601p-2,
+        0x1.294062ed59f06p-2, 0x1.111d262b1f677p-2, 0x1.f19f97b215f1bp-3, 0x1.c0b826a7e4f63p-3,
+        0x1.8f8b83c69a60bp-3, 0x1.5e214448b3fc6p-3, 0x1.2c8106e8e613ap-3, 0x1.f564e56a9730ep-4,
+        0x1.917a6bc29b42cp-4, 0x1.2d52092ce19f6p-4, 0x1.91f65f10dd814p-5, 0x1.92155f7a3667ep-6,
     },
     {
-        0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
-        0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
-        0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
-        0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
-        0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
-        0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
-        0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
-        0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
-        0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
-        0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
-        0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
-        0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
-        0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
-        0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
-        0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
-        0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+        0x1.0000000000000p+0, 0x1.0163da9fb3335p+0, 0x1.02c9a3e778061p+0, 0x1.04315e86e7f85p+0,
+        0x1.059b0d3158574p+0, 0x1.0706b29ddf6dep+0, 0x1.0874518759bc8p+0, 0x1.09e3ecac6f383p+0,
+        0x1.0b5586cf9890fp+0, 0x1.0cc922b7247f7p+0, 0x1.0e3ec32d3d1a2p+0, 0x1.0fb66affed31bp+0,
+        0x1.11301d0125b51p+0, 0x1.12abdc06c31ccp+0, 0x1.1429aaea92de0p+0, 0x1.15a98c8a58e51p+0,
+        0x1.172b83c7d517bp+0, 0x1.18af9388c8deap+0, 0x1.1a35beb6fcb75p+0, 0x1.1bbe084045cd4p+0,
+        0x1.1d4873168b9aap+0, 0x1.1ed5022fcd91dp+0, 0x1.2063b88628cd6p+0, 0x1.21f49917ddc96p+0,
+        0x1.2387a6e756238p+0, 0x1.251ce4fb2a63fp+0, 0x1.26b4565e27cddp+0, 0x1.284dfe1f56381p+0,
+        0x1.29e9df51fdee1p+0, 0x1.2b87fd0dad990p+0, 0x1.2d285a6e4030bp+0, 0x1.2ecafa93e2f56p+0,
+        0x1.306fe0a31b715p+0, 0x1.32170fc4cd831p+0, 0x1.33c08b26416ffp+0, 0x1.356c55f929ff1p+0,
+        0x1.371a7373aa9cbp+0, 0x1.38cae6d05d866p+0, 0x1.3a7db34e59ff7p+0, 0x1.3c32dc313a8e5p+0,
+        0x1.3dea64c123422p+0, 0x1.3fa4504ac801cp+0, 0x1.4160a21f72e2ap+0, 0x1.431f5d950a897p+0,
+        0x1.44e086061892dp+0, 0x1.46a41ed1d0057p+0, 0x1.486a2b5c13cd0p+0, 0x1.4a32af0d7d3dep+0,
+        0x1.4bfdad5362a27p+0, 0x1.4dcb299fddd0dp+0, 0x1.4f9b2769d2ca7p+0, 0x1.516daa2cf6642p+0,
+        0x1.5342b569d4f82p+0, 0x1.551a4ca5d920fp+0, 0x1.56f4736b527dap+0, 0x1.58d12d497c7fdp+0,
+        0x1.5ab07dd485429p+0, 0x1.5c9268a5946b7p+0, 0x1.5e76f15ad2148p+0, 0x1.605e1b976dc09p+0,
+        0x1.6247eb03a5585p+0, 0x1.6434634ccc320p+0, 0x1.6623882552225p+0, 0x1.68155d44ca973p+0,
+        0x1.6a09e667f3bcdp+0, 0x1.6c012750bdabfp+0, 0x1.6dfb23c651a2fp+0, 0x1.6ff7df9519484p+0,
+        0x1.71f75e8ec5f74p+0, 0x1.73f9a48a58174p+0, 0x1.75feb564267c9p+0, 0x1.780694fde5d3fp+0,
+        0x1.7a11473eb0187p+0, 0x1.7c1ed0130c132p+0, 0x1.7e2f336cf4e62p+0, 0x1.80427543e1a12p+0,
+        0x1.82589994cce13p+0, 0x1.8471a4623c7adp+0, 0x1.868d99b4492edp+0, 0x1.88ac7d98a6699p+0,
+        0x1.8ace5422aa0dbp+0, 0x1.8cf3216b5448cp+0, 0x1.8f1ae99157736p+0, 0x1.9145b0b91ffc6p+0,
+        0x1.93737b0cdc5e5p+0, 0x1.95a44cbc8520fp+0, 0x1.97d829fde4e50p+0, 0x1.9a0f170ca07bap+0,
+        0x1.9c49182a3f090p+0, 0x1.9e86319e32323p+0, 0x1.a0c667b5de565p+0, 0x1.a309bec4a2d33p+0,
+        0x1.a5503b23e255dp+0, 0x1.a799e1330b358p+0, 0x1.a9e6b5579fdbfp+0, 0x1.ac36bbfd3f37ap+0,
+        0x1.ae89f995ad3adp+0, 0x1.b0e07298db666p+0, 0x1.b33a2b84f15fbp+0, 0x1.b59728de5593ap+0,
+        0x1.b7f76f2fb5e47p+0, 0x1.ba5b030a1064ap+0, 0x1.bcc1e904bc1d2p+0, 0x1.bf2c25bd71e09p+0,
+        0x1.c199bdd85529cp+0, 0x1.c40ab5fffd07ap+0, 0x1.c67f12e57d14bp+0, 0x1.c8f6d9406e7b5p+0,
+        0x1.cb720dcef9069p+0, 0x1.cdf0b555dc3fap+0, 0x1.d072d4a07897cp+0, 0x1.d2f87080d89f2p+0,
+        0x1.d5818dcfba487p+0, 0x1.d80e316c98398p+0, 0x1.da9e603db3285p+0, 0x1.dd321f301b460p+0,
+        0x1.dfc97337b9b5fp+0, 0x1.e264614f5a129p+0, 0x1.e502ee78b3ff6p+0, 0x1.e7a51fbc74c83p+0,
+        0x1.ea4afa2a490dap+0, 0x1.ecf482d8e67f1p+0, 0x1.efa1bee615a27p+0, 0x1.f252b376bba97p+0,
+        0x1.f50765b6e4540p+0, 0x1.f7bfdad9cbe14p+0, 0x1.fa7c1819e90d8p+0, 0x1.fd3c22b8f71f1p+0,
     },
     {
         {0x1.6d1a62681c861p+0, -0x1.6b85b4cffa3fdp-2}, {0x1.6b1490aa31a3dp+0, -0x1.65d558d4ce00bp-2},
@@ -162,11 +226,10 @@ __device__ const ExactTables kExactTables = {
     },
 };
 
-// Taylor coefficients: exp(r) - 1 to r^5; log1p(r) to r^7; sin(r) to r^7; cos(r) - 1 to r^6.
+// Taylor coefficients: exp(r) - 1 to r^4; log1p(r) to r^7; sin(r) to r^5; cos(r) - 1 to r^6.
 constexpr double kE2 = 0x1.0000000000000p-1;
 constexpr double kE3 = 0x1.5555555555555p-3;
 constexpr double kE4 = 0x1.5555555555555p-5;
-constexpr double kE5 = 0x1.1111111111111p-7;
 constexpr double kL2 = -0x1.0000000000000p-1;
 constexpr double kL3 = 0x1.5555555555555p-2;
 constexpr double kL4 = -0x1.0000000000000p-2;
@@ -175,7 +238,6 @@ constexpr double kL6 = -0x1.5555555555555p-3;
 constexpr double kL7 = 0x1.2492492492492p-3;
 constexpr double kS3 = -0x1.5555555555555p-3;
 constexpr double kS5 = 0x1.1111111111111p-7;
-constexpr double kS7 = -0x1.a01a01a01a01ap-13;
 constexpr double kC2 = -0x1.0000000000000p-1;
 constexpr double kC4 = 0x1.5555555555555p-5;
 constexpr double kC6 = -0x1.6c16c16c16c17p-10;

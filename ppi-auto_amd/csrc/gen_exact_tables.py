@@ -33,16 +33,16 @@ def main() -> None:
     w("namespace dev {")
     w("")
 
-    # exp: x = k*ln2/64 + r, exp(x) = 2^(k>>6) * 2^((k&63)/64) * exp(r).
+    # exp: x = k*ln2/128 + r, exp(x) = 2^(k>>7) * 2^((k&127)/128) * exp(r).
     ln2 = mp.log(2)
-    l1, l2 = split(ln2 / 64)
-    w("// exp(x) = 2^(k >> 6) * 2^((k & 63)/64) * exp(r),  k = rint(x * 64/ln2),  r = x - k*(L1 + L2)")
-    w(f"constexpr double kExpInvL = {hexd(64 / ln2)};  // 64/ln2")
-    w(f"constexpr double kExpL1 = {hexd(l1)};  // ln2/64 rounded")
-    w(f"constexpr double kExpL2 = {hexd(l2)};  // ln2/64 - kExpL1 rounded")
+    l1, l2 = split(ln2 / 128)
+    w("// exp(x) = 2^(k >> 7) * 2^((k & 127)/128) * exp(r),  k = rint(x * 128/ln2),  r = x - k*(L1 + L2)")
+    w(f"constexpr double kExpInvL = {hexd(128 / ln2)};  // 128/ln2")
+    w(f"constexpr double kExpL1 = {hexd(l1)};  // ln2/128 rounded")
+    w(f"constexpr double kExpL2 = {hexd(l2)};  // ln2/128 - kExpL1 rounded")
     tables = []  # (comment, member declaration, rows of initialiser text); the sine table goes first (below)
-    tables.append(("2^(j/64)", "double exp2j[64];",
-                   [", ".join(hexd(mp.power(2, mp.mpf(i) / 64)) for i in range(j, j + 4)) for j in range(0, 64, 4)]))
+    tables.append(("2^(j/128)", "double exp2j[128];",
+                   [", ".join(hexd(mp.power(2, mp.mpf(i) / 128)) for i in range(j, j + 4)) for j in range(0, 128, 4)]))
 
     # log: x = 2^k * z, z in [OFF, 2 OFF) with OFF = 0x3f330000 (0.69921875); bucket i = top 7 bits
     # of z's offset from OFF; log(z) = log1p(z*invc - 1) - log(invc).
@@ -72,20 +72,20 @@ def main() -> None:
                    ["{" + hexd(invs[i]) + ", " + hexd(logs[i]) + "}, {" + hexd(invs[i + 1]) + ", " + hexd(logs[i + 1])
                     + "}" for i in range(0, 128, 2)]))
 
-    # sin/cos: |x| = k*pi/64 + r, sin(|x|) = S_k cos r + C_k sin r with S_k = sin(k pi/64), C_k = S_{k+32}.
-    p1, p2 = split(mp.pi / 64)
+    # sin/cos: |x| = k*pi/128 + r, sin(|x|) = S_k cos r + C_k sin r with S_k = sin(k pi/128), C_k = S_{k+64}.
+    p1, p2 = split(mp.pi / 128)
 
-    def sin_pi64(i):  # sin(i pi/64); exactly 0 at multiples of pi (mpmath leaves ~2^-200 there)
-        return mp.mpf(0) if i % 64 == 0 else mp.sin(mp.mpf(i) * mp.pi / 64)
+    def sin_pi128(i):  # sin(i pi/128); exactly 0 at multiples of pi (mpmath leaves ~2^-200 there)
+        return mp.mpf(0) if i % 128 == 0 else mp.sin(mp.mpf(i) * mp.pi / 128)
 
-    w("// sin(a) = S_j cos(r) + S_{j+32} sin(r),  a = |x| = k*pi/64 + r,  j = k mod 128,  S_j = sin(j pi/64)")
-    w(f"constexpr double kTrigInvP = {hexd(64 / mp.pi)};  // 64/pi")
-    w(f"constexpr double kTrigP1 = {hexd(p1)};  // pi/64 rounded")
-    w(f"constexpr double kTrigP2 = {hexd(p2)};  // pi/64 - kTrigP1 rounded")
+    w("// sin(a) = S_j cos(r) + S_{j+64} sin(r),  a = |x| = k*pi/128 + r,  j = k mod 256,  S_j = sin(j pi/128)")
+    w(f"constexpr double kTrigInvP = {hexd(128 / mp.pi)};  // 128/pi")
+    w(f"constexpr double kTrigP1 = {hexd(p1)};  // pi/128 rounded")
+    w(f"constexpr double kTrigP2 = {hexd(p2)};  // pi/128 - kTrigP1 rounded")
     # First in the struct (LDS offset 0): both of its reads then fit the 8-bit offsets of one ds_read2_b64.
-    tables.insert(0, ("sin(j pi/64), j < 192: sin and cos read entries j and j + 32 (cos: j + 32 and j + 64) "
-                      "of one index j < 128", "double sinpi64[192];",
-                      [", ".join(hexd(sin_pi64(i)) for i in range(j, j + 4)) for j in range(0, 192, 4)]))
+    tables.insert(0, ("sin(j pi/128), j < 384: sin and cos read entries j and j + 64 (cos: j + 64 and j + 128) "
+                      "of one index j < 256", "double sinpi128[384];",
+                      [", ".join(hexd(sin_pi128(i)) for i in range(j, j + 4)) for j in range(0, 384, 4)]))
 
     # One struct, so that kernels can stage all tables into LDS with one copy.
     w("")
@@ -104,12 +104,12 @@ def main() -> None:
 
     # Taylor coefficients.
     fac = mp.factorial
-    w("// Taylor coefficients: exp(r) - 1 to r^5; log1p(r) to r^7; sin(r) to r^7; cos(r) - 1 to r^6.")
-    for n in range(2, 6):
+    w("// Taylor coefficients: exp(r) - 1 to r^4; log1p(r) to r^7; sin(r) to r^5; cos(r) - 1 to r^6.")
+    for n in range(2, 5):
         w(f"constexpr double kE{n} = {hexd(1 / fac(n))};")
     for n in range(2, 8):
         w(f"constexpr double kL{n} = {hexd(mp.mpf((-1) ** (n + 1)) / n)};")
-    for n in (3, 5, 7):
+    for n in (3, 5):
         w(f"constexpr double kS{n} = {hexd(mp.mpf((-1) ** ((n - 1) // 2)) / fac(n))};")
     for n in (2, 4, 6):
         w(f"constexpr double kC{n} = {hexd(mp.mpf((-1) ** (n // 2)) / fac(n))};")
