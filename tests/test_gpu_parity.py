@@ -174,3 +174,30 @@ def test_constant_folding_same_bits(eng, prot, native):
     assert res[True][2] < res[False][2]
     same = res[True][0].view(np.uint32) == res[False][0].view(np.uint32)
     assert same.all(), f"{(~same).sum()} programs differ, e.g. {np.nonzero(~same)[0][:5]}"
+
+
+@pytest.mark.parametrize("nlin", [1, 2, 63, 65, 257, 1023])
+def test_ragged_row_counts(eng, nlin):
+    """Row counts that leave a single or partial row tile, with 37 programs and with one: every form
+    against the oracle (PP and ordered bit-exact, FP and PDP within the float-sum bar)."""
+    gr = full_grammar(4)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 7, nlin, 4)
+    for npro in (37, 1):
+        pop = ppiamd.grow(gr, npro, 41 + nlin, 6, 48, p_leaf=0.3)
+        dpl, ppl, bpl = oracle.reference_geometry(nlin, pop.n)
+        cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS, False, True, None, dpl, ppl, bpl)
+        ovec, oidx = oracle.interpret(cfg, gr.table, pop, data, alpha=ALPHA, best_size=1)
+        for form in (ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_ORDERED, ppiamd.FORM_PP):
+            print(f"case ragged nlin={nlin} programs={npro} form={form}", flush=True)
+            eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_ABS, protected=True, form=form,
+                          compute_units=256)
+            eng.set_dataset(data)
+            vec, idx = eng.evaluate(pop, alpha=ALPHA, best_size=1)
+            if form in (ppiamd.FORM_PP, ppiamd.FORM_ORDERED):
+                same = vec.view(np.uint32) == ovec.view(np.uint32)
+                assert same.all(), f"form {form}: programs {np.nonzero(~same)[0][:5]}"
+                assert idx[0] == oidx[0]
+            else:
+                ok = rel_close(vec, ovec, RTOL)
+                assert ok.all(), f"form {form}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
+                check_rank(vec, idx, ovec, oidx)
