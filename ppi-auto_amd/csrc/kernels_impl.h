@@ -101,8 +101,8 @@ struct GlobalFeat {  // straight from the column-major dataset (more than 32 col
 };
 template <int R, int F>
 struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP form).  The tile holds the
-                  // input columns (every input column a program reads is < F); columns >= ncol are
-                  // derived columns and come from the dataset through the caches.
+                  // first ncol = min(input columns, F) columns; the others (input columns beyond F and
+                  // the derived columns) come from the dataset through the caches.
     typedef typename VecT<R>::type VR;
     typedef __attribute__((address_space(3))) const VR LdsVR;
     LdsVR* base;  // &tile[0][lane][0]: a lane's R rows of column c are one vector at base[c * 64]
@@ -127,7 +127,7 @@ struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP for
 template <int R, int F>
 __device__ __forceinline__ auto make_feat(const EvalArgs& A, const float* lds_lane, const int (&row)[R]) {
     if constexpr (F > 0)
-        return LdsFeat<R, F>{(typename LdsFeat<R, F>::LdsVR*)lds_lane, A.X, A.ldx, (uint32_t)A.ncol, row};
+        return LdsFeat<R, F>{(typename LdsFeat<R, F>::LdsVR*)lds_lane, A.X, A.ldx, (uint32_t)min(A.ncol, F), row};
     else return GlobalFeat<R>{A.X, A.ldx, row};
 }
 

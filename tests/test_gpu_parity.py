@@ -187,6 +187,7 @@ def test_ragged_row_counts(eng, nlin):
         dpl, ppl, bpl = oracle.reference_geometry(nlin, pop.n)
         cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS, False, True, None, dpl, ppl, bpl)
         ovec, oidx = oracle.interpret(cfg, gr.table, pop, data, alpha=ALPHA, best_size=1)
+        ex, bad = oracle.exact_sums(oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS), gr.table, pop, data)
         for form in (ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_ORDERED, ppiamd.FORM_PP):
             print(f"case ragged nlin={nlin} programs={npro} form={form}", flush=True)
             eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_ABS, protected=True, form=form,
@@ -198,6 +199,32 @@ def test_ragged_row_counts(eng, nlin):
                 assert same.all(), f"form {form}: programs {np.nonzero(~same)[0][:5]}"
                 assert idx[0] == oidx[0]
             else:
-                ok = rel_close(vec, ovec, RTOL)
+                ok = sums_agree(vec, ovec, ex, bad, nlin, ALPHA, pop.size)
                 assert ok.all(), f"form {form}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
                 check_rank(vec, idx, ovec, oidx)
+
+
+@pytest.mark.parametrize("nfeat", [24, 40])
+def test_wide_datasets(eng, nfeat):
+    """Datasets wider than the 16 columns the FP form stages in LDS (the rest are read through the
+    caches) and wider than 32: FP, ordered and PDP forms against the oracle."""
+    gr = full_grammar(nfeat)
+    pop = ppiamd.grow(gr, 600, 51 + nfeat, 7, 64, p_leaf=0.35)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 8, 3000, nfeat) * np.float32(2.0)
+    dpl, ppl, bpl = oracle.reference_geometry(data.shape[0], pop.n)
+    cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_SQUARED, False, True, None, dpl, ppl, bpl)
+    ovec, oidx = oracle.interpret(cfg, gr.table, pop, data, alpha=ALPHA, best_size=1)
+    ex, bad = oracle.exact_sums(oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_SQUARED), gr.table, pop, data)
+    for form in (ppiamd.FORM_FP, ppiamd.FORM_ORDERED, ppiamd.FORM_PDP):
+        print(f"case wide nfeat={nfeat} form={form}", flush=True)
+        eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_SQUARED, protected=True, form=form,
+                      compute_units=256)
+        eng.set_dataset(data)
+        vec, idx = eng.evaluate(pop, alpha=ALPHA, best_size=1)
+        if form == ppiamd.FORM_ORDERED:
+            same = vec.view(np.uint32) == ovec.view(np.uint32)
+            assert same.all(), f"ordered: programs {np.nonzero(~same)[0][:5]}"
+        else:
+            ok = sums_agree(vec, ovec, ex, bad, data.shape[0], ALPHA, pop.size)
+            assert ok.all(), f"form {form}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
+            check_rank(vec, idx, ovec, oidx)
