@@ -263,7 +263,10 @@ ppi_status enqueue_run(ppi_batch* b, int32_t n_items, bool with_finalize, bool w
     B_HIP(launch_fill_u32(b->dBadHi, 0u, n, b->stream));
     B_HIP(launch_fill_u32(b->dCounter, 0u, 1, b->stream));
     B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
-    if (with_finalize) B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
+    if (with_finalize) {
+        B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
+        if (b->cfg.strategy == PPI_STRATEGY_PDP) B_HIP(launch_batch_pdp_exact(B, b->dPairOf, n, b->P, b->N, b->stream));
+    }
     if (with_download)
         B_HIP(hipMemcpyAsync(b->hRaw, b->dRaw, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     return PPI_OK;
@@ -327,6 +330,10 @@ ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_sy
     if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
     const bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
     if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs a matrix"); return PPI_E_ARG; }
+    if (cfg->max_local_size < 0 || cfg->max_local_size > kPdpMaxLanes) {
+        set_error("max_local_size must be in [0, " + std::to_string(kPdpMaxLanes) + "]");
+        return PPI_E_ARG;
+    }
     if (cfg->rows_per_lane != 0 && cfg->rows_per_lane != 2 && cfg->rows_per_lane != 4 && cfg->rows_per_lane != 8) {
         set_error("batched rows_per_lane must be 0 (autotune), 2, 4 or 8");
         return PPI_E_ARG;
@@ -647,6 +654,7 @@ ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
         B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
         B_HIP(hipEventRecord(b->ev[3], b->stream));
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
+        if (b->cfg.strategy == PPI_STRATEGY_PDP) B_HIP(launch_batch_pdp_exact(B, b->dPairOf, n, b->P, b->N, b->stream));
         B_HIP(hipEventSynchronize(b->ev[3]));
         float t = 0.0f;
         B_HIP(hipEventElapsedTime(&t, b->ev[2], b->ev[3]));

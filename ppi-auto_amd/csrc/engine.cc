@@ -353,6 +353,10 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
         return PPI_E_ARG;
     }
     if (cfg->ordered_pass_rows < 0) { set_error("bad ordered_pass_rows"); return PPI_E_ARG; }
+    if (cfg->max_local_size < 0 || cfg->max_local_size > kPdpMaxLanes) {
+        set_error("max_local_size must be in [0, " + std::to_string(kPdpMaxLanes) + "]");
+        return PPI_E_ARG;
+    }
     if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
     bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
     if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs cost_n > 0 and cost"); return PPI_E_ARG; }
@@ -580,6 +584,8 @@ static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
         PPI_HIP(launch_pp(A, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
     }
+    // PDP strategy: the programs whose result depends on the reference's lane partition.
+    if (E->cfg.strategy == PPI_STRATEGY_PDP) PPI_HIP(launch_pdp_exact(A, P, N, E->stream));
     return PPI_OK;
 }
 

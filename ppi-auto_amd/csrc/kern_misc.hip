@@ -51,6 +51,28 @@ hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_pdp_exact(const EvalArgs& A, bool P, bool N, hipStream_t st) {
+    if (A.nprog <= 0) return hipSuccess;
+    if (A.pdp_lanes < 1 || A.pdp_lanes > kPdpMaxLanes) return hipErrorInvalidValue;
+    dim3 grid((unsigned)A.nprog);
+    if (P && N) hipLaunchKernelGGL((dev::pdp_exact_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::pdp_exact_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::pdp_exact_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::pdp_exact_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_pdp_exact(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
+                                  hipStream_t st) {
+    if (n_total <= 0) return hipSuccess;
+    dim3 grid((unsigned)n_total);
+    if (P && N) hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
+    else if (P) hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
+    else if (N) hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
+    else hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
+    return hipGetLastError();
+}
+
 hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st) {
     dim3 grid((unsigned)((A.nprog + 255) / 256));
     if (P && N) hipLaunchKernelGGL((dev::pp_kernel<true, true>), grid, dim3(256), 0, st, A);

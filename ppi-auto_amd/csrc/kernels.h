@@ -11,6 +11,7 @@
 namespace ppi {
 
 constexpr int kMaxChunk = 1024;  // programs per FP workgroup
+constexpr int kPdpMaxLanes = 1024;  // PDP strategy work-group width at most (the reference's -cl-mls)
 
 // Rows of one FP-form workgroup tile (64 lanes x R rows per lane, shared by its waves).
 inline constexpr int fp_rows_per_block(int R) { return 64 * R; }
@@ -161,6 +162,11 @@ hipError_t launch_ordered_sum(const EvalArgs& A, int rows_per_block, bool last, 
 hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
 hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st);
 hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st);
+// PDP strategy: recomputes, lane by lane as the reference does, the programs whose epilogue marked
+// them (raw[p] holding the recheck NaN); one workgroup per program, unmarked ones leave at once.
+hipError_t launch_pdp_exact(const EvalArgs& A, bool P, bool N, hipStream_t st);
+hipError_t launch_batch_pdp_exact(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
+                                  hipStream_t st);
 hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 // Folded constant subtrees: out[j] = value of fold program j (meta[j], code).

@@ -358,6 +358,32 @@ __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
 
 __device__ __forceinline__ bool finite_f(float x) { return !(isinf(x) || isnan(x)); }
 
+// PDP strategy (accelerator.cl:173-268): lane L of the reference's work-group
+// (pdp_lanes of them) takes rows L, L + pdp_lanes, ...; a non-finite row error
+// sets the lane's partial to MAXFLOAT and ends the lane; the partials are
+// combined by a tree, and an infinite result becomes MAXFLOAT.  Whether the
+// result is MAXFLOAT or MAXFLOAT/nlin can therefore depend on which lane an
+// overflowing row falls in (REDUCEMAX: an infinite error*nlin survives only in a
+// lane with no bad row) or on the tree's partial sums near FLT_MAX.  The
+// epilogue settles the cases the (total, bad lanes) summary decides and marks
+// the rest by writing kRecheckBits, a NaN no epilogue produces, into raw[p];
+// pdp_exact_kernel then recomputes those programs lane by lane.
+constexpr uint32_t kRecheckBits = 0x7fc0e3c1u;
+
+// Whether a PDP-strategy result needs the lane-exact recomputation.  The error
+// functions other than the cost matrix give non-negative errors, so a partial
+// holding MAXFLOAT stays at or above it and two of them sum to inf; with a cost
+// matrix (entries of either sign) every program with a bad row is recomputed.
+__device__ __forceinline__ bool pdp_needs_lanes(const EvalArgs& A, float total, bool bad, uint32_t kmin,
+                                                uint32_t kmax) {
+    if (A.reducemax) return bad && !finite_f(total);  // no bad row: max is order-free
+    if (!bad) return !(total < 0x1.0p126f);           // a reordered sum near overflow
+    if (A.error_fn >= PPI_EF_COST) return true;
+    // One bad lane: MAXFLOAT + x rounds to inf only for x >= 2^103 (half an ulp
+    // of FLT_MAX); every tree partial is at most the total of the other rows.
+    return kmin == kmax && !(total < 0x1.0p102f);
+}
+
 // Per-program result from (reduced value, bad-row record).  Restates the
 // reference's epilogues: sequential.cc:135-189, accelerator.cl:64-84 +
 // accelerator.cc:849 (PP), accelerator.cl:157-171 + accelerator.cc:773-805 (DP),
@@ -376,11 +402,16 @@ __device__ __forceinline__ void semantics(const EvalArgs& A, float total, uint32
             raw = FLT_MAX;
             vec = (A.strategy == PPI_STRATEGY_SEQ) ? FLT_MAX : raw + as;
         }
+    } else if (A.strategy == PPI_STRATEGY_PDP && pdp_needs_lanes(A, total, bad, kmin, kmax)) {
+        raw = __uint_as_float(kRecheckBits);
+        vec = raw;
     } else if (bad) {
         if (A.strategy == PPI_STRATEGY_PDP) {
+            // Settled cases: REDUCEMAX with no infinite row (a lane's MAXFLOAT is the
+            // maximum), two or more bad lanes (MAXFLOAT + MAXFLOAT = inf), or one bad
+            // lane whose MAXFLOAT absorbs the rest.
             if (A.reducemax) raw = FLT_MAX / fn;
-            else if (kmin != kmax) raw = FLT_MAX;                 // >= 2 lanes hold MAXFLOAT: inf
-            else raw = (total >= 0x1.0p103f) ? FLT_MAX : FLT_MAX / fn;  // MAXFLOAT absorbs the rest
+            else raw = (kmin != kmax) ? FLT_MAX : FLT_MAX / fn;
             vec = raw + as;
         } else if (A.strategy == PPI_STRATEGY_PP) {
             raw = FLT_MAX / fn;
@@ -649,6 +680,100 @@ __global__ void finalize_kernel(EvalArgs A, int nrb) {
     }
     const ProgMeta m = A.meta[p];
     semantics(A, total, m.flags, A.bad_lo[p], A.bad_hi[p], m.size, &A.raw[p], &A.vec[p]);
+}
+
+// The PDP strategy's result, computed as the reference's evaluate_pdp does
+// (accelerator.cl:196-267): thread t plays lanes t, t + kBlock, ... of the
+// reference's pdp_lanes-wide work-group; each lane sums (REDUCEMAX: maximum of
+// error*nlin) its rows in order and stops at its first non-finite error with
+// MAXFLOAT; the lanes' partials meet in the reference's tree (strides from half
+// the next power of two down, lane + stride < pdp_lanes), and an infinite or NaN
+// result becomes MAXFLOAT.  Every thread runs the program on every pass (rows
+// past the end or after a lane's stop are evaluated on row 0 and dropped), so
+// the tape loop stays wave-uniform.  The host adds alpha*size as accelerator.cc:849.
+template <bool P, bool N>
+__device__ __forceinline__ void pdp_exact_body(const EvalArgs& A, const ProgMeta& m, float* raw_out,
+                                               float* vec_out, float* PE) {
+    typedef Core<1, kMaxSlots, P, N> C;
+    const int lanes = A.pdp_lanes;
+    const int passes = (A.nlin + lanes - 1) / lanes;
+    const float fn = (float)A.nlin;
+    for (int l0 = 0; l0 < lanes; l0 += kBlock) {
+        const int l = l0 + (int)threadIdx.x;
+        float pe = 0.0f;
+        bool stop = l >= lanes;
+        for (int j = 0; j < passes; ++j) {
+            const int n = j * lanes + l;
+            const bool live = !stop && n < A.nlin;
+            int row[1] = {live ? n : 0};
+            float acc[1];
+            typename C::Stk stk;
+            C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<1>{A.X, A.ldx, row}, acc, stk);
+            if (live) {
+                const float e = error_fn(A.error_fn, acc[0], A.X[(int64_t)A.ycol * A.ldx + n], A.cost, A.cost_n);
+                if (!finite_f(e)) {
+                    pe = FLT_MAX;
+                    stop = true;
+                } else if (A.reducemax) {
+                    pe = (e * fn > pe) ? e * fn : pe;
+                } else {
+                    pe += e;
+                }
+            }
+        }
+        if (l < lanes) PE[l] = pe;
+    }
+    __syncthreads();
+    int np2 = 1;
+    while (np2 < lanes) np2 <<= 1;
+    for (int s = np2 >> 1; s > 0; s >>= 1) {
+        // Lanes below s read PE[l + s] (>= s, not written at this stride) and write PE[l].
+        for (int l = (int)threadIdx.x; l < s; l += kBlock) {
+            if (l + s < lanes) PE[l] = A.reducemax ? (PE[l + s] > PE[l] ? PE[l + s] : PE[l]) : PE[l] + PE[l + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float raw = finite_f(PE[0]) ? PE[0] / fn : FLT_MAX;
+        *raw_out = raw;
+        *vec_out = raw + A.alpha * (float)m.size;
+    }
+}
+
+__device__ __forceinline__ bool marked_for_lanes(const float* raw) {
+    return __float_as_uint(*raw) == kRecheckBits;
+}
+
+// One workgroup per program; programs whose epilogue did not mark them leave at once.
+template <bool P, bool N>
+__global__ void __launch_bounds__(kBlock) pdp_exact_kernel(EvalArgs A) {
+    __shared__ float PE[kPdpMaxLanes];
+    const int p = blockIdx.x;
+    if (!marked_for_lanes(&A.raw[p])) return;  // uniform over the workgroup
+    stage_math<N>();
+    __syncthreads();
+    const ProgMeta m = load_meta(A.meta, p);
+    pdp_exact_body<P, N>(A, m, &A.raw[p], &A.vec[p], PE);
+}
+
+// The same for the batched form: program g belongs to pair pair_of[g].
+template <bool P, bool N>
+__global__ void __launch_bounds__(kBlock) batch_pdp_exact_kernel(BatchArgs B, const int32_t* pair_of) {
+    __shared__ float PE[kPdpMaxLanes];
+    const int g = blockIdx.x;
+    if (!marked_for_lanes(&B.base.raw[g])) return;
+    stage_math<N>();
+    __syncthreads();
+    const PairDev pd = B.pairs[pair_of[g]];
+    EvalArgs A = B.base;
+    A.X = pd.X;
+    A.ldx = pd.ldx;
+    A.nlin = pd.nlin;
+    A.ncol = pd.ncol;
+    A.ycol = pd.ncol - 1;
+    A.pdp_lanes = pd.pdp_lanes;
+    const ProgMeta m = load_meta(B.base.meta, g);
+    pdp_exact_body<P, N>(A, m, &B.base.raw[g], &B.base.vec[g], PE);
 }
 
 // Ordered form: one thread per program continues its sum (REDUCEMAX: maximum) over the

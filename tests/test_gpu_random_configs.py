@@ -1,0 +1,78 @@
+"""GPU parity over randomly drawn configurations.
+
+Each seed draws a grammar (a random subset of the operator set), a dataset shape (1 to 40
+features, 1 to 3,000 rows), a population (1 to 1,500 programs, 16 to 200 symbols), the
+result semantics (SEQ, PP, FP or PDP strategy, PROTECTED on or off, REDUCEMAX on or off,
+absolute or squared error) and a kernel form, and compares the engine with the oracle:
+bit for bit where the form keeps the reference's row order (PP form; ordered form), and
+with the float-sum bar (within 1e-5 relative, or at least as close to the exactly summed
+row errors as the oracle's own sum) elsewhere.  The seed, shape and choices are printed
+before each case.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import ppiamd
+from helpers import ALL_BINARY, ALL_UNARY, rel_close, sums_agree
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = np.float32(1e-6)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    if ppiamd.device_count() == 0:
+        pytest.fail("no HIP device visible")
+    e = ppiamd.Engine(0)
+    yield e
+    e.close()
+
+
+def draw(seed):
+    rng = np.random.default_rng(1000 + seed)
+    nfeat = int(rng.integers(1, 41))
+    nlin = int(rng.choice([1, 7, 64, 255, 256, 257, 1000, 3000]))
+    nprog = int(rng.choice([1, 5, 300, 1500]))
+    binary = list(rng.choice(ALL_BINARY, size=int(rng.integers(1, len(ALL_BINARY) + 1)), replace=False))
+    unary = list(rng.choice(ALL_UNARY, size=int(rng.integers(0, len(ALL_UNARY) + 1)), replace=False))
+    strategy = int(rng.choice([ppiamd.SEQ, ppiamd.PP, ppiamd.FP, ppiamd.PDP]))
+    forms = [ppiamd.FORM_AUTO, ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_PP]
+    if strategy in (ppiamd.SEQ, ppiamd.PP):
+        forms.append(ppiamd.FORM_ORDERED)
+    return dict(nfeat=nfeat, nlin=nlin, nprog=nprog, binary=binary, unary=unary, strategy=strategy,
+                form=int(rng.choice(forms)), protected=bool(rng.integers(0, 2)), reducemax=bool(rng.integers(0, 2)),
+                error_fn=int(rng.choice([ppiamd.EF_ABS, ppiamd.EF_SQUARED])),
+                max_symbols=int(rng.choice([16, 64, 200])), p_leaf=float(rng.uniform(0.25, 0.5)))
+
+
+@pytest.mark.parametrize("seed", range(96))
+def test_random_configuration(eng, seed):
+    c = draw(seed)
+    print(f"case random seed={seed} {c}", flush=True)
+    gr = ppiamd.Grammar(c["nfeat"], binary=c["binary"], unary=c["unary"])
+    pop = ppiamd.grow(gr, c["nprog"], 7 + seed, 9, c["max_symbols"], p_leaf=c["p_leaf"])
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 3 + seed, c["nlin"], c["nfeat"]) * np.float32(2.0)
+    eng.configure(gr.table, strategy=c["strategy"], error_fn=c["error_fn"], reducemax=c["reducemax"],
+                  protected=c["protected"], form=c["form"], compute_units=256)
+    eng.set_dataset(data)
+    vec, idx = eng.evaluate(pop, alpha=ALPHA, best_size=1)
+    dpl, ppl, bpl = oracle.reference_geometry(data.shape[0], pop.n)
+    cfg = oracle.make_cfg(c["strategy"], c["error_fn"], c["reducemax"], c["protected"], None, dpl, ppl, bpl)
+    ovec, oidx = oracle.interpret(cfg, gr.table, pop, data, alpha=ALPHA, best_size=1)
+    in_order = c["strategy"] in (ppiamd.SEQ, ppiamd.PP) and c["form"] in (ppiamd.FORM_PP, ppiamd.FORM_ORDERED)
+    if in_order:
+        same = vec.view(np.uint32) == ovec.view(np.uint32)
+        assert same.all(), f"programs {np.nonzero(~same)[0][:5]}: {vec[~same][:5]} vs {ovec[~same][:5]}"
+        assert idx[0] == oidx[0]
+    else:
+        ecfg = oracle.make_cfg(ppiamd.SEQ, c["error_fn"], c["reducemax"], c["protected"])
+        ex, bad = oracle.exact_sums(ecfg, gr.table, pop, data)
+        if c["reducemax"]:
+            ok = rel_close(vec, ovec, 1e-5)  # a max is order-independent: only the sentinel rules differ
+        else:
+            ok = sums_agree(vec, ovec, ex, bad, data.shape[0], ALPHA, pop.size)
+        assert ok.all(), f"programs {np.nonzero(~ok)[0][:5]}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
+        best = ovec[oidx[0]]
+        assert np.float32(ovec[idx[0]]) == np.float32(best) or rel_close(ovec[idx[0]], best, 1e-5)
