@@ -149,13 +149,14 @@ int feature_slots(int max_col) {
 
 // Reference geometry of one pair (accelerator.cc:272-355), as the plain engine derives it.
 void pair_geometry(const ppi_batch* b, int32_t nlin, int32_t nprog, int32_t* pdp_lanes, int32_t* dp_groups,
-                   int32_t* best_local) {
+                   int32_t* dp_local_out, int32_t* best_local) {
     const int cu = b->cfg.compute_units > 0 ? b->cfg.compute_units : b->dev_cu;
     int mls1 = b->cfg.max_local_size > 0 ? b->cfg.max_local_size
                                           : (int)std::min<size_t>((size_t)b->dev_max_wg, b->dev_lds / 4);
     int mls2 = (int)std::min<size_t>((size_t)b->dev_max_wg, b->dev_lds / 8);
     int dp_local = std::max(1, std::min(mls1, (int)std::ceil(nlin / (float)cu)));
     *dp_groups = (nlin + dp_local - 1) / dp_local;
+    *dp_local_out = dp_local;
     *pdp_lanes = std::max(1, nlin < mls1 ? nlin : mls1);
     *best_local = std::max(1, std::min(mls2, (int)std::ceil(nprog / (float)cu)));
 }
@@ -195,7 +196,7 @@ void plan_items(const ppi_batch* b, int R, double items_per_slot, std::vector<Pa
         pd.partial_base = part;
         pd.nrb = nrb[k];
         int32_t bl;
-        pair_geometry(b, d.nlin, p.n, &pd.pdp_lanes, &pd.dp_groups, &bl);
+        pair_geometry(b, d.nlin, p.n, &pd.pdp_lanes, &pd.dp_groups, &pd.dp_local, &bl);
         pairs->push_back(pd);
         part += (int64_t)nrb[k] * p.n;
         // Program ranges of about `target` instructions each (an item covers one row tile).
@@ -265,7 +266,7 @@ ppi_status enqueue_run(ppi_batch* b, int32_t n_items, bool with_finalize, bool w
     B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
     if (with_finalize) {
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
-        if (b->cfg.strategy == PPI_STRATEGY_PDP) B_HIP(launch_batch_pdp_exact(B, b->dPairOf, n, b->P, b->N, b->stream));
+        B_HIP(launch_batch_recheck(B, b->dPairOf, n, b->P, b->N, b->stream));
     }
     if (with_download)
         B_HIP(hipMemcpyAsync(b->hRaw, b->dRaw, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
@@ -617,8 +618,8 @@ ppi_status ppi_batch_results(ppi_batch* b, int32_t pop_id, float alpha, float* v
     }
     if (vector) std::memcpy(vector, vec.data(), vec.size() * sizeof(float));
     if (index && best_size && *best_size > 0) {
-        int32_t pdp, dpg, bl;
-        pair_geometry(b, b->datasets[p.dataset].nlin, p.n, &pdp, &dpg, &bl);
+        int32_t pdp, dpg, dpl, bl;
+        pair_geometry(b, b->datasets[p.dataset].nlin, p.n, &pdp, &dpg, &dpl, &bl);
         if (b->cfg.strategy == PPI_STRATEGY_SEQ) {
             if (*best_size > p.n) *best_size = p.n;
             pick_n_best(*best_size, index, p.n, vec.data(), nullptr);
@@ -654,7 +655,7 @@ ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
         B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
         B_HIP(hipEventRecord(b->ev[3], b->stream));
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
-        if (b->cfg.strategy == PPI_STRATEGY_PDP) B_HIP(launch_batch_pdp_exact(B, b->dPairOf, n, b->P, b->N, b->stream));
+        B_HIP(launch_batch_recheck(B, b->dPairOf, n, b->P, b->N, b->stream));
         B_HIP(hipEventSynchronize(b->ev[3]));
         float t = 0.0f;
         B_HIP(hipEventElapsedTime(&t, b->ev[2], b->ev[3]));

@@ -540,6 +540,7 @@ static EvalArgs make_args(ppi_engine* E, float alpha) {
     A.cost_n = E->cfg.cost_n;
     A.pdp_lanes = std::max(1, E->pdp_local);
     A.dp_groups = E->dp_groups;
+    A.dp_local = std::max(1, E->dp_local);
     A.alpha = alpha;
     A.order = E->dOrder;
     A.partial = E->dPartial;
@@ -584,8 +585,8 @@ static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
         PPI_HIP(launch_pp(A, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
     }
-    // PDP strategy: the programs whose result depends on the reference's lane partition.
-    if (E->cfg.strategy == PPI_STRATEGY_PDP) PPI_HIP(launch_pdp_exact(A, P, N, E->stream));
+    // The programs whose result depends on the reference's partition or order of the rows.
+    PPI_HIP(launch_recheck(A, P, N, E->stream));
     return PPI_OK;
 }
 

@@ -51,25 +51,27 @@ hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_pdp_exact(const EvalArgs& A, bool P, bool N, hipStream_t st) {
+hipError_t launch_recheck(const EvalArgs& A, bool P, bool N, hipStream_t st) {
     if (A.nprog <= 0) return hipSuccess;
-    if (A.pdp_lanes < 1 || A.pdp_lanes > kPdpMaxLanes) return hipErrorInvalidValue;
-    dim3 grid((unsigned)A.nprog);
-    if (P && N) hipLaunchKernelGGL((dev::pdp_exact_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, A);
-    else if (P) hipLaunchKernelGGL((dev::pdp_exact_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, A);
-    else if (N) hipLaunchKernelGGL((dev::pdp_exact_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, A);
-    else hipLaunchKernelGGL((dev::pdp_exact_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, A);
+    if (A.pdp_lanes < 1 || A.pdp_lanes > kPdpMaxLanes || A.dp_local < 1 || A.dp_local > kPdpMaxLanes)
+        return hipErrorInvalidValue;
+    dim3 grid((unsigned)((A.nprog + dev::kRecheckSpan - 1) / dev::kRecheckSpan));
+    if (P && N) hipLaunchKernelGGL((dev::recheck_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::recheck_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::recheck_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::recheck_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, A);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_pdp_exact(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
-                                  hipStream_t st) {
+// Every pair's pdp_lanes and dp_local come from the configured work-group limit (<= kPdpMaxLanes).
+hipError_t launch_batch_recheck(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
+                                hipStream_t st) {
     if (n_total <= 0) return hipSuccess;
-    dim3 grid((unsigned)n_total);
-    if (P && N) hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
-    else if (P) hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
-    else if (N) hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
-    else hipLaunchKernelGGL((dev::batch_pdp_exact_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, B, pair_of);
+    dim3 grid((unsigned)((n_total + dev::kRecheckSpan - 1) / dev::kRecheckSpan));
+    if (P && N) hipLaunchKernelGGL((dev::batch_recheck_kernel<true, true>), grid, dim3(dev::kBlock), 0, st, B, pair_of, n_total);
+    else if (P) hipLaunchKernelGGL((dev::batch_recheck_kernel<true, false>), grid, dim3(dev::kBlock), 0, st, B, pair_of, n_total);
+    else if (N) hipLaunchKernelGGL((dev::batch_recheck_kernel<false, true>), grid, dim3(dev::kBlock), 0, st, B, pair_of, n_total);
+    else hipLaunchKernelGGL((dev::batch_recheck_kernel<false, false>), grid, dim3(dev::kBlock), 0, st, B, pair_of, n_total);
     return hipGetLastError();
 }
 

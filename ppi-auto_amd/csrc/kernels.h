@@ -11,7 +11,7 @@
 namespace ppi {
 
 constexpr int kMaxChunk = 1024;  // programs per FP workgroup
-constexpr int kPdpMaxLanes = 1024;  // PDP strategy work-group width at most (the reference's -cl-mls)
+constexpr int kPdpMaxLanes = 1024;  // PDP / DP work-group width at most (the reference's -cl-mls)
 
 // Rows of one FP-form workgroup tile (64 lanes x R rows per lane, shared by its waves).
 inline constexpr int fp_rows_per_block(int R) { return 64 * R; }
@@ -38,6 +38,7 @@ struct EvalArgs {
     int32_t cost_n;
     int32_t pdp_lanes;    // reference evaluate_pdp work-group size (bad-lane bookkeeping)
     int32_t dp_groups;    // reference evaluate_dp work-group count (empty-program rule)
+    int32_t dp_local;     // reference evaluate_dp work-group size (rows per group)
     float alpha;
     const int32_t* order; // PP form: lane -> program (length-sorted), may be null
     float* partial;       // FP form: [row_block][nprog]
@@ -63,7 +64,7 @@ struct PairDev {
     int64_t partial_base;  // first partial of the pair: [row_block][program] block of nrb*nprog
     int32_t nrb;           // row tiles of 64*R rows
     int32_t pdp_lanes, dp_groups;
-    int32_t pad_;
+    int32_t dp_local;
 };
 
 // One work item: a row tile of a pair x a program range [p0, p1) of that pair.
@@ -162,11 +163,11 @@ hipError_t launch_ordered_sum(const EvalArgs& A, int rows_per_block, bool last, 
 hipError_t launch_pdp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hipStream_t st);
 hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st);
 hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st);
-// PDP strategy: recomputes, lane by lane as the reference does, the programs whose epilogue marked
-// them (raw[p] holding the recheck NaN); one workgroup per program, unmarked ones leave at once.
-hipError_t launch_pdp_exact(const EvalArgs& A, bool P, bool N, hipStream_t st);
-hipError_t launch_batch_pdp_exact(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
-                                  hipStream_t st);
+// Recomputes, in the reference's own partition and order (PDP lanes, DP work-groups, row order), the
+// programs whose epilogue marked them (raw[p] holding the recheck NaN); unmarked programs cost one load.
+hipError_t launch_recheck(const EvalArgs& A, bool P, bool N, hipStream_t st);
+hipError_t launch_batch_recheck(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
+                                hipStream_t st);
 hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 // Folded constant subtrees: out[j] = value of fold program j (meta[j], code).

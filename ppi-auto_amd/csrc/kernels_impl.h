@@ -358,30 +358,88 @@ __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
 
 __device__ __forceinline__ bool finite_f(float x) { return !(isinf(x) || isnan(x)); }
 
-// PDP strategy (accelerator.cl:173-268): lane L of the reference's work-group
-// (pdp_lanes of them) takes rows L, L + pdp_lanes, ...; a non-finite row error
-// sets the lane's partial to MAXFLOAT and ends the lane; the partials are
-// combined by a tree, and an infinite result becomes MAXFLOAT.  Whether the
-// result is MAXFLOAT or MAXFLOAT/nlin can therefore depend on which lane an
-// overflowing row falls in (REDUCEMAX: an infinite error*nlin survives only in a
-// lane with no bad row) or on the tree's partial sums near FLT_MAX.  The
-// epilogue settles the cases the (total, bad lanes) summary decides and marks
+// Results that depend on the reference's partition or order of the rows.  The
+// engine reduces rows in its own order; the reference does it
+//  * PDP (accelerator.cl:173-268): lane L of a pdp_lanes-wide work-group takes
+//    rows L, L + pdp_lanes, ...; a non-finite error sets the lane's partial to
+//    MAXFLOAT and ends the lane; the partials meet in a tree; inf -> MAXFLOAT.
+//    MAXFLOAT or MAXFLOAT/nlin then depends on the lane of an overflowing row
+//    (REDUCEMAX: an infinite error*nlin survives only in a lane with no bad row)
+//    and on the tree's partial sums next to FLT_MAX;
+//  * DP (accelerator.cl:88-171, accelerator.cc:773-805): work-groups of dp_local
+//    rows reduce by a tree; the host stops at the first non-finite group result
+//    with FLT_MAX (-> FLT_MAX/nlin).  A group sum that overflows thus gives
+//    FLT_MAX/nlin where an overflow of the host's sum gives FLT_MAX, and the
+//    REDUCEMAX tree, (PE[l+s] > PE[l]) ? PE[l+s] : PE[l], drops a NaN met as the
+//    upper operand and keeps one met as the lower, which then also hides the
+//    upper operand (an inf row included);
+//  * SEQ and PP (sequential.cc:169-179, accelerator.cl:7-86): rows in order, so
+//    only a sum near overflow can differ from a reordered one.
+// The epilogue settles every case its (total, bad-row record) decides and marks
 // the rest by writing kRecheckBits, a NaN no epilogue produces, into raw[p];
-// pdp_exact_kernel then recomputes those programs lane by lane.
+// recheck_kernel recomputes those programs in the reference's partition.  The
+// bad-row record holds, per program, the minimum and maximum of bad_key over
+// the rows with a non-finite error.
 constexpr uint32_t kRecheckBits = 0x7fc0e3c1u;
 
-// Whether a PDP-strategy result needs the lane-exact recomputation.  The error
-// functions other than the cost matrix give non-negative errors, so a partial
-// holding MAXFLOAT stays at or above it and two of them sum to inf; with a cost
-// matrix (entries of either sign) every program with a bad row is recomputed.
-__device__ __forceinline__ bool pdp_needs_lanes(const EvalArgs& A, float total, bool bad, uint32_t kmin,
-                                                uint32_t kmax) {
-    if (A.reducemax) return bad && !finite_f(total);  // no bad row: max is order-free
-    if (!bad) return !(total < 0x1.0p126f);           // a reordered sum near overflow
-    if (A.error_fn >= PPI_EF_COST) return true;
-    // One bad lane: MAXFLOAT + x rounds to inf only for x >= 2^103 (half an ulp
-    // of FLT_MAX); every tree partial is at most the total of the other rows.
-    return kmin == kmax && !(total < 0x1.0p102f);
+// PDP: the row's reference lane; other strategies: 1 for a NaN error, 0 for an infinite one.
+__device__ __forceinline__ uint32_t bad_key(const EvalArgs& A, int row, float e) {
+    return A.strategy == PPI_STRATEGY_PDP ? (uint32_t)row % (uint32_t)A.pdp_lanes : (isnan(e) ? 1u : 0u);
+}
+
+// A sum of row errors computed in another order than the reference's: the
+// epilogue's float total and whether the reference's own rounding decides if its
+// sum overflows.  The engine's sum arrives as D, the sum in double of its finite
+// float partials, plus ninf partials that overflowed; gb bounds a partial's
+// relative rounding error.  For non-negative terms any order's float sum lies
+// within a relative (n-1)u of the exact sum S (u = 2^-24; recursive summation is
+// the worst case, the reference's lane and group trees are tighter) and
+// overflows exactly when it reaches thr = 2^128 - 2^103, so S(1 + nu) < thr is
+// finite in every order and S(1 - nu) >= thr infinite in every order; in between
+// the program is rechecked.  An overflowed partial's exact sum is at least
+// thr(1 - gb).  Cost-matrix errors may be negative, so their sums are trusted
+// only far from overflow.
+struct SumTotal {
+    float total;
+    bool undecided;
+};
+__device__ __forceinline__ SumTotal decide_sum(const EvalArgs& A, double D, double gb, int ninf = 0) {
+    const double thr = 0x1.0p128 - 0x1.0p103;
+    const double gr = 1.01 * (double)A.nlin * 0x1.0p-24 + 1e-12;
+    if (!(D == D)) return {(float)D, true};
+    if (A.error_fn >= PPI_EF_COST) return {(float)D, ninf > 0 || !(fabs(D) < 0x1.0p120)};
+    if (ninf > 0 || isinf(D)) {
+        const double lower = ((double)ninf * thr + (isinf(D) ? 0.0 : D)) * (1.0 - gb);
+        return {__builtin_inff(), !(lower * (1.0 - gr) >= thr)};
+    }
+    if (D * (1.0 + gr) < thr * (1.0 - gb)) return {(float)D, false};
+    if (D * (1.0 - gb) * (1.0 - gr) >= thr) return {__builtin_inff(), false};
+    return {(float)D, true};
+}
+constexpr double kUlp = 0x1.0p-24 * 1.01;
+
+// Whether a result needs the recomputation.  undecided: from decide_sum (false for
+// maxima and for the row-order sums of the PP and ordered forms under SEQ / PP).
+// The error functions other than the cost matrix give non-negative errors, so a
+// PDP partial holding MAXFLOAT stays at or above it and two of them sum to inf;
+// with a cost matrix (entries of either sign) every PDP program with a bad row is
+// recomputed.
+__device__ __forceinline__ bool needs_recheck(const EvalArgs& A, float total, bool bad, uint32_t kmin,
+                                              uint32_t kmax, bool undecided) {
+    if (A.strategy == PPI_STRATEGY_PDP) {
+        if (A.reducemax) return bad && !finite_f(total);  // no bad row: a maximum is order-free
+        if (!bad) return undecided;
+        if (A.error_fn >= PPI_EF_COST) return true;
+        // One bad lane: MAXFLOAT + x rounds to inf only for x >= 2^103 (half an ulp
+        // of FLT_MAX); every tree partial is at most the total of the other rows.
+        return kmin == kmax && !(total < 0x1.0p102f);
+    }
+    if (A.strategy == PPI_STRATEGY_FP) {
+        if (A.reducemax) return bad && kmax == 1u;  // a NaN row
+        // An overflow gives FLT_MAX/nlin in a work-group's sum and FLT_MAX in the host's.
+        return !bad && (undecided || !finite_f(total));
+    }
+    return !A.reducemax && !bad && undecided;
 }
 
 // Per-program result from (reduced value, bad-row record).  Restates the
@@ -389,7 +447,8 @@ __device__ __forceinline__ bool pdp_needs_lanes(const EvalArgs& A, float total, 
 // accelerator.cc:849 (PP), accelerator.cl:157-171 + accelerator.cc:773-805 (DP),
 // accelerator.cl:236-267 + accelerator.cc:849 (PDP).
 __device__ __forceinline__ void semantics(const EvalArgs& A, float total, uint32_t flags, uint32_t kmin,
-                                          uint32_t kmax, int32_t size, float* raw_out, float* vec_out) {
+                                          uint32_t kmax, int32_t size, float* raw_out, float* vec_out,
+                                          bool undecided = false) {
     const float fn = (float)A.nlin;
     const float as = A.alpha * (float)size;
     float raw, vec;
@@ -402,7 +461,7 @@ __device__ __forceinline__ void semantics(const EvalArgs& A, float total, uint32
             raw = FLT_MAX;
             vec = (A.strategy == PPI_STRATEGY_SEQ) ? FLT_MAX : raw + as;
         }
-    } else if (A.strategy == PPI_STRATEGY_PDP && pdp_needs_lanes(A, total, bad, kmin, kmax)) {
+    } else if (needs_recheck(A, total, bad, kmin, kmax, undecided)) {
         raw = __uint_as_float(kRecheckBits);
         vec = raw;
     } else if (bad) {
@@ -470,7 +529,7 @@ __device__ __forceinline__ void accumulate(const EvalArgs& A, const float (&acc)
         if (!valid[r]) continue;
         const float e = error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n);
         if (!finite_f(e)) {
-            const uint32_t key = (uint32_t)row[r] % (uint32_t)A.pdp_lanes;
+            const uint32_t key = bad_key(A, row[r], e);
             kmin = min(kmin, key);
             kmax = max(kmax, key);
         } else if (A.reducemax) {
@@ -651,6 +710,27 @@ __global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S>::value)) batch_kern
 }
 
 #ifdef PPI_KERNELS_MISC
+// A program's row-tile partials (stride apart) reduced in tile order: the maximum,
+// or the sum in double.  A partial is the wave's DPP tree (6 levels) over lane
+// sums of at most 8 rows, so every row error passes through at most 14 float
+// roundings before the double sum.
+__device__ __forceinline__ SumTotal reduce_partials(const EvalArgs& A, const float* partial, int nrb,
+                                                    int64_t stride) {
+    if (A.reducemax) {
+        float mx = 0.0f;
+        for (int rb = 0; rb < nrb; ++rb) mx = fmaxf(mx, partial[(int64_t)rb * stride]);
+        return {mx, false};
+    }
+    double sum = 0.0;
+    int ninf = 0;
+    for (int rb = 0; rb < nrb; ++rb) {
+        const float v = partial[(int64_t)rb * stride];
+        if (isinf(v) && v > 0.0f) ++ninf;
+        else sum += (double)v;
+    }
+    return decide_sum(A, sum, 14.0 * kUlp + (double)nrb * 0x1.0p-53, ninf);
+}
+
 // Per-program epilogue of the batched form: program g belongs to pair pair_of[g].
 __global__ void batch_finalize_kernel(BatchArgs B, const int32_t* pair_of, int32_t n_total) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -661,119 +741,18 @@ __global__ void batch_finalize_kernel(BatchArgs B, const int32_t* pair_of, int32
     A.nlin = pd.nlin;
     A.dp_groups = pd.dp_groups;
     A.pdp_lanes = pd.pdp_lanes;
-    float total = 0.0f;
-    for (int rb = 0; rb < pd.nrb; ++rb) {
-        const float v = B.base.partial[pd.partial_base + (int64_t)rb * pd.nprog + p];
-        total = A.reducemax ? fmaxf(total, v) : total + v;
-    }
+    const SumTotal t = reduce_partials(A, B.base.partial + pd.partial_base + p, pd.nrb, pd.nprog);
     const ProgMeta m = B.base.meta[g];
-    semantics(A, total, m.flags, B.base.bad_lo[g], B.base.bad_hi[g], m.size, &B.base.raw[g], &B.base.vec[g]);
+    semantics(A, t.total, m.flags, B.base.bad_lo[g], B.base.bad_hi[g], m.size, &B.base.raw[g], &B.base.vec[g],
+              t.undecided);
 }
 
 __global__ void finalize_kernel(EvalArgs A, int nrb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.nprog) return;
-    float total = 0.0f;
-    for (int rb = 0; rb < nrb; ++rb) {
-        const float v = A.partial[(int64_t)rb * A.nprog + p];
-        total = A.reducemax ? fmaxf(total, v) : total + v;
-    }
+    const SumTotal t = reduce_partials(A, A.partial + p, nrb, A.nprog);
     const ProgMeta m = A.meta[p];
-    semantics(A, total, m.flags, A.bad_lo[p], A.bad_hi[p], m.size, &A.raw[p], &A.vec[p]);
-}
-
-// The PDP strategy's result, computed as the reference's evaluate_pdp does
-// (accelerator.cl:196-267): thread t plays lanes t, t + kBlock, ... of the
-// reference's pdp_lanes-wide work-group; each lane sums (REDUCEMAX: maximum of
-// error*nlin) its rows in order and stops at its first non-finite error with
-// MAXFLOAT; the lanes' partials meet in the reference's tree (strides from half
-// the next power of two down, lane + stride < pdp_lanes), and an infinite or NaN
-// result becomes MAXFLOAT.  Every thread runs the program on every pass (rows
-// past the end or after a lane's stop are evaluated on row 0 and dropped), so
-// the tape loop stays wave-uniform.  The host adds alpha*size as accelerator.cc:849.
-template <bool P, bool N>
-__device__ __forceinline__ void pdp_exact_body(const EvalArgs& A, const ProgMeta& m, float* raw_out,
-                                               float* vec_out, float* PE) {
-    typedef Core<1, kMaxSlots, P, N> C;
-    const int lanes = A.pdp_lanes;
-    const int passes = (A.nlin + lanes - 1) / lanes;
-    const float fn = (float)A.nlin;
-    for (int l0 = 0; l0 < lanes; l0 += kBlock) {
-        const int l = l0 + (int)threadIdx.x;
-        float pe = 0.0f;
-        bool stop = l >= lanes;
-        for (int j = 0; j < passes; ++j) {
-            const int n = j * lanes + l;
-            const bool live = !stop && n < A.nlin;
-            int row[1] = {live ? n : 0};
-            float acc[1];
-            typename C::Stk stk;
-            C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<1>{A.X, A.ldx, row}, acc, stk);
-            if (live) {
-                const float e = error_fn(A.error_fn, acc[0], A.X[(int64_t)A.ycol * A.ldx + n], A.cost, A.cost_n);
-                if (!finite_f(e)) {
-                    pe = FLT_MAX;
-                    stop = true;
-                } else if (A.reducemax) {
-                    pe = (e * fn > pe) ? e * fn : pe;
-                } else {
-                    pe += e;
-                }
-            }
-        }
-        if (l < lanes) PE[l] = pe;
-    }
-    __syncthreads();
-    int np2 = 1;
-    while (np2 < lanes) np2 <<= 1;
-    for (int s = np2 >> 1; s > 0; s >>= 1) {
-        // Lanes below s read PE[l + s] (>= s, not written at this stride) and write PE[l].
-        for (int l = (int)threadIdx.x; l < s; l += kBlock) {
-            if (l + s < lanes) PE[l] = A.reducemax ? (PE[l + s] > PE[l] ? PE[l + s] : PE[l]) : PE[l] + PE[l + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const float raw = finite_f(PE[0]) ? PE[0] / fn : FLT_MAX;
-        *raw_out = raw;
-        *vec_out = raw + A.alpha * (float)m.size;
-    }
-}
-
-__device__ __forceinline__ bool marked_for_lanes(const float* raw) {
-    return __float_as_uint(*raw) == kRecheckBits;
-}
-
-// One workgroup per program; programs whose epilogue did not mark them leave at once.
-template <bool P, bool N>
-__global__ void __launch_bounds__(kBlock) pdp_exact_kernel(EvalArgs A) {
-    __shared__ float PE[kPdpMaxLanes];
-    const int p = blockIdx.x;
-    if (!marked_for_lanes(&A.raw[p])) return;  // uniform over the workgroup
-    stage_math<N>();
-    __syncthreads();
-    const ProgMeta m = load_meta(A.meta, p);
-    pdp_exact_body<P, N>(A, m, &A.raw[p], &A.vec[p], PE);
-}
-
-// The same for the batched form: program g belongs to pair pair_of[g].
-template <bool P, bool N>
-__global__ void __launch_bounds__(kBlock) batch_pdp_exact_kernel(BatchArgs B, const int32_t* pair_of) {
-    __shared__ float PE[kPdpMaxLanes];
-    const int g = blockIdx.x;
-    if (!marked_for_lanes(&B.base.raw[g])) return;
-    stage_math<N>();
-    __syncthreads();
-    const PairDev pd = B.pairs[pair_of[g]];
-    EvalArgs A = B.base;
-    A.X = pd.X;
-    A.ldx = pd.ldx;
-    A.nlin = pd.nlin;
-    A.ncol = pd.ncol;
-    A.ycol = pd.ncol - 1;
-    A.pdp_lanes = pd.pdp_lanes;
-    const ProgMeta m = load_meta(B.base.meta, g);
-    pdp_exact_body<P, N>(A, m, &B.base.raw[g], &B.base.vec[g], PE);
+    semantics(A, t.total, m.flags, A.bad_lo[p], A.bad_hi[p], m.size, &A.raw[p], &A.vec[p], t.undecided);
 }
 
 // Ordered form: one thread per program continues its sum (REDUCEMAX: maximum) over the
@@ -830,6 +809,267 @@ __global__ void __launch_bounds__(256) ordered_sum_kernel(EvalArgs A, int rows_p
     A.ostop[p] = stop;
     if (last) semantics(A, part, m.flags, stop ? 0u : 0xffffffffu, 0u, m.size, &A.raw[p], &A.vec[p]);
 }
+
+// ---------------------------------------------------------------------------
+// Recheck: programs whose epilogue marked them (kRecheckBits in raw[p]) are
+// recomputed in the reference's own partition and order of the rows.  Every
+// thread of the workgroup runs the program on every pass (rows past the end or
+// after a stop are evaluated on row 0 and dropped), so the tape loop stays
+// wave-uniform.  Each body ends with thread 0 writing the result; the caller's
+// barrier follows.
+// ---------------------------------------------------------------------------
+constexpr int kRecheckSpan = 16;  // programs per recheck workgroup
+
+// Error of program m at row n.
+template <bool P, bool N>
+__device__ __forceinline__ float row_error(const EvalArgs& A, const ProgMeta& m, int n) {
+    typedef Core<1, kMaxSlots, P, N> C;
+    int row[1] = {n};
+    float acc[1];
+    typename C::Stk stk;
+    C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<1>{A.X, A.ldx, row}, acc, stk);
+    return error_fn(A.error_fn, acc[0], A.X[(int64_t)A.ycol * A.ldx + n], A.cost, A.cost_n);
+}
+
+// The reference's work-group reduction over PE[0, width) (accelerator.cl:157-166,
+// 251-260): strides from half the next power of two of width down to 1; at stride
+// s, lane l < s with l + s < width takes PE[l + s] (sum, or REDUCEMAX's
+// (PE[l+s] > PE[l]) ? PE[l+s] : PE[l]).  Lanes below s read PE[l + s] (>= s, not
+// written at this stride).  Entered after a barrier; leaves the result in PE[0].
+__device__ __forceinline__ void reference_tree(float* PE, int width, bool rmax) {
+    int np2 = 1;
+    while (np2 < width) np2 <<= 1;
+    for (int s = np2 >> 1; s > 0; s >>= 1) {
+        for (int l = (int)threadIdx.x; l < s; l += kBlock) {
+            if (l + s < width) PE[l] = rmax ? ((PE[l + s] > PE[l]) ? PE[l + s] : PE[l]) : PE[l] + PE[l + s];
+        }
+        __syncthreads();
+    }
+}
+
+// PDP (accelerator.cl:196-267): thread t plays lanes t, t + kBlock, ... of the
+// pdp_lanes-wide work-group; a lane sums (REDUCEMAX: maximum of error*nlin) its
+// rows in order and stops at its first non-finite error with MAXFLOAT; an
+// infinite or NaN tree result becomes MAXFLOAT.  The host adds alpha*size
+// (accelerator.cc:849).
+template <bool P, bool N>
+__device__ __forceinline__ void pdp_recheck(const EvalArgs& A, const ProgMeta& m, float* raw_out, float* vec_out,
+                                            float* PE) {
+    const int lanes = A.pdp_lanes;
+    const int passes = (A.nlin + lanes - 1) / lanes;
+    const float fn = (float)A.nlin;
+    for (int l0 = 0; l0 < lanes; l0 += kBlock) {
+        const int l = l0 + (int)threadIdx.x;
+        float pe = 0.0f;
+        bool stop = l >= lanes;
+        for (int j = 0; j < passes; ++j) {
+            const int n = j * lanes + l;
+            const bool live = !stop && n < A.nlin;
+            const float e = row_error<P, N>(A, m, live ? n : 0);
+            if (live) {
+                if (!finite_f(e)) {
+                    pe = FLT_MAX;
+                    stop = true;
+                } else if (A.reducemax) {
+                    pe = (e * fn > pe) ? e * fn : pe;
+                } else {
+                    pe += e;
+                }
+            }
+        }
+        if (l < lanes) PE[l] = pe;
+    }
+    __syncthreads();
+    reference_tree(PE, lanes, A.reducemax != 0);
+    if (threadIdx.x == 0) {
+        const float raw = finite_f(PE[0]) ? PE[0] / fn : FLT_MAX;
+        *raw_out = raw;
+        *vec_out = raw + A.alpha * (float)m.size;
+    }
+}
+
+// DP (accelerator.cl:113-167, accelerator.cc:773-805): work-group g holds the
+// error of row g*dp_local + l in lane l (0 past the last row) and reduces by the
+// tree; the host walks the groups in order, stops at the first non-finite group
+// result with FLT_MAX, and otherwise sums them (REDUCEMAX: maximum of
+// result*nlin); a non-finite total becomes FLT_MAX, else total/nlin + alpha*size.
+template <bool P, bool N>
+__device__ __forceinline__ void dp_recheck(const EvalArgs& A, const ProgMeta& m, float* raw_out, float* vec_out,
+                                           float* PE, int* stop_s) {
+    const int width = A.dp_local;
+    const int groups = (A.nlin + width - 1) / width;
+    const float fn = (float)A.nlin;
+    float sum = 0.0f;  // thread 0's
+    for (int g = 0; g < groups; ++g) {
+        for (int l0 = 0; l0 < width; l0 += kBlock) {
+            const int l = l0 + (int)threadIdx.x;
+            const int n = g * width + l;
+            const bool live = l < width && n < A.nlin;
+            const float e = row_error<P, N>(A, m, live ? n : 0);
+            if (l < width) PE[l] = live ? e : 0.0f;
+        }
+        __syncthreads();
+        reference_tree(PE, width, A.reducemax != 0);
+        if (threadIdx.x == 0) {
+            const float v = PE[0];
+            if (!finite_f(v)) {
+                sum = FLT_MAX;
+                *stop_s = 1;
+            } else if (A.reducemax) {
+                sum = (v * fn > sum) ? v * fn : sum;
+            } else {
+                sum += v;
+            }
+        }
+        __syncthreads();  // PE[0] read and the stop published before PE is rewritten
+        if (*stop_s) break;
+    }
+    if (threadIdx.x == 0) {
+        const float r = finite_f(sum) ? sum / fn + A.alpha * (float)m.size : FLT_MAX;
+        *raw_out = r;
+        *vec_out = r;
+    }
+}
+
+// SEQ / PP (sequential.cc:169-179, accelerator.cl:40-84): rows in order, the
+// errors of kBlock rows at a time staged in LDS and taken by thread 0.
+template <bool P, bool N>
+__device__ __forceinline__ void ordered_recheck(const EvalArgs& A, const ProgMeta& m, float* raw_out,
+                                                float* vec_out, float* PE, int* stop_s) {
+    const float fn = (float)A.nlin;
+    const bool rmax = A.reducemax != 0;
+    float part = 0.0f;  // thread 0's
+    uint32_t stop = 0;
+    for (int n0 = 0; n0 < A.nlin; n0 += kBlock) {
+        const int n = n0 + (int)threadIdx.x;
+        const float e = row_error<P, N>(A, m, n < A.nlin ? n : 0);
+        PE[threadIdx.x] = e;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // 64 errors per batch, read as float4 before the serial takes.
+            const int cnt = min(kBlock, A.nlin - n0);
+            const float4* v4 = reinterpret_cast<const float4*>(PE);
+            for (int b = 0; b < cnt && !stop; b += 64) {
+                float4 q[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) q[k] = v4[b / 4 + k];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int j = b + 4 * k;
+                    if (j + 0 < cnt) ordered_take(q[k].x, fn, rmax, part, stop);
+                    if (j + 1 < cnt) ordered_take(q[k].y, fn, rmax, part, stop);
+                    if (j + 2 < cnt) ordered_take(q[k].z, fn, rmax, part, stop);
+                    if (j + 3 < cnt) ordered_take(q[k].w, fn, rmax, part, stop);
+                }
+            }
+            *stop_s = (int)stop;
+        }
+        __syncthreads();
+        if (*stop_s) break;
+    }
+    if (threadIdx.x == 0) semantics(A, part, m.flags, stop ? 0u : 0xffffffffu, 0u, m.size, raw_out, vec_out);
+}
+
+// The sum of every row error in double, in every thread (relative error about
+// nlin/256 * 2^-53 against the exact sum).  red: kBlock doubles of LDS.
+template <bool P, bool N>
+__device__ __forceinline__ double sum_rows_double(const EvalArgs& A, const ProgMeta& m, double* red) {
+    double s = 0.0;
+    for (int n0 = 0; n0 < A.nlin; n0 += kBlock) {
+        const int n = n0 + (int)threadIdx.x;
+        const float e = row_error<P, N>(A, m, n < A.nlin ? n : 0);
+        if (n < A.nlin) s += (double)e;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();  // every thread has read red[0] before the buffer is reused
+    return r;
+}
+
+// A program marked for its sum is first settled from the double sum when that sum
+// is clear of float overflow by more than any order's rounding (and, under DP,
+// finite: an overflow's sentinel depends on the work-groups); the rest take the
+// partition-exact path.  bad: the bad-row record where the form keeps one (FP and
+// batched forms); elsewhere a bad row makes the double sum NaN or infinite, which
+// also sends the program on.
+template <bool P, bool N>
+__device__ __forceinline__ void recheck_program(const EvalArgs& A, const ProgMeta& m, bool bad, float* raw_out,
+                                                float* vec_out, float* PE, int* stop_s) {
+    if (!bad && !A.reducemax) {
+        const SumTotal t = decide_sum(A, sum_rows_double<P, N>(A, m, reinterpret_cast<double*>(PE)), 1e-9);
+        if (!t.undecided && (A.strategy != PPI_STRATEGY_FP || finite_f(t.total))) {
+            if (threadIdx.x == 0) semantics(A, t.total, m.flags, 0xffffffffu, 0u, m.size, raw_out, vec_out);
+            return;
+        }
+    }
+    if (A.strategy == PPI_STRATEGY_PDP) pdp_recheck<P, N>(A, m, raw_out, vec_out, PE);
+    else if (A.strategy == PPI_STRATEGY_FP) dp_recheck<P, N>(A, m, raw_out, vec_out, PE, stop_s);
+    else ordered_recheck<P, N>(A, m, raw_out, vec_out, PE, stop_s);
+}
+
+__device__ __forceinline__ bool marked(const float* raw) { return __float_as_uint(*raw) == kRecheckBits; }
+
+// Workgroup b looks at programs [b*kRecheckSpan, (b+1)*kRecheckSpan); the mark is
+// read by every thread from the same address, so the branches are uniform.
+template <bool P, bool N>
+__global__ void __launch_bounds__(kBlock) recheck_kernel(EvalArgs A) {
+    __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
+    __shared__ int stop_s;
+    bool staged = false;
+    for (int q = 0; q < kRecheckSpan; ++q) {
+        const int p = (int)blockIdx.x * kRecheckSpan + q;
+        if (p >= A.nprog) break;
+        if (!marked(&A.raw[p])) continue;
+        if (!staged) {
+            stage_math<N>();
+            staged = true;
+        }
+        if (threadIdx.x == 0) stop_s = 0;
+        __syncthreads();
+        recheck_program<P, N>(A, load_meta(A.meta, p), A.bad_lo[p] != 0xffffffffu, &A.raw[p], &A.vec[p], PE,
+                              &stop_s);
+        __syncthreads();
+    }
+}
+
+// The same for the batched form: program g belongs to pair pair_of[g].
+template <bool P, bool N>
+__global__ void __launch_bounds__(kBlock) batch_recheck_kernel(BatchArgs B, const int32_t* pair_of,
+                                                              int32_t n_total) {
+    __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
+    __shared__ int stop_s;
+    bool staged = false;
+    for (int q = 0; q < kRecheckSpan; ++q) {
+        const int g = (int)blockIdx.x * kRecheckSpan + q;
+        if (g >= n_total) break;
+        if (!marked(&B.base.raw[g])) continue;
+        if (!staged) {
+            stage_math<N>();
+            staged = true;
+        }
+        if (threadIdx.x == 0) stop_s = 0;
+        __syncthreads();
+        const PairDev pd = B.pairs[pair_of[g]];
+        EvalArgs A = B.base;
+        A.X = pd.X;
+        A.ldx = pd.ldx;
+        A.nlin = pd.nlin;
+        A.ncol = pd.ncol;
+        A.ycol = pd.ncol - 1;
+        A.nprog = pd.nprog;
+        A.pdp_lanes = pd.pdp_lanes;
+        A.dp_groups = pd.dp_groups;
+        A.dp_local = pd.dp_local;
+        recheck_program<P, N>(A, load_meta(B.base.meta, g), B.base.bad_lo[g] != 0xffffffffu, &B.base.raw[g],
+                              &B.base.vec[g], PE, &stop_s);
+        __syncthreads();
+    }
+}
 #endif  // PPI_KERNELS_MISC
 
 // ---------------------------------------------------------------------------
@@ -873,14 +1113,21 @@ __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
     if (lane == 0) { red[wave] = part; rlo[wave] = kmin; rhi[wave] = kmax; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        float v = red[0];
+        float mx = red[0];
+        double sum = 0.0;
+        int ninf = 0;
         uint32_t lo = rlo[0], hi = rhi[0];
-        for (int w = 1; w < kWaves; ++w) {
-            v = A.reducemax ? fmaxf(v, red[w]) : v + red[w];
+        for (int w = 0; w < kWaves; ++w) {
+            mx = fmaxf(mx, red[w]);
+            if (isinf(red[w]) && red[w] > 0.0f) ++ninf;
+            else sum += (double)red[w];
             lo = min(lo, rlo[w]);
             hi = max(hi, rhi[w]);
         }
-        semantics(A, v, m.flags, lo, hi, m.size, &A.raw[p], &A.vec[p]);
+        // A lane sums its rows serially (at most nlin / 64 of them), then the wave's DPP tree (6 levels).
+        const SumTotal t = A.reducemax ? SumTotal{mx, false}
+                                       : decide_sum(A, sum, ((double)A.nlin / 64.0 + 8.0) * kUlp + 1e-15, ninf);
+        semantics(A, t.total, m.flags, lo, hi, m.size, &A.raw[p], &A.vec[p], t.undecided);
     }
 }
 
@@ -898,7 +1145,11 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
     const ProgMeta m = A.meta[p];
     float part = 0.0f;
     uint32_t kmin = 0xffffffffu, kmax = 0u;
-    const bool stop_at_bad = A.strategy != PPI_STRATEGY_PDP;
+    // SEQ / PP stop at the first bad row (sequential.cc:173, accelerator.cl:64); DP sums
+    // only need to know that one exists; the DP maximum keeps looking for a NaN row;
+    // PDP needs every bad lane.
+    const bool in_order = A.strategy == PPI_STRATEGY_SEQ || A.strategy == PPI_STRATEGY_PP;
+    const bool stop_at_bad = in_order || (A.strategy == PPI_STRATEGY_FP && !A.reducemax);
     const float fn = (float)A.nlin;
     if (!(m.flags & kEmpty)) {
         const Ins* code = A.code + m.code_off;
@@ -935,10 +1186,10 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
             }
             const float e = error_fn(A.error_fn, acc, A.X[(int64_t)A.ycol * A.ldx + row], A.cost, A.cost_n);
             if (!finite_f(e)) {
-                const uint32_t key = (uint32_t)row % (uint32_t)A.pdp_lanes;
+                const uint32_t key = bad_key(A, row, e);
                 kmin = min(kmin, key);
                 kmax = max(kmax, key);
-                if (stop_at_bad) break;  // sequential.cc:173 / accelerator.cl:64
+                if (stop_at_bad || (A.strategy == PPI_STRATEGY_FP && kmax == 1u)) break;
             } else if (A.reducemax) {
                 part = (e * fn > part) ? e * fn : part;
             } else {
@@ -946,7 +1197,10 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
             }
         }
     }
-    semantics(A, part, m.flags, kmin, kmax, m.size, &A.raw[p], &A.vec[p]);
+    // SEQ / PP: the row-order sum is the reference's own; otherwise a serial sum over nlin rows.
+    const SumTotal tot = (in_order || A.reducemax) ? SumTotal{part, false}
+                                                   : decide_sum(A, (double)part, (double)A.nlin * kUlp);
+    semantics(A, tot.total, m.flags, kmin, kmax, m.size, &A.raw[p], &A.vec[p], tot.undecided);
 }
 
 // ---------------------------------------------------------------------------
