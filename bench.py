@@ -294,6 +294,9 @@ def main():
             wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(500), w.data, w.error_fn)
             pp = bench_engine(eng, wpp, ppiamd.FORM_PP, 1, 0)
             line["pp_gpops"] = round(pp["node_evals"] / (pp["ms_per_step"] * 1e-3) / 1e9, 3)
+            line["pp_note"] = ("the reference's PP geometry, one lane per program with rows in order: 500 programs "
+                               "fill 2 workgroups of 256 CUs; kept for fidelity, the ordered form gives the same "
+                               "bits at ordered_gpops")
             log("timing whole generations (compile + upload + evaluate + results)")
             gen = bench_generations(eng, w, 10)
             gen["gpops_end_to_end"] = round(fp["node_evals"] / (gen["ms_per_generation"] * 1e-3) / 1e9, 3)
