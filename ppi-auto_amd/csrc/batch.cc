@@ -335,6 +335,10 @@ ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_sy
         set_error("max_local_size must be in [0, " + std::to_string(kPdpMaxLanes) + "]");
         return PPI_E_ARG;
     }
+    if (cfg->max_local_size == 0 && std::min<size_t>((size_t)b->dev_max_wg, b->dev_lds / 4) > (size_t)kPdpMaxLanes) {
+        set_error("this device's work-group limit exceeds " + std::to_string(kPdpMaxLanes) + "; set max_local_size");
+        return PPI_E_ARG;
+    }
     if (cfg->rows_per_lane != 0 && cfg->rows_per_lane != 2 && cfg->rows_per_lane != 4 && cfg->rows_per_lane != 8) {
         set_error("batched rows_per_lane must be 0 (autotune), 2, 4 or 8");
         return PPI_E_ARG;

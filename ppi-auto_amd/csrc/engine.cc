@@ -357,6 +357,10 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
         set_error("max_local_size must be in [0, " + std::to_string(kPdpMaxLanes) + "]");
         return PPI_E_ARG;
     }
+    if (cfg->max_local_size == 0 && std::min<size_t>((size_t)E->dev_max_wg, E->dev_lds / 4) > (size_t)kPdpMaxLanes) {
+        set_error("this device's work-group limit exceeds " + std::to_string(kPdpMaxLanes) + "; set max_local_size");
+        return PPI_E_ARG;
+    }
     if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
     bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
     if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs cost_n > 0 and cost"); return PPI_E_ARG; }

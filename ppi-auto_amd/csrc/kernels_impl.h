@@ -523,19 +523,29 @@ template <int R>
 __device__ __forceinline__ void accumulate(const EvalArgs& A, const float (&acc)[R], const float (&y)[R],
                                            const bool (&valid)[R], const int (&row)[R], float& part,
                                            uint32_t& kmin, uint32_t& kmax) {
+    // Branch-free over the rows (a row that is past the end or not finite adds 0, which
+    // leaves the partial's bits unchanged: it is never -0 and a maximum of non-negative
+    // values is at least 0); the bad-row record is one rarely taken branch.
     const float fn = (float)A.nlin;
+    float e[R];
+    bool any_bad = false;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        if (!valid[r]) continue;
-        const float e = error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n);
-        if (!finite_f(e)) {
-            const uint32_t key = bad_key(A, row[r], e);
-            kmin = min(kmin, key);
-            kmax = max(kmax, key);
-        } else if (A.reducemax) {
-            part = fmaxf(part, e * fn);
-        } else {
-            part += e;
+        e[r] = error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n);
+        const bool fin = finite_f(e[r]);
+        any_bad = any_bad || (valid[r] && !fin);
+        const bool take = valid[r] && fin;
+        if (A.reducemax) part = fmaxf(part, take ? e[r] * fn : 0.0f);
+        else part += take ? e[r] : 0.0f;
+    }
+    if (any_bad) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (valid[r] && !finite_f(e[r])) {
+                const uint32_t key = bad_key(A, row[r], e[r]);
+                kmin = min(kmin, key);
+                kmax = max(kmax, key);
+            }
         }
     }
 }
