@@ -116,7 +116,7 @@ typedef struct ppi_config {
     const float* cost;
     /* Reference launch geometry that shapes result semantics (accelerator.cc:272-355).
        <= 0 selects what acc_interpret_init computes on this device. */
-    int32_t max_local_size; /* -cl-mls */
+    int32_t max_local_size; /* -cl-mls: 0 = the device's limit; at most 1024 (PPI_E_ARG above) */
     int32_t compute_units;  /* CL_DEVICE_MAX_COMPUTE_UNITS */
     /* Tuning; 0 = autotune. */
     int32_t rows_per_lane;  /* 1, 2, 4 or 8 */
