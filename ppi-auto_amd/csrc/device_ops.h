@@ -190,6 +190,14 @@ template <int R, bool P, bool N> __device__ __forceinline__ void unary_rows(uint
     for (int r = 0; r < R; ++r) v[r] = unary<P, N>(u, v[r]);
 }
 
+// fmin / fmax as the reference's x86-64 C library computes them (minsd / maxsd): the
+// first operand when it is strictly smaller (larger), otherwise the second; a NaN
+// operand yields the other one.  v_min_f32 / v_max_f32 order -0 below +0, so on equal
+// zeros they can return the other operand than the reference, and the sign then
+// reaches results such as gamma(+-0) = +-inf.
+__device__ __forceinline__ float ref_fmin(float x, float y) { return (x < y || isnan(y)) ? x : y; }
+__device__ __forceinline__ float ref_fmax(float x, float y) { return (x > y || isnan(y)) ? x : y; }
+
 // Binary operators; `a` is the accumulator, `b` the folded or parked operand.
 // Non-R forms compute f(a, b); R forms compute f(b, a).
 template <bool P, bool N> __device__ __forceinline__ float binary(uint32_t op, float a, float b) {
@@ -201,10 +209,10 @@ template <bool P, bool N> __device__ __forceinline__ float binary(uint32_t op, f
     case B_DIV: return f_div<P, N>(a, b);
     case B_RDIV: return f_div<P, N>(b, a);
     case B_MEAN: return (a + b) / 2.0f;
-    case B_MAX: return fmaxf(a, b);
-    case B_RMAX: return fmaxf(b, a);
-    case B_MIN: return fminf(a, b);
-    case B_RMIN: return fminf(b, a);
+    case B_MAX: return ref_fmax(a, b);
+    case B_RMAX: return ref_fmax(b, a);
+    case B_MIN: return ref_fmin(a, b);
+    case B_RMIN: return ref_fmin(b, a);
     case B_MOD: return fmod_float(a, b);
     case B_RMOD: return fmod_float(b, a);
     case B_POW: return f_pow<N>(a, b);

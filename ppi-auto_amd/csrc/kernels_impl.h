@@ -1154,6 +1154,7 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
     const int p = A.order ? A.order[t] : t;
     const ProgMeta m = A.meta[p];
     float part = 0.0f;
+    double dsum = 0.0;  // DP / PDP sums: the reference adds in its own tree order, not row order
     uint32_t kmin = 0xffffffffu, kmax = 0u;
     // SEQ / PP stop at the first bad row (sequential.cc:173, accelerator.cl:64); DP sums
     // only need to know that one exists; the DP maximum keeps looking for a NaN row;
@@ -1202,14 +1203,19 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
                 if (stop_at_bad || (A.strategy == PPI_STRATEGY_FP && kmax == 1u)) break;
             } else if (A.reducemax) {
                 part = (e * fn > part) ? e * fn : part;
-            } else {
+            } else if (in_order) {
                 part += e;
+            } else {
+                dsum += (double)e;
             }
         }
     }
-    // SEQ / PP: the row-order sum is the reference's own; otherwise a serial sum over nlin rows.
+    // SEQ / PP: the row-order float sum is the reference's own.  DP / PDP: a row-order
+    // float sum can stray from the reference's tree-ordered one by up to nlin roundings
+    // when a few rows dominate, so the rows are summed in double (relative error about
+    // nlin * 2^-53) and the float total is that sum rounded.
     const SumTotal tot = (in_order || A.reducemax) ? SumTotal{part, false}
-                                                   : decide_sum(A, (double)part, (double)A.nlin * kUlp);
+                                                   : decide_sum(A, dsum, (double)A.nlin * 0x1.0p-52);
     semantics(A, tot.total, m.flags, kmin, kmax, m.size, &A.raw[p], &A.vec[p], tot.undecided);
 }
 

@@ -228,3 +228,37 @@ def test_wide_datasets(eng, nfeat):
             ok = sums_agree(vec, ovec, ex, bad, data.shape[0], ALPHA, pop.size)
             assert ok.all(), f"form {form}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
             check_rank(vec, idx, ovec, oidx)
+
+
+@pytest.mark.parametrize("prot", [False, True])
+def test_min_max_signed_zero(eng, prot):
+    """MIN / MAX on equal operands return the second one, as the x86-64 C library's fmin / fmax
+    (minsd / maxsd) do for the reference: min(-0, +0) = +0 and max(+0, -0) = -0.  Checked per row
+    (prediction) on columns holding -0 and +0, and on constants (folded at upload)."""
+    b = Builder(4)
+    data = np.zeros((300, 5), dtype=np.float32)
+    data[:, 0] = -0.0
+    data[1::2, 0] = 0.0          # x0 alternates +0 / -0, x1 = +0, x2 = -0
+    data[:, 2] = -0.0
+    data[:, 3] = np.linspace(-2.0, 2.0, 300, dtype=np.float32)
+    progs = [["min", "x0", "x1"], ["min", "x1", "x0"], ["max", "x0", "x2"], ["max", "x2", "x0"],
+             ["min", "x2", "x1"], ["max", "x1", "x2"], ["gamma", "min", "x2", "x1"], ["gamma", "max", "x1", "x2"],
+             ["min", "x3", "x3"], ["gamma", "min", "round", -0.25, "floor", 0.75],
+             ["+", "x3", "gamma", "max", "floor", 0.75, "round", -0.25]]
+    print(f"case signed-zero min/max protected={prot}", flush=True)
+    eng.configure(b.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_ABS, protected=prot, compute_units=256)
+    eng.set_dataset(data)
+    cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS, protected=prot)
+    for prog in progs:
+        ph, ep = b.encode(prog)
+        ph = np.array(ph, dtype=np.int32)
+        ep = np.array(ep, dtype=np.float32)
+        got = eng.predict(ph, ep, len(ph), data.shape[0])
+        one = ppiamd.Population(ph, ep, np.array([len(ph)], dtype=np.int32), len(ph))
+        want, _ = oracle.interpret(cfg, b.table, one, data, ppp_mode=1, prediction_mode=1)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"{prog}: {got[:4]} vs {want[:4]}"
+    pop = b.population(progs)
+    w = ppiamd.Workload("signed-zero", None, pop, data, ppiamd.EF_ABS, protected=prot)
+    for form in (ppiamd.FORM_FP, ppiamd.FORM_PP):
+        vec, idx, ovec, oidx = run_both(eng, w, ppiamd.SEQ, form, table=b.table)
+        assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32)), f"form {form}: {vec} vs {ovec}"

@@ -87,7 +87,7 @@ def oracle_run(c, gr, pop, data):
     return ovec, oidx, ex, bad
 
 
-@pytest.mark.parametrize("seed", range(160))
+@pytest.mark.parametrize("seed", range(1160))
 def test_random_configuration(eng, seed):
     c = draw(seed)
     print(f"case random seed={seed} {c}", flush=True)
