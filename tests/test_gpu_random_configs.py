@@ -80,8 +80,8 @@ def check(c, vec, idx, ovec, oidx, ex, bad, nlin, size):
         assert np.float32(ovec[idx[0]]) == np.float32(best) or rel_close(ovec[idx[0]], best, 1e-5)
 
 
-def oracle_run(c, gr, pop, data):
-    dpl, ppl, bpl = oracle.reference_geometry(data.shape[0], pop.n)
+def oracle_run(c, gr, pop, data, mls=0, cu=256):
+    dpl, ppl, bpl = oracle.reference_geometry(data.shape[0], pop.n, cu=cu, mls=mls)
     cfg = oracle.make_cfg(c["strategy"], c["error_fn"], c["reducemax"], c["protected"], c["cost"], dpl, ppl, bpl)
     ovec, oidx = oracle.interpret(cfg, gr.table, pop, data, alpha=ALPHA, best_size=1)
     ecfg = oracle.make_cfg(ppiamd.SEQ, c["error_fn"], c["reducemax"], c["protected"], c["cost"])
@@ -89,10 +89,7 @@ def oracle_run(c, gr, pop, data):
     return ovec, oidx, ex, bad
 
 
-@pytest.mark.parametrize("seed", range(1660))
-def test_random_configuration(eng, seed):
-    c = draw(seed)
-    print(f"case random seed={seed} {c}", flush=True)
+def run_case(eng, seed, c, mls=0, cu=256):
     gr = ppiamd.Grammar(c["nfeat"], binary=c["binary"], unary=c["unary"])
     pop = ppiamd.grow(gr, c["nprog"], 7 + seed, 9, c["max_symbols"], p_leaf=c["p_leaf"])
     if classification(c):
@@ -102,10 +99,11 @@ def test_random_configuration(eng, seed):
         if seed >= 1160 and seed % 2 == 0:  # quantized: exact zeros, integers and halves in every column
             data = np.ascontiguousarray(np.round(data * np.float32(2.0)) / np.float32(2.0), dtype=np.float32)
     eng.configure(gr.table, strategy=c["strategy"], error_fn=c["error_fn"], reducemax=c["reducemax"],
-                  protected=c["protected"], cost=c["cost"], form=c["form"], compute_units=256)
+                  protected=c["protected"], cost=c["cost"], form=c["form"], compute_units=cu,
+                  max_local_size=mls)
     eng.set_dataset(data)
     vec, idx = eng.evaluate(pop, alpha=ALPHA, best_size=1)
-    ovec, oidx, ex, bad = oracle_run(c, gr, pop, data)
+    ovec, oidx, ex, bad = oracle_run(c, gr, pop, data, mls, cu)
     check(c, vec, idx, ovec, oidx, ex, bad, data.shape[0], pop.size)
     if not c["batched"]:
         return
@@ -116,13 +114,35 @@ def test_random_configuration(eng, seed):
     b = ppiamd.Batch(0)
     try:
         b.configure(gr.table, strategy=c["strategy"], error_fn=c["error_fn"], reducemax=c["reducemax"],
-                    protected=c["protected"], cost=c["cost"], compute_units=256)
+                    protected=c["protected"], cost=c["cost"], compute_units=cu,
+                    max_local_size=mls)
         ids = [b.add_population(b.add_dataset(d), pop) for d in datas]
         b.build()
         b.run()
         for k, d in enumerate(datas):
             bvec, _ = b.results(ids[k], alpha=ALPHA)
-            ovec, oidx, ex, bad = oracle_run(c, gr, pop, d)
+            ovec, oidx, ex, bad = oracle_run(c, gr, pop, d, mls, cu)
             check(cb, bvec, None, ovec, oidx, ex, bad, d.shape[0], pop.size)
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("seed", range(1660))
+def test_random_configuration(eng, seed):
+    c = draw(seed)
+    print(f"case random seed={seed} {c}", flush=True)
+    run_case(eng, seed, c)
+
+
+@pytest.mark.parametrize("seed", range(6000, 6200))
+def test_random_geometry(eng, seed):
+    """DP and PDP strategies under the reference geometry of other devices: work-group limits of
+    1 to 1,000 (PDP lanes, not powers of two included) and 1 to 256 compute units (DP group size)."""
+    c = draw(seed)
+    r = np.random.default_rng(90000 + seed)
+    mls, cu = int(r.choice([1, 3, 7, 100, 256, 1000])), int(r.choice([1, 7, 104, 256]))
+    c["strategy"] = [ppiamd.FP, ppiamd.PDP][seed % 2]
+    if c["form"] == ppiamd.FORM_ORDERED:
+        c["form"] = ppiamd.FORM_FP
+    print(f"case random geometry seed={seed} mls={mls} cu={cu} {c}", flush=True)
+    run_case(eng, seed, c, mls, cu)
