@@ -11,9 +11,15 @@
 // and round.  Basic arithmetic is float with no FMA contraction (the file is
 // compiled with -ffp-contract=off).
 //
-// NATIVE mode mirrors the reference's -DNATIVE OpenCL build (functions.h:19-57,
-// native_* built-ins): fast float intrinsics, not bit-compatible with the
-// sequential path.
+// NATIVE mode mirrors the reference's -DNATIVE OpenCL build (functions.h:59-193,
+// native_* built-ins): fast float intrinsics for exactly the operators that
+// functions.h switches (SIN, COS, EXP, EXP10, EXP2, DIV, LOG, LOG10, LOG2, TAN;
+// SQRT stays exact, which native_sqrt may be), not bit-compatible with the
+// sequential path.  POW, GAMMA's sinh and pow, and MOD call pow, sinh and fmod
+// directly in interpreter_core:68-75,127-129, which NATIVE does not touch, so
+// they keep the sequential path's double routines in both modes.  The
+// sequential drop-in never selects NATIVE: its native_* shims are the double
+// routines (sequential.cc:32-74).
 #ifndef PPI_DEVICE_OPS_H
 #define PPI_DEVICE_OPS_H
 
@@ -100,10 +106,8 @@ template <bool P, bool N> __device__ __forceinline__ float f_tan(float x) {
     if constexpr (N) return __tanf(x);
     else return tan_of_double(x);
 }
-template <bool N> __device__ __forceinline__ float f_pow(float x, float y) {
-    if constexpr (N) return __powf(x, y);
-    else return pow_of_double(x, y);
-}
+// interpreter_core:72-75 (POW): pow is outside functions.h's NATIVE set.
+__device__ __forceinline__ float f_pow(float x, float y) { return pow_of_double(x, y); }
 
 // GAMMA (interpreter_core:127-129): sinh and pow are double routines in the reference,
 // and the last product is formed in double before it is rounded.  Out of line, like the
@@ -136,15 +140,7 @@ template <bool P, bool N> __device__ __forceinline__ float unary(uint32_t u, flo
     case U_STEP: return b2f(a >= 0.0f);
     case U_SIGN: return a > 0.0f ? 1.0f : (a < 0.0f ? -1.0f : 0.0f);
     case U_LOGISTIC: return 1.0f / (1.0f + f_exp<N>(-a));
-    case U_GAMMA:
-        if constexpr (N) {
-            const float s1 = f_sqrt<P, N>(a * sinhf(1.0f / a));
-            const float base = (a / kE_F) * s1;
-            const float s2 = f_sqrt<P, N>(2.0f * kPI_F / a);
-            return __powf(base, a) * s2;
-        } else {
-            return gamma_of_double<P>(a);
-        }
+    case U_GAMMA: return gamma_of_double<P>(a);  // sinh, pow and SQRT: the same routines in both modes
     case U_GAUSS: return f_exp<N>(-a * a);
     case U_SQRT: return f_sqrt<P, N>(a);
     case U_LOG: return f_log<P, N>(a);
@@ -215,8 +211,8 @@ template <bool P, bool N> __device__ __forceinline__ float binary(uint32_t op, f
     case B_RMIN: return ref_fmin(b, a);
     case B_MOD: return fmod_float(a, b);
     case B_RMOD: return fmod_float(b, a);
-    case B_POW: return f_pow<N>(a, b);
-    case B_RPOW: return f_pow<N>(b, a);
+    case B_POW: return f_pow(a, b);
+    case B_RPOW: return f_pow(b, a);
     case B_AND: return b2f((a != 0.0f) && (b != 0.0f));
     case B_OR: return b2f((a != 0.0f) || (b != 0.0f));
     case B_XOR: return b2f((a != 0.0f) != (b != 0.0f));

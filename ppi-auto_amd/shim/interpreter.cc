@@ -169,8 +169,12 @@ Interp g_acc, g_seq;
     throw std::runtime_error(std::string(where) + ": " + ppi_last_error() + " (status " + std::to_string((int)s) + ")");
 }
 
+// `native`: the accelerator entry points honour -DNATIVE (the OpenCL native_* built-ins of
+// functions.h:59-193); the sequential ones never do, because their native_* shims are the
+// double routines themselves (sequential.cc:32-74), so a NATIVE build of seq_interpret gives
+// the same bits as one without it.
 int setup(Interp& I, int device, int strategy, int form, int mls, int rows_per_lane, const unsigned size,
-          float** input, int nlin, int ncol, bool verbose) {
+          float** input, int nlin, int ncol, bool verbose, bool native) {
     ErrorFnInfo ef = detect_error_fn();
     if (ef.ef < 0) {
         fprintf(stderr, "Unsupported error function ERROR(X,Y) = %s: the engine evaluates fabs((X)-(Y)), "
@@ -194,9 +198,7 @@ int setup(Interp& I, int device, int strategy, int form, int mls, int rows_per_l
 #ifdef PROTECTED
     cfg.protected_ops = 1;
 #endif
-#ifdef NATIVE
-    cfg.native_math = 1;
-#endif
+    cfg.native_math = native ? 1 : 0;
     cfg.cost_n = ef.cost_n;
     cfg.cost = ef.cost.empty() ? nullptr : ef.cost.data();
     cfg.max_local_size = mls > 0 ? mls : 0;
@@ -289,7 +291,17 @@ int acc_interpret_init(int argc, char** argv, const unsigned size, const unsigne
         g_acc.eng = nullptr;
     }
     const int strategy = o.strategy == "PP" ? PPI_STRATEGY_PP : o.strategy == "DP" ? PPI_STRATEGY_FP : PPI_STRATEGY_PDP;
-    if (setup(g_acc, o.device, strategy, o.form, o.mls, o.rows_per_lane, size, input, nlin, ncol, o.verbose)) return 1;
+    // evaluate_pp adds each program's rows in order (accelerator.cl:22-84): the ordered form gives
+    // the same sums bit for bit.  -acc-form picks another form (FP: faster, sums in tile order).
+    int form = o.form;
+    if (form == PPI_FORM_AUTO && strategy == PPI_STRATEGY_PP) form = PPI_FORM_ORDERED;
+#ifdef NATIVE
+    const bool native = true;
+#else
+    const bool native = false;
+#endif
+    if (setup(g_acc, o.device, strategy, form, o.mls, o.rows_per_lane, size, input, nlin, ncol, o.verbose, native))
+        return 1;
     static bool registered = false;
     if (!registered) { std::atexit(cleanup); registered = true; }
     return 0;
@@ -332,9 +344,14 @@ void seq_interpret_init(const unsigned size, float** input, int nlin, int ncol) 
         ppi_engine_destroy(g_seq.eng);
         g_seq.eng = nullptr;
     }
-    // sequential.cc has no options; PPI_DEVICE selects the GPU (default 0).
+    // sequential.cc has no options; PPI_DEVICE selects the GPU (default 0).  The loop of
+    // sequential.cc:133-190 adds each program's rows in order: the ordered form gives the same sums
+    // bit for bit.  PPI_SEQ_FORM=FP selects the faster FP form (sums in tile order, within a relative
+    // 1e-5 of the sequential ones).
     const char* dev = std::getenv("PPI_DEVICE");
-    if (setup(g_seq, dev ? std::atoi(dev) : 0, PPI_STRATEGY_SEQ, PPI_FORM_AUTO, 0, 0, size, input, nlin, ncol, false))
+    const char* sf = std::getenv("PPI_SEQ_FORM");
+    const int form = (sf && std::strcmp(sf, "FP") == 0) ? PPI_FORM_FP : PPI_FORM_ORDERED;
+    if (setup(g_seq, dev ? std::atoi(dev) : 0, PPI_STRATEGY_SEQ, form, 0, 0, size, input, nlin, ncol, false, false))
         throw std::runtime_error("seq_interpret_init: engine initialisation failed");
     static bool registered = false;
     if (!registered) { std::atexit(cleanup); registered = true; }

@@ -104,6 +104,32 @@ def test_full_operator_set(eng, form):
         assert ok.all(), f"{(~ok).sum()} mismatches at {np.nonzero(~ok)[0][:5]}: {vec[~ok][:5]} vs {ovec[~ok][:5]}"
 
 
+NON_NATIVE_BINARY = ["+", "-", "*", "mean", "max", "min", "mod", "pow", "xor", "=", "!=", "<", "<=", ">", ">=", "or",
+                     "and"]
+NON_NATIVE_UNARY = ["abs", "pow2", "pow3", "pow4", "pow5", "neg", "round", "ceil", "floor", "step", "sign", "gamma",
+                    "not"]
+
+
+@pytest.mark.parametrize("native", [False, True])
+def test_operators_outside_the_native_set(eng, native):
+    """functions.h:59-193 switch SIN, COS, EXP, EXP10, EXP2, DIV, SQRT, LOG, LOG10, LOG2 and TAN under
+    NATIVE; POW, GAMMA (sinh, pow), MOD and the rest call the same routines in both builds
+    (interpreter_core:44-132).  Programs over those operators alone give the sequential path's bits
+    with NATIVE on: ordered form, SEQ semantics, bit for bit."""
+    gr = ppiamd.Grammar(6, binary=NON_NATIVE_BINARY, unary=NON_NATIVE_UNARY)
+    pop = ppiamd.grow(gr, 1500, 17, 7, 96, p_leaf=0.3)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 18, 2500, 6) * np.float32(3.0)
+    n_pow = int(np.isin(pop.phenotype, [gr.values["T_POW"], gr.values["T_GAMMA"]]).sum())
+    assert n_pow > 500
+    for prot in (True, False):
+        w = ppiamd.Workload("non-native", gr, pop, data, ppiamd.EF_ABS, protected=prot)
+        print(f"case non-native-ops native={native} protected={prot}", flush=True)
+        vec, idx, ovec, oidx = run_both(eng, w, ppiamd.SEQ, ppiamd.FORM_ORDERED, native=native)
+        same = vec.view(np.uint32) == ovec.view(np.uint32)
+        assert same.all(), f"{(~same).sum()} differ at {np.nonzero(~same)[0][:5]}: {vec[~same][:5]} vs {ovec[~same][:5]}"
+        assert list(idx) == list(oidx)
+
+
 def test_edge_programs(eng):
     """Empty, invalid, unknown-symbol, out-of-range attribute, ternary and dead-code programs."""
     b = Builder(4)

@@ -66,7 +66,13 @@ def test_ge_generation(tmp_path, mode, strategy, argv):
     cfg = oracle.make_cfg(strategy, ppiamd.EF_ABS, False, True, None, dpl, ppl, bpl)
     ovec, oidx = oracle.interpret(cfg, table, pop, data, alpha=ALPHA, best_size=1)
     ex, bad = oracle.exact_sums(oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS), table, pop, data)
-    if strategy in (ppiamd.SEQ, ppiamd.PP):
+    if strategy in (ppiamd.SEQ, ppiamd.PP) and "-acc-form" not in argv:
+        # seq_* and -strategy PP default to the ordered form: the reference's row-order sums, bit for bit
+        assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32)), \
+            f"{int((vec.view(np.uint32) != ovec.view(np.uint32)).sum())} differ"
+        assert list(idx) == list(oidx)
+        ok = np.ones(pop.n, bool)
+    elif strategy in (ppiamd.SEQ, ppiamd.PP):
         ok = sums_agree(vec, ovec, ex, bad, data.shape[0], ALPHA, pop.size)
     else:
         ok = rel_close(vec, ovec, 1e-5) | sums_agree(vec, ovec, ex, bad, data.shape[0], ALPHA, pop.size)
@@ -76,6 +82,45 @@ def test_ge_generation(tmp_path, mode, strategy, argv):
         assert list(cb) == [1, 1, 3]  # send + receive called once, nImmigrants from receive
     else:
         assert list(cb[:2]) == [0, 0]
+
+
+@pytest.mark.parametrize("mode", ["seq", "seq-ppp", "seq-pred"])
+def test_native_build_of_the_sequential_path(tmp_path, mode):
+    """A -DNATIVE build of seq_interpret gives the same bits as a build without it: the sequential
+    path's native_* shims are the double routines (sequential.cc:32-74).  Compared with the oracle
+    (no NATIVE mode) and with the non-NATIVE build, bit for bit."""
+    table, pop = population(1 if mode != "seq" else 700, 21)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 22, 1300, 4)
+    vec, idx, _ = run_harness(tmp_path, "native", mode, data, pop, best_size=2)
+    plain, pidx, _ = run_harness(tmp_path, "abs", mode, data, pop, best_size=2)
+    assert np.array_equal(vec.view(np.uint32), plain.view(np.uint32))
+    cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS)
+    if mode == "seq":
+        ovec, oidx = oracle.interpret(cfg, table, pop, data, alpha=ALPHA, best_size=2)
+        assert list(idx) == list(oidx) == list(pidx)
+    elif mode == "seq-ppp":
+        ovec, _ = oracle.interpret(cfg, table, pop, data, alpha=0.0, ppp_mode=1)
+    else:
+        ovec, _ = oracle.interpret(cfg, table, pop, data, ppp_mode=1, prediction_mode=1)
+    assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32))
+
+
+def test_native_build_of_the_accelerator_path(tmp_path):
+    """-DNATIVE on acc_*: the functions.h set (SIN, COS, EXP, LOG, DIV, ...) takes the hardware
+    approximations, whose results are implementation-defined (OpenCL native_*), so only closeness is
+    required; programs without those operators give the reference's bits."""
+    table, pop = population(400, 23)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 24, 1000, 4)
+    vec, idx, _ = run_harness(tmp_path, "native", "acc", data, pop, ["-strategy", "PP"])
+    ovec, _ = oracle.interpret(oracle.make_cfg(ppiamd.PP, ppiamd.EF_ABS), table, pop, data, alpha=ALPHA)
+    close = rel_close(vec, ovec, 1e-3)
+    assert close.mean() > 0.9, close.mean()
+    _, vals = shim_grammar()
+    fast = {vals[k] for k in ("T_SIN", "T_COS", "T_EXP", "T_LOG", "T_DIV", "T_SQRT") if k in vals}
+    plain = np.array([not (set(pop.phenotype[i * pop.stride:i * pop.stride + pop.size[i]].tolist()) & fast)
+                      for i in range(pop.n)])
+    assert plain.sum() > 10
+    assert np.array_equal(vec[plain].view(np.uint32), ovec[plain].view(np.uint32))
 
 
 @pytest.mark.parametrize("mode", ["acc", "seq"])
