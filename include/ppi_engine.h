@@ -182,6 +182,20 @@ ppi_status ppi_engine_evaluate(ppi_engine* eng, const int32_t* phenotype, const 
 /* Prediction mode (ppp -pred; accelerator.cl:72-75): out[row] = value of program 0 on every row. */
 ppi_status ppi_engine_predict(ppi_engine* eng, const int32_t* phenotype, const float* ephemeral,
                               int32_t size, float* out);
+/* ppi_engine_evaluate without allocating: no counterpart in the reference.  For callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same arguments and results as ppi_engine_evaluate (vector and, when
+   index/best_size are given, the ranking), for the FP and PDP forms (PPI_E_UNSUPPORTED for the
+   others).  The programs, constant-subtree programs, partials, bad-row records and results live
+   in a static 64 MiB device array per GPU (held until the process ends); no device or pinned host
+   memory is allocated (the host compile uses ordinary heap memory) and no derived columns are
+   added.  A population whose buffers exceed 64 MiB (about 16 M programs x row tiles of 256 rows,
+   e.g. 30,000 programs x 100,000 rows fits) is rejected with PPI_E_ARG.  The call is
+   synchronous; calls that overlap on one GPU share the array and race.  The engine's resident
+   program set (upload / launch / wait) is left as it was. */
+ppi_status ppi_engine_evaluate_noalloc(ppi_engine* eng, const int32_t* phenotype, const float* ephemeral,
+                                      const int32_t* size, int32_t stride, int32_t n_programs, float alpha,
+                                      float* vector, int32_t* index, int32_t* best_size);
 ppi_status ppi_engine_get_stats(const ppi_engine* eng, ppi_stats* out);
 
 /* Device-resident timing loop used by benchmarks: run `iters` evaluations of the resident

@@ -222,8 +222,14 @@ static void reference_geometry(ppi_engine* E) {
 
 static void reset_derived(ppi_engine* E);
 
-static bool plan(ppi_engine* E, std::string* why) {
-    const CompileStats& cs = E->tape.stats;
+// Kernel form and shape for a compiled population of `nprog` programs (`cs` its compile
+// statistics) on this engine's dataset and configuration.
+struct LaunchPlan {
+    int form = PPI_FORM_FP, R = 4, F = 16, S = 9, chunk = 64, nrb = 1;
+};
+
+static bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t nprog, int requested_form,
+                         LaunchPlan* L, std::string* why) {
     int need_f = cs.max_col + 1;
     int F = need_f <= 8 ? 8 : (need_f <= 16 ? 16 : (need_f <= 32 ? 32 : 0));
     int S = cs.max_slots <= 9 ? 9 : 16;
@@ -231,7 +237,7 @@ static bool plan(ppi_engine* E, std::string* why) {
         *why = "program needs more than 16 register slots";
         return false;
     }
-    int form = E->cfg.form;
+    int form = requested_form;
     if (form == PPI_FORM_AUTO) form = PPI_FORM_FP;
     // The FP and ordered forms stage at most 16 columns in LDS and read a wider dataset's other columns
     // through the caches: 6 waves per SIMD instead of the 4 a 32-column tile leaves room for (7% faster
@@ -258,22 +264,41 @@ static bool plan(ppi_engine* E, std::string* why) {
     }
     // The 4-slot stack where the shape has one: 20 VGPRs fewer at 4 rows, 40 at 8.
     if (S == 9 && cs.max_slots <= 4 && shape_supported(R, F, 4)) S = 4;
-    E->form = form;
-    E->R = R;
-    E->F = F;
-    E->S = S;
+    L->form = form;
+    L->R = R;
+    L->F = F;
+    L->S = S;
     // Programs per FP tile: enough workgroups to fill 256 CUs several times over.
     int64_t rows_per_block = fp_rows_per_block(R);
     int64_t nrb = (E->nlin + rows_per_block - 1) / rows_per_block;
     int chunk = E->cfg.programs_per_tile;
     if (chunk <= 0) {
         int64_t target = (int64_t)E->dev_cu * 16;
-        int64_t per = (E->nprog * nrb + target - 1) / target;
+        int64_t per = ((int64_t)nprog * nrb + target - 1) / target;
         chunk = (int)std::max<int64_t>(8, std::min<int64_t>(kMaxChunk, per));
     }
-    E->chunk = std::min(chunk, kMaxChunk);
-    E->nrb = (int)nrb;
+    L->chunk = std::min(chunk, kMaxChunk);
+    L->nrb = (int)nrb;
     return true;
+}
+
+static bool plan(ppi_engine* E, std::string* why) {
+    LaunchPlan L;
+    if (!compute_plan(E, E->tape.stats, E->nprog, E->cfg.form, &L, why)) return false;
+    E->form = L.form;
+    E->R = L.R;
+    E->F = L.F;
+    E->S = L.S;
+    E->chunk = L.chunk;
+    E->nrb = L.nrb;
+    return true;
+}
+
+// best_individual's work-group size for `nprog` programs (accelerator.cc:331-337).
+static int best_local_for(const ppi_engine* E, int32_t nprog) {
+    const int cu = E->cfg.compute_units > 0 ? E->cfg.compute_units : E->dev_cu;
+    const int mls2 = (int)std::min<size_t>((size_t)E->dev_max_wg, E->dev_lds / 8);
+    return std::max(1, std::min(mls2, (int)std::ceil(nprog / (float)cu)));
 }
 
 extern "C" {
@@ -315,6 +340,14 @@ ppi_status ppi_engine_create(ppi_engine** out, int32_t device) {
         set_error(std::string("device init: ") + hipGetErrorString(e));
         delete E;
         return PPI_E_DEVICE;
+    }
+    // The no-allocation entry point's work area on this device (its address, once per device).
+    if (ppi_status ss = scratch_setup(device)) {
+        for (auto& ev : E->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        (void)hipStreamDestroy(E->stream);
+        delete E;
+        return ss;
     }
     E->dev_cu = prop.multiProcessorCount;
     E->dev_max_wg = prop.maxThreadsPerBlock;
@@ -560,30 +593,30 @@ static EvalArgs make_args(ppi_engine* E, float alpha) {
     return A;
 }
 
-// Enqueue one evaluation of the resident programs on the engine stream.
-static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
-    EvalArgs A = make_args(E, alpha);
+// Enqueue one evaluation on the engine stream: programs, work and result buffers as `A` names them,
+// kernel form and shape as `L` says.
+static ppi_status enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, bool time_main) {
     const bool P = E->cfg.protected_ops != 0, N = E->cfg.native_math != 0;
-    PPI_HIP(launch_fill_u32(E->dBadLo, 0xffffffffu, E->nprog, E->stream));
-    PPI_HIP(launch_fill_u32(E->dBadHi, 0u, E->nprog, E->stream));
+    PPI_HIP(launch_fill_u32(A.bad_lo, 0xffffffffu, A.nprog, E->stream));
+    PPI_HIP(launch_fill_u32(A.bad_hi, 0u, A.nprog, E->stream));
     if (time_main) PPI_HIP(hipEventRecord(E->ev[2], E->stream));
-    if (E->form == PPI_FORM_FP) {
-        PPI_HIP(launch_fp(A, E->R, E->F, E->S, P, N, E->stream));
+    if (L.form == PPI_FORM_FP) {
+        PPI_HIP(launch_fp(A, L.R, L.F, L.S, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
-        PPI_HIP(launch_finalize(A, E->nrb, E->stream));
-    } else if (E->form == PPI_FORM_ORDERED) {
+        PPI_HIP(launch_finalize(A, L.nrb, E->stream));
+    } else if (L.form == PPI_FORM_ORDERED) {
         // Passes of nrb_pass row tiles: per-row errors, then each program's row-order sum continues.
         PPI_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(E->dOSum), 0u, E->nprog, E->stream));  // 0.0f
         PPI_HIP(launch_fill_u32(E->dOStop, 0u, E->nprog, E->stream));
-        for (int rb0 = 0; rb0 < E->nrb; rb0 += E->nrb_pass) {
+        for (int rb0 = 0; rb0 < L.nrb; rb0 += E->nrb_pass) {
             A.rb0 = rb0;
-            A.nrb_pass = std::min(E->nrb_pass, E->nrb - rb0);
-            PPI_HIP(launch_fp(A, E->R, E->F, E->S, P, N, E->stream, true));
-            PPI_HIP(launch_ordered_sum(A, fp_rows_per_block(E->R), rb0 + A.nrb_pass >= E->nrb, E->stream));
+            A.nrb_pass = std::min(E->nrb_pass, L.nrb - rb0);
+            PPI_HIP(launch_fp(A, L.R, L.F, L.S, P, N, E->stream, true));
+            PPI_HIP(launch_ordered_sum(A, fp_rows_per_block(L.R), rb0 + A.nrb_pass >= L.nrb, E->stream));
         }
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
-    } else if (E->form == PPI_FORM_PDP) {
-        PPI_HIP(launch_pdp(A, E->R, E->F, E->S, P, N, E->stream));
+    } else if (L.form == PPI_FORM_PDP) {
+        PPI_HIP(launch_pdp(A, L.R, L.F, L.S, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
     } else {
         PPI_HIP(launch_pp(A, P, N, E->stream));
@@ -592,6 +625,18 @@ static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
     // The programs whose result depends on the reference's partition or order of the rows.
     PPI_HIP(launch_recheck(A, P, N, E->stream));
     return PPI_OK;
+}
+
+// Enqueue one evaluation of the resident programs on the engine stream.
+static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
+    LaunchPlan L;
+    L.form = E->form;
+    L.R = E->R;
+    L.F = E->F;
+    L.S = E->S;
+    L.chunk = E->chunk;
+    L.nrb = E->nrb;
+    return enqueue_with(E, make_args(E, alpha), L, time_main);
 }
 
 ppi_status ppi_engine_launch(ppi_engine* E) {
@@ -697,6 +742,123 @@ ppi_status ppi_engine_predict(ppi_engine* E, const int32_t* phenotype, const flo
     PPI_HIP(launch_predict(A, F, 16, E->cfg.protected_ops != 0, E->cfg.native_math != 0, E->dPred, E->stream));
     PPI_HIP(hipMemcpyAsync(out, E->dPred, (size_t)E->nlin * sizeof(float), hipMemcpyDeviceToHost, E->stream));
     PPI_HIP(hipStreamSynchronize(E->stream));
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_evaluate_noalloc(ppi_engine* E, const int32_t* phenotype, const float* ephemeral,
+                                      const int32_t* size, int32_t stride, int32_t n, float alpha, float* vector,
+                                      int32_t* index, int32_t* best_size) {
+    if (!E) { set_error("engine is NULL"); return PPI_E_ARG; }
+    if (!E->configured) { set_error("configure the engine first"); return PPI_E_STATE; }
+    if (!E->dX) { set_error("set a dataset first"); return PPI_E_STATE; }
+    if (n <= 0 || stride <= 0 || !phenotype || !ephemeral || !size || !vector) {
+        set_error("bad program arrays");
+        return PPI_E_ARG;
+    }
+    int form = E->cfg.form == PPI_FORM_AUTO ? PPI_FORM_FP : E->cfg.form;
+    if (form != PPI_FORM_FP && form != PPI_FORM_PDP) {
+        set_error("the no-allocation entry point runs the FP and PDP forms (configured form " +
+                  std::to_string(E->cfg.form) + ")");
+        return PPI_E_UNSUPPORTED;
+    }
+    (void)hipSetDevice(E->device);
+    uint8_t* base = static_cast<uint8_t*>(scratch_region(E->device));
+    if (!base) { set_error("no scratch region on this device"); return PPI_E_STATE; }
+    // Host-side compile (ordinary heap memory); no derived columns, which would grow the dataset buffer.
+    Tape t;
+    std::string err;
+    int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (!compile_population_mt(E->symtab, phenotype, ephemeral, size, stride, n, E->ncol, false, threads, &t, &err,
+                               nullptr, E->cfg.no_constant_folding == 0)) {
+        set_error(err);
+        return PPI_E_PROGRAM;
+    }
+    t.code.insert(t.code.end(), kTapePad, Ins{H_END, kEmpty});
+    LaunchPlan L;
+    if (!compute_plan(E, t.stats, n, form, &L, &err)) { set_error(err); return PPI_E_PROGRAM; }
+    // Sections of the scratch region, 256-byte aligned.
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t nf = t.folds.meta.size();
+    const size_t o_meta = carve((size_t)n * sizeof(ProgMeta));
+    const size_t o_code = carve(t.code.size() * sizeof(Ins));
+    const size_t o_part = carve(L.form == PPI_FORM_FP ? (size_t)L.nrb * (size_t)n * sizeof(float) : 0);
+    const size_t o_blo = carve((size_t)n * sizeof(uint32_t));
+    const size_t o_bhi = carve((size_t)n * sizeof(uint32_t));
+    const size_t o_raw = carve((size_t)n * sizeof(float));
+    const size_t o_vec = carve((size_t)n * sizeof(float));
+    const size_t o_fmeta = carve(nf * sizeof(ProgMeta));
+    const size_t o_fcode = carve(t.folds.code.size() * sizeof(Ins));
+    const size_t o_fout = carve(nf * sizeof(float));
+    if (off > kScratchBytes) {
+        set_error("evaluate_noalloc: " + std::to_string(n) + " programs over " + std::to_string(E->nlin) +
+                  " rows need " + std::to_string(off) + " bytes of device scratch; the limit is " +
+                  std::to_string(kScratchBytes) + " bytes");
+        return PPI_E_ARG;
+    }
+    const bool P = E->cfg.protected_ops != 0, N = E->cfg.native_math != 0;
+    if (nf > 0) {
+        // Constant subtrees: evaluated once on the device, values written into the tape.
+        std::vector<float> vals(nf);
+        auto* fmeta = reinterpret_cast<ProgMeta*>(base + o_fmeta);
+        auto* fcode = reinterpret_cast<Ins*>(base + o_fcode);
+        auto* fout = reinterpret_cast<float*>(base + o_fout);
+        PPI_HIP(hipMemcpyAsync(fmeta, t.folds.meta.data(), nf * sizeof(ProgMeta), hipMemcpyHostToDevice, E->stream));
+        PPI_HIP(hipMemcpyAsync(fcode, t.folds.code.data(), t.folds.code.size() * sizeof(Ins), hipMemcpyHostToDevice,
+                               E->stream));
+        PPI_HIP(launch_fold(fmeta, fcode, (int32_t)nf, fout, P, N, E->stream));
+        PPI_HIP(hipMemcpyAsync(vals.data(), fout, nf * sizeof(float), hipMemcpyDeviceToHost, E->stream));
+        PPI_HIP(hipStreamSynchronize(E->stream));
+        apply_folds(&t, vals.data());
+    }
+    EvalArgs A = make_args(E, 0.0f);
+    A.meta = reinterpret_cast<const ProgMeta*>(base + o_meta);
+    A.code = reinterpret_cast<const Ins*>(base + o_code);
+    A.nprog = n;
+    A.chunk = L.chunk;
+    A.order = nullptr;
+    A.partial = L.form == PPI_FORM_FP ? reinterpret_cast<float*>(base + o_part) : nullptr;
+    A.bad_lo = reinterpret_cast<uint32_t*>(base + o_blo);
+    A.bad_hi = reinterpret_cast<uint32_t*>(base + o_bhi);
+    A.raw = reinterpret_cast<float*>(base + o_raw);
+    A.vec = reinterpret_cast<float*>(base + o_vec);
+    A.rb0 = 0;
+    A.nrb_pass = L.nrb;
+    A.rowerr = nullptr;
+    A.osum = nullptr;
+    A.ostop = nullptr;
+    PPI_HIP(hipMemcpyAsync(base + o_meta, t.meta.data(), t.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice,
+                           E->stream));
+    PPI_HIP(hipMemcpyAsync(base + o_code, t.code.data(), t.code.size() * sizeof(Ins), hipMemcpyHostToDevice,
+                           E->stream));
+    ppi_status s = enqueue_with(E, A, L, false);
+    if (s) return s;
+    PPI_HIP(hipMemcpyAsync(vector, A.raw, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, E->stream));
+    PPI_HIP(hipStreamSynchronize(E->stream));
+    // The epilogue of ppi_engine_wait: PP / PDP rank the kernel's values before alpha*size is added,
+    // SEQ / FP rank the alpha-inclusive vector (accelerator.cc:843-889, sequential.cc:197-208).
+    const int strategy = E->cfg.strategy;
+    const bool rank = index && best_size && *best_size > 0;
+    if (rank && (strategy == PPI_STRATEGY_PP || strategy == PPI_STRATEGY_PDP))
+        best_acc(vector, n, best_local_for(E, n), index, best_size);
+    for (int i = 0; i < n; ++i) {
+        const float raw = vector[i];
+        const float as = alpha * (float)t.meta[i].size;
+        if (strategy == PPI_STRATEGY_PP || strategy == PPI_STRATEGY_PDP)
+            vector[i] = raw + as;
+        else
+            vector[i] = (raw == FLT_MAX || (t.meta[i].flags & kEmpty)) ? raw : raw + as;
+    }
+    if (rank && strategy == PPI_STRATEGY_SEQ) {
+        if (*best_size > n) *best_size = n;
+        pick_n_best(*best_size, index, n, vector, nullptr);
+    } else if (rank && strategy == PPI_STRATEGY_FP) {
+        best_acc(vector, n, best_local_for(E, n), index, best_size);
+    }
     return PPI_OK;
 }
 

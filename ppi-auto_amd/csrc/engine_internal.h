@@ -34,6 +34,13 @@ struct FoldBuffers {
 // (P = PROTECTED, N = NATIVE) and writes their values into the tape (apply_folds).
 ppi_status evaluate_folds(Tape* t, FoldBuffers* B, bool P, bool N, hipStream_t st);
 void pick_n_best(int n, int32_t* best, int num, const float* errors, const int32_t* indices);
+// Device work area of ppi_engine_evaluate_noalloc (kern_scratch.hip): a static array of
+// kScratchBytes on every device the library runs on.  scratch_setup(device), with that device
+// current, takes its device address once (hipGetSymbolAddress); scratch_region(device) returns it
+// (null before set-up).  Calls that use it at the same time on one device race.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+ppi_status scratch_setup(int device);
+void* scratch_region(int device);
 void best_acc(const float* vec, int P, int local2, int32_t* index, int32_t* best_size);
 }  // namespace ppi
 

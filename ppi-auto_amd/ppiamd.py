@@ -96,6 +96,7 @@ ENGINE_SYMBOLS = [
     "ppi_abi_version", "ppi_last_error", "ppi_device_count", "ppi_engine_create", "ppi_engine_destroy",
     "ppi_engine_configure", "ppi_engine_set_dataset", "ppi_engine_set_dataset_rowmajor", "ppi_engine_upload",
     "ppi_engine_launch", "ppi_engine_wait", "ppi_engine_evaluate", "ppi_engine_predict", "ppi_engine_get_stats",
+    "ppi_engine_evaluate_noalloc",
     "ppi_engine_time", "ppi_batch_create", "ppi_batch_destroy", "ppi_batch_configure", "ppi_batch_add_dataset",
     "ppi_batch_add_population", "ppi_batch_clear_populations", "ppi_batch_build", "ppi_batch_launch",
     "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
@@ -128,6 +129,8 @@ def lib() -> C.CDLL:
         L.ppi_engine_evaluate.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
                                           C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
         L.ppi_engine_predict.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32, P(C.c_float)]
+        L.ppi_engine_evaluate_noalloc.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
+                                                  C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
         L.ppi_engine_get_stats.argtypes = [C.c_void_p, P(Stats)]
         L.ppi_engine_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
         L.ppi_batch_create.argtypes = [P(C.c_void_p), C.c_int32]
@@ -277,6 +280,19 @@ class Engine:
         self.upload(pop)
         self.launch()
         return self.wait(pop.n, alpha, best_size)
+
+    def evaluate_noalloc(self, pop: "Population", alpha: float = 0.0, best_size: int = 0):
+        """ppi_engine_evaluate_noalloc: the evaluation with every device buffer in the library's static
+        scratch array (no device allocation); the resident program set is left as it was."""
+        vec = np.empty(pop.n, dtype=np.float32)
+        idx = np.zeros(max(1, best_size), dtype=np.int32)
+        bs = C.c_int32(best_size)
+        _check(lib().ppi_engine_evaluate_noalloc(self._h, _ptr(pop.phenotype, C.c_int32),
+                                                 _ptr(pop.ephemeral, C.c_float), _ptr(pop.size, C.c_int32),
+                                                 pop.stride, pop.n, alpha, _ptr(vec, C.c_float),
+                                                 _ptr(idx, C.c_int32) if best_size else None,
+                                                 C.byref(bs) if best_size else None))
+        return vec, idx[:bs.value] if best_size else None
 
     def predict(self, phenotype: np.ndarray, ephemeral: np.ndarray, size: int, nlin: int) -> np.ndarray:
         ph = np.ascontiguousarray(phenotype, dtype=np.int32)
