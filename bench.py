@@ -47,6 +47,28 @@ def host_info():
     return model, os.cpu_count()
 
 
+def host_record():
+    """The host the CPU legs ran on: lscpu's model name, nproc, the CPUs this process may use, and the
+    threads the CPU legs use (this process's CPUs, at most OMP_NUM_THREADS: a GPU box allots 16 CPUs to
+    one GPU)."""
+    model, ncpu = host_info()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    aff = len(os.sched_getaffinity(0))
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    threads = max(1, min(aff, omp) if omp > 0 else aff)
+    return {"model": model, "nproc": ncpu, "affinity_cpus": aff, "omp_num_threads": omp or None,
+            "threads": threads}
+
+
 def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile=0, strategy=ppiamd.SEQ):
     eng.configure(w.grammar.table, strategy=strategy, error_fn=w.error_fn, reducemax=w.reducemax,
                   protected=w.protected, cost=w.cost, form=form, rows_per_lane=rows_per_lane,
@@ -104,6 +126,17 @@ def bench_batched(steps, warmup):
         b.run()
     ms = b.time(steps)
     st = b.stats()
+    # FP64 error on three pairs (the first 100 programs of the largest, a middle and the smallest dataset)
+    rows = [d.shape[0] for d in w.datas]
+    picks = sorted({int(np.argmax(rows)), int(np.argsort(rows)[len(rows) // 2]), int(np.argmin(rows))})
+    b.run()
+    errs = {}
+    threads = host_record()["threads"]
+    for k in picks:
+        v, _ = b.results(k, alpha=0.0)
+        sub = w.pops[k].subset(min(100, w.pops[k].n))
+        errs[f"pair{k}_{rows[k]}rows"] = fp64_error(w.grammars[k].table, sub, w.datas[k], w.error_fn, True, False,
+                                                    None, threads, v[:sub.n])
     b.close()
     eng = ppiamd.Engine(0)
     plain_ms = 0.0
@@ -114,16 +147,16 @@ def bench_batched(steps, warmup):
         eng.time(1)
         plain_ms += eng.time(1)
     eng.close()
-    return {"node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
+    return {"fp64_error": errs, "node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
             "rows_per_lane": st.rows_per_lane, "items": st.n_items, "grid": st.grid,
             "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs,
             "programs": int(sum(p.n for p in w.pops))}
 
 
-def bench_adjoint(steps, warmup):
+def bench_adjoint(steps, warmup, threads):
     """Adjoint workload (30,000 programs with <= 8 T_CONST each, 1,000,000 x 8 rows): device time
     of one backward pass (forward recomputation + reverse sweep + epilogue), and 20 on-device
-    descent steps on the first 100,000 rows."""
+    descent steps on all 1,000,000 rows."""
     w = ppiamd.workload("adjoint")
     a = ppiamd.Adjoint(0)
     a.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
@@ -134,10 +167,10 @@ def bench_adjoint(steps, warmup):
         a.time(warmup)
     ms = a.time(steps)
     st = a.stats()
-    sub = 100000
+    sub = w.data.shape[0]
     d = ppiamd.Adjoint(0)
     d.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
-    d.set_dataset(w.data[:sub])
+    d.set_dataset(w.data)
     f0 = d.forward(w.pop, alpha=1e-6)
     _, f1 = d.descend(20, step=0.1)
     dst = d.stats()
@@ -148,7 +181,14 @@ def bench_adjoint(steps, warmup):
     rel = (f0[sel] - f1[sel]) / f0[sel]
     d.close()
     a.close()
-    return {"ms": ms, "kernel_ms": st.kernel_ms, "n": w.pop.n, "node_evals": st.node_evals,
+    k = 100
+    e = ppiamd.Engine(0)
+    e.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True, form=ppiamd.FORM_FP)
+    e.set_dataset(w.data)
+    gv, _ = e.evaluate(w.pop.subset(k), alpha=0.0)
+    e.close()
+    err = fp64_error(w.grammar.table, w.pop.subset(k), w.data, w.error_fn, True, False, None, threads, gv)
+    return {"fp64_error": err, "ms": ms, "kernel_ms": st.kernel_ms, "n": w.pop.n, "node_evals": st.node_evals,
             "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins, "partials": st.partials,
             "tune_ms": [round(x, 3) for x in st.tune_ms], "consts": st.n_consts, "max_segments": st.max_segments,
             "descend_ms": dst.descend_ms, "descend_rows": sub, "improved": float((rel > 0).mean()),
@@ -156,19 +196,70 @@ def bench_adjoint(steps, warmup):
             "reverted": dst.descend_reverted}
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle  # cpu_baseline and FP64-error legs only: the oracle is the checker, never the thing measured
+    return oracle
+
+
 def cpu_baseline(w, budget_s, threads):
     """Oracle (restated sequential path) on the first k programs x all rows."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle  # cpu_baseline leg only
+    oracle = _oracle()
     cfg = oracle.make_cfg(ppiamd.SEQ, w.error_fn, w.reducemax, w.protected, w.cost)
-    k = 50
+    k = min(50, w.pop.n)
     t, _ = oracle.time_seq(cfg, w.grammar.table, w.pop.subset(k), w.data, threads=threads)
-    rate = w.pop.subset(k).total_size * w.data.shape[0] / max(t, 1e-9)
     k2 = int(min(w.pop.n, max(k, k * budget_s / max(t, 1e-6))))
     sub = w.pop.subset(k2)
     t2, _ = oracle.time_seq(cfg, w.grammar.table, sub, w.data, threads=threads)
     rate = sub.total_size * w.data.shape[0] / t2
     return rate, t2, k2
+
+
+def _err_stats(v, f64):
+    """Relative (and absolute) error of float results v against FP64 results f64."""
+    v = np.asarray(v, np.float64)
+    a = np.abs(v - f64)
+    r = a / np.maximum(np.abs(f64), 1e-30)
+    if not r.size:
+        return None
+    return {"median_rel": float(np.median(r)), "p90_rel": float(np.quantile(r, 0.90)),
+            "within_1e-5": float((r <= 1e-5).mean()), "within_1e-3": float((r <= 1e-3).mean()),
+            "differs": float((a > 0).mean())}
+
+
+def fp64_error(table, pop, data, error_fn, protected, reducemax, cost, threads, gpu_vec):
+    """Error of the GPU's and of the reference's (the oracle's restated sequential path) per-program
+    error against an FP64 CPU evaluation of the same programs (every operator and the error function
+    in double, no alpha); programs whose float result is a non-finite sentinel are counted apart."""
+    oracle = _oracle()
+    cfg = oracle.make_cfg(ppiamd.SEQ, error_fn, reducemax, protected, cost)
+    _, ref = oracle.time_seq(cfg, table, pop, data, alpha=0.0, threads=threads)
+    f64 = oracle.fitness_f64(cfg, table, pop, data)
+    g = np.asarray(gpu_vec, np.float32)
+    ok = np.isfinite(f64) & (g < 3.0e38) & (ref < 3.0e38)
+    return {"programs": int(pop.n), "compared": int(ok.sum()),
+            "sentinel_gpu": int((g >= 3.0e38).sum()), "sentinel_reference": int((ref >= 3.0e38).sum()),
+            "gpu": _err_stats(g[ok], f64[ok]), "reference": _err_stats(ref[ok], f64[ok])}
+
+
+def config_leg(eng, w, steps, threads, cpu_budget, sample, note, reducemax=None):
+    """One configuration: GPU GPop/s (FP form, programs resident), the oracle's CPU GPop/s on a bounded
+    sample, and the FP64 error of both on the first `sample` programs."""
+    rm = w.reducemax if reducemax is None else reducemax
+    wl = ppiamd.Workload(w.name, w.grammar, w.pop, w.data, w.error_fn, w.cost, rm, w.protected)
+    r = bench_engine(eng, wl, ppiamd.FORM_FP, steps, 1)
+    gp = r["node_evals"] / (r["ms_per_step"] * 1e-3) / 1e9
+    cpu, t, k = cpu_baseline(wl, cpu_budget, threads)
+    sub = w.pop.subset(min(sample, w.pop.n))
+    eng.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, reducemax=rm, protected=w.protected,
+                  cost=w.cost, form=ppiamd.FORM_FP)
+    eng.set_dataset(w.data)
+    gv, _ = eng.evaluate(sub, alpha=0.0)
+    return {"workload": note, "gpu_gpops": round(gp, 3), "gpu_ms_per_step": round(r["ms_per_step"], 4),
+            "cpu_gpops": round(cpu / 1e9, 4), "cpu_threads": threads,
+            "cpu_sample": f"first {k} of {w.pop.n} programs x all {w.data.shape[0]} rows, {t:.2f} s",
+            "fp64_error": fp64_error(w.grammar.table, sub, w.data, w.error_fn, w.protected, rm, w.cost, threads,
+                                     gv)}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fp_tile_regression_traffic.md")
@@ -290,13 +381,13 @@ def main():
             line["ordered_gpops"] = round(od["node_evals"] / (od["ms_per_step"] * 1e-3) / 1e9, 3)
             line["ordered_note"] = ("FP row tiles store per-row errors and each program's errors are summed in row "
                                     "order: SEQ/PP results bit-identical to the sequential path")
-            log("timing PP form (500-program sample)")
-            wpp = ppiamd.Workload(w.name, w.grammar, w.pop.subset(500), w.data, w.error_fn)
-            pp = bench_engine(eng, wpp, ppiamd.FORM_PP, 1, 0)
+            log("timing evaluate_pp semantics (ordered form, the whole population)")
+            pp = bench_engine(eng, w, ppiamd.FORM_ORDERED, max(1, a.steps // 2), 1, strategy=ppiamd.PP)
             line["pp_gpops"] = round(pp["node_evals"] / (pp["ms_per_step"] * 1e-3) / 1e9, 3)
-            line["pp_note"] = ("the reference's PP geometry, one lane per program with rows in order: 500 programs "
-                               "fill 2 workgroups of 256 CUs; kept for fidelity, the ordered form gives the same "
-                               "bits at ordered_gpops")
+            line["pp_note"] = ("evaluate_pp's results (row-order sums, raw-error ranking) for all 30,000 programs, "
+                               "bit for bit, by the ordered form: the default of -strategy PP; the reference's "
+                               "own lane-per-program geometry (one program per lane, rows in order) is kept as "
+                               "FORM_PP for fidelity tests and not timed here")
             log("timing whole generations (compile + upload + evaluate + results)")
             gen = bench_generations(eng, w, 10)
             gen["gpops_end_to_end"] = round(fp["node_evals"] / (gen["ms_per_generation"] * 1e-3) / 1e9, 3)
@@ -311,17 +402,49 @@ def main():
         eng.close()
         if a.only is None:
             log("CPU baseline (oracle restatement of seq_interpret)")
-            model, ncpu = host_info()
-            threads = min(ncpu or 1, 16)
+            host = host_record()
+            threads = host["threads"]
             cpu_rate, cpu_t, cpu_k = cpu_baseline(w, a.cpu_budget, threads)
             seq_rate, seq_t, seq_k = cpu_baseline(w, a.cpu_budget, 1)
             line["cpu_baseline"] = {"value": round(cpu_rate / 1e9, 4), "unit": "GPop/s", "cores": threads,
                                     "kind": "port",
                                     "sample": f"first {cpu_k} programs x all 100,000 rows, {cpu_t:.1f} s, OpenMP "
-                                              f"over programs, host {model}"}
+                                              f"over programs on {threads} threads, host {host['model']}"}
             line["cpu_baseline_1thread"] = {"value": round(seq_rate / 1e9, 4), "unit": "GPop/s", "cores": 1,
                                             "sample": f"first {seq_k} programs x all rows, {seq_t:.1f} s"}
+            line["host"] = host
             line["vs_cpu"] = round(value / (cpu_rate / 1e9), 2)
+            if extras:
+                log("per configuration: GPU and CPU GPop/s, FP64 error of both")
+                eng = ppiamd.Engine(0)
+                cfgs = {}
+                ws = ppiamd.workload("small")
+                cfgs["cfg1_small"] = config_leg(
+                    eng, ws, a.steps, threads, 2.0, ws.pop.n,
+                    "1,000 programs (depth <= 6, <= 64 symbols) x 1,000 rows x 4 U[-1,1], EF = |X-Y|, sum")
+                cfgs["cfg1_small"]["cpu_1thread_gpops"] = round(cpu_baseline(ws, 2.0, 1)[0] / 1e9, 4)
+                rg = {"workload": "the headline line above", "gpu_gpops": line["value"],
+                      "cpu_gpops": line["cpu_baseline"]["value"], "cpu_threads": threads}
+                eng.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=w.protected,
+                              form=ppiamd.FORM_FP)
+                eng.set_dataset(w.data)
+                sub = w.pop.subset(300)
+                gv, _ = eng.evaluate(sub, alpha=0.0)
+                rg["fp64_error"] = fp64_error(w.grammar.table, sub, w.data, w.error_fn, w.protected, False, None,
+                                              threads, gv)
+                cfgs["cfg2_regression"] = rg
+                wc = ppiamd.workload("classification")
+                for rm in (False, True):
+                    cfgs["cfg3_classification_" + ("max" if rm else "sum")] = config_leg(
+                        eng, wc, max(1, a.steps // 2), threads, 3.0, 300,
+                        "30,000 programs x 150,000 rows x 4 (4 clusters), EF = COST[(int)clamp(X,0,3)][(int)Y], "
+                        + ("REDUCEMAX" if rm else "sum"), reducemax=rm)
+                eng.close()
+                line["configs"] = cfgs
+                line["configs_note"] = ("fp64_error: |float result - FP64 result| / |FP64 result| per program on a "
+                                        "sample (the first programs, all rows), for the GPU and for the reference's "
+                                        "sequential path (the oracle), with every operator and the error function "
+                                        "in double for FP64; sentinel (non-finite) programs are counted apart")
     if "batched" in legs and (a.only == "batched" or extras):
         log("batched: 64 uneven pairs (populations 500-20,000, lengths 8-256, rows 1e3-1e6, 2-32 features)")
         batched = bench_batched(max(1, a.steps // 5), 1)
@@ -335,12 +458,13 @@ def main():
             "vs_plain": round(bg / pg, 3),
             "kernel_frac_of_valu_peak": round(batched["node_evals"] / (batched["kernel_ms"] * 1e-3) / VALU_PEAK, 5),
             "rows_per_lane": batched["rows_per_lane"], "items": batched["items"], "grid": batched["grid"],
-            "autotune_ms_per_Gnode_R2_R4_R8": batched["tune_ms"]}
+            "autotune_ms_per_Gnode_R2_R4_R8": batched["tune_ms"],
+            "fp64_error_sample": batched["fp64_error"]}
         if cpu_rate:
             line["batched"]["vs_cpu"] = round(bg / (cpu_rate / 1e9), 2)
     if "adjoint" in legs and (a.only == "adjoint" or extras):
         log("adjoint: 30,000 programs (<= 8 constants) x 1,000,000 rows x 8 features, backward pass + 20 descent steps")
-        adjoint = bench_adjoint(2, 1)
+        adjoint = bench_adjoint(2, 1, host_record()["threads"])
         ms = adjoint["ms"]
         line["adjoint"] = {
             "workload": "30,000 programs (depth <= 10, <= 128 symbols, <= 8 T_CONST) x 1,000,000 rows x 8 "
@@ -357,6 +481,7 @@ def main():
                         "median_relative_fitness_gain": round(adjoint["median_rel_gain"], 5),
                         "programs_worse": adjoint["worse"],
                         "programs_kept_original": adjoint["reverted"]},
+            "fp64_error_forward": adjoint["fp64_error"],
             "tolerance": "gradients within 1e-4*(|g|+m) of the float computation's exact derivative and "
                          "1e-3*(|g|+m) of FP64 (m = sum |EF' dpred/dc| / nlin); central differences h=1e-3"}
     print(json.dumps(line), flush=True)

@@ -91,6 +91,34 @@ double oracle_time_seq(const oracle_cfg* cfg, const ppi_symbol* table, int32_t n
                        int32_t stride, int32_t n_programs, const float* data, int32_t nlin, int32_t ncol,
                        float alpha, float* vector, int32_t threads);
 
+/* ---- grammatical evolution (ppi_ge_oracle.c) ---- */
+
+/* XorShift128Plus of src/util/Random.h:130-162. */
+typedef struct oracle_rng { uint64_t s[2]; } oracle_rng;
+void oracle_rng_seed(oracle_rng* r, uint64_t seed);
+uint64_t oracle_rng_int(oracle_rng* r);
+double oracle_rng_real(oracle_rng* r);
+int oracle_rng_probability(oracle_rng* r, double p);
+/* n draws of Int() and of Real() from two generators seeded `seed`. */
+int oracle_rng_draws(uint64_t seed, int32_t n, uint64_t* ints, double* reals);
+
+/* Effective bits per gene (raised to the grammar's need) and phenotype stride (ppi.cc:258-264,300-307). */
+int oracle_ge_effective(const ppi_ge_grammar* g, const ppi_ge_params* p, int32_t* bits_per_gene,
+                        int32_t* max_size_phenotype);
+/* decode (ppi.cc:106-182) of n genomes (bytes 0/1, n x number_of_bits): phenotype / ephemeral
+   n x max_size_phenotype (0 where the reference leaves the array untouched), size n. */
+int oracle_ge_decode(const ppi_ge_grammar* g, const ppi_ge_params* p, const uint8_t* genomes, int32_t n,
+                     int32_t* phenotype, float* ephemeral, int32_t* size);
+/* One breeding step (ppi.cc:896-936) from parents / their fitness into children[n_immigrants..P). */
+int oracle_ge_breed(const ppi_ge_grammar* g, const ppi_ge_params* p, const uint8_t* parents, const float* fitness,
+                    uint8_t* children, int32_t n_immigrants);
+/* ppi_evolve (ppi.cc:823-945) without islands: the initial population and `generations` generations,
+   each evaluated by oracle_interpret with cfg.  Returns the final population (genomes P x nb,
+   fitness P), the best fitness after each generation (generations + 1 values) and the best genome. */
+int oracle_ge_run(const oracle_cfg* cfg, const ppi_symbol* table, int32_t ntable, const ppi_ge_grammar* g,
+                  const ppi_ge_params* p, const float* data, int32_t nlin, int32_t ncol, int32_t generations,
+                  uint8_t* genomes, float* fitness, float* best_history, uint8_t* best_genome);
+
 #ifdef __cplusplus
 }
 #endif

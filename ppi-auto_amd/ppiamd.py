@@ -92,6 +92,52 @@ class AdjointStats(C.Structure):
 
 PARTIALS_AUTO, PARTIALS_RECOMPUTE, PARTIALS_STORE = 0, 1, 2
 
+
+class GeGrammar(C.Structure):
+    _fields_ = [("n_nonterminals", C.c_int32), ("n_rules", C.POINTER(C.c_int32)), ("rule_len", C.POINTER(C.c_int32)),
+                ("rule_symbols", C.POINTER(C.c_int32)), ("max_rule", C.c_int32), ("t_const", C.c_int32),
+                ("t_attribute", C.c_int32)]
+
+
+class GeParams(C.Structure):
+    _fields_ = [("population_size", C.c_int32), ("number_of_bits", C.c_int32), ("bits_per_gene", C.c_int32),
+                ("bits_per_constant", C.c_int32), ("min_constant", C.c_float), ("max_constant", C.c_float),
+                ("crossover_rate", C.c_float), ("mutation_rate", C.c_float), ("tournament_size", C.c_int32),
+                ("elitism", C.c_int32), ("max_phenotype", C.c_int32), ("threads", C.c_int32),
+                ("seed", C.c_int64), ("alpha", C.c_float)]
+
+
+class GeStats(C.Structure):
+    _fields_ = [("generation", C.c_int32), ("bits_per_gene", C.c_int32), ("max_size_phenotype", C.c_int32),
+                ("max_slots", C.c_int32), ("best_fitness", C.c_float), ("breed_ms", C.c_double),
+                ("decode_ms", C.c_double), ("compile_ms", C.c_double), ("evaluate_ms", C.c_double),
+                ("rank_ms", C.c_double), ("generation_ms", C.c_double), ("node_evals", C.c_double)]
+
+
+class GeSetup:
+    """ppi_ge_grammar + ppi_ge_params of a BNF grammar (shim/grammar.py numbering) and ppi_init's
+    options (defaults of ppi.cc:200-237: -nb 2000 -bg 8 -bc 16 -min -10 -max 10 -cp 0.9 -mr 0.01
+    -ts 3), with the arrays kept alive."""
+
+    def __init__(self, g, population_size=1024, number_of_bits=2000, bits_per_gene=8, bits_per_constant=16,
+                 min_constant=-10.0, max_constant=10.0, crossover_rate=0.9, mutation_rate=0.01, tournament_size=3,
+                 elitism=False, max_phenotype=0, threads=1, seed=1, alpha=1e-6):
+        nts, alts = g.rule_tables()
+        vals = g.values()
+        self.max_rule = g.max_rule
+        self._n_rules = np.array([len(a) for a in alts], dtype=np.int32)
+        rules = [r for a in alts for r in a]
+        self._rule_len = np.array([len(r) for r in rules], dtype=np.int32)
+        self._rule_sym = np.zeros(len(rules) * g.max_rule, dtype=np.int32)
+        for k, r in enumerate(rules):
+            self._rule_sym[k * g.max_rule:k * g.max_rule + len(r)] = r
+        self.grammar = GeGrammar(len(nts), _ptr(self._n_rules, C.c_int32), _ptr(self._rule_len, C.c_int32),
+                                 _ptr(self._rule_sym, C.c_int32), g.max_rule, vals.get("T_CONST", -1),
+                                 vals.get("T_ATTRIBUTE", -1))
+        self.params = GeParams(population_size, number_of_bits, bits_per_gene, bits_per_constant, min_constant,
+                               max_constant, crossover_rate, mutation_rate, tournament_size, int(elitism),
+                               max_phenotype, threads, seed, alpha)
+
 ENGINE_SYMBOLS = [
     "ppi_abi_version", "ppi_last_error", "ppi_device_count", "ppi_engine_create", "ppi_engine_destroy",
     "ppi_engine_configure", "ppi_engine_set_dataset", "ppi_engine_set_dataset_rowmajor", "ppi_engine_upload",

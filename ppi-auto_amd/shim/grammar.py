@@ -118,6 +118,28 @@ class Grammar:
             table.append((vals["T_ATTRIBUTE"], OPS["ATTRIBUTE"], 0.0))
         return table
 
+    def rule_tables(self) -> Tuple[List[int], List[List[List[int]]]]:
+        """The decoder's grammar (the ``grammar`` header's ``gramatica[]`` / ``tamanhos[]``,
+        read_grammar.py:160-230): per non-terminal in enum order, its alternatives in BNF order,
+        each a list of Symbol values (an attribute keeps its ``T_ATTRK`` value; PPI's decoder
+        rewrites it, ppi.cc:167-171).  Returns (non-terminal values, alternatives)."""
+        vals = self.values()
+        nts, alts_all = [], []
+        for name in self.nonterminals:
+            head = name[3:]
+            nts.append(vals[name])
+            alts = []
+            for alt in self.rules[head]:
+                syms = []
+                for t in alt:
+                    if t.startswith("<") and t.endswith(">"):
+                        syms.append(vals["NT_" + t[1:-1].upper()])
+                    else:
+                        syms.append(vals[_terminal_name(t)])
+                alts.append(syms)
+            alts_all.append(alts)
+        return nts, alts_all
+
     def symbol_header(self) -> str:
         """C header equivalent to the reference's generated ``symbol`` file."""
         parts = list(self.nonterminals)

@@ -20,6 +20,9 @@ class OracleCfg(C.Structure):
     ]
 
 
+GeGrammar = ppiamd.GeGrammar
+GeParams = ppiamd.GeParams
+
 _o = None
 
 
@@ -59,6 +62,14 @@ def lib():
                                       P(C.c_int32), C.c_int32, C.c_int32, P(C.c_float), C.c_int32, C.c_int32,
                                       C.c_float, P(C.c_float), C.c_int32]
         L.oracle_time_seq.restype = C.c_double
+        L.oracle_rng_draws.argtypes = [C.c_uint64, C.c_int32, P(C.c_uint64), P(C.c_double)]
+        L.oracle_ge_effective.argtypes = [P(GeGrammar), P(GeParams), P(C.c_int32), P(C.c_int32)]
+        L.oracle_ge_decode.argtypes = [P(GeGrammar), P(GeParams), P(C.c_uint8), C.c_int32, P(C.c_int32),
+                                       P(C.c_float), P(C.c_int32)]
+        L.oracle_ge_breed.argtypes = [P(GeGrammar), P(GeParams), P(C.c_uint8), P(C.c_float), P(C.c_uint8), C.c_int32]
+        L.oracle_ge_run.argtypes = [P(OracleCfg), P(ppiamd.Symbol), C.c_int32, P(GeGrammar), P(GeParams),
+                                    P(C.c_float), C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8), P(C.c_float),
+                                    P(C.c_float), P(C.c_uint8)]
         _o = L
     return _o
 
@@ -189,3 +200,52 @@ def time_seq(cfg, table, pop, data, alpha=0.0, threads=1):
                               _p(pop.ephemeral, C.c_float), _p(pop.size, C.c_int32), pop.stride, pop.n,
                               _p(data, C.c_float), data.shape[0], data.shape[1], alpha, _p(out, C.c_float), threads)
     return t, out
+
+
+def rng_draws(seed, n):
+    ints = np.zeros(n, dtype=np.uint64)
+    reals = np.zeros(n, dtype=np.float64)
+    lib().oracle_rng_draws(seed, n, _p(ints, C.c_uint64), _p(reals, C.c_double))
+    return ints, reals
+
+
+def ge_effective(ge):
+    bg, mx = C.c_int32(), C.c_int32()
+    assert lib().oracle_ge_effective(C.byref(ge.grammar), C.byref(ge.params), C.byref(bg), C.byref(mx)) == 0
+    return bg.value, mx.value
+
+
+def ge_decode(ge, genomes):
+    """genomes: n x nb uint8 (0/1) -> phenotype, ephemeral (n x stride), size (n), stride."""
+    g = np.ascontiguousarray(genomes, dtype=np.uint8)
+    n = g.shape[0]
+    _, mx = ge_effective(ge)
+    ph = np.zeros((n, mx), dtype=np.int32)
+    ep = np.zeros((n, mx), dtype=np.float32)
+    sz = np.zeros(n, dtype=np.int32)
+    assert lib().oracle_ge_decode(C.byref(ge.grammar), C.byref(ge.params), _p(g, C.c_uint8), n, _p(ph, C.c_int32),
+                                  _p(ep, C.c_float), _p(sz, C.c_int32)) == 0
+    return ph, ep, sz, mx
+
+
+def ge_breed(ge, parents, fitness, n_immigrants=0):
+    par = np.ascontiguousarray(parents, dtype=np.uint8)
+    fit = np.ascontiguousarray(fitness, dtype=np.float32)
+    out = np.zeros_like(par)
+    assert lib().oracle_ge_breed(C.byref(ge.grammar), C.byref(ge.params), _p(par, C.c_uint8), _p(fit, C.c_float),
+                                 _p(out, C.c_uint8), n_immigrants) == 0
+    return out
+
+
+def ge_run(cfg, table, ge, data, generations):
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    P, nb = ge.params.population_size, ge.params.number_of_bits
+    genomes = np.zeros((P, nb), dtype=np.uint8)
+    fit = np.zeros(P, dtype=np.float32)
+    hist = np.zeros(generations + 1, dtype=np.float32)
+    best = np.zeros(nb, dtype=np.uint8)
+    r = lib().oracle_ge_run(C.byref(cfg), _table(table), len(table), C.byref(ge.grammar), C.byref(ge.params),
+                            _p(data, C.c_float), data.shape[0], data.shape[1], generations, _p(genomes, C.c_uint8),
+                            _p(fit, C.c_float), _p(hist, C.c_float), _p(best, C.c_uint8))
+    assert r == 0
+    return genomes, fit, hist, best

@@ -55,3 +55,16 @@ def test_terminal_op_integer_and_named_constants():
     assert grammar.terminal_op("T_LN10")[0] == grammar.OPS["VALUE"]
     assert grammar.terminal_op("T_FOO") is None
     assert ppiamd.OPS["TAN"] == 64
+
+
+def test_rule_tables_match_the_reference_generator_output():
+    """The decoder's tables (gramatica[] / tamanhos[], ppi.cc:106-182 reads them) equal what the
+    reference's generator wrote for problem/teste/grammar_bnf1 (its checked-in problem/teste/grammar)."""
+    gv = json.load(open(os.path.join(HERE, "golden", "grammar_rules.json")))
+    g = grammar.parse_bnf(gv["bnf"])
+    nts, alts = g.rule_tables()
+    name = {v: k for k, v in g.values().items()}
+    assert [name[n] for n in nts] == gv["nonterminals"]
+    assert nts == list(range(len(nts)))  # gramatica[] is indexed by the non-terminal's enum value
+    assert [[[name[s] for s in alt] for alt in rules] for rules in alts] == gv["rules"]
+    assert g.max_rule == gv["max_rule"]
