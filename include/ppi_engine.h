@@ -314,6 +314,100 @@ ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_
 ppi_status ppi_adjoint_time(ppi_adjoint* a, int32_t iters, int32_t partials, double* ms_per_pass);
 ppi_status ppi_adjoint_get_stats(const ppi_adjoint* a, ppi_adjoint_stats* out);
 
+/*
+ * Evaluation with the reference's epilogue on the device: upload, launch, then alpha*size and the
+ * strategy's ranking (best_individual + PickNBest, or PickNBest over the vector: accelerator.cl:270-313,
+ * accelerator.cc:843-889, sequential.cc:197-208) computed by kernels; only the vector and the index
+ * come back.  Same arguments and results as ppi_engine_evaluate (best_size at most 64).
+ */
+ppi_status ppi_engine_evaluate_ranked(ppi_engine* eng, const int32_t* phenotype, const float* ephemeral,
+                                      const int32_t* size, int32_t stride, int32_t n_programs, float alpha,
+                                      float* vector, int32_t* index, int32_t* best_size);
+/* The resident compiled tape (test and inspection): meta[4*i..] = {code_off, ncode, size, flags} per
+   program, code[2*k..] = {code, arg} per instruction; *n_code receives the instruction count (when
+   code is NULL only the count is returned). */
+ppi_status ppi_engine_resident_tape(const ppi_engine* eng, uint32_t* meta, uint32_t* code, int64_t* n_code);
+
+/*
+ * Grammatical evolution on the device: the genome -> phenotype decoder (ppi.cc:106-182), the
+ * generation loop of ppi_evolve (ppi.cc:823-945: elitism, tournament selection, one- and two-point
+ * crossover, bit-flip and shrink mutation, ppi.cc:422-452,680-814) and the per-generation
+ * evaluation and best-individual tracking of ppi_evaluate (ppi.cc:570-640), with the programs
+ * compiled into the engine's tape by the device (no host round trip).  Genomes, phenotypes,
+ * tapes and fitness stay in HBM; the host reads back only what it asks for.
+ *
+ * Random numbers reproduce the reference's: XorShift128+ (util/Random.h:130-162), one generator
+ * per OpenMP thread seeded seed ^ thread (ppi.cc:388-396), each thread drawing for the
+ * iterations the static schedule gives it.  With the same seed, thread count, grammar, dataset
+ * and evaluation semantics, the device run follows the reference's run genome for genome (the
+ * evaluation must give the reference's bits: SEQ or PP strategy with the ordered form).
+ * Islands are the caller's: immigrants are handed over with ppi_ge_set_immigrants.
+ */
+typedef struct ppi_ge ppi_ge;
+
+/* The decoder's grammar: the grammar header's gramatica[] and tamanhos[] (script/read_grammar.py). */
+typedef struct ppi_ge_grammar {
+    int32_t n_nonterminals;        /* non-terminals are Symbol values 0 .. n-1; NT_INICIAL = 0 starts */
+    const int32_t* n_rules;        /* tamanhos[nt] */
+    const int32_t* rule_len;       /* quantity of each rule; the rules of nt 0 first, then nt 1, ... */
+    const int32_t* rule_symbols;   /* rule r: Symbols at [r * max_rule, r * max_rule + rule_len[r]) */
+    int32_t max_rule;              /* MAX_QUANT_SIMBOLOS_POR_REGRA */
+    int32_t t_const;               /* T_CONST's Symbol value, -1 without one (NOT_USING_T_CONST) */
+    int32_t t_attribute;           /* T_ATTRIBUTE's Symbol value (T_ATTRk = 20000 + k decode to it) */
+} ppi_ge_grammar;
+
+/* ppi_init's options (ppi.cc:200-300). */
+typedef struct ppi_ge_params {
+    int32_t population_size;       /* -ps */
+    int32_t number_of_bits;        /* -nb */
+    int32_t bits_per_gene;         /* -bg (raised to what the grammar needs, ppi.cc:258-264) */
+    int32_t bits_per_constant;     /* -bc */
+    float min_constant, max_constant; /* -min, -max (swapped when min > max) */
+    float crossover_rate;          /* -cp */
+    float mutation_rate;           /* -mr */
+    int32_t tournament_size;       /* -ts */
+    int32_t elitism;               /* -e */
+    int32_t max_phenotype;         /* -mps: 0 = MAX_QUANT_SIMBOLOS_POR_REGRA * nb / bg */
+    int32_t threads;               /* -t: the reference's OpenMP threads = random streams (1 = sequential) */
+    int64_t seed;                  /* -s; seed ^ thread must be non-zero (0 seeds from the clock) */
+    float alpha;                   /* ALPHA (ppi.cc:63): 1e-6 */
+} ppi_ge_params;
+
+typedef struct ppi_ge_stats {
+    int32_t generation;            /* generations evolved since ppi_ge_initialize */
+    int32_t bits_per_gene;         /* effective */
+    int32_t max_size_phenotype;    /* phenotype stride */
+    int32_t max_slots;             /* register slots of the last compiled population */
+    float best_fitness;            /* best individual so far */
+    double breed_ms, decode_ms, compile_ms, evaluate_ms, rank_ms; /* last generation, device time */
+    double generation_ms;          /* last generation, host wall time */
+    double node_evals;             /* last generation: sum(size) * nlin */
+} ppi_ge_stats;
+
+/* Evaluates through eng's configuration and dataset; destroy it before eng. */
+ppi_status ppi_ge_create(ppi_ge** out, ppi_engine* eng);
+void ppi_ge_destroy(ppi_ge* ge);
+ppi_status ppi_ge_configure(ppi_ge* ge, const ppi_ge_params* params, const ppi_ge_grammar* grammar);
+/* ppi_generate_population (ppi.cc:656-672): random genomes, decoded and evaluated. */
+ppi_status ppi_ge_initialize(ppi_ge* ge);
+/* `generations` iterations of ppi_evolve's loop (ppi.cc:878-945). */
+ppi_status ppi_ge_evolve(ppi_ge* ge, int32_t generations);
+/* The current population as the reference holds it: genomes as GENOME_TYPE bytes (0/1, P x nb) and
+   fitness (P); either may be NULL. */
+ppi_status ppi_ge_get_population(const ppi_ge* ge, uint8_t* genomes, float* fitness);
+/* Immigrants for the next generation (ppi_receive_individual, ppi.cc:528-560): the next
+   ppi_ge_evolve generation starts its offspring population with these n genomes (bool bytes,
+   n x nb), then the elite (with elitism), then the bred individuals, as ppi_evolve does with
+   nImmigrants (ppi.cc:882-896). */
+ppi_status ppi_ge_set_immigrants(ppi_ge* ge, const uint8_t* genomes, int32_t n);
+ppi_status ppi_ge_best(const ppi_ge* ge, float* fitness, uint8_t* genome);
+/* The current population decoded (ppi.cc:578-591): phenotype and ephemeral P x stride (ephemeral
+   is 0 off T_CONST / T_ATTRIBUTE positions), size P; *stride = max_size_phenotype. */
+ppi_status ppi_ge_decoded(const ppi_ge* ge, int32_t* phenotype, float* ephemeral, int32_t* size, int32_t* stride);
+/* The device-compiled tape of the current population, laid out as ppi_engine_resident_tape. */
+ppi_status ppi_ge_tape(const ppi_ge* ge, uint32_t* meta, uint32_t* code, int64_t* n_code);
+ppi_status ppi_ge_get_stats(const ppi_ge* ge, ppi_ge_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

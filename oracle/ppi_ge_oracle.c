@@ -18,8 +18,8 @@
  * Undefined behaviour of the reference is given one meaning, shared with the engine: a terminal
  * written at a position >= max_size_phenotype (possible only when -mps caps the size) kills the
  * individual (size 0), like the position check of decode (ppi.cc:143); a seed ^ thread of 0 (which
- * the reference replaces by the clock) is rejected; a random index that rounds to n (probability
- * 2^-54 per draw) is clamped to n - 1.
+ * the reference replaces by the clock) is rejected; a draw that rounds to 1.0 (probability 2^-54)
+ * is clamped: a random index to n - 1, the number of mutated bits to its maximum.
  */
 #include "ppi_oracle.h"
 
@@ -231,6 +231,7 @@ static void mutation(ge_run* R, oracle_rng* r, uint8_t* genome) {
     /* ceil(mutation_rate * number_of_bits) in float (a float times an int) */
     const int max_bits = (int)ceilf(R->p->mutation_rate * (float)nb);
     int num = (int)(oracle_rng_real(r) * (max_bits + 1));
+    if (num > max_bits) num = max_bits;  /* a draw that rounds to 1.0 */
     if (num == 0) return;
     if (oracle_rng_probability(r, 0.75)) {
         while (num-- > 0) {

@@ -18,6 +18,7 @@
 
 #include "../../include/ppi_engine.h"
 #include "engine_internal.h"
+#include "ge.h"
 #include "kernels.h"
 #include "tape.h"
 
@@ -222,13 +223,9 @@ static void reference_geometry(ppi_engine* E) {
 
 static void reset_derived(ppi_engine* E);
 
-// Kernel form and shape for a compiled population of `nprog` programs (`cs` its compile
-// statistics) on this engine's dataset and configuration.
-struct LaunchPlan {
-    int form = PPI_FORM_FP, R = 4, F = 16, S = 9, chunk = 64, nrb = 1;
-};
+namespace ppi {
 
-static bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t nprog, int requested_form,
+bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t nprog, int requested_form,
                          LaunchPlan* L, std::string* why) {
     int need_f = cs.max_col + 1;
     int F = need_f <= 8 ? 8 : (need_f <= 16 ? 16 : (need_f <= 32 ? 32 : 0));
@@ -282,6 +279,8 @@ static bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t np
     return true;
 }
 
+}  // namespace ppi
+
 static bool plan(ppi_engine* E, std::string* why) {
     LaunchPlan L;
     if (!compute_plan(E, E->tape.stats, E->nprog, E->cfg.form, &L, why)) return false;
@@ -295,7 +294,7 @@ static bool plan(ppi_engine* E, std::string* why) {
 }
 
 // best_individual's work-group size for `nprog` programs (accelerator.cc:331-337).
-static int best_local_for(const ppi_engine* E, int32_t nprog) {
+int ppi::best_local_for(const ppi_engine* E, int32_t nprog) {
     const int cu = E->cfg.compute_units > 0 ? E->cfg.compute_units : E->dev_cu;
     const int mls2 = (int)std::min<size_t>((size_t)E->dev_max_wg, E->dev_lds / 8);
     return std::max(1, std::min(mls2, (int)std::ceil(nprog / (float)cu)));
@@ -365,7 +364,8 @@ void ppi_engine_destroy(ppi_engine* E) {
     (void)hipSetDevice(E->device);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void* dev[] = {E->dX, E->dMeta, E->dCode, E->dPartial, E->dBadLo, E->dBadHi, E->dRaw, E->dVec,
-                   E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop};
+                   E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop,
+                   E->dRankVec, E->dRankGv, E->dRankGi, E->dRankIdx};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     E->folds.release();
@@ -559,7 +559,9 @@ ppi_status ppi_engine_upload(ppi_engine* E, const int32_t* phenotype, const floa
     return PPI_OK;
 }
 
-static EvalArgs make_args(ppi_engine* E, float alpha) {
+}  // extern "C"
+
+EvalArgs ppi::make_args(ppi_engine* E, float alpha) {
     EvalArgs A{};
     A.meta = E->dMeta;
     A.code = E->dCode;
@@ -595,7 +597,7 @@ static EvalArgs make_args(ppi_engine* E, float alpha) {
 
 // Enqueue one evaluation on the engine stream: programs, work and result buffers as `A` names them,
 // kernel form and shape as `L` says.
-static ppi_status enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, bool time_main) {
+ppi_status ppi::enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, bool time_main) {
     const bool P = E->cfg.protected_ops != 0, N = E->cfg.native_math != 0;
     PPI_HIP(launch_fill_u32(A.bad_lo, 0xffffffffu, A.nprog, E->stream));
     PPI_HIP(launch_fill_u32(A.bad_hi, 0u, A.nprog, E->stream));
@@ -606,11 +608,11 @@ static ppi_status enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, b
         PPI_HIP(launch_finalize(A, L.nrb, E->stream));
     } else if (L.form == PPI_FORM_ORDERED) {
         // Passes of nrb_pass row tiles: per-row errors, then each program's row-order sum continues.
-        PPI_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(E->dOSum), 0u, E->nprog, E->stream));  // 0.0f
-        PPI_HIP(launch_fill_u32(E->dOStop, 0u, E->nprog, E->stream));
-        for (int rb0 = 0; rb0 < L.nrb; rb0 += E->nrb_pass) {
+        PPI_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(A.osum), 0u, A.nprog, E->stream));  // 0.0f
+        PPI_HIP(launch_fill_u32(A.ostop, 0u, A.nprog, E->stream));
+        for (int rb0 = 0; rb0 < L.nrb; rb0 += L.nrb_pass) {
             A.rb0 = rb0;
-            A.nrb_pass = std::min(E->nrb_pass, L.nrb - rb0);
+            A.nrb_pass = std::min(L.nrb_pass, L.nrb - rb0);
             PPI_HIP(launch_fp(A, L.R, L.F, L.S, P, N, E->stream, true));
             PPI_HIP(launch_ordered_sum(A, fp_rows_per_block(L.R), rb0 + A.nrb_pass >= L.nrb, E->stream));
         }
@@ -636,8 +638,11 @@ static ppi_status enqueue_eval(ppi_engine* E, float alpha, bool time_main) {
     L.S = E->S;
     L.chunk = E->chunk;
     L.nrb = E->nrb;
+    L.nrb_pass = E->nrb_pass;
     return enqueue_with(E, make_args(E, alpha), L, time_main);
 }
+
+extern "C" {
 
 ppi_status ppi_engine_launch(ppi_engine* E) {
     if (!E) { set_error("engine is NULL"); return PPI_E_ARG; }
@@ -859,6 +864,52 @@ ppi_status ppi_engine_evaluate_noalloc(ppi_engine* E, const int32_t* phenotype, 
     } else if (rank && strategy == PPI_STRATEGY_FP) {
         best_acc(vector, n, best_local_for(E, n), index, best_size);
     }
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_evaluate_ranked(ppi_engine* E, const int32_t* phenotype, const float* ephemeral,
+                                      const int32_t* size, int32_t stride, int32_t n, float alpha, float* vector,
+                                      int32_t* index, int32_t* best_size) {
+    if (!vector) { set_error("vector is NULL"); return PPI_E_ARG; }
+    if (index && best_size && *best_size > kRankMax) {
+        set_error("best_size above " + std::to_string(kRankMax));
+        return PPI_E_ARG;
+    }
+    ppi_status s = ppi_engine_upload(E, phenotype, ephemeral, size, stride, n);
+    if (s) return s;
+    if (E->cap_rank < (size_t)n || !E->dRankVec) {
+        size_t c = 0;
+        if ((s = ensure(&E->dRankVec, &c, (size_t)n))) return s;
+        c = 0;
+        if ((s = ensure(&E->dRankGv, &c, (size_t)n))) return s;
+        c = 0;
+        if ((s = ensure(&E->dRankGi, &c, (size_t)n))) return s;
+        c = 0;
+        if ((s = ensure(&E->dRankIdx, &c, (size_t)kRankMax))) return s;
+        E->cap_rank = (size_t)n;
+    }
+    if ((s = enqueue_eval(E, 0.0f, true))) return s;
+    // the epilogue of ppi_engine_wait, on the device
+    int32_t k = (index && best_size && *best_size > 0) ? *best_size : 0;
+    if (k > 0) {
+        const int32_t limit = E->cfg.strategy == PPI_STRATEGY_SEQ ? n : (n + E->best_local - 1) / E->best_local;
+        k = std::min(k, limit);
+    }
+    PPI_HIP(launch_rank(E->dRaw, E->dMeta, n, E->cfg.strategy, alpha, E->best_local, k, E->dRankVec, E->dRankGv,
+                        E->dRankGi, E->dRankIdx, E->stream));
+    PPI_HIP(hipMemcpyAsync(vector, E->dRankVec, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, E->stream));
+    if (k > 0) PPI_HIP(hipMemcpyAsync(index, E->dRankIdx, (size_t)k * sizeof(int32_t), hipMemcpyDeviceToHost, E->stream));
+    PPI_HIP(hipStreamSynchronize(E->stream));
+    if (index && best_size && *best_size > 0) *best_size = k;
+    return PPI_OK;
+}
+
+ppi_status ppi_engine_resident_tape(const ppi_engine* E, uint32_t* meta, uint32_t* code, int64_t* n_code) {
+    if (!E || !n_code) { set_error("bad arguments"); return PPI_E_ARG; }
+    if (E->nprog <= 0) { set_error("upload programs first"); return PPI_E_STATE; }
+    *n_code = (int64_t)E->tape.code.size();
+    if (meta) std::memcpy(meta, E->tape.meta.data(), E->tape.meta.size() * sizeof(ProgMeta));
+    if (code) std::memcpy(code, E->tape.code.data(), E->tape.code.size() * sizeof(Ins));
     return PPI_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/ppi_engine.h"
+#include "kernels.h"
 #include "tape.h"
 
 namespace ppi {
@@ -111,6 +112,31 @@ struct ppi_engine {
 
     bool launched = false;
     ppi_stats stats{};
+    // device epilogue (ppi_engine_evaluate_ranked): alpha-inclusive vector, group winners, index
+    float* dRankVec = nullptr;
+    float* dRankGv = nullptr;
+    int32_t* dRankGi = nullptr;
+    int32_t* dRankIdx = nullptr;
+    size_t cap_rank = 0;
 };
+
+namespace ppi {
+// Kernel form and shape of one evaluation (engine.cc: compute_plan).
+struct LaunchPlan {
+    int form = PPI_FORM_FP, R = 4, F = 16, S = 9, chunk = 64, nrb = 1;
+    int nrb_pass = 1;  // ordered form: row tiles per pass of stored row errors
+};
+// The form and shape for a compiled population of `nprog` programs (`cs` its compile statistics)
+// on the engine's dataset and configuration, form `requested_form` (AUTO: FP).
+bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t nprog, int requested_form, LaunchPlan* L,
+                  std::string* why);
+// best_individual's work-group size for `nprog` programs (accelerator.cc:331-337).
+int best_local_for(const ppi_engine* E, int32_t nprog);
+// Kernel arguments naming the engine's own resident buffers.
+EvalArgs make_args(ppi_engine* E, float alpha);
+// Enqueues one evaluation (fills, form kernels, epilogue, recheck) on the engine stream with the
+// buffers `A` names.
+ppi_status enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, bool time_main);
+}  // namespace ppi
 
 #endif

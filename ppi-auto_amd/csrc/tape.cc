@@ -35,56 +35,6 @@ const ppi_symbol* SymbolTable::find(int32_t symbol) const {
 
 namespace {
 
-int arity(int32_t op) {
-    if (op == PPI_OP_IF_THEN_ELSE) return 3;
-    if (op >= PPI_OP_AND && op <= PPI_OP_DIV) return 2;
-    if (op >= PPI_OP_NOT && op <= PPI_OP_TAN) return 1;
-    return 0;
-}
-
-uint32_t unop_of(int32_t op) {
-    switch (op) {
-    case PPI_OP_NOT: return U_NOT;     case PPI_OP_ABS: return U_ABS;
-    case PPI_OP_POW2: return U_POW2;   case PPI_OP_POW3: return U_POW3;
-    case PPI_OP_POW4: return U_POW4;   case PPI_OP_POW5: return U_POW5;
-    case PPI_OP_NEG: return U_NEG;     case PPI_OP_ROUND: return U_ROUND;
-    case PPI_OP_CEIL: return U_CEIL;   case PPI_OP_FLOOR: return U_FLOOR;
-    case PPI_OP_EXP: return U_EXP;     case PPI_OP_EXP10: return U_EXP10;
-    case PPI_OP_EXP2: return U_EXP2;   case PPI_OP_SIN: return U_SIN;
-    case PPI_OP_COS: return U_COS;     case PPI_OP_STEP: return U_STEP;
-    case PPI_OP_SIGN: return U_SIGN;   case PPI_OP_LOGISTIC: return U_LOGISTIC;
-    case PPI_OP_GAMMA: return U_GAMMA; case PPI_OP_GAUSS: return U_GAUSS;
-    case PPI_OP_SQRT: return U_SQRT;   case PPI_OP_LOG: return U_LOG;
-    case PPI_OP_LOG10: return U_LOG10; case PPI_OP_LOG2: return U_LOG2;
-    default: return U_TAN;
-    }
-}
-
-// Binary operator with the accumulator as the left operand (rev=false) or as the
-// right operand (rev=true); the other operand is the folded/parked value.
-uint32_t binop_of(int32_t op, bool rev) {
-    switch (op) {
-    case PPI_OP_AND: return B_AND;
-    case PPI_OP_OR: return B_OR;
-    case PPI_OP_XOR: return B_XOR;
-    case PPI_OP_GREATER: return rev ? B_LT : B_GT;
-    case PPI_OP_GREATEREQUAL: return rev ? B_LE : B_GE;
-    case PPI_OP_LESS: return rev ? B_GT : B_LT;
-    case PPI_OP_LESSEQUAL: return rev ? B_GE : B_LE;
-    case PPI_OP_EQUAL: return B_EQ;
-    case PPI_OP_NOTEQUAL: return B_NE;
-    case PPI_OP_ADD: return B_ADD;
-    case PPI_OP_SUB: return rev ? B_RSUB : B_SUB;
-    case PPI_OP_MULT: return B_MULT;
-    case PPI_OP_MEAN: return B_MEAN;
-    case PPI_OP_MAX: return rev ? B_RMAX : B_MAX;
-    case PPI_OP_MIN: return rev ? B_RMIN : B_MIN;
-    case PPI_OP_MOD: return rev ? B_RMOD : B_MOD;
-    case PPI_OP_POW: return rev ? B_RPOW : B_POW;
-    default: return rev ? B_RDIV : B_DIV;
-    }
-}
-
 struct Node {
     int32_t op;       // ppi_op of an operator, or -1 for a leaf
     int32_t ch[3];

@@ -142,7 +142,10 @@ ENGINE_SYMBOLS = [
     "ppi_abi_version", "ppi_last_error", "ppi_device_count", "ppi_engine_create", "ppi_engine_destroy",
     "ppi_engine_configure", "ppi_engine_set_dataset", "ppi_engine_set_dataset_rowmajor", "ppi_engine_upload",
     "ppi_engine_launch", "ppi_engine_wait", "ppi_engine_evaluate", "ppi_engine_predict", "ppi_engine_get_stats",
-    "ppi_engine_evaluate_noalloc",
+    "ppi_engine_evaluate_noalloc", "ppi_engine_evaluate_ranked", "ppi_engine_resident_tape",
+    "ppi_ge_create", "ppi_ge_destroy", "ppi_ge_configure", "ppi_ge_initialize", "ppi_ge_evolve",
+    "ppi_ge_get_population", "ppi_ge_set_immigrants", "ppi_ge_best", "ppi_ge_decoded", "ppi_ge_tape",
+    "ppi_ge_get_stats",
     "ppi_engine_time", "ppi_batch_create", "ppi_batch_destroy", "ppi_batch_configure", "ppi_batch_add_dataset",
     "ppi_batch_add_population", "ppi_batch_clear_populations", "ppi_batch_build", "ppi_batch_launch",
     "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
@@ -175,6 +178,21 @@ def lib() -> C.CDLL:
         L.ppi_engine_evaluate.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
                                           C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
         L.ppi_engine_predict.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), C.c_int32, P(C.c_float)]
+        L.ppi_engine_evaluate_ranked.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
+                                                 C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
+        L.ppi_engine_resident_tape.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_uint32), P(C.c_int64)]
+        L.ppi_ge_create.argtypes = [P(C.c_void_p), C.c_void_p]
+        L.ppi_ge_destroy.argtypes = [C.c_void_p]
+        L.ppi_ge_destroy.restype = None
+        L.ppi_ge_configure.argtypes = [C.c_void_p, P(GeParams), P(GeGrammar)]
+        L.ppi_ge_initialize.argtypes = [C.c_void_p]
+        L.ppi_ge_evolve.argtypes = [C.c_void_p, C.c_int32]
+        L.ppi_ge_get_population.argtypes = [C.c_void_p, P(C.c_uint8), P(C.c_float)]
+        L.ppi_ge_set_immigrants.argtypes = [C.c_void_p, P(C.c_uint8), C.c_int32]
+        L.ppi_ge_best.argtypes = [C.c_void_p, P(C.c_float), P(C.c_uint8)]
+        L.ppi_ge_decoded.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), P(C.c_int32)]
+        L.ppi_ge_tape.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_uint32), P(C.c_int64)]
+        L.ppi_ge_get_stats.argtypes = [C.c_void_p, P(GeStats)]
         L.ppi_engine_evaluate_noalloc.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int32,
                                                   C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
         L.ppi_engine_get_stats.argtypes = [C.c_void_p, P(Stats)]
@@ -253,8 +271,14 @@ class Engine:
         self._h = C.c_void_p()
         _check(lib().ppi_engine_create(C.byref(self._h), device))
         self._cost = None
+        self._children = []  # objects that evaluate through this engine (GE): closed before it
 
     def close(self) -> None:
+        for ref in self._children:
+            child = ref()
+            if child is not None:
+                child.close()
+        self._children = []
         if self._h:
             lib().ppi_engine_destroy(self._h)
             self._h = C.c_void_p()
@@ -307,6 +331,7 @@ class Engine:
         _check(lib().ppi_engine_set_dataset(self._h, rows, d.shape[0], d.shape[1]))
 
     def upload(self, pop: "Population") -> None:
+        self._nprog = pop.n
         _check(lib().ppi_engine_upload(self._h, _ptr(pop.phenotype, C.c_int32), _ptr(pop.ephemeral, C.c_float),
                                        _ptr(pop.size, C.c_int32), pop.stride, pop.n))
 
@@ -339,6 +364,28 @@ class Engine:
                                                  _ptr(idx, C.c_int32) if best_size else None,
                                                  C.byref(bs) if best_size else None))
         return vec, idx[:bs.value] if best_size else None
+
+    def evaluate_ranked(self, pop: "Population", alpha: float = 0.0, best_size: int = 0):
+        """ppi_engine_evaluate_ranked: alpha*size and the strategy's ranking computed on the device."""
+        vec = np.empty(pop.n, dtype=np.float32)
+        idx = np.zeros(max(1, best_size), dtype=np.int32)
+        bs = C.c_int32(best_size)
+        _check(lib().ppi_engine_evaluate_ranked(self._h, _ptr(pop.phenotype, C.c_int32),
+                                                _ptr(pop.ephemeral, C.c_float), _ptr(pop.size, C.c_int32),
+                                                pop.stride, pop.n, alpha, _ptr(vec, C.c_float),
+                                                _ptr(idx, C.c_int32) if best_size else None,
+                                                C.byref(bs) if best_size else None))
+        return vec, idx[:bs.value] if best_size else None
+
+    def resident_tape(self):
+        """(meta n x 4 uint32, code m x 2 uint32) of the resident compiled programs."""
+        n = C.c_int64(0)
+        _check(lib().ppi_engine_resident_tape(self._h, None, None, C.byref(n)))
+        nprog = self._nprog
+        meta = np.zeros((nprog, 4), np.uint32)
+        code = np.zeros((n.value, 2), np.uint32)
+        _check(lib().ppi_engine_resident_tape(self._h, _ptr(meta, C.c_uint32), _ptr(code, C.c_uint32), C.byref(n)))
+        return meta, code
 
     def predict(self, phenotype: np.ndarray, ephemeral: np.ndarray, size: int, nlin: int) -> np.ndarray:
         ph = np.ascontiguousarray(phenotype, dtype=np.int32)
@@ -707,3 +754,75 @@ def batched_workload(n_pairs: int = 64, seed: int = 4, p_range=(500, 20000), len
         datas.append(dataset(DS_POLY_UNIFORM, seed * 1000 + 2 * k + 1, nlin, nfeat))
         grammars.append(gr)
     return BatchedWorkload(grammars, pops, datas)
+
+
+class GE:
+    """Grammatical evolution on the device (ppi_ge_*), evaluating through an Engine."""
+
+    def __init__(self, engine: Engine, setup: "GeSetup"):
+        import weakref
+        self._h = C.c_void_p()
+        self.engine = engine
+        self.setup = setup
+        _check(lib().ppi_ge_create(C.byref(self._h), engine._h))
+        engine._children.append(weakref.ref(self))
+        _check(lib().ppi_ge_configure(self._h, C.byref(setup.params), C.byref(setup.grammar)))
+        self.P = setup.params.population_size
+        self.nb = setup.params.number_of_bits
+
+    def close(self):
+        if self._h:
+            lib().ppi_ge_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self):
+        _check(lib().ppi_ge_initialize(self._h))
+
+    def evolve(self, generations: int = 1):
+        _check(lib().ppi_ge_evolve(self._h, generations))
+
+    def population(self):
+        g = np.zeros((self.P, self.nb), np.uint8)
+        f = np.zeros(self.P, np.float32)
+        _check(lib().ppi_ge_get_population(self._h, _ptr(g, C.c_uint8), _ptr(f, C.c_float)))
+        return g, f
+
+    def set_immigrants(self, genomes: np.ndarray):
+        g = np.ascontiguousarray(genomes, dtype=np.uint8)
+        _check(lib().ppi_ge_set_immigrants(self._h, _ptr(g, C.c_uint8), g.shape[0]))
+
+    def best(self):
+        f = C.c_float(0.0)
+        g = np.zeros(self.nb, np.uint8)
+        _check(lib().ppi_ge_best(self._h, C.byref(f), _ptr(g, C.c_uint8)))
+        return f.value, g
+
+    def decoded(self):
+        st = self.stats()
+        mx = st.max_size_phenotype
+        ph = np.zeros((self.P, mx), np.int32)
+        ep = np.zeros((self.P, mx), np.float32)
+        sz = np.zeros(self.P, np.int32)
+        stride = C.c_int32(0)
+        _check(lib().ppi_ge_decoded(self._h, _ptr(ph, C.c_int32), _ptr(ep, C.c_float), _ptr(sz, C.c_int32),
+                                    C.byref(stride)))
+        return ph, ep, sz
+
+    def tape(self):
+        n = C.c_int64(0)
+        _check(lib().ppi_ge_tape(self._h, None, None, C.byref(n)))
+        meta = np.zeros((self.P, 4), np.uint32)
+        code = np.zeros((n.value, 2), np.uint32)
+        _check(lib().ppi_ge_tape(self._h, _ptr(meta, C.c_uint32), _ptr(code, C.c_uint32), C.byref(n)))
+        return meta, code
+
+    def stats(self) -> GeStats:
+        s = GeStats()
+        _check(lib().ppi_ge_get_stats(self._h, C.byref(s)))
+        return s
