@@ -153,6 +153,24 @@ def bench_batched(steps, warmup):
             "programs": int(sum(p.n for p in w.pops))}
 
 
+def bench_adjoint_pass(steps, warmup):
+    """The adjoint workload's backward pass alone (forward, autotuned backward, `steps` timed passes):
+    the leg profiler runs use."""
+    w = ppiamd.workload("adjoint")
+    a = ppiamd.Adjoint(0)
+    a.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+    a.set_dataset(w.data)
+    a.forward(w.pop, alpha=1e-6)
+    a.backward()
+    if warmup > 0:
+        a.time(warmup)
+    ms = a.time(steps)
+    st = a.stats()
+    a.close()
+    return {"ms_per_backward": round(ms, 3), "grads_per_s": round(w.pop.n / (ms * 1e-3), 1),
+            "partials": st.partials, "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins}
+
+
 def bench_adjoint(steps, warmup, threads):
     """Adjoint workload (30,000 programs with <= 8 T_CONST each, 1,000,000 x 8 rows): device time
     of one backward pass (forward recomputation + reverse sweep + epilogue), and 20 on-device
@@ -194,6 +212,67 @@ def bench_adjoint(steps, warmup, threads):
             "descend_ms": dst.descend_ms, "descend_rows": sub, "improved": float((rel > 0).mean()),
             "median_rel_gain": float(np.median(rel)), "worse": int((f1[finite] > f0[finite]).sum()),
             "reverted": dst.descend_reverted}
+
+
+def bench_ge(gens):
+    """Grammatical evolution at the regression configuration's scale: 30,000 individuals (PPI's
+    defaults -nb 2000 -bg 8 -bc 16 -cp 0.9 -mr 0.01 -ts 3, elitism) over the 100,000 x 16 rows, the
+    config grammar (+ - * / sin cos exp log sqrt, attr0..15, const).  Device: every generation on the
+    GPU (breeding plan with 1 or 16 random streams = the reference's -t, offspring, decode, tape
+    compile, evaluation, epilogue).  Beside it the drop-in path of the reference's own loop: breeding
+    and decoding on the host (the oracle's restatement of ppi.cc, one thread) and the engine's
+    evaluate (host compile + upload + kernels + results)."""
+    w = ppiamd.workload("regression")
+    gr = w.grammar
+    eng = ppiamd.Engine(0)
+    eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True, form=ppiamd.FORM_FP)
+    eng.set_dataset(w.data)
+    out = {}
+    for T in (1, 16):
+        setup = ppiamd.GeSetup(gr.g, population_size=30000, threads=T, seed=12345, elitism=True)
+        ge = ppiamd.GE(eng, setup)
+        t0 = time.perf_counter()
+        ge.initialize()
+        init_s = time.perf_counter() - t0
+        ge.evolve(2)
+        t0 = time.perf_counter()
+        ge.evolve(gens)
+        ms = (time.perf_counter() - t0) * 1e3 / gens
+        st = ge.stats()
+        out[f"threads_{T}"] = {"ms_per_generation": round(ms, 3), "init_ms": round(init_s * 1e3, 1),
+                               "gpops_end_to_end": round(st.node_evals / (ms * 1e-3) / 1e9, 3),
+                               "last_generation_device_ms": {"breed": round(st.breed_ms, 3),
+                                                             "decode": round(st.decode_ms, 3),
+                                                             "compile": round(st.compile_ms, 3),
+                                                             "evaluate": round(st.evaluate_ms, 3),
+                                                             "rank": round(st.rank_ms, 3)},
+                               "node_evals": st.node_evals, "max_size_phenotype": st.max_size_phenotype,
+                               "best_fitness": st.best_fitness}
+        if T == 1:
+            genomes, fit = ge.population()
+        ge.close()
+    # the drop-in path: host breeding + decode (one thread), engine evaluate
+    oracle = _oracle()
+    setup = ppiamd.GeSetup(gr.g, population_size=30000, threads=1, seed=12345, elitism=True)
+    host = {"breed_ms": 0.0, "decode_ms": 0.0, "evaluate_ms": 0.0}
+    for _ in range(2):
+        t0 = time.perf_counter()
+        kids = oracle.ge_breed(setup, genomes, fit, n_immigrants=1)
+        t1 = time.perf_counter()
+        ph, ep, sz, mx = oracle.ge_decode(setup, kids)
+        t2 = time.perf_counter()
+        pop = ppiamd.Population(ph.ravel(), ep.ravel(), sz, mx)
+        fit, _ = eng.evaluate(pop, alpha=1e-6, best_size=1)
+        t3 = time.perf_counter()
+        genomes = kids
+        host["breed_ms"] += (t1 - t0) * 500
+        host["decode_ms"] += (t2 - t1) * 500
+        host["evaluate_ms"] += (t3 - t2) * 500
+    eng.close()
+    host = {k: round(v, 3) for k, v in host.items()}
+    host["ms_per_generation"] = round(sum(host.values()), 3)
+    out["host_loop_with_drop_in"] = host
+    return out
 
 
 def _oracle():
@@ -301,7 +380,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline workload and CPU baseline only")
-    ap.add_argument("--only", choices=["fp", "batched", "adjoint"],
+    ap.add_argument("--only", choices=["fp", "batched", "adjoint", "adjoint_pass", "ge"],
                     help="one leg, no CPU baseline (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the 1 GiB copy kernel used to scale the memory counters, then exit")
@@ -316,7 +395,7 @@ def main():
     if ppiamd.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    legs = {"fp", "batched", "adjoint"} if a.only is None else {a.only}
+    legs = {"fp", "batched", "adjoint", "ge"} if a.only is None else {a.only}
     extras = a.only is None and not a.quick
 
     line = {
@@ -445,6 +524,19 @@ def main():
                                         "sample (the first programs, all rows), for the GPU and for the reference's "
                                         "sequential path (the oracle), with every operator and the error function "
                                         "in double for FP64; sentinel (non-finite) programs are counted apart")
+    if a.only == "adjoint_pass":
+        log("adjoint backward pass: 30,000 programs x 1,000,000 rows")
+        line["adjoint_pass"] = bench_adjoint_pass(a.steps, a.warmup)
+    if "ge" in legs and (a.only == "ge" or extras):
+        log("grammatical evolution on the device: 30,000 individuals x 100,000 rows, 10 generations")
+        ge = bench_ge(10)
+        ge["workload"] = ("30,000 individuals, -nb 2000 -bg 8 -bc 16 -cp 0.9 -mr 0.01 -ts 3 -e, config grammar over "
+                          "100,000 x 16 rows, EF = (X-Y)^2, sequential-path semantics (FP form)")
+        ge["note"] = ("ms per generation of ppi_evolve's loop: device = breeding plan + offspring + decode + "
+                      "tape compile + evaluation + epilogue, all in HBM (threads = the reference's -t random "
+                      "streams); host_loop_with_drop_in = the reference's loop with this engine as its "
+                      "interpreter: breeding and decode on one host thread, then evaluate")
+        line["ge"] = ge
     if "batched" in legs and (a.only == "batched" or extras):
         log("batched: 64 uneven pairs (populations 500-20,000, lengths 8-256, rows 1e3-1e6, 2-32 features)")
         batched = bench_batched(max(1, a.steps // 5), 1)
