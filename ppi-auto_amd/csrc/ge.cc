@@ -60,6 +60,42 @@ double now_ms() {
 
 }  // namespace
 
+// XorShift128+'s state update without the output addition is linear over GF(2): s0' = s1,
+// s1' = x ^ s1 ^ (x >> 17) ^ (s1 >> 26) with x = s0 ^ (s0 << 23).  Matrices are 128 columns of
+// (s0, s1) word pairs: column b < 64 is the image of s0's bit b, 64 + b of s1's bit b.
+void ppi::ge_jump_tables(uint64_t* out) {
+    struct V { uint64_t a, b; };
+    auto step = [](V v) {
+        uint64_t x = v.a;
+        const uint64_t y = v.b;
+        x ^= x << 23;
+        return V{y, x ^ y ^ (x >> 17) ^ (y >> 26)};
+    };
+    auto apply = [](const std::vector<V>& M, V v) {
+        V r{0, 0};
+        for (int b = 0; b < 64; ++b) {
+            if ((v.a >> b) & 1) { r.a ^= M[b].a; r.b ^= M[b].b; }
+            if ((v.b >> b) & 1) { r.a ^= M[64 + b].a; r.b ^= M[64 + b].b; }
+        }
+        return r;
+    };
+    auto square = [&](const std::vector<V>& M) {
+        std::vector<V> R(128);
+        for (int j = 0; j < 128; ++j) R[j] = apply(M, M[j]);
+        return R;
+    };
+    std::vector<V> M(128);
+    for (int j = 0; j < 128; ++j) M[j] = step(j < 64 ? V{1ull << j, 0} : V{0, 1ull << (j - 64)});
+    for (int k = 1; k < kDrawBlock; k <<= 1) M = square(M);  // T^kDrawBlock
+    for (int lvl = 0; lvl < kJumpLevels; ++lvl) {
+        for (int j = 0; j < 128; ++j) {
+            out[(size_t)lvl * 256 + 2 * j] = M[j].a;
+            out[(size_t)lvl * 256 + 2 * j + 1] = M[j].b;
+        }
+        M = square(M);
+    }
+}
+
 struct ppi_ge {
     ppi_engine* E = nullptr;
     bool configured = false, initialized = false;
@@ -83,6 +119,13 @@ struct ppi_ge {
     int32_t n_immigrants = 0;
     GePlan* d_plan = nullptr;
     int32_t* d_flips = nullptr;
+    // parallel random streams: jump tables, draws, block states, iteration chains, consumed counts
+    uint64_t* d_jump = nullptr;
+    double* d_draw = nullptr;
+    uint64_t* d_bstate = nullptr;
+    int32_t* d_nxt = nullptr;
+    int64_t* d_end = nullptr;
+    GeDraws breed_draws{}, gen_draws{};
     // decode
     int32_t *d_ph = nullptr, *d_size = nullptr, *d_frames = nullptr;
     float* d_eph = nullptr;
@@ -107,7 +150,8 @@ struct ppi_ge {
 
     void release() {
         void* ptrs[] = {d_first, d_nrules, d_len, d_sym, d_symop, d_symval, d_rng, d_gen[0], d_gen[1], d_fit[0],
-                        d_fit[1], d_best_genome, d_best_fit, d_best_idx, d_immigrants, d_plan, d_flips, d_ph,
+                        d_fit[1], d_best_genome, d_best_fit, d_best_idx, d_immigrants, d_plan, d_flips, d_jump,
+                        d_draw, d_bstate, d_nxt, d_end, d_ph,
                         d_size, d_frames, d_eph, X.node, X.stk, X.fval, X.stage, d_comp, d_tot, d_offs, d_meta,
                         d_code, d_partial, d_raw, d_vec, d_rowerr, d_osum, d_bad_lo, d_bad_hi, d_ostop, d_gval,
                         d_gidx};
@@ -124,6 +168,11 @@ struct ppi_ge {
         d_immigrants = nullptr;
         d_plan = nullptr;
         d_flips = nullptr;
+        d_jump = nullptr;
+        d_draw = nullptr;
+        d_bstate = nullptr;
+        d_nxt = nullptr;
+        d_end = nullptr;
         d_ph = d_size = d_frames = nullptr;
         d_eph = nullptr;
         X = GeCompileScratch{};
@@ -360,6 +409,7 @@ ppi_status ppi_ge_configure(ppi_ge* g, const ppi_ge_params* prm, const ppi_ge_gr
         (s = dalloc(&g->d_best_fit, 1)) || (s = dalloc(&g->d_best_idx, kRankMax)) ||
         (s = dalloc(&g->d_immigrants, (size_t)P * S.W)) || (s = dalloc(&g->d_plan, (size_t)n_it)) ||
         (s = dalloc(&g->d_flips, (size_t)n_it * 2 * std::max(1, S.max_bits))) ||
+        (s = dalloc(&g->d_jump, (size_t)kJumpLevels * 256)) || (s = dalloc(&g->d_end, (size_t)S.T)) ||
         (s = dalloc(&g->d_ph, (size_t)P * S.max_size)) || (s = dalloc(&g->d_eph, (size_t)P * S.max_size)) ||
         (s = dalloc(&g->d_size, (size_t)P)) || (s = dalloc(&g->d_frames, (size_t)P * g->fstride)) ||
         (s = dalloc(&g->X.node, (size_t)P * NN * 4)) || (s = dalloc(&g->X.stk, (size_t)P * NN * 4)) ||
@@ -372,6 +422,31 @@ ppi_status ppi_ge_configure(ppi_ge* g, const ppi_ge_params* prm, const ppi_ge_gr
         (s = dalloc(&g->d_ostop, (size_t)P)) || (s = dalloc(&g->d_gval, (size_t)P)) ||
         (s = dalloc(&g->d_gidx, (size_t)P)))
         return s;
+    // Random streams: a breeding step's draws per stream (its share of the iterations times the most
+    // one iteration draws), the chain tables, and the block states of either phase.
+    {
+        const int64_t m_max = ((int64_t)n_it + S.T - 1) / S.T;
+        const int64_t cap = ((m_max * ge_draws_per_iteration(S) + kDrawBlock - 1) / kDrawBlock) * kDrawBlock;
+        int levels = 1;
+        while (((int64_t)1 << levels) < m_max) ++levels;
+        const int64_t gen_draws = (((int64_t)P + S.T - 1) / S.T) * S.nb;
+        const int64_t gen_blk = (gen_draws + kDrawBlock - 1) / kDrawBlock;
+        if (gen_blk >= ((int64_t)1 << kJumpLevels) || cap >= ((int64_t)1 << 31)) {
+            set_error("population x number_of_bits too large for the random-stream tables");
+            return PPI_E_ARG;
+        }
+        const int64_t blk_max = std::max<int64_t>(cap / kDrawBlock, gen_blk);
+        if ((s = dalloc(&g->d_draw, (size_t)S.T * cap)) || (s = dalloc(&g->d_bstate, (size_t)S.T * (blk_max + 1) * 2)) ||
+            (s = dalloc(&g->d_nxt, (size_t)levels * S.T * (cap + 1))))
+            return s;
+        std::vector<uint64_t> jump((size_t)kJumpLevels * 256);
+        ge_jump_tables(jump.data());
+        GE_HIP(hipMemcpy(g->d_jump, jump.data(), jump.size() * 8, hipMemcpyHostToDevice));
+        g->breed_draws = GeDraws{g->d_jump, g->d_rng, g->d_draw, g->d_bstate, cap, (int32_t)(cap / kDrawBlock),
+                                 g->d_nxt, levels, g->d_end};
+        g->gen_draws = GeDraws{g->d_jump, g->d_rng, nullptr, g->d_bstate, gen_blk * kDrawBlock, (int32_t)gen_blk,
+                               nullptr, 0, g->d_end};
+    }
     g->X.node_stride = NN * 4;
     g->X.stk_stride = NN * 4;
     g->X.fval_stride = NN;
@@ -403,7 +478,8 @@ ppi_status ppi_ge_initialize(ppi_ge* g) {
     GE_HIP(hipMemsetAsync(g->d_best_genome, 0, (size_t)g->S.W * 4, E->stream));
     g->cur = 0;
     GE_HIP(hipEventRecord(g->ev[0], E->stream));
-    GE_HIP(ge_launch_generate(g->S, g->d_rng, g->d_gen[0], E->stream));
+    GE_HIP(hipMemsetAsync(g->d_gen[0], 0, (size_t)g->S.P * g->S.W * 4, E->stream));
+    GE_HIP(ge_launch_generate(g->S, g->gen_draws, g->d_gen[0], E->stream));
     ppi_status s = evaluate_population(g, 0);
     if (s) return s;
     GE_HIP(hipStreamSynchronize(E->stream));
@@ -436,7 +512,7 @@ ppi_status ppi_ge_evolve(ppi_ge* g, int32_t generations) {
             ++nimm;
         }
         if (nimm < S.P) {
-            GE_HIP(ge_launch_plan(S, g->d_rng, g->d_fit[g->cur], nimm, g->d_plan, g->d_flips, st));
+            GE_HIP(ge_launch_plan(S, g->breed_draws, g->d_fit[g->cur], nimm, g->d_plan, g->d_flips, st));
             GE_HIP(ge_launch_offspring(S, g->d_plan, g->d_flips, g->d_gen[g->cur], g->d_gen[nxt], nimm, st));
         }
         g->n_immigrants = 0;

@@ -78,9 +78,43 @@ struct GeCompileTotals {
     int32_t pad_;
 };
 
-hipError_t ge_launch_generate(const GeShape& S, uint64_t* rng, uint32_t* genomes, hipStream_t st);
-hipError_t ge_launch_plan(const GeShape& S, uint64_t* rng, const float* fitness, int32_t nimm, GePlan* plan,
+// The random streams in parallel.  A stream's state advances by a linear map T over GF(2)^128
+// (XorShift128+'s shifts and XORs; the output addition is applied to a state, not part of it), so the
+// state kDrawBlock * k draws ahead is T^(kDrawBlock * k) applied to it: `jump` holds the columns of
+// T^(kDrawBlock * 2^j), j < kJumpLevels, and one thread per block of draws reaches its block by at
+// most kJumpLevels matrix-vector products, then draws its block in order.  A breeding step draws a
+// data-dependent number of values per iteration (the mutation draws as many positions as it drew
+// bits); the iteration that starts at draw p ends at p + L(p), L computed for every p in parallel,
+// and the start of each stream's l-th iteration is found by pointer doubling over p -> p + L(p).
+constexpr int kDrawBlock = 256;
+constexpr int kJumpLevels = 24;
+
+struct GeDraws {
+    const uint64_t* jump;   // [kJumpLevels][128][2]
+    uint64_t* rng;          // [T][2] stream states, advanced past the draws a phase consumed
+    double* draw;           // [T][cap] Real() of each stream's draws, in order (breeding)
+    uint64_t* bstate;       // [T][nblk + 1][2] state at the start of each block
+    int64_t cap;            // draws per stream, a multiple of kDrawBlock
+    int32_t nblk;           // cap / kDrawBlock
+    int32_t* nxt;           // [levels][T][cap + 1] start of the iteration 2^j iterations after one at p
+    int32_t levels;
+    int64_t* end;           // [T] draws the phase consumed
+};
+
+// Host: columns of T^(kDrawBlock * 2^j), j < kJumpLevels (kJumpLevels * 256 words).
+void ge_jump_tables(uint64_t* out);
+// Generation 0 (ppi_generate_population): each stream draws its individuals' alleles in order.
+hipError_t ge_launch_generate(const GeShape& S, const GeDraws& D, uint32_t* genomes, hipStream_t st);
+// One breeding step's plan (every draw of ppi_evolve's loop body, tournaments resolved), offspring
+// built separately by ge_launch_offspring.
+hipError_t ge_launch_plan(const GeShape& S, const GeDraws& D, const float* fitness, int32_t nimm, GePlan* plan,
                           int32_t* flips, hipStream_t st);
+// Draws per stream a breeding step may consume at most, per iteration.
+inline int64_t ge_draws_per_iteration(const GeShape& S) {
+    // tournaments, crossover test + Probability(0.66) + two cuts, per child: bit count + Probability(0.75)
+    // + the positions, or Probability(0.10) + a new count + the start of the shrink
+    return 2 * (int64_t)S.ts + 4 + 2 * (2 + (int64_t)(S.max_bits > 3 ? S.max_bits : 3));
+}
 hipError_t ge_launch_offspring(const GeShape& S, const GePlan* plan, const int32_t* flips, const uint32_t* parents,
                                uint32_t* children, int32_t nimm, hipStream_t st);
 hipError_t ge_launch_decode(const GeShape& S, const GeDevGrammar& G, const uint32_t* genomes, int32_t* ph,

@@ -29,12 +29,6 @@ __device__ __forceinline__ double rng_real(uint64_t& s0, uint64_t& s1) {
     return d * 0x1p-64;
 }
 
-// (int)(random_number() * n), a draw that rounds to n (Int() within 2^10 of 2^64) clamped to n - 1.
-__device__ __forceinline__ int draw_index(uint64_t& s0, uint64_t& s1, int n) {
-    const int k = (int)(rng_real(s0, s1) * n);
-    return k < n ? k : n - 1;
-}
-
 // OpenMP's static schedule: thread t of T gets iterations [lo, hi) of n.
 __device__ __forceinline__ void static_range(int n, int T, int t, int& lo, int& hi) {
     int q = n / T, r = n % T;
@@ -46,104 +40,260 @@ __device__ __forceinline__ void static_range(int n, int T, int t, int& lo, int& 
     hi = lo + q;
 }
 
+// State of stream t advanced by k blocks of draws: the product of the jump tables of k's set bits.
+__device__ void jump_blocks(const uint64_t* jump, uint64_t& s0, uint64_t& s1, int64_t k) {
+    for (int j = 0; j < kJumpLevels && k; ++j, k >>= 1) {
+        if (!(k & 1)) continue;
+        const uint64_t* col = jump + (size_t)j * 256;
+        uint64_t r0 = 0, r1 = 0;
+        for (int b = 0; b < 64; ++b) {
+            const uint64_t m = 0 - ((s0 >> b) & 1ull);
+            r0 ^= col[2 * b] & m;
+            r1 ^= col[2 * b + 1] & m;
+        }
+        for (int b = 0; b < 64; ++b) {
+            const uint64_t m = 0 - ((s1 >> b) & 1ull);
+            r0 ^= col[128 + 2 * b] & m;
+            r1 ^= col[128 + 2 * b + 1] & m;
+        }
+        s0 = r0;
+        s1 = r1;
+    }
+}
+
+// OpenMP's static schedule, inverted: the thread and local index of iteration `it` of n over T threads.
+__device__ __forceinline__ void static_owner(int n, int T, int it, int& t, int& l) {
+    const int q = n / T, r = n % T;
+    if (it < r * (q + 1)) {
+        t = it / (q + 1);
+        l = it % (q + 1);
+    } else {
+        t = r + (it - r * (q + 1)) / q;
+        l = (it - r * (q + 1)) % q;
+    }
+}
+
 // ppi_generate_population (ppi.cc:656-672): allele j of individual i is random_number() < 0.5, each
-// stream filling its own individuals allele by allele.
-__global__ void ge_generate_kernel(GeShape S, uint64_t* rng, uint32_t* genomes) {
-    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+// stream drawing its individuals' alleles in order.  One thread per (stream, block of draws); draw d of
+// stream t is allele d % nb of its (d / nb)-th individual.  The genomes must be zero.
+__global__ void ge_generate_kernel(GeShape S, GeDraws D, uint32_t* genomes) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(gid / (D.nblk + 1)), k = (int)(gid % (D.nblk + 1));
     if (t >= S.T) return;
-    uint64_t s0 = rng[2 * t], s1 = rng[2 * t + 1];
     int lo, hi;
     static_range(S.P, S.T, t, lo, hi);
-    for (int i = lo; i < hi; ++i)
-        for (int w = 0; w < S.W; ++w) {
-            uint32_t word = 0;
-            const int nbits = min(32, S.nb - 32 * w);
-            for (int b = 0; b < nbits; ++b)
-                if (rng_real(s0, s1) < 0.5) word |= 1u << b;
-            genomes[(size_t)i * S.W + w] = word;
+    const int64_t n = (int64_t)(hi - lo) * S.nb;
+    uint64_t s0 = D.rng[2 * t], s1 = D.rng[2 * t + 1];
+    jump_blocks(D.jump, s0, s1, k);
+    uint64_t* bs = D.bstate + ((size_t)t * (D.nblk + 1) + k) * 2;
+    bs[0] = s0;
+    bs[1] = s1;
+    if (k == 0) D.end[t] = n;
+    if (k == D.nblk) return;  // the state after the last block only
+    const int64_t d0 = (int64_t)k * kDrawBlock;
+    uint32_t word = 0;
+    int64_t wi = -1;
+    for (int i = 0; i < kDrawBlock && d0 + i < n; ++i) {
+        const int64_t d = d0 + i;
+        const double r = rng_real(s0, s1);
+        const int64_t ind = lo + d / S.nb;
+        const int a = (int)(d % S.nb);
+        const int64_t w = ind * S.W + (a >> 5);
+        if (w != wi) {
+            if (wi >= 0 && word) atomicOr(genomes + wi, word);
+            wi = w;
+            word = 0;
         }
-    rng[2 * t] = s0;
-    rng[2 * t + 1] = s1;
+        if (r < 0.5) word |= 1u << (a & 31);
+    }
+    if (wi >= 0 && word) atomicOr(genomes + wi, word);
 }
 
-// ppi_tournament (ppi.cc:422-452).
-__device__ int tournament(const GeShape& S, uint64_t& s0, uint64_t& s1, const float* fit) {
-    int w = draw_index(s0, s1, S.P);
-    float fw = fit[w];
-    for (int t = 1; t < S.ts; ++t) {
-        const int k = draw_index(s0, s1, S.P);
-        const float fk = fit[k];
-        if (fk < fw) {
-            fw = fk;
-            w = k;
+// Draws of every stream for a breeding step: one thread per (stream, block).
+__global__ void ge_draws_kernel(GeDraws D, int T) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(gid / (D.nblk + 1)), k = (int)(gid % (D.nblk + 1));
+    if (t >= T) return;
+    uint64_t s0 = D.rng[2 * t], s1 = D.rng[2 * t + 1];
+    jump_blocks(D.jump, s0, s1, k);
+    uint64_t* bs = D.bstate + ((size_t)t * (D.nblk + 1) + k) * 2;
+    bs[0] = s0;
+    bs[1] = s1;
+    if (k == 0) D.end[t] = 0;  // a stream without iterations consumes nothing
+    if (k == D.nblk) return;
+    double* out = D.draw + (size_t)t * D.cap + (size_t)k * kDrawBlock;
+    for (int i = 0; i < kDrawBlock; ++i) out[i] = rng_real(s0, s1);
+}
+
+// The draws of one loop iteration of ppi_evolve (ppi.cc:896-936), in the reference's order: two
+// tournaments (ppi.cc:422-452), the crossover test, the crossover's own draws (ppi.cc:686-724), then
+// each child's mutation (ppi.cc:738-808).  Reads draws from v (at most `cap` of them); with `q`
+// non-null it fills the plan, reading fitness for the tournaments.  Returns the draws consumed, or -1
+// when the iteration would run past the buffer (a start that is not on a stream's chain).
+__device__ int64_t iteration(const GeShape& S, const double* v, int64_t p, int64_t cap, int nchild, const float* fit,
+                             GePlan* q, int32_t* flips) {
+    int64_t k = p;
+    auto next = [&](double& x) -> bool {
+        if (k >= cap) return false;
+        x = v[k++];
+        return true;
+    };
+    // (int)(random_number() * n), a draw that rounds to n (Int() within 2^10 of 2^64) clamped to n - 1
+    auto index = [&](double x, int n) {
+        const int i = (int)(x * n);
+        return i < n ? i : n - 1;
+    };
+    double x;
+    for (int parent = 0; parent < 2; ++parent) {
+        if (!next(x)) return -1;
+        int w = index(x, S.P);
+        float fw = q ? fit[w] : 0.0f;
+        for (int t = 1; t < S.ts; ++t) {
+            if (!next(x)) return -1;
+            const int c = index(x, S.P);
+            if (q) {
+                const float fc = fit[c];
+                if (fc < fw) {
+                    fw = fc;
+                    w = c;
+                }
+            }
+        }
+        if (q) {
+            if (parent == 0) q->father = w;
+            else q->mother = w;
         }
     }
-    return w;
+    int xo = 0, a = 0, b = S.nb;
+    if (!next(x)) return -1;
+    if (x < (double)S.cp) {
+        if (!next(x)) return -1;
+        if (x < 0.66) {
+            double y, z;
+            if (!next(y) || !next(z)) return -1;
+            a = ((int)(y * S.nb)) / S.bg * S.bg;
+            b = ((int)(z * S.nb)) / S.bg * S.bg;
+            if (a > b) {
+                const int tmp = a;
+                a = b;
+                b = tmp;
+            }
+            xo = 2;
+        } else {
+            if (!next(x)) return -1;
+            a = (int)(x * S.nb);
+            xo = 1;
+        }
+    }
+    if (q) {
+        q->xo = xo;
+        q->a = a;
+        q->b = b;
+    }
+    for (int c = 0; c < 2; ++c) {
+        if (q) {
+            q->mut[c] = 0;
+            q->nflip[c] = 0;
+            q->s0[c] = q->s1[c] = 0;
+        }
+        if (c >= nchild) continue;
+        if (!next(x)) return -1;
+        int num = (int)(x * (S.max_bits + 1));
+        if (num > S.max_bits) num = S.max_bits;  // a draw that rounds to 1.0
+        if (num == 0) continue;
+        if (!next(x)) return -1;
+        if (x < 0.75) {
+            if (k + num > cap) return -1;
+            if (q) {
+                q->mut[c] = 1;
+                q->nflip[c] = num;
+                for (int j = 0; j < num; ++j) flips[(size_t)c * S.max_bits + j] = index(v[k + j], S.nb);
+            }
+            k += num;
+        } else {
+            if (!next(x)) return -1;
+            if (x < 0.10) {
+                if (!next(x)) return -1;
+                num = (int)(x * S.nb);
+            }
+            if (!next(x)) return -1;
+            const int shrink = ((num + (S.bg - 1)) / S.bg) * S.bg;
+            const int start = ((int)(x * S.nb)) / S.bg * S.bg;
+            if (q) {
+                q->mut[c] = 2;
+                q->s0[c] = start;
+                q->s1[c] = min(start + shrink, S.nb);
+            }
+        }
+    }
+    return k - p;
 }
 
-// The draws of one loop iteration of ppi_evolve (ppi.cc:896-936) in the reference's order:
-// tournaments, the crossover test, the crossover's own draws (ppi.cc:686-724), then the mutation of
-// each child (ppi.cc:738-808).  Each stream walks the iterations its static share gives it.
-__global__ void ge_plan_kernel(GeShape S, uint64_t* rng, const float* fit, int32_t nimm, GePlan* plan,
-                               int32_t* flips) {
-    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+// nxt_0[t][p] = the start of the iteration after one that starts at draw p (two children), or cap.
+__global__ void ge_len_kernel(GeShape S, GeDraws D) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = D.cap + 1;
+    const int t = (int)(gid / stride);
+    const int64_t p = gid % stride;
     if (t >= S.T) return;
-    uint64_t s0 = rng[2 * t], s1 = rng[2 * t + 1];
+    int64_t n = D.cap;
+    if (p < D.cap) {
+        const int64_t len = iteration(S, D.draw + (size_t)t * D.cap, p, D.cap, 2, nullptr, nullptr, nullptr);
+        if (len >= 0) n = p + len;
+    }
+    D.nxt[(size_t)t * stride + p] = (int32_t)n;
+}
+
+// nxt_j = nxt_{j-1} applied twice.
+__global__ void ge_double_kernel(GeShape S, GeDraws D, int j) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = D.cap + 1;
+    if (gid >= (int64_t)S.T * stride) return;
+    const int t = (int)(gid / stride);
+    const int32_t* prev = D.nxt + ((size_t)(j - 1) * S.T + t) * stride;
+    int32_t* cur = D.nxt + ((size_t)j * S.T + t) * stride;
+    cur[gid % stride] = prev[prev[gid % stride]];
+}
+
+// One thread per iteration of the breeding loop: its stream's chain gives its first draw, then the plan.
+__global__ void ge_plan_kernel(GeShape S, GeDraws D, const float* fit, int32_t nimm, GePlan* plan, int32_t* flips) {
     const int n_it = (S.P - nimm + 1) / 2;
+    const int it = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (it >= n_it) return;
+    int t, l;
+    static_owner(n_it, S.T, it, t, l);
+    const int64_t stride = D.cap + 1;
+    int64_t p = 0;
+    for (int j = 0, ll = l; ll; ++j, ll >>= 1)
+        if (ll & 1) p = D.nxt[((size_t)j * S.T + t) * stride + p];
+    const int i = nimm + 2 * it;
+    GePlan q;
+    q.father = q.mother = 0;  // (every iteration on a stream's chain fits its buffer and fills the plan)
+    q.xo = 0;
+    q.a = 0;
+    q.b = S.nb;
+    q.mut[0] = q.mut[1] = 0;
+    q.nflip[0] = q.nflip[1] = 0;
+    q.s0[0] = q.s0[1] = q.s1[0] = q.s1[1] = 0;
+    const int64_t len = iteration(S, D.draw + (size_t)t * D.cap, p, D.cap, i < S.P - 1 ? 2 : 1, fit, &q,
+                                  flips + (size_t)it * 2 * S.max_bits);
+    plan[it] = q;
     int lo, hi;
     static_range(n_it, S.T, t, lo, hi);
-    for (int it = lo; it < hi; ++it) {
-        const int i = nimm + 2 * it;
-        GePlan q;
-        q.father = tournament(S, s0, s1, fit);
-        q.mother = tournament(S, s0, s1, fit);
-        q.xo = 0;
-        q.a = 0;
-        q.b = S.nb;
-        if (rng_real(s0, s1) < (double)S.cp) {
-            if (rng_real(s0, s1) < 0.66) {
-                int a = ((int)(rng_real(s0, s1) * S.nb)) / S.bg * S.bg;
-                int b = ((int)(rng_real(s0, s1) * S.nb)) / S.bg * S.bg;
-                if (a > b) {
-                    const int x = a;
-                    a = b;
-                    b = x;
-                }
-                q.xo = 2;
-                q.a = a;
-                q.b = b;
-            } else {
-                q.xo = 1;
-                q.a = (int)(rng_real(s0, s1) * S.nb);
-            }
-        }
-        const int nchild = i < S.P - 1 ? 2 : 1;
-        for (int c = 0; c < 2; ++c) {
-            q.mut[c] = 0;
-            q.nflip[c] = 0;
-            q.s0[c] = q.s1[c] = 0;
-            if (c >= nchild) continue;
-            int num = (int)(rng_real(s0, s1) * (S.max_bits + 1));
-            if (num > S.max_bits) num = S.max_bits;  // a draw that rounds to 1.0
-            if (num == 0) continue;
-            if (rng_real(s0, s1) < 0.75) {
-                q.mut[c] = 1;
-                q.nflip[c] = num;
-                int32_t* f = flips + ((size_t)it * 2 + c) * S.max_bits;
-                for (int k = 0; k < num; ++k) f[k] = draw_index(s0, s1, S.nb);
-            } else {
-                if (rng_real(s0, s1) < 0.10) num = (int)(rng_real(s0, s1) * S.nb);
-                const int shrink = ((num + (S.bg - 1)) / S.bg) * S.bg;
-                const int start = ((int)(rng_real(s0, s1) * S.nb)) / S.bg * S.bg;
-                q.mut[c] = 2;
-                q.s0[c] = start;
-                q.s1[c] = min(start + shrink, S.nb);
-            }
-        }
-        plan[it] = q;
-    }
-    rng[2 * t] = s0;
-    rng[2 * t + 1] = s1;
+    if (it == hi - 1) D.end[t] = p + (len > 0 ? len : 0);
+}
+
+// Each stream's state after the draws its phase consumed: its block's start state, stepped.
+__global__ void ge_state_kernel(GeDraws D, int T) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= T) return;
+    const int64_t e = D.end[t];
+    const int64_t k = e / kDrawBlock;
+    const uint64_t* bs = D.bstate + ((size_t)t * (D.nblk + 1) + k) * 2;
+    uint64_t s0 = bs[0], s1 = bs[1];
+    for (int64_t i = 0; i < e % kDrawBlock; ++i) (void)rng_next(s0, s1);
+    D.rng[2 * t] = s0;
+    D.rng[2 * t + 1] = s1;
 }
 
 __device__ __forceinline__ uint32_t gbit(const uint32_t* g, int k) { return (g[k >> 5] >> (k & 31)) & 1u; }
@@ -814,15 +964,33 @@ __global__ void __launch_bounds__(1024) pick_n_best_kernel(const float* val, con
 // launchers
 // ---------------------------------------------------------------------------
 
-hipError_t ge_launch_generate(const GeShape& S, uint64_t* rng, uint32_t* genomes, hipStream_t st) {
-    hipLaunchKernelGGL(dev::ge_generate_kernel, dim3((unsigned)((S.T + 63) / 64)), dim3(64), 0, st, S, rng, genomes);
+hipError_t ge_launch_generate(const GeShape& S, const GeDraws& D, uint32_t* genomes, hipStream_t st) {
+    const int64_t n = (int64_t)S.T * (D.nblk + 1);
+    hipLaunchKernelGGL(dev::ge_generate_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, S, D, genomes);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dev::ge_state_kernel, dim3((unsigned)((S.T + 63) / 64)), dim3(64), 0, st, D, S.T);
     return hipGetLastError();
 }
 
-hipError_t ge_launch_plan(const GeShape& S, uint64_t* rng, const float* fitness, int32_t nimm, GePlan* plan,
+hipError_t ge_launch_plan(const GeShape& S, const GeDraws& D, const float* fitness, int32_t nimm, GePlan* plan,
                           int32_t* flips, hipStream_t st) {
-    hipLaunchKernelGGL(dev::ge_plan_kernel, dim3((unsigned)((S.T + 63) / 64)), dim3(64), 0, st, S, rng, fitness, nimm,
-                       plan, flips);
+    const int n_it = (S.P - nimm + 1) / 2;
+    const int64_t nb = (int64_t)S.T * (D.nblk + 1);
+    hipLaunchKernelGGL(dev::ge_draws_kernel, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, st, D, S.T);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t np = (int64_t)S.T * (D.cap + 1);
+    hipLaunchKernelGGL(dev::ge_len_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, S, D);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (int j = 1; j < D.levels; ++j) {
+        hipLaunchKernelGGL(dev::ge_double_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, S, D, j);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(dev::ge_plan_kernel, dim3((unsigned)((n_it + 255) / 256)), dim3(256), 0, st, S, D, fitness,
+                       nimm, plan, flips);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(dev::ge_state_kernel, dim3((unsigned)((S.T + 63) / 64)), dim3(64), 0, st, D, S.T);
     return hipGetLastError();
 }
 
