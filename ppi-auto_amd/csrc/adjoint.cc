@@ -108,7 +108,7 @@ struct ppi_adjoint {
     DBuf<AdjJob> dJobs;
     DBuf<AdjSeg> dSegs;
     DBuf<RevIns> dRev;
-    DBuf<uint32_t> dRec;
+    DBuf<AdjStore> dStores;
     DBuf<Ins> dCode;
     DBuf<int32_t> dOrder, dSample;
     DBuf<ProgMeta> dMeta;
@@ -140,7 +140,7 @@ AdjArgs make_args(ppi_adjoint* a) {
     A.jobs = a->dJobs.p;
     A.segs = a->dSegs.p;
     A.rev = a->dRev.p;
-    A.rec = a->dRec.p;
+    A.stores = a->dStores.p;
     A.code = a->dCode.p;
     A.order = a->dOrder.p;
     A.njobs = (int32_t)a->T.jobs.size();
@@ -201,7 +201,10 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
             return PPI_E_PROGRAM;
         }
     T.fwd.code.insert(T.fwd.code.end(), kTapePad, Ins{H_END, kEmpty});
-    T.rec.push_back(0u);  // look-ahead of the last job's forward pass
+    // Window padding: the kernels read reverse entries up to 127 past a segment's
+    // first and store entries up to 63 past its first.
+    T.rev.insert(T.rev.end(), 128, RevIns{0u, 0u});
+    T.stores.insert(T.stores.end(), 64, AdjStore{0u, 0u});
     a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
     a->F = (T.fwd.stats.max_col < 12 && a->S == 12) ? 12 : 0;
     a->R = 1;
@@ -233,7 +236,7 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
     ppi_status s;
     hipStream_t st = a->stream;
     if ((s = upload(a->dJobs, T.jobs, st)) || (s = upload(a->dSegs, T.segs, st)) || (s = upload(a->dRev, T.rev, st)) ||
-        (s = upload(a->dRec, T.rec, st)) || (s = upload(a->dCode, T.fwd.code, st)) ||
+        (s = upload(a->dStores, T.stores, st)) || (s = upload(a->dCode, T.fwd.code, st)) ||
         (s = upload(a->dOrder, a->order, st)) || (s = upload(a->dMeta, T.fwd.meta, st)) ||
         (s = upload(a->dConsts, T.consts, st)) || (s = upload(a->dProgConsts, T.prog_consts, st)))
         return s;
@@ -367,7 +370,7 @@ void ppi_adjoint_destroy(ppi_adjoint* a) {
     if (!a) return;
     (void)hipSetDevice(a->device);
     if (a->stream) (void)hipStreamSynchronize(a->stream);
-    a->dJobs.release(); a->dSegs.release(); a->dRev.release(); a->dRec.release(); a->dCode.release();
+    a->dJobs.release(); a->dSegs.release(); a->dRev.release(); a->dStores.release(); a->dCode.release();
     a->dOrder.release(); a->dSample.release(); a->dMeta.release(); a->dConsts.release(); a->dProgConsts.release();
     a->dGpart.release(); a->dFpart.release(); a->dFit.release(); a->dGraw.release(); a->dGscale.release();
     a->dGout.release(); a->dCur.release(); a->dCbest.release(); a->dGbest.release(); a->dFbest.release();

@@ -135,7 +135,6 @@ __device__ __forceinline__ AdjJob load_job(const AdjJob* jobs, int i) {
     AdjJob j;
     j.code_off = q->code_off;
     j.ncode = q->ncode;
-    j.rec_off = q->rec_off;
     j.seg_off = q->seg_off;
     j.nseg = q->nseg;
     j.prog = q->prog;
@@ -143,7 +142,7 @@ __device__ __forceinline__ AdjJob load_job(const AdjJob* jobs, int i) {
     j.nconst = q->nconst;
     j.flags = q->flags;
     j.cost = 0;
-    j.pad_[0] = j.pad_[1] = 0;
+    j.pad_[0] = j.pad_[1] = j.pad_[2] = 0;
     return j;
 }
 
@@ -239,23 +238,43 @@ struct AdjCore {
         }
     }
 
-    // Forward pass over instructions [0, n) storing the partials of segment seg.
-    // code[n] and rec[n] must be readable (look-ahead).
+    // Forward pass over the program's instructions [0, n), fetched through a
+    // 64-entry window across the wave's lanes as in the FP form (lane L holds
+    // tape entry blk + L, read back with v_readlane; the next block is loaded
+    // when the current one starts).  The segment's stores (at most B < 64, in
+    // increasing pc) are one window too: instruction pc = st_pc[ks] also writes
+    // its partials.  code is the whole tape, the program starts at c0; the tape
+    // is padded so that blk + 127 and the store window stay in bounds.
     template <class FeatT, class PSt>
-    static __device__ __forceinline__ void forward(ConstPtr<Ins> code, ConstPtr<uint32_t> rec, uint32_t n,
-                                                   uint32_t seg, const FeatT& ft, float (&acc)[R], Stk& stk,
-                                                   PSt& ps) {
-        uint32_t nc = code[0].code, na = code[0].arg, nr = rec[0];
-        for (uint32_t pc = 0; pc < n; ++pc) {
-            const uint32_t c = nc, a = na, rc = nr;
-            nc = code[pc + 1].code;
-            na = code[pc + 1].arg;
-            nr = rec[pc + 1];
-            const uint32_t ea = rc & 0xffffu, eb = rc >> 16;
-            const uint32_t sa = (ea != 0 && ((ea - 1) >> LOGB) == seg) ? ((ea - 1) & (B - 1)) : kNone;
-            const uint32_t sb = (eb != 0 && ((eb - 1) >> LOGB) == seg) ? ((eb - 1) & (B - 1)) : kNone;
-            if (sa == kNone && sb == kNone) C::step(c, a, ft, acc, stk, typename C::NoEnd{});
-            else step_rec(c, a, sa, sb, ft, acc, stk, ps);
+    static __device__ __forceinline__ void forward(const Ins* code, uint32_t c0, uint32_t n, uint32_t st_pc,
+                                                   uint32_t st_sl, uint32_t nst, const FeatT& ft, float (&acc)[R],
+                                                   Stk& stk, PSt& ps) {
+        const int lane = threadIdx.x & 63;
+        const uint32_t gend = c0 + n;
+        uint32_t blk = c0 & ~63u;
+        uint32_t wc = code[blk + lane].code, wa = code[blk + lane].arg;
+        uint32_t k = c0 - blk;
+        uint32_t ks = 0;
+        uint32_t next = nst > 0 ? c0 + (uint32_t)__builtin_amdgcn_readlane((int)st_pc, 0) : 0xffffffffu;
+        for (; blk < gend; blk += 64) {
+            const uint32_t nc = code[blk + 64 + lane].code, na = code[blk + 64 + lane].arg;
+            const uint32_t kend = min(64u, gend - blk);
+            for (; k < kend; ++k) {
+                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)wc, (int)k);
+                const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)wa, (int)k);
+                if (blk + k != next) {
+                    C::step(c, a, ft, acc, stk, typename C::NoEnd{});
+                    continue;
+                }
+                // slot + 1 per side, 0 = not stored: the subtraction gives kNone for 0.
+                const uint32_t sl = (uint32_t)__builtin_amdgcn_readlane((int)st_sl, (int)ks);
+                step_rec(c, a, (sl & 0xffffu) - 1u, (sl >> 16) - 1u, ft, acc, stk, ps);
+                ++ks;
+                next = ks < nst ? c0 + (uint32_t)__builtin_amdgcn_readlane((int)st_pc, (int)ks) : 0xffffffffu;
+            }
+            k = 0;
+            wc = nc;
+            wa = na;
         }
     }
 };
@@ -270,6 +289,7 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     typedef AdjCore<R, S, PS, P, N> AC;
     typedef typename AC::Stk Stk;
     constexpr int B = AC::B;
+    static_assert(B < 64, "a segment's stores must fit one window");
     typedef typename VecT<kAdjSlots>::type VA;
     typedef typename VecT<kAdjMaxConsts>::type VG;
     constexpr int LDS_WAVE = PS == 1 ? B * R * 64 : 1;
@@ -294,9 +314,7 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     float part = 0.0f;
     bool bad = false;
     if (!(J.flags & kEmpty)) {
-        ConstPtr<Ins> code = as_const(A.code) + J.code_off;
-        ConstPtr<uint32_t> rec = as_const(A.rec) + J.rec_off;
-        ConstPtr<RevIns> rv = as_const(A.rev);
+        const RevIns* rv = A.rev;
         for (int64_t row0 = rbeg; row0 < rend; row0 += 64 * R) {
             int row[R];
             float y[R];
@@ -325,9 +343,12 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
             for (uint32_t s = 0; s < J.nseg; ++s) {
                 ConstPtr<AdjSeg> sq = as_const(A.segs) + (J.seg_off + s);
                 const uint32_t rb = sq->rev_begin, re = sq->rev_end, fe = sq->fwd_end;
+                const uint32_t sb0 = sq->st_begin, nst = sq->st_end - sb0;
+                // The segment's stores across the lanes (the store list is padded by 64).
+                const uint32_t st_pc = A.stores[sb0 + lane].pc, st_sl = A.stores[sb0 + lane].slots;
                 float acc[R];
                 Stk stk;
-                AC::forward(code, rec, fe, s, ft, acc, stk, ps);
+                AC::forward(A.code, J.code_off, fe, st_pc, st_sl, nst, ft, acc, stk, ps);
                 if (s == 0) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
@@ -342,37 +363,53 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
                         adj[r][0] = seed;
                     }
                 }
-                for (uint32_t i = rb; i < re; ++i) {
-                    const uint32_t w = rv[i].w, pb = rv[i].p;
-                    const uint32_t src = (w >> 4) & 15u, dst = (w >> 8) & 31u;
-                    // A zero adjoint stays zero whatever the partial (a NaN or inf
-                    // partial inside a subtree that MAX/MIN/IF_THEN_ELSE did not
-                    // select does not reach the constants): forward-mode semantics.
-                    float v[R];
-                    if (w & kRevStored) {
-                        const uint32_t sl = (w >> 16) & (uint32_t)(B - 1);
+                // Reverse instructions [rb, re) through a 64-entry window like the
+                // forward tape (the reverse tape is padded by 128 entries).
+                uint32_t blk = rb & ~63u;
+                uint32_t ww = rv[blk + lane].w, wp = rv[blk + lane].p;
+                uint32_t k = rb - blk;
+                for (; blk < re; blk += 64) {
+                    const uint32_t nw = rv[blk + 64 + lane].w, np = rv[blk + 64 + lane].p;
+                    const uint32_t kend = min(64u, re - blk);
+                    for (; k < kend; ++k) {
+                        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)ww, (int)k);
+                        const uint32_t pb = (uint32_t)__builtin_amdgcn_readlane((int)wp, (int)k);
+                        const uint32_t src = (w >> 4) & 15u, dst = (w >> 8) & 31u;
+                        // A zero adjoint stays zero whatever the partial (a NaN or inf
+                        // partial inside a subtree that MAX/MIN/IF_THEN_ELSE did not
+                        // select does not reach the constants): forward-mode semantics.
+                        // The stored partial is read whatever the adjoint (in bounds),
+                        // so the zero test is a select, not a branch around the read.
+                        float v[R];
+                        if (w & kRevStored) {
+                            const uint32_t sl = (w >> 16) & (uint32_t)(B - 1);
 #pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            const float ad = adj[r][src];
-                            v[r] = ad == 0.0f ? 0.0f : ad * ps.get(sl, r);
+                            for (int r = 0; r < R; ++r) {
+                                const float p = ps.get(sl, r);
+                                const float ad = adj[r][src];
+                                v[r] = ad == 0.0f ? 0.0f : ad * p;
+                            }
+                        } else {
+                            const float pc = __uint_as_float(pb);
+#pragma unroll
+                            for (int r = 0; r < R; ++r) {
+                                const float ad = adj[r][src];
+                                v[r] = ad == 0.0f ? 0.0f : ad * pc;
+                            }
                         }
-                    } else {
-                        const float pc = __uint_as_float(pb);
+                        if ((w & 3u) == kRevAdj) {
 #pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            const float ad = adj[r][src];
-                            v[r] = ad == 0.0f ? 0.0f : ad * pc;
+                            for (int r = 0; r < R; ++r) adj[r][dst] = v[r];
+                        } else {
+                            float t = 0.0f;
+#pragma unroll
+                            for (int r = 0; r < R; ++r) t += valid[r] ? v[r] : 0.0f;
+                            g[dst] += t;
                         }
                     }
-                    if ((w & 3u) == kRevAdj) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r) adj[r][dst] = v[r];
-                    } else {
-                        float t = 0.0f;
-#pragma unroll
-                        for (int r = 0; r < R; ++r) t += valid[r] ? v[r] : 0.0f;
-                        g[dst] += t;
-                    }
+                    k = 0;
+                    ww = nw;
+                    wp = np;
                 }
             }
         }

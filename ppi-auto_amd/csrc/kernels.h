@@ -87,7 +87,7 @@ struct AdjArgs {
     const AdjJob* jobs;
     const AdjSeg* segs;
     const RevIns* rev;
-    const uint32_t* rec;
+    const AdjStore* stores;
     const Ins* code;
     const int32_t* order;  // unit u -> job order[u % njobs] (heaviest first), range u / njobs
     int32_t njobs;

@@ -277,8 +277,15 @@ struct Core {
     template <class FeatT>
     static __device__ __forceinline__ void run(ConstPtr<Ins> code, uint32_t n, const FeatT& ft, float (&acc)[R],
                                                Stk& stk) {
-        uint32_t nc = code[0].code, na = code[0].arg;
-        for (uint32_t pc = 0; pc < n; ++pc) {
+        run_range(code, 0, n, ft, acc, stk);
+    }
+
+    // Instructions [pc0, pc1) of a program; code[pc1] must be readable (look-ahead).
+    template <class FeatT>
+    static __device__ __forceinline__ void run_range(ConstPtr<Ins> code, uint32_t pc0, uint32_t pc1, const FeatT& ft,
+                                                     float (&acc)[R], Stk& stk) {
+        uint32_t nc = code[pc0].code, na = code[pc0].arg;
+        for (uint32_t pc = pc0; pc < pc1; ++pc) {
             const uint32_t c = nc, a = na;
             nc = code[pc + 1].code;  // code[code_off + ncode] is the program's H_END
             na = code[pc + 1].arg;

@@ -715,11 +715,10 @@ struct AdjBuilder {
         J.const_base = (uint32_t)out->consts.size();
         J.nconst = (uint32_t)ng;
         J.flags = (m.flags & (kEmpty | kInvalid)) | (first ? kAdjFitness : 0u);
-        J.rec_off = (uint32_t)out->rec.size();
-        out->rec.resize(out->rec.size() + m.ncode, 0u);
         const size_t rev0 = out->rev.size();
         std::vector<int32_t> seg_of;     // per reverse instruction
         std::vector<uint32_t> e_ins;     // per stored partial: instruction (relative)
+        std::vector<uint8_t> e_hi;       // per stored partial: 1 = other-operand side
         int32_t cur_seg = 0;
         int max_slot = 0;
         auto put_rev = [&](uint32_t kind, int src, int dst, int kindp, float v, int32_t e) {
@@ -766,7 +765,7 @@ struct AdjBuilder {
                         const uint32_t rel = (uint32_t)(c.ins_at[(size_t)v] - (int32_t)m.code_off);
                         e_ins.push_back(rel);
                         const bool hi = arity(n.op) == 3 ? (jj == 2) : (c.side[(size_t)ch] == 1);
-                        out->rec[J.rec_off + rel] |= (uint32_t)(e + 1) << (hi ? 16 : 0);
+                        e_hi.push_back(hi ? 1 : 0);
                     }
                     if (c.nodes[(size_t)ch].op < 0) {
                         put_rev(kRevGrad, slot, kidx[(size_t)ch], kind, sv, e);
@@ -793,13 +792,19 @@ struct AdjBuilder {
             sg.rev_begin = (uint32_t)r;
             while (r < out->rev.size() && seg_of[r - rev0] == s) ++r;
             sg.rev_end = (uint32_t)r;
-            if (s == 0) {
-                sg.fwd_end = m.ncode;
-            } else {
-                uint32_t hi = 0;
-                for (int32_t e = s * B; e < std::min(nst, (s + 1) * B); ++e) hi = std::max(hi, e_ins[(size_t)e] + 1);
-                sg.fwd_end = hi;
+            // The segment's stores, one per instruction, in instruction order.
+            std::vector<AdjStore> ss;
+            for (int32_t e = s * B; e < std::min(nst, (s + 1) * B); ++e)
+                ss.push_back(AdjStore{e_ins[(size_t)e], (uint32_t)(e - s * B + 1) << (e_hi[(size_t)e] ? 16 : 0)});
+            std::sort(ss.begin(), ss.end(), [](const AdjStore& x, const AdjStore& y) { return x.pc < y.pc; });
+            sg.st_begin = (uint32_t)out->stores.size();
+            for (const AdjStore& q : ss) {
+                if (out->stores.size() > sg.st_begin && out->stores.back().pc == q.pc) out->stores.back().slots |= q.slots;
+                else out->stores.push_back(q);
             }
+            sg.st_end = (uint32_t)out->stores.size();
+            if (s == 0) sg.fwd_end = m.ncode;
+            else sg.fwd_end = sg.st_end > sg.st_begin ? out->stores.back().pc + 1 : 0;
             out->segs.push_back(sg);
         }
         J.cost = m.ncode * (uint32_t)nseg + (uint32_t)(out->rev.size() - rev0);

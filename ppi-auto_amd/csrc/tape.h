@@ -251,12 +251,14 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
 // ROUND/CEIL/FLOOR/STEP/SIGN, the condition of IF_THEN_ELSE) are cut: those
 // constants have gradient 0 and are left out.
 //
-// The forward tape is the ordinary accumulator tape.  A parallel record word per
-// instruction names the partials that instruction must store while it executes
-// (derivative with respect to the accumulator operand, and with respect to the
-// other operand; for IF_THEN_ELSE the then- and else-selectors).  Partials that
-// are constants (ADD, SUB, MEAN, NEG, one side of MOD) are not stored.  Stored
-// partials are numbered e = 0, 1, ... in the order the reverse sweep reads them.
+// The forward tape is the ordinary accumulator tape.  Stored partials are
+// numbered e = 0, 1, ... in the order the reverse sweep reads them; each is the
+// derivative of one instruction with respect to its accumulator operand or its
+// other operand (for IF_THEN_ELSE the then- and else-selectors).  Partials that
+// are constants (ADD, SUB, MEAN, NEG, one side of MOD) are not stored.  Each
+// segment lists, in instruction order, the instructions that store one of its
+// partials and the slots they store to, so the forward pass runs plain steps
+// between them and checks nothing per instruction.
 //
 // The reverse sweep walks the active tree root-first.  Adjoints live in
 // register slots with stack discipline (a node's last child reuses its slot), and
@@ -276,7 +278,6 @@ enum : uint32_t { kAdjFitness = 1u << 4 };  // AdjJob::flags: this job also accu
 struct AdjJob {
     uint32_t code_off;    // forward tape of the program (as ProgMeta::code_off)
     uint32_t ncode;
-    uint32_t rec_off;     // record words of this job: rec[rec_off + pc], pc < ncode
     uint32_t seg_off;     // segments of this job: segs[seg_off + s], s < nseg
     uint32_t nseg;
     int32_t prog;
@@ -284,13 +285,22 @@ struct AdjJob {
     uint32_t nconst;
     uint32_t flags;       // kEmpty / kInvalid of the program, kAdjFitness
     uint32_t cost;        // ncode * nseg + reverse instructions (scheduling weight)
-    uint32_t pad_[2];
+    uint32_t pad_[3];
 };
 
 struct AdjSeg {
     uint32_t rev_begin, rev_end;  // reverse instructions [rev_begin, rev_end) (absolute)
     uint32_t fwd_end;             // forward instructions [0, fwd_end) needed by the segment
-    uint32_t pad_;
+    uint32_t st_begin, st_end;    // stores [st_begin, st_end) (absolute), increasing pc
+    uint32_t pad_[3];
+};
+
+// A forward instruction that stores partials of its segment: pc relative to the
+// program; slots = (slot of the accumulator-side partial + 1) | (slot of the
+// other-side partial + 1) << 16, 0 = that side is not stored in the segment.
+struct AdjStore {
+    uint32_t pc;
+    uint32_t slots;
 };
 
 // Reverse instruction.  w: bits 0-1 kind (kRevAdj: adj[dst] = adj[src] * p;
@@ -313,7 +323,7 @@ struct AdjConst {
 struct AdjTape {
     Tape fwd;                          // forward tape (identical to compile_population's), with const_pos
     std::vector<AdjJob> jobs;          // program order; a program's jobs are consecutive
-    std::vector<uint32_t> rec;
+    std::vector<AdjStore> stores;
     std::vector<AdjSeg> segs;
     std::vector<RevIns> rev;
     std::vector<AdjConst> consts;      // gradient index -> constant

@@ -160,12 +160,15 @@ extern "C" int adj_eval(const ppi_symbol* table, int ntable, const int32_t* ph, 
                 const AdjSeg& sg = T.segs[J.seg_off + s];
                 float acc = NAN, reg[kMaxSlots];
                 for (int k = 0; k < kMaxSlots; ++k) reg[k] = NAN;
+                uint32_t next = sg.st_begin;
                 for (uint32_t pc = 0; pc < sg.fwd_end; ++pc) {
                     const Ins in = T.fwd.code[J.code_off + pc];
-                    const uint32_t rc = T.rec[J.rec_off + pc];
-                    const uint32_t ea = rc & 0xffffu, eb = rc >> 16;
-                    const int sa = (ea && (ea - 1) / (uint32_t)B == s) ? (int)((ea - 1) % (uint32_t)B) : -1;
-                    const int sb = (eb && (eb - 1) / (uint32_t)B == s) ? (int)((eb - 1) % (uint32_t)B) : -1;
+                    int sa = -1, sb = -1;
+                    if (next < sg.st_end && T.stores[next].pc == pc) {
+                        const uint32_t sl = T.stores[next++].slots;
+                        sa = (int)(sl & 0xffffu) - 1;
+                        sb = (int)(sl >> 16) - 1;
+                    }
                     const uint32_t h = in.code & 0xffu, s1 = (in.code >> 8) & 31u, s2 = (in.code >> 13) & 31u;
                     float cv;
                     std::memcpy(&cv, &in.arg, 4);
@@ -199,6 +202,7 @@ extern "C" int adj_eval(const ppi_symbol* table, int ntable, const int32_t* ph, 
                         acc = (c != 0.0f) ? th : el;
                     }
                 }
+                if (next != sg.st_end) return -2;  // a store outside [0, fwd_end) or out of order
                 if (s == 0) {
                     const float d = acc - y;
                     const float e = ef == PPI_EF_SQUARED ? d * d : std::fabs(d);
