@@ -168,7 +168,8 @@ def bench_adjoint_pass(steps, warmup):
     st = a.stats()
     a.close()
     return {"ms_per_backward": round(ms, 3), "grads_per_s": round(w.pop.n / (ms * 1e-3), 1),
-            "partials": st.partials, "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins}
+            "partials": st.partials, "rows_per_lane": st.rows_per_lane,
+            "tune_ms": [round(x, 3) for x in st.tune_ms], "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins}
 
 
 def bench_adjoint(steps, warmup, threads):
@@ -209,6 +210,7 @@ def bench_adjoint(steps, warmup, threads):
     return {"fp64_error": err, "ms": ms, "kernel_ms": st.kernel_ms, "n": w.pop.n, "node_evals": st.node_evals,
             "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins, "partials": st.partials,
             "tune_ms": [round(x, 3) for x in st.tune_ms], "consts": st.n_consts, "max_segments": st.max_segments,
+            "rows_per_lane": st.rows_per_lane,
             "descend_ms": dst.descend_ms, "descend_rows": sub, "improved": float((rel > 0).mean()),
             "median_rel_gain": float(np.median(rel)), "worse": int((f1[finite] > f0[finite]).sum()),
             "reverted": dst.descend_reverted}
@@ -566,7 +568,8 @@ def main():
             "fwd_gpops": round(adjoint["node_evals"] / (ms * 1e-3) / 1e9, 3),
             "fwd_plus_rev_ins_per_s": round((adjoint["fwd_ins"] + adjoint["rev_ins"]) / (ms * 1e-3) / 1e9, 3),
             "partials": {1: "recompute (registers)", 2: "store (LDS)"}.get(adjoint["partials"], "?"),
-            "autotune_ms_recompute_store": adjoint["tune_ms"],
+            "rows_per_lane": adjoint["rows_per_lane"],
+            "autotune_ms_r1_recompute_r1_store_r2_recompute_r2_store": adjoint["tune_ms"],
             "constants": adjoint["consts"], "max_segments": adjoint["max_segments"],
             "descend": {"steps": 20, "rows": adjoint["descend_rows"], "ms": round(adjoint["descend_ms"], 2),
                         "programs_improved": round(adjoint["improved"], 4),

@@ -272,8 +272,9 @@ typedef struct ppi_adjoint ppi_adjoint;
 
 typedef enum ppi_partials {
     PPI_PARTIALS_AUTO = 0,       /* pick by measurement */
-    PPI_PARTIALS_RECOMPUTE = 1,  /* 16 partials per row in registers; longer programs re-run the forward pass */
-    PPI_PARTIALS_STORE = 2       /* 32 partials per row in LDS; most programs need one forward pass */
+    PPI_PARTIALS_RECOMPUTE = 1,  /* 16 / R partials per row in registers (R rows per lane); longer programs
+                                    re-run the forward pass */
+    PPI_PARTIALS_STORE = 2       /* 32 / R partials per row in LDS; most programs need one forward pass */
 } ppi_partials;
 
 typedef struct ppi_adjoint_stats {
@@ -292,7 +293,8 @@ typedef struct ppi_adjoint_stats {
     int32_t ranges;          /* row ranges per job */
     int32_t descend_steps;
     int32_t descend_reverted; /* programs whose refined constants did not lower the plain fitness (kept) */
-    double tune_ms[2];       /* AUTO: sampled ms of RECOMPUTE and STORE */
+    double tune_ms[4];       /* sampled ms per candidate (R - 1) * 2 + (STORE ? 1 : 0), R rows per lane;
+                                0 = not sampled, -2 = no kernel shape holds the programs at that R */
 } ppi_adjoint_stats;
 
 ppi_status ppi_adjoint_create(ppi_adjoint** out, int32_t device);
@@ -313,6 +315,10 @@ ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_
 /* `iters` backward passes back to back (no host transfers); mean device ms per pass. */
 ppi_status ppi_adjoint_time(ppi_adjoint* a, int32_t iters, int32_t partials, double* ms_per_pass);
 ppi_status ppi_adjoint_get_stats(const ppi_adjoint* a, ppi_adjoint_stats* out);
+/* Rows per lane of the adjoint kernel: 0 = chosen by measurement with the partial store (default),
+   1, or 2 (programs of at most 5 stack and 5 adjoint slots; otherwise backward returns
+   PPI_E_UNSUPPORTED).  Kept across forward calls. */
+ppi_status ppi_adjoint_set_rows_per_lane(ppi_adjoint* a, int32_t rows_per_lane);
 
 /*
  * Evaluation with the reference's epilogue on the device: upload, launch, then alpha*size and the

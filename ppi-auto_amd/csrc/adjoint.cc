@@ -95,11 +95,14 @@ struct ppi_adjoint {
     float alpha = 0.0f;
     bool have_programs = false;
 
-    // compiled adjoint tape for partial store `tape_ps` (-1: none); store picked by AUTO (-1: not yet)
-    int tape_ps = -1;
-    int tuned_ps = -1;
+    // compiled adjoint tape for partial store `tape_ps` at `tape_r` rows per lane (-1: none)
+    int tape_ps = -1, tape_r = 0;
+    // rows per lane requested (0: chosen by measurement); sampled ms per candidate
+    // (rows per lane - 1) * 2 + store, -1 = not sampled, -2 = no kernel shape for these programs
+    int want_r = 0;
+    double cand_ms[4] = {-1.0, -1.0, -1.0, -1.0};
     AdjTape T;
-    int R = 1, F = 12, S = 12;
+    int R = 1, F = 12, S = 12, AS = kAdjSlots;
     int32_t nranges = 1;
     int64_t range_rows = 0;
     int grid = 0;
@@ -181,16 +184,25 @@ AdjFinal make_final(ppi_adjoint* a, const float* gscale, float* gout) {
     return F;
 }
 
-// Compile the kept programs for partial store `ps` and upload the tape.
-ppi_status prepare(ppi_adjoint* a, int ps) {
-    if (a->tape_ps == ps) return PPI_OK;
+void forget_tuning(ppi_adjoint* a) {
+    a->tape_ps = -1;
+    a->tape_r = 0;
+    for (double& t : a->cand_ms) t = -1.0;
+}
+
+// Compile the kept programs for partial store `ps` at `rows` rows per lane and
+// upload the tape.  PPI_E_UNSUPPORTED: no kernel shape holds these programs at
+// that many rows per lane (their stack or adjoint slots, see kern_adjoint*.hip).
+ppi_status prepare(ppi_adjoint* a, int ps, int rows) {
+    if (a->tape_ps == ps && a->tape_r == rows) return PPI_OK;
     const double t0 = now_ms();
     std::string err;
-    const int32_t B = adjoint_store_partials(ps);
+    const int32_t B = adjoint_store_partials(ps, rows);
     if (!compile_adjoint(a->eng->symtab, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->eng->ncol, B,
                          &a->T, &err)) {
         set_error(err);
         a->tape_ps = -1;
+        a->tape_r = 0;
         return PPI_E_PROGRAM;
     }
     AdjTape& T = a->T;
@@ -198,6 +210,7 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
         if (c.ins >= T.fwd.code.size()) {
             set_error("program " + std::to_string(c.prog) + ": constant at " + std::to_string(c.pos) + " has no tape slot");
             a->tape_ps = -1;
+            a->tape_r = 0;
             return PPI_E_PROGRAM;
         }
     T.fwd.code.insert(T.fwd.code.end(), kTapePad, Ins{H_END, kEmpty});
@@ -205,12 +218,26 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
     // first and store entries up to 63 past its first.
     T.rev.insert(T.rev.end(), 128, RevIns{0u, 0u});
     T.stores.insert(T.stores.end(), 64, AdjStore{0u, 0u});
-    a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
-    a->F = (T.fwd.stats.max_col < 12 && a->S == 12) ? 12 : 0;
-    a->R = 1;
-    if (!adjoint_shape_supported(a->R, a->F, a->S, ps)) {
+    a->tape_ps = -1;  // the device tape is replaced below
+    a->tape_r = 0;
+    if (rows == 2) {
+        // One register tuple of 10 per stack: 5 slots per row.
+        a->S = 5;
+        a->AS = 5;
+        a->F = T.fwd.stats.max_col < 12 ? 12 : 0;
+        if (T.fwd.stats.max_slots > 5 || T.max_adj_slots > 5) {
+            set_error("programs need more than 5 stack or adjoint slots: one row per lane");
+            return PPI_E_UNSUPPORTED;
+        }
+    } else {
+        a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
+        a->AS = kAdjSlots;
+        a->F = (T.fwd.stats.max_col < 12 && a->S == 12) ? 12 : 0;
+    }
+    a->R = rows;
+    if (!adjoint_shape_supported(a->R, a->F, a->S, a->AS, ps)) {
         set_error("no adjoint kernel shape for these programs");
-        return PPI_E_PROGRAM;
+        return PPI_E_UNSUPPORTED;
     }
     const size_t nj = T.jobs.size();
     a->order.resize(nj);
@@ -219,7 +246,7 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
                      [&](int32_t x, int32_t y) { return T.jobs[(size_t)x].cost > T.jobs[(size_t)y].cost; });
     // Row ranges: enough units to fill every wave slot several times over.
     const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
-    const int occ = std::max(1, adjoint_occupancy(a->R, a->F, a->S, ps, P, N));
+    const int occ = std::max(1, adjoint_occupancy(a->R, a->F, a->S, a->AS, ps, P, N));
     const int64_t slots = (int64_t)a->dev_cu * 4 * occ;
     const int64_t nlin = a->eng->nlin;
     const int64_t tile = 64 * a->R;
@@ -249,6 +276,7 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
         return s;
     A_HIP(hipStreamSynchronize(st));
     a->tape_ps = ps;
+    a->tape_r = rows;
     double fwd = 0.0, rev = 0.0;
     for (const AdjJob& j : T.jobs) {
         if (j.flags & kEmpty) continue;
@@ -272,7 +300,7 @@ ppi_status prepare(ppi_adjoint* a, int ps) {
 
 ppi_status enqueue_adjoint(ppi_adjoint* a, const AdjArgs& A, int grid) {
     const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
-    A_HIP(launch_adjoint(A, grid, a->R, a->F, a->S, a->tape_ps, P, N, a->stream));
+    A_HIP(launch_adjoint(A, grid, a->R, a->F, a->S, a->AS, a->tape_ps, P, N, a->stream));
     return PPI_OK;
 }
 
@@ -305,21 +333,56 @@ ppi_status sample_ms(ppi_adjoint* a, double* ms) {
     return PPI_OK;
 }
 
-// Resolve PPI_PARTIALS_* to a partial store (0: VGPR + recompute, 1: LDS).
-ppi_status choose_store(ppi_adjoint* a, int32_t partials, int* ps) {
-    if (partials == PPI_PARTIALS_RECOMPUTE) { *ps = 0; return PPI_OK; }
-    if (partials == PPI_PARTIALS_STORE) { *ps = 1; return PPI_OK; }
-    if (partials != PPI_PARTIALS_AUTO) { set_error("bad partials mode"); return PPI_E_ARG; }
-    if (a->tuned_ps >= 0) { *ps = a->tuned_ps; return PPI_OK; }  // tuned for the kept programs
-    double t[2] = {0.0, 0.0};
-    for (int k = 0; k < 2; ++k) {
-        ppi_status s = prepare(a, k);
+// Resolve PPI_PARTIALS_* and the rows per lane to a kernel: every candidate
+// (store: both for AUTO; rows per lane: 1 and 2 unless one was requested) whose
+// shape holds the programs is compiled and timed on a sample of the units, and
+// the fastest is prepared.  Timings are kept until the programs or data change.
+ppi_status choose(ppi_adjoint* a, int32_t partials, int* ps_out) {
+    int ps_lo = 0, ps_hi = 1;
+    if (partials == PPI_PARTIALS_RECOMPUTE) ps_lo = ps_hi = 0;
+    else if (partials == PPI_PARTIALS_STORE) ps_lo = ps_hi = 1;
+    else if (partials != PPI_PARTIALS_AUTO) { set_error("bad partials mode"); return PPI_E_ARG; }
+    const int r_lo = a->want_r ? a->want_r : 1, r_hi = a->want_r ? a->want_r : 2;
+    if (ps_lo == ps_hi && r_lo == r_hi) {  // one candidate: nothing to time
+        ppi_status s = prepare(a, ps_lo, r_lo);
         if (s) return s;
-        if ((s = sample_ms(a, &t[k]))) return s;
-        a->stats.tune_ms[k] = t[k];
+        *ps_out = ps_lo;
+        a->stats.partials = ps_lo ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+        return PPI_OK;
     }
-    *ps = t[1] < t[0] ? 1 : 0;
-    a->tuned_ps = *ps;
+    int best = -1;
+    double best_ms = 0.0;
+    for (int r = r_lo; r <= r_hi; ++r)
+        for (int ps = ps_lo; ps <= ps_hi; ++ps) {
+            const int c = (r - 1) * 2 + ps;
+            if (a->cand_ms[c] == -1.0) {
+                ppi_status s = prepare(a, ps, r);
+                if (s == PPI_E_UNSUPPORTED) {
+                    a->cand_ms[c] = -2.0;
+                    a->stats.tune_ms[c] = -2.0;
+                    continue;
+                }
+                if (s) return s;
+                double t = 0.0;
+                if ((s = sample_ms(a, &t))) return s;
+                a->cand_ms[c] = t;
+                a->stats.tune_ms[c] = t;
+            }
+            if (a->cand_ms[c] < 0.0) continue;
+            if (best < 0 || a->cand_ms[c] < best_ms) {
+                best = c;
+                best_ms = a->cand_ms[c];
+            }
+        }
+    if (best < 0) {
+        set_error(a->want_r == 2 ? "programs need more than 5 stack or adjoint slots: one row per lane"
+                                 : "no adjoint kernel shape for these programs");
+        return PPI_E_UNSUPPORTED;
+    }
+    *ps_out = best & 1;
+    ppi_status s = prepare(a, best & 1, best / 2 + 1);
+    if (s) return s;
+    a->stats.partials = (best & 1) ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
     return PPI_OK;
 }
 
@@ -392,8 +455,7 @@ ppi_status ppi_adjoint_configure(ppi_adjoint* a, const ppi_config* cfg, const pp
     a->cfg.cost = nullptr;
     a->configured = true;
     a->have_programs = false;
-    a->tape_ps = -1;
-    a->tuned_ps = -1;
+    forget_tuning(a);
     return PPI_OK;
 }
 
@@ -402,8 +464,7 @@ ppi_status ppi_adjoint_set_dataset_rowmajor(ppi_adjoint* a, const float* data, i
     ppi_status s = ppi_engine_set_dataset_rowmajor(a->eng, data, nlin, ncol);
     if (s) return s;
     a->have_programs = false;
-    a->tape_ps = -1;
-    a->tuned_ps = -1;
+    forget_tuning(a);
     return PPI_OK;
 }
 
@@ -422,8 +483,7 @@ ppi_status ppi_adjoint_forward(ppi_adjoint* a, const int32_t* phenotype, const f
     a->n = n;
     a->alpha = alpha;
     a->have_programs = true;
-    a->tape_ps = -1;
-    a->tuned_ps = -1;
+    forget_tuning(a);
     if (fitness) std::memcpy(fitness, fit.data(), (size_t)n * sizeof(float));
     return PPI_OK;
 }
@@ -433,9 +493,7 @@ ppi_status ppi_adjoint_backward(ppi_adjoint* a, const float* grad_fitness, int32
     if (s) return s;
     if (!grad) { set_error("grad is NULL"); return PPI_E_ARG; }
     int ps = 0;
-    if ((s = choose_store(a, partials, &ps))) return s;
-    if ((s = prepare(a, ps))) return s;
-    a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    if ((s = choose(a, partials, &ps))) return s;
     const size_t total = (size_t)a->n * a->stride;
     if ((s = a->dGout.reserve(total))) return s;
     const float* gs = nullptr;
@@ -464,9 +522,7 @@ ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_
     if (s) return s;
     if (steps < 0 || !(step > 0.0f) || !std::isfinite(step)) { set_error("need steps >= 0 and a finite step > 0"); return PPI_E_ARG; }
     int ps = 0;
-    if ((s = choose_store(a, partials, &ps))) return s;
-    if ((s = prepare(a, ps))) return s;
-    a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    if ((s = choose(a, partials, &ps))) return s;
     // Plain-evaluator fitness of the starting point, for the final no-regression check.
     std::vector<float> f_start((size_t)a->n), f_end((size_t)a->n);
     if ((s = ppi_engine_evaluate(a->eng, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->alpha,
@@ -521,7 +577,10 @@ ppi_status ppi_adjoint_descend(ppi_adjoint* a, int32_t steps, float step, int32_
         f_end[(size_t)p] = f_start[(size_t)p];
         ++reverted;
     }
-    if (reverted) a->tape_ps = -1;  // the device tape holds cbest for the reverted programs: recompile on next use
+    if (reverted) {  // the device tape holds cbest for the reverted programs: recompile on next use
+        a->tape_ps = -1;
+        a->tape_r = 0;
+    }
     a->stats.descend_reverted = reverted;
     if (fitness_out) std::memcpy(fitness_out, f_end.data(), f_end.size() * sizeof(float));
     if (ephemeral_out) std::memcpy(ephemeral_out, a->eph.data(), a->eph.size() * sizeof(float));
@@ -533,9 +592,7 @@ ppi_status ppi_adjoint_time(ppi_adjoint* a, int32_t iters, int32_t partials, dou
     if (s) return s;
     if (iters <= 0 || !ms_per_pass) { set_error("bad arguments"); return PPI_E_ARG; }
     int ps = 0;
-    if ((s = choose_store(a, partials, &ps))) return s;
-    if ((s = prepare(a, ps))) return s;
-    a->stats.partials = ps == 1 ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    if ((s = choose(a, partials, &ps))) return s;
     double kern = 0.0;
     A_HIP(hipEventRecord(a->ev[0], a->stream));
     for (int32_t k = 0; k < iters; ++k) {
@@ -552,6 +609,13 @@ ppi_status ppi_adjoint_time(ppi_adjoint* a, int32_t iters, int32_t partials, dou
     *ms_per_pass = ms / iters;
     a->stats.backward_ms = ms / iters;
     a->stats.kernel_ms = kern / iters;
+    return PPI_OK;
+}
+
+ppi_status ppi_adjoint_set_rows_per_lane(ppi_adjoint* a, int32_t rows_per_lane) {
+    if (!a) { set_error("adjoint is NULL"); return PPI_E_ARG; }
+    if (rows_per_lane < 0 || rows_per_lane > 2) { set_error("rows_per_lane must be 0, 1 or 2"); return PPI_E_ARG; }
+    a->want_r = rows_per_lane;
     return PPI_OK;
 }
 

@@ -113,11 +113,11 @@ __device__ __forceinline__ void dbinary(uint32_t op, float a, float b, float z, 
 
 // Partial stores: slot s (< B) of row r.
 template <int R, int B>
-struct PStoreV {  // VGPRs
-    typedef typename VecT<B>::type V;
-    V v[R];
-    __device__ __forceinline__ void set(uint32_t s, int r, float x) { v[r][s] = x; }
-    __device__ __forceinline__ float get(uint32_t s, int r) const { return v[r][s]; }
+struct PStoreV {  // VGPRs, one tuple of B*R (>= 16: indexed moves, see RegStack)
+    typedef typename VecT<B * R>::type V;
+    V v;
+    __device__ __forceinline__ void set(uint32_t s, int r, float x) { v[s * R + r] = x; }
+    __device__ __forceinline__ float get(uint32_t s, int r) const { return v[s * R + r]; }
 };
 template <int R>
 struct PStoreL {  // this lane's column of the wave's [B][R][64] LDS block
@@ -126,9 +126,8 @@ struct PStoreL {  // this lane's column of the wave's [B][R][64] LDS block
     __device__ __forceinline__ float get(uint32_t s, int r) const { return base[(s * R + r) * 64]; }
 };
 
-template <int PS> struct StoreCfg;
-template <> struct StoreCfg<0> { static constexpr int B = 16, LOGB = 4; };
-template <> struct StoreCfg<1> { static constexpr int B = 32, LOGB = 5; };
+// Partials per row: 16 / R in registers (PS 0), 32 / R in LDS (PS 1, 8 KB per wave).
+template <int PS, int R> struct StoreCfg { static constexpr int B = (PS == 1 ? 32 : 16) / R; };
 
 __device__ __forceinline__ AdjJob load_job(const AdjJob* jobs, int i) {
     ConstPtr<AdjJob> q = as_const(jobs) + i;
@@ -150,8 +149,7 @@ template <int R, int S, int PS, bool P, bool N>
 struct AdjCore {
     typedef Core<R, S, P, N> C;
     typedef typename C::Stk Stk;
-    static constexpr int B = StoreCfg<PS>::B;
-    static constexpr int LOGB = StoreCfg<PS>::LOGB;
+    static constexpr int B = StoreCfg<PS, R>::B;
     static constexpr uint32_t kNone = 0xffffffffu;
 
     // One instruction that stores partials: sa / sb = store slot of the partial
@@ -241,23 +239,37 @@ struct AdjCore {
     // Forward pass over the program's instructions [0, n), fetched through a
     // 64-entry window across the wave's lanes as in the FP form (lane L holds
     // tape entry blk + L, read back with v_readlane; the next block is loaded
-    // when the current one starts).  The segment's stores (at most B < 64, in
-    // increasing pc) are one window too: instruction pc = st_pc[ks] also writes
-    // its partials.  code is the whole tape, the program starts at c0; the tape
-    // is padded so that blk + 127 and the store window stay in bounds.
+    // when the current one starts).  A program shorter than 64 instructions is
+    // one window loaded once per unit (res: wc0 / wa0 hold code[c0 + lane]).
+    // The segment's stores are entries [ks, ks_end) of the store window st_pc /
+    // st_sl, in increasing pc: instruction pc = st_pc[ks] also writes its
+    // partials.  code is the whole tape, the program starts at c0; the tape is
+    // padded so that blk + 127 stays in bounds.
     template <class FeatT, class PSt>
-    static __device__ __forceinline__ void forward(const Ins* code, uint32_t c0, uint32_t n, uint32_t st_pc,
-                                                   uint32_t st_sl, uint32_t nst, const FeatT& ft, float (&acc)[R],
-                                                   Stk& stk, PSt& ps) {
+    static __device__ __forceinline__ void forward(const Ins* code, uint32_t c0, uint32_t n, bool res, uint32_t wc0,
+                                                   uint32_t wa0, uint32_t st_pc, uint32_t st_sl, uint32_t ks,
+                                                   uint32_t ks_end, const FeatT& ft, float (&acc)[R], Stk& stk,
+                                                   PSt& ps) {
         const int lane = threadIdx.x & 63;
         const uint32_t gend = c0 + n;
-        uint32_t blk = c0 & ~63u;
-        uint32_t wc = code[blk + lane].code, wa = code[blk + lane].arg;
+        uint32_t blk, wc, wa;
+        if (res) {
+            blk = c0;
+            wc = wc0;
+            wa = wa0;
+        } else {
+            blk = c0 & ~63u;
+            wc = code[blk + lane].code;
+            wa = code[blk + lane].arg;
+        }
         uint32_t k = c0 - blk;
-        uint32_t ks = 0;
-        uint32_t next = nst > 0 ? c0 + (uint32_t)__builtin_amdgcn_readlane((int)st_pc, 0) : 0xffffffffu;
+        uint32_t next = ks < ks_end ? c0 + (uint32_t)__builtin_amdgcn_readlane((int)st_pc, (int)ks) : 0xffffffffu;
         for (; blk < gend; blk += 64) {
-            const uint32_t nc = code[blk + 64 + lane].code, na = code[blk + 64 + lane].arg;
+            uint32_t nc = 0, na = 0;
+            if (!res) {
+                nc = code[blk + 64 + lane].code;
+                na = code[blk + 64 + lane].arg;
+            }
             const uint32_t kend = min(64u, gend - blk);
             for (; k < kend; ++k) {
                 const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)wc, (int)k);
@@ -270,7 +282,7 @@ struct AdjCore {
                 const uint32_t sl = (uint32_t)__builtin_amdgcn_readlane((int)st_sl, (int)ks);
                 step_rec(c, a, (sl & 0xffffu) - 1u, (sl >> 16) - 1u, ft, acc, stk, ps);
                 ++ks;
-                next = ks < nst ? c0 + (uint32_t)__builtin_amdgcn_readlane((int)st_pc, (int)ks) : 0xffffffffu;
+                next = ks < ks_end ? c0 + (uint32_t)__builtin_amdgcn_readlane((int)st_pc, (int)ks) : 0xffffffffu;
             }
             k = 0;
             wc = nc;
@@ -284,13 +296,17 @@ struct AdjFeat {
     typedef typename std::conditional<(F > 0), VgprFeat<R, F>, GlobalFeat<R>>::type type;
 };
 
-template <int R, int F, int S, int PS, bool P, bool N>
+// R rows per lane, F features in VGPRs (0: read from the dataset), S stack
+// slots, AS adjoint slots (both register stacks: RegStack's layout), PS
+// partial store (0: VGPRs, 1: LDS).
+template <int R, int F, int S, int AS, int PS, bool P, bool N>
 __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     typedef AdjCore<R, S, PS, P, N> AC;
     typedef typename AC::Stk Stk;
     constexpr int B = AC::B;
     static_assert(B < 64, "a segment's stores must fit one window");
-    typedef typename VecT<kAdjSlots>::type VA;
+    static_assert(AS <= kAdjSlots, "adjoint slots");
+    typedef RegStack<R, AS> VA;
     typedef typename VecT<kAdjMaxConsts>::type VG;
     constexpr int LDS_WAVE = PS == 1 ? B * R * 64 : 1;
     __shared__ float lds[kWaves][LDS_WAVE];
@@ -315,40 +331,67 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     bool bad = false;
     if (!(J.flags & kEmpty)) {
         const RevIns* rv = A.rev;
+        const Ins* code = A.code;
+        const AdjStore* stores = A.stores;
+        // Windows that serve every row block of the unit: the program when it is
+        // shorter than 64 instructions, the job's reverse tape and store list
+        // (all segments, consecutive) when each holds at most 64 entries.  The
+        // reverse tape and the store list are padded by 128 and 64 entries.
+        const uint32_t c0 = J.code_off;
+        const bool fres = J.ncode < 64u;
+        const uint32_t fwc = fres ? code[c0 + lane].code : 0u, fwa = fres ? code[c0 + lane].arg : 0u;
+        ConstPtr<AdjSeg> sg0 = as_const(A.segs) + J.seg_off;
+        ConstPtr<AdjSeg> sgl = sg0 + (J.nseg - 1);
+        const uint32_t rb0 = sg0->rev_begin, st0 = sg0->st_begin;
+        const bool rres = sgl->rev_end - rb0 <= 64u, sres = sgl->st_end - st0 <= 64u;
+        const uint32_t rw0 = rres ? rv[rb0 + lane].w : 0u, rp0 = rres ? rv[rb0 + lane].p : 0u;
+        const uint32_t sp0 = sres ? stores[st0 + lane].pc : 0u, ss0 = sres ? stores[st0 + lane].slots : 0u;
+
+        typedef typename AdjFeat<R, F>::type FeatT;
+        typedef typename VgprFeat<R, (F > 0 ? F : 1)>::VF VF;
+        // Row block features and targets.  (Loading them one block ahead cost
+        // 13 VGPRs and a wave per SIMD, and measured slower.)
+        VF xf[R];
+        float y[R];
+        auto load_rows = [&](int64_t r0, VF (&xv)[R], float (&yv)[R]) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t rw = r0 + r * 64 + lane;  // < ldx: rows are padded past nlin
+                yv[r] = A.X[(int64_t)A.ycol * A.ldx + rw];
+                if constexpr (F > 0) {
+#pragma unroll
+                    for (int f = 0; f < F; ++f) xv[r][f] = f < A.ncol ? A.X[(int64_t)f * A.ldx + rw] : 0.0f;
+                }
+            }
+        };
         for (int64_t row0 = rbeg; row0 < rend; row0 += 64 * R) {
+            load_rows(row0, xf, y);
             int row[R];
-            float y[R];
             bool valid[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 row[r] = (int)(row0 + r * 64 + lane);
                 valid[r] = row[r] < rend;
-                y[r] = A.X[(int64_t)A.ycol * A.ldx + row[r]];
-            }
-            typedef typename AdjFeat<R, F>::type FeatT;
-            typename VgprFeat<R, (F > 0 ? F : 1)>::VF xf[R];
-            if constexpr (F > 0) {
-#pragma unroll
-                for (int f = 0; f < F; ++f) {
-                    const bool have = f < A.ncol;
-#pragma unroll
-                    for (int r = 0; r < R; ++r) xf[r][f] = have ? A.X[(int64_t)f * A.ldx + row[r]] : 0.0f;
-                }
             }
             FeatT ft = [&]() {
                 if constexpr (F > 0) return FeatT{xf};
                 else return FeatT{A.X, A.ldx, row};
             }();
-            VA adj[R];
+            VA adj;
             for (uint32_t s = 0; s < J.nseg; ++s) {
-                ConstPtr<AdjSeg> sq = as_const(A.segs) + (J.seg_off + s);
+                ConstPtr<AdjSeg> sq = sg0 + s;
                 const uint32_t rb = sq->rev_begin, re = sq->rev_end, fe = sq->fwd_end;
                 const uint32_t sb0 = sq->st_begin, nst = sq->st_end - sb0;
-                // The segment's stores across the lanes (the store list is padded by 64).
-                const uint32_t st_pc = A.stores[sb0 + lane].pc, st_sl = A.stores[sb0 + lane].slots;
+                // The segment's stores across the lanes (one window: nst <= B < 64).
+                uint32_t st_pc = sp0, st_sl = ss0, ks = sb0 - st0;
+                if (!sres) {
+                    st_pc = stores[sb0 + lane].pc;
+                    st_sl = stores[sb0 + lane].slots;
+                    ks = 0;
+                }
                 float acc[R];
                 Stk stk;
-                AC::forward(A.code, J.code_off, fe, st_pc, st_sl, nst, ft, acc, stk, ps);
+                AC::forward(code, c0, fe, fres, fwc, fwa, st_pc, st_sl, ks, ks + nst, ft, acc, stk, ps);
                 if (s == 0) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
@@ -360,16 +403,28 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
                             else part += e;
                             seed = A.error_fn == PPI_EF_SQUARED ? 2.0f * d : (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
                         }
-                        adj[r][0] = seed;
+                        adj.set(r, 0, seed);
                     }
                 }
-                // Reverse instructions [rb, re) through a 64-entry window like the
-                // forward tape (the reverse tape is padded by 128 entries).
-                uint32_t blk = rb & ~63u;
-                uint32_t ww = rv[blk + lane].w, wp = rv[blk + lane].p;
+                // Reverse instructions [rb, re) through a window like the forward tape:
+                // the unit's resident window, or 64-entry blocks streamed from rb.
+                uint32_t blk, ww, wp;
+                if (rres) {
+                    blk = rb0;
+                    ww = rw0;
+                    wp = rp0;
+                } else {
+                    blk = rb & ~63u;
+                    ww = rv[blk + lane].w;
+                    wp = rv[blk + lane].p;
+                }
                 uint32_t k = rb - blk;
                 for (; blk < re; blk += 64) {
-                    const uint32_t nw = rv[blk + 64 + lane].w, np = rv[blk + 64 + lane].p;
+                    uint32_t nw = 0, np = 0;
+                    if (!rres) {
+                        nw = rv[blk + 64 + lane].w;
+                        np = rv[blk + 64 + lane].p;
+                    }
                     const uint32_t kend = min(64u, re - blk);
                     for (; k < kend; ++k) {
                         const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)ww, (int)k);
@@ -386,20 +441,20 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
                                 const float p = ps.get(sl, r);
-                                const float ad = adj[r][src];
+                                const float ad = adj.get(r, src);
                                 v[r] = ad == 0.0f ? 0.0f : ad * p;
                             }
                         } else {
                             const float pc = __uint_as_float(pb);
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
-                                const float ad = adj[r][src];
+                                const float ad = adj.get(r, src);
                                 v[r] = ad == 0.0f ? 0.0f : ad * pc;
                             }
                         }
                         if ((w & 3u) == kRevAdj) {
 #pragma unroll
-                            for (int r = 0; r < R; ++r) adj[r][dst] = v[r];
+                            for (int r = 0; r < R; ++r) adj.set(r, dst, v[r]);
                         } else {
                             float t = 0.0f;
 #pragma unroll
@@ -515,23 +570,23 @@ __global__ void adj_descend_kernel(AdjDescend D) {
 
 namespace launch {
 
-template <int R, int F, int S, int PS>
+template <int R, int F, int S, int AS, int PS>
 hipError_t launch_adjoint_t(const AdjArgs& A, int grid, bool P, bool N, hipStream_t st) {
-    if (P && N) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, true, true>), dim3(grid), dim3(dev::kBlock), 0, st, A);
-    else if (P) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, true, false>), dim3(grid), dim3(dev::kBlock), 0, st, A);
-    else if (N) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, false, true>), dim3(grid), dim3(dev::kBlock), 0, st, A);
-    else hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, PS, false, false>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    if (P && N) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, AS, PS, true, true>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, AS, PS, true, false>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, AS, PS, false, true>), dim3(grid), dim3(dev::kBlock), 0, st, A);
+    else hipLaunchKernelGGL((dev::adjoint_kernel<R, F, S, AS, PS, false, false>), dim3(grid), dim3(dev::kBlock), 0, st, A);
     return hipGetLastError();
 }
 
-template <int R, int F, int S, int PS>
+template <int R, int F, int S, int AS, int PS>
 int adjoint_occupancy_t(bool P, bool N) {
     int n = 0;
     hipError_t e;
-    if (P && N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, true, true>, dev::kBlock, 0);
-    else if (P) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, true, false>, dev::kBlock, 0);
-    else if (N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, false, true>, dev::kBlock, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, PS, false, false>, dev::kBlock, 0);
+    if (P && N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, AS, PS, true, true>, dev::kBlock, 0);
+    else if (P) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, AS, PS, true, false>, dev::kBlock, 0);
+    else if (N) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, AS, PS, false, true>, dev::kBlock, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::adjoint_kernel<R, F, S, AS, PS, false, false>, dev::kBlock, 0);
     return e == hipSuccess ? n : 0;
 }
 

@@ -3,28 +3,39 @@
 #include "adjoint_impl.h"
 
 namespace ppi {
-// (rows per lane, features in VGPRs (0: read from the dataset), stack slots, partial store (0: VGPR, 1: LDS))
-#define PPI_ADJ_SHAPES(X) X(1, 12, 12, 0) X(1, 0, 12, 0) X(1, 0, 16, 0) X(1, 12, 12, 1) X(1, 0, 12, 1) X(1, 0, 16, 1)
+// (rows per lane, features in VGPRs (0: read from the dataset), stack slots, adjoint slots,
+// partial store (0: VGPR, 1: LDS)); two rows per lane: kern_adjoint_r2.hip.
+#define PPI_ADJ_SHAPES(X) \
+    X(1, 12, 12, 12, 0) X(1, 0, 12, 12, 0) X(1, 0, 16, 12, 0) X(1, 12, 12, 12, 1) X(1, 0, 12, 12, 1) X(1, 0, 16, 12, 1)
 
-bool adjoint_shape_supported(int R, int F, int S, int PS) {
-#define PPI_X(r, f, s, ps) if (R == r && F == f && S == s && PS == ps) return true;
+bool adjoint_shape_supported(int R, int F, int S, int AS, int PS) {
+    if (R == 2) return adjoint_shape_supported_r2(F, S, AS, PS);
+#define PPI_X(r, f, s, as, ps) if (R == r && F == f && S == s && AS == as && PS == ps) return true;
     PPI_ADJ_SHAPES(PPI_X)
 #undef PPI_X
     return false;
 }
 
-int adjoint_store_partials(int PS) { return PS == 1 ? dev::StoreCfg<1>::B : dev::StoreCfg<0>::B; }
+int adjoint_store_partials(int PS, int R) {
+    if (R == 2) return PS == 1 ? dev::StoreCfg<1, 2>::B : dev::StoreCfg<0, 2>::B;
+    return PS == 1 ? dev::StoreCfg<1, 1>::B : dev::StoreCfg<0, 1>::B;
+}
 
-hipError_t launch_adjoint(const AdjArgs& A, int grid, int R, int F, int S, int PS, bool P, bool N, hipStream_t st) {
-#define PPI_X(r, f, s, ps) \
-    if (R == r && F == f && S == s && PS == ps) return launch::launch_adjoint_t<r, f, s, ps>(A, grid, P, N, st);
+hipError_t launch_adjoint(const AdjArgs& A, int grid, int R, int F, int S, int AS, int PS, bool P, bool N,
+                          hipStream_t st) {
+    if (R == 2) return launch_adjoint_r2(A, grid, F, S, AS, PS, P, N, st);
+#define PPI_X(r, f, s, as, ps)                                 \
+    if (R == r && F == f && S == s && AS == as && PS == ps) \
+        return launch::launch_adjoint_t<r, f, s, as, ps>(A, grid, P, N, st);
     PPI_ADJ_SHAPES(PPI_X)
 #undef PPI_X
     return hipErrorInvalidValue;
 }
 
-int adjoint_occupancy(int R, int F, int S, int PS, bool P, bool N) {
-#define PPI_X(r, f, s, ps) if (R == r && F == f && S == s && PS == ps) return launch::adjoint_occupancy_t<r, f, s, ps>(P, N);
+int adjoint_occupancy(int R, int F, int S, int AS, int PS, bool P, bool N) {
+    if (R == 2) return adjoint_occupancy_r2(F, S, AS, PS, P, N);
+#define PPI_X(r, f, s, as, ps) \
+    if (R == r && F == f && S == s && AS == as && PS == ps) return launch::adjoint_occupancy_t<r, f, s, as, ps>(P, N);
     PPI_ADJ_SHAPES(PPI_X)
 #undef PPI_X
     return 0;

@@ -143,10 +143,17 @@ struct AdjDescend {
     int32_t last;  // 1: write back cbest and stop
 };
 
-bool adjoint_shape_supported(int R, int F, int S, int PS);
-int adjoint_store_partials(int PS);
-hipError_t launch_adjoint(const AdjArgs& A, int grid, int R, int F, int S, int PS, bool P, bool N, hipStream_t st);
-int adjoint_occupancy(int R, int F, int S, int PS, bool P, bool N);
+// Adjoint kernel shapes: R rows per lane, F features in VGPRs (0: from the dataset),
+// S stack slots, AS adjoint slots, PS partial store (0: VGPRs, 1: LDS).
+bool adjoint_shape_supported(int R, int F, int S, int AS, int PS);
+int adjoint_store_partials(int PS, int R);
+hipError_t launch_adjoint(const AdjArgs& A, int grid, int R, int F, int S, int AS, int PS, bool P, bool N,
+                          hipStream_t st);
+int adjoint_occupancy(int R, int F, int S, int AS, int PS, bool P, bool N);
+// The two-rows-per-lane shapes (their own instantiation unit).
+bool adjoint_shape_supported_r2(int F, int S, int AS, int PS);
+hipError_t launch_adjoint_r2(const AdjArgs& A, int grid, int F, int S, int AS, int PS, bool P, bool N, hipStream_t st);
+int adjoint_occupancy_r2(int F, int S, int AS, int PS, bool P, bool N);
 hipError_t launch_adjoint_finalize(const AdjFinal& F, hipStream_t st);
 hipError_t launch_adjoint_descend(const AdjDescend& D, hipStream_t st);
 

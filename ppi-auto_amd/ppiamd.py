@@ -86,7 +86,7 @@ class AdjointStats(C.Structure):
         ("reverse_ins", C.c_double), ("n_jobs", C.c_int32), ("n_consts", C.c_int32),
         ("max_segments", C.c_int32), ("partials", C.c_int32), ("rows_per_lane", C.c_int32),
         ("ranges", C.c_int32), ("descend_steps", C.c_int32), ("descend_reverted", C.c_int32),
-        ("tune_ms", C.c_double * 2),
+        ("tune_ms", C.c_double * 4),
     ]
 
 
@@ -151,7 +151,7 @@ ENGINE_SYMBOLS = [
     "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
     "ppi_adjoint_create", "ppi_adjoint_destroy", "ppi_adjoint_configure", "ppi_adjoint_set_dataset_rowmajor",
     "ppi_adjoint_forward", "ppi_adjoint_backward", "ppi_adjoint_descend", "ppi_adjoint_time",
-    "ppi_adjoint_get_stats",
+    "ppi_adjoint_get_stats", "ppi_adjoint_set_rows_per_lane",
 ]
 
 
@@ -222,6 +222,7 @@ def lib() -> C.CDLL:
         L.ppi_adjoint_descend.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_int32, P(C.c_float), P(C.c_float)]
         L.ppi_adjoint_time.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double)]
         L.ppi_adjoint_get_stats.argtypes = [C.c_void_p, P(AdjointStats)]
+        L.ppi_adjoint_set_rows_per_lane.argtypes = [C.c_void_p, C.c_int32]
         for name in ENGINE_SYMBOLS:
             if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy", "ppi_batch_destroy",
                             "ppi_adjoint_destroy"):
@@ -560,6 +561,10 @@ class Adjoint:
         s = AdjointStats()
         _check(lib().ppi_adjoint_get_stats(self._h, C.byref(s)))
         return s
+
+    def set_rows_per_lane(self, rows: int) -> None:
+        """0: chosen by measurement (default), 1, or 2 (programs of at most 5 stack and adjoint slots)."""
+        _check(lib().ppi_adjoint_set_rows_per_lane(self._h, rows))
 
 
 # ---------------------------------------------------------------------------

@@ -36,13 +36,18 @@ def adj_case():
     return w, a, fit
 
 
+@pytest.mark.parametrize("rows", [1, 2])
 @pytest.mark.parametrize("partials", [ppiamd.PARTIALS_RECOMPUTE, ppiamd.PARTIALS_STORE])
-def test_gradient_matches_oracle(adj_case, partials):
+def test_gradient_matches_oracle(adj_case, partials, rows):
     w, a, fit = adj_case
-    print(f"adjoint parity: 1500 programs x 20000 rows, partials={partials}", flush=True)
-    g = a.backward(partials=partials)
+    print(f"adjoint parity: 1500 programs x 20000 rows, partials={partials}, rows per lane {rows}", flush=True)
+    a.set_rows_per_lane(rows)
+    try:
+        g = a.backward(partials=partials)
+    finally:
+        a.set_rows_per_lane(0)
     st = a.stats()
-    assert st.partials == partials and st.n_consts > 2000
+    assert st.partials == partials and st.rows_per_lane == rows and st.n_consts > 2000
     sentinel = fit >= 3.0e38 / 20000
     gm = g.reshape(w.pop.n, w.pop.stride)
     assert (gm[sentinel] == 0).all()
@@ -67,9 +72,13 @@ def test_stores_agree(adj_case):
     up to the order the row sums are formed in (the two kernels may split rows into
     different ranges)."""
     w, a, fit = adj_case
-    g1 = a.backward(partials=ppiamd.PARTIALS_RECOMPUTE).astype(np.float64)
-    seg1 = a.stats().max_segments
-    g2 = a.backward(partials=ppiamd.PARTIALS_STORE).astype(np.float64)
+    a.set_rows_per_lane(1)
+    try:
+        g1 = a.backward(partials=ppiamd.PARTIALS_RECOMPUTE).astype(np.float64)
+        seg1 = a.stats().max_segments
+        g2 = a.backward(partials=ppiamd.PARTIALS_STORE).astype(np.float64)
+    finally:
+        a.set_rows_per_lane(0)
     Gv, Mv, _ = oracle_layout(w.grammar.table, w.pop, w.data, True, ppiamd.EF_SQUARED, w.grammar.t_const,
                               f32_values=True)
     assert seg1 >= 2, "the register store must recompute some programs"
@@ -98,8 +107,10 @@ def test_edge_programs():
     a.configure(b.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_SQUARED, protected=True)
     a.set_dataset(data)
     fit = a.forward(pop)
-    for partials in (ppiamd.PARTIALS_RECOMPUTE, ppiamd.PARTIALS_STORE):
+    for partials, rows in [(p, r) for p in (ppiamd.PARTIALS_RECOMPUTE, ppiamd.PARTIALS_STORE) for r in (1, 2)]:
+        a.set_rows_per_lane(rows)
         g = a.backward(partials=partials).reshape(pop.n, pop.stride)
+        assert a.stats().rows_per_lane == rows
         x0 = data[:, 0].astype(np.float64)
         y = data[:, 4].astype(np.float64)
         cf = np.float64(np.float32(c))
@@ -115,6 +126,40 @@ def test_edge_programs():
     sentinel = (v >= 3.0e38 / 3000) | (fit >= 3.0e38 / 3000)
     assert np.array_equal(fit[sentinel], v[sentinel])
     assert (np.abs(fit - v) <= 1e-5 * np.abs(v))[~sentinel].all()
+
+
+def test_rows_per_lane_needs_small_programs():
+    """Two rows per lane hold programs of at most 5 stack and 5 adjoint slots: a complete
+    binary tree of depth 7 needs 6 stack slots (a node whose operands are both leaves needs
+    none), so a request for two rows fails with
+    PPI_E_UNSUPPORTED while the measured choice falls back to one row per lane, with the
+    same gradients as an explicit one-row run."""
+    b = Builder(4)
+
+    def tree(d, k):
+        if d == 0:
+            return [1.0 + 0.01 * k] if k % 3 == 0 else [f"x{k % 4}"]
+        return ["+"] + tree(d - 1, 2 * k) + ["*"] + tree(d - 1, 2 * k + 1) + [0.5]
+
+    progs = [tree(7, 0), ["*", 1.5, "x0"]]
+    pop = b.population(progs)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 9, 2000, 4)
+    a = ppiamd.Adjoint(0)
+    a.configure(b.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_SQUARED, protected=True)
+    a.set_dataset(data)
+    a.forward(pop)
+    a.set_rows_per_lane(2)
+    with pytest.raises(ppiamd.PpiError):
+        a.backward(partials=ppiamd.PARTIALS_STORE)
+    a.set_rows_per_lane(0)
+    g0 = a.backward()
+    st = a.stats()
+    assert st.rows_per_lane == 1 and st.tune_ms[2] == -2.0 and st.tune_ms[3] == -2.0
+    a.set_rows_per_lane(1)
+    g1 = a.backward(partials=st.partials)
+    assert np.array_equal(g0, g1, equal_nan=True)
+    assert (g0.reshape(pop.n, pop.stride) != 0).any(axis=1).all()
+    a.close()
 
 
 def test_gradient_against_finite_differences():
