@@ -86,7 +86,7 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
     st = eng.stats()
     return {"ms_per_step": ms, "wall_s": wall, "main_kernel_ms": st.main_kernel_ms,
             "node_evals": st.node_evals, "executed_nodes": st.executed_nodes,
-            "rows_per_lane": st.rows_per_lane, "programs_per_tile": st.programs_per_tile,
+            "rows_per_lane": st.rows_per_lane, "feature_slots": st.feature_slots, "programs_per_tile": st.programs_per_tile,
             "form": st.form, "derived_columns": st0.derived_columns, "folded_subtrees": st0.folded_subtrees,
             "folded_nodes": st0.folded_nodes, "upload_ms": round(upload_s * 1e3, 2)}
 
@@ -148,7 +148,7 @@ def bench_batched(steps, warmup):
         plain_ms += eng.time(1)
     eng.close()
     return {"fp64_error": errs, "node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
-            "rows_per_lane": st.rows_per_lane, "items": st.n_items, "grid": st.grid,
+            "rows_per_lane": st.rows_per_lane, "feature_slots": st.feature_slots, "items": st.n_items, "grid": st.grid,
             "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs,
             "programs": int(sum(p.n for p in w.pops))}
 
@@ -168,7 +168,7 @@ def bench_adjoint_pass(steps, warmup):
     st = a.stats()
     a.close()
     return {"ms_per_backward": round(ms, 3), "grads_per_s": round(w.pop.n / (ms * 1e-3), 1),
-            "partials": st.partials, "rows_per_lane": st.rows_per_lane,
+            "partials": st.partials, "rows_per_lane": st.rows_per_lane, "feature_slots": st.feature_slots,
             "tune_ms": [round(x, 3) for x in st.tune_ms], "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins}
 
 
@@ -210,7 +210,7 @@ def bench_adjoint(steps, warmup, threads):
     return {"fp64_error": err, "ms": ms, "kernel_ms": st.kernel_ms, "n": w.pop.n, "node_evals": st.node_evals,
             "fwd_ins": st.forward_ins, "rev_ins": st.reverse_ins, "partials": st.partials,
             "tune_ms": [round(x, 3) for x in st.tune_ms], "consts": st.n_consts, "max_segments": st.max_segments,
-            "rows_per_lane": st.rows_per_lane,
+            "rows_per_lane": st.rows_per_lane, "feature_slots": st.feature_slots,
             "descend_ms": dst.descend_ms, "descend_rows": sub, "improved": float((rel > 0).mean()),
             "median_rel_gain": float(np.median(rel)), "worse": int((f1[finite] > f0[finite]).sum()),
             "reverted": dst.descend_reverted}
@@ -569,7 +569,10 @@ def main():
             "fwd_plus_rev_ins_per_s": round((adjoint["fwd_ins"] + adjoint["rev_ins"]) / (ms * 1e-3) / 1e9, 3),
             "partials": {1: "recompute (registers)", 2: "store (LDS)"}.get(adjoint["partials"], "?"),
             "rows_per_lane": adjoint["rows_per_lane"],
-            "autotune_ms_r1_recompute_r1_store_r2_recompute_r2_store": adjoint["tune_ms"],
+            "feature_slots": adjoint["feature_slots"],
+            "autotune_ms": {"order": "R1 recompute (F12, F0), R1 store (F12, F0), R2 recompute (F12, F0), "
+                                     "R2 store (F12, F0); -2 = shape does not hold the programs",
+                            "ms": adjoint["tune_ms"]},
             "constants": adjoint["consts"], "max_segments": adjoint["max_segments"],
             "descend": {"steps": 20, "rows": adjoint["descend_rows"], "ms": round(adjoint["descend_ms"], 2),
                         "programs_improved": round(adjoint["improved"], 4),

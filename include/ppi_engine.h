@@ -293,8 +293,10 @@ typedef struct ppi_adjoint_stats {
     int32_t ranges;          /* row ranges per job */
     int32_t descend_steps;
     int32_t descend_reverted; /* programs whose refined constants did not lower the plain fitness (kept) */
-    double tune_ms[4];       /* sampled ms per candidate (R - 1) * 2 + (STORE ? 1 : 0), R rows per lane;
-                                0 = not sampled, -2 = no kernel shape holds the programs at that R */
+    int32_t feature_slots;   /* 12: a row's first 12 columns held in registers; 0: read from the dataset */
+    int32_t pad_;
+    double tune_ms[8];       /* sampled ms per candidate ((R - 1) * 2 + (STORE ? 1 : 0)) * 2 + (feature_slots == 0),
+                                R rows per lane; 0 = not sampled, -2 = no kernel shape holds the programs so */
 } ppi_adjoint_stats;
 
 ppi_status ppi_adjoint_create(ppi_adjoint** out, int32_t device);

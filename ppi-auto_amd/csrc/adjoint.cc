@@ -98,9 +98,10 @@ struct ppi_adjoint {
     // compiled adjoint tape for partial store `tape_ps` at `tape_r` rows per lane (-1: none)
     int tape_ps = -1, tape_r = 0;
     // rows per lane requested (0: chosen by measurement); sampled ms per candidate
-    // (rows per lane - 1) * 2 + store, -1 = not sampled, -2 = no kernel shape for these programs
+    // ((rows per lane - 1) * 2 + store) * 2 + (features read from the dataset),
+    // -1 = not sampled, -2 = no kernel shape for these programs
     int want_r = 0;
-    double cand_ms[4] = {-1.0, -1.0, -1.0, -1.0};
+    double cand_ms[8] = {-1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0};
     AdjTape T;
     int R = 1, F = 12, S = 12, AS = kAdjSlots;
     int32_t nranges = 1;
@@ -118,7 +119,7 @@ struct ppi_adjoint {
     DBuf<AdjConst> dConsts;
     DBuf<uint32_t> dProgConsts;
     DBuf<float> dGpart, dFpart, dFit, dGraw, dGscale, dGout, dCur, dCbest, dGbest, dFbest, dLam;
-    DBuf<uint32_t> dFbad, dBad;
+    DBuf<uint32_t> dFbad, dBad, dQueue;
 
     ppi_adjoint_stats stats{};
 };
@@ -160,6 +161,7 @@ AdjArgs make_args(ppi_adjoint* a) {
     A.gpart = a->dGpart.p;
     A.fpart = a->dFpart.p;
     A.fbad = a->dFbad.p;
+    A.queue = a->dQueue.p;
     return A;
 }
 
@@ -191,26 +193,23 @@ void forget_tuning(ppi_adjoint* a) {
 }
 
 // Compile the kept programs for partial store `ps` at `rows` rows per lane and
-// upload the tape.  PPI_E_UNSUPPORTED: no kernel shape holds these programs at
-// that many rows per lane (their stack or adjoint slots, see kern_adjoint*.hip).
-ppi_status prepare(ppi_adjoint* a, int ps, int rows) {
+// upload the tape (skipped when that tape is current).
+ppi_status compile_tape(ppi_adjoint* a, int ps, int rows) {
     if (a->tape_ps == ps && a->tape_r == rows) return PPI_OK;
     const double t0 = now_ms();
     std::string err;
+    a->tape_ps = -1;  // the device tape is replaced below
+    a->tape_r = 0;
     const int32_t B = adjoint_store_partials(ps, rows);
     if (!compile_adjoint(a->eng->symtab, a->ph.data(), a->eph.data(), a->sz.data(), a->stride, a->n, a->eng->ncol, B,
                          &a->T, &err)) {
         set_error(err);
-        a->tape_ps = -1;
-        a->tape_r = 0;
         return PPI_E_PROGRAM;
     }
     AdjTape& T = a->T;
     for (const AdjConst& c : T.consts)
         if (c.ins >= T.fwd.code.size()) {
             set_error("program " + std::to_string(c.prog) + ": constant at " + std::to_string(c.pos) + " has no tape slot");
-            a->tape_ps = -1;
-            a->tape_r = 0;
             return PPI_E_PROGRAM;
         }
     T.fwd.code.insert(T.fwd.code.end(), kTapePad, Ins{H_END, kEmpty});
@@ -218,48 +217,11 @@ ppi_status prepare(ppi_adjoint* a, int ps, int rows) {
     // first and store entries up to 63 past its first.
     T.rev.insert(T.rev.end(), 128, RevIns{0u, 0u});
     T.stores.insert(T.stores.end(), 64, AdjStore{0u, 0u});
-    a->tape_ps = -1;  // the device tape is replaced below
-    a->tape_r = 0;
-    if (rows == 2) {
-        // One register tuple of 10 per stack: 5 slots per row.
-        a->S = 5;
-        a->AS = 5;
-        a->F = T.fwd.stats.max_col < 12 ? 12 : 0;
-        if (T.fwd.stats.max_slots > 5 || T.max_adj_slots > 5) {
-            set_error("programs need more than 5 stack or adjoint slots: one row per lane");
-            return PPI_E_UNSUPPORTED;
-        }
-    } else {
-        a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
-        a->AS = kAdjSlots;
-        a->F = (T.fwd.stats.max_col < 12 && a->S == 12) ? 12 : 0;
-    }
-    a->R = rows;
-    if (!adjoint_shape_supported(a->R, a->F, a->S, a->AS, ps)) {
-        set_error("no adjoint kernel shape for these programs");
-        return PPI_E_UNSUPPORTED;
-    }
     const size_t nj = T.jobs.size();
     a->order.resize(nj);
     std::iota(a->order.begin(), a->order.end(), 0);
     std::stable_sort(a->order.begin(), a->order.end(),
                      [&](int32_t x, int32_t y) { return T.jobs[(size_t)x].cost > T.jobs[(size_t)y].cost; });
-    // Row ranges: enough units to fill every wave slot several times over.
-    const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
-    const int occ = std::max(1, adjoint_occupancy(a->R, a->F, a->S, a->AS, ps, P, N));
-    const int64_t slots = (int64_t)a->dev_cu * 4 * occ;
-    const int64_t nlin = a->eng->nlin;
-    const int64_t tile = 64 * a->R;
-    const int64_t max_ranges = std::max<int64_t>(1, nlin / (tile * 8));
-    int64_t want = (8 * slots + (int64_t)nj - 1) / std::max<int64_t>(1, (int64_t)nj);
-    want = std::max<int64_t>(1, std::min(want, max_ranges));
-    int64_t rr = (nlin + want - 1) / want;
-    rr = (rr + tile - 1) / tile * tile;
-    a->range_rows = rr;
-    a->nranges = (int32_t)((nlin + rr - 1) / rr);
-    const int64_t units = (int64_t)nj * a->nranges;
-    a->grid = (int)((units + 3) / 4);
-
     ppi_status s;
     hipStream_t st = a->stream;
     if ((s = upload(a->dJobs, T.jobs, st)) || (s = upload(a->dSegs, T.segs, st)) || (s = upload(a->dRev, T.rev, st)) ||
@@ -268,15 +230,14 @@ ppi_status prepare(ppi_adjoint* a, int ps, int rows) {
         (s = upload(a->dConsts, T.consts, st)) || (s = upload(a->dProgConsts, T.prog_consts, st)))
         return s;
     const size_t nc = T.consts.size();
-    if ((s = a->dGpart.reserve((size_t)a->nranges * nc)) || (s = a->dFpart.reserve((size_t)a->nranges * a->n)) ||
-        (s = a->dFbad.reserve((size_t)a->nranges * a->n)) || (s = a->dFit.reserve((size_t)a->n)) ||
-        (s = a->dBad.reserve((size_t)a->n)) || (s = a->dGraw.reserve(nc)) || (s = a->dCur.reserve(nc)) ||
-        (s = a->dCbest.reserve(nc)) || (s = a->dGbest.reserve(nc)) || (s = a->dFbest.reserve((size_t)a->n)) ||
-        (s = a->dLam.reserve((size_t)a->n)))
+    if ((s = a->dFit.reserve((size_t)a->n)) || (s = a->dBad.reserve((size_t)a->n)) || (s = a->dGraw.reserve(nc)) ||
+        (s = a->dCur.reserve(nc)) || (s = a->dCbest.reserve(nc)) || (s = a->dGbest.reserve(nc)) ||
+        (s = a->dFbest.reserve((size_t)a->n)) || (s = a->dLam.reserve((size_t)a->n)))
         return s;
     A_HIP(hipStreamSynchronize(st));
     a->tape_ps = ps;
     a->tape_r = rows;
+    const int64_t nlin = a->eng->nlin;
     double fwd = 0.0, rev = 0.0;
     for (const AdjJob& j : T.jobs) {
         if (j.flags & kEmpty) continue;
@@ -291,34 +252,108 @@ ppi_status prepare(ppi_adjoint* a, int ps, int rows) {
     a->stats.n_jobs = (int32_t)nj;
     a->stats.n_consts = (int32_t)nc;
     a->stats.max_segments = T.max_seg;
-    a->stats.rows_per_lane = a->R;
-    a->stats.ranges = a->nranges;
     a->stats.node_evals = (double)T.fwd.stats.total_size * (double)nlin;
     a->stats.compile_ms = now_ms() - t0;
     return PPI_OK;
 }
 
+// Tape for (ps, rows) plus the kernel shape and launch plan with features in
+// VGPRs (feat 12), read from the dataset (feat 0), or the default (-1: in VGPRs
+// when the programs read only the first 12 columns).  PPI_E_UNSUPPORTED: no
+// kernel shape holds these programs that way (their stack or adjoint slots, or
+// the columns they read; see kern_adjoint*.hip).
+ppi_status prepare(ppi_adjoint* a, int ps, int rows, int feat = -1) {
+    ppi_status s = compile_tape(a, ps, rows);
+    if (s) return s;
+    const AdjTape& T = a->T;
+    const bool cols_fit = T.fwd.stats.max_col < 12;
+    if (rows == 2) {
+        // One register tuple of 10 per stack: 5 slots per row.
+        a->S = 5;
+        a->AS = 5;
+        if (T.fwd.stats.max_slots > 5 || T.max_adj_slots > 5) {
+            set_error("programs need more than 5 stack or adjoint slots: one row per lane");
+            return PPI_E_UNSUPPORTED;
+        }
+    } else {
+        a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
+        a->AS = kAdjSlots;
+    }
+    const bool vgpr_ok = cols_fit && (rows == 2 || a->S == 12);
+    if (feat == 12 && !vgpr_ok) {
+        set_error("the programs read columns beyond the first 12 (or need 16 stack slots)");
+        return PPI_E_UNSUPPORTED;
+    }
+    a->F = feat == 0 ? 0 : (vgpr_ok ? 12 : 0);
+    a->R = rows;
+    if (!adjoint_shape_supported(a->R, a->F, a->S, a->AS, ps)) {
+        set_error("no adjoint kernel shape for these programs");
+        return PPI_E_UNSUPPORTED;
+    }
+    // Persistent grid: the workgroups resident at once.  Row ranges: units
+    // outnumber the wave slots 32 times over (at least 8 row tiles per range), so
+    // the launch's tail, one unit long, is short.
+    const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
+    const int occ = std::max(1, adjoint_occupancy(a->R, a->F, a->S, a->AS, ps, P, N));
+    a->grid = a->dev_cu * occ;
+    const int64_t slots = (int64_t)a->grid * 4;
+    const int64_t nlin = a->eng->nlin;
+    const int64_t nj = (int64_t)T.jobs.size();
+    const int64_t tile = 64 * a->R;
+    const int64_t max_ranges = std::max<int64_t>(1, nlin / (tile * 8));
+    int64_t want = (32 * slots + nj - 1) / std::max<int64_t>(1, nj);
+    want = std::max<int64_t>(1, std::min(want, max_ranges));
+    int64_t rr = (nlin + want - 1) / want;
+    rr = (rr + tile - 1) / tile * tile;
+    a->range_rows = rr;
+    a->nranges = (int32_t)((nlin + rr - 1) / rr);
+    if ((uint64_t)nj * (uint64_t)a->nranges + (uint64_t)slots >= 0xffffffffull) {
+        set_error("too many adjoint units");
+        return PPI_E_ARG;
+    }
+    const size_t nc = T.consts.size();
+    if ((s = a->dGpart.reserve((size_t)a->nranges * nc)) || (s = a->dFpart.reserve((size_t)a->nranges * a->n)) ||
+        (s = a->dFbad.reserve((size_t)a->nranges * a->n)) || (s = a->dQueue.reserve(1)))
+        return s;
+    a->stats.rows_per_lane = a->R;
+    a->stats.feature_slots = a->F;
+    a->stats.ranges = a->nranges;
+    return PPI_OK;
+}
+
 ppi_status enqueue_adjoint(ppi_adjoint* a, const AdjArgs& A, int grid) {
     const bool P = a->cfg.protected_ops != 0, N = a->cfg.native_math != 0;
+    A_HIP(launch_fill_u32(a->dQueue.p, 0u, 1, a->stream));
     A_HIP(launch_adjoint(A, grid, a->R, a->F, a->S, a->AS, a->tape_ps, P, N, a->stream));
     return PPI_OK;
 }
 
-// Mean device ms of the adjoint kernel over an evenly spaced sample of the units
-// (one range of at most 4096 rows).
+// Device ms of the adjoint kernel over a sample of 64 units per CU: an evenly
+// spaced subset of the jobs (heaviest first) times ranges of at most 4096 rows
+// spread over the whole dataset (up to 256 ranges).  That fills every wave slot of
+// any shape twice over, so shapes of different occupancy compare as in the full
+// launch, and the waves in flight stream rows from all over the dataset as the full
+// launch's long ranges do, rather than rereading a few cached ones.
 ppi_status sample_ms(ppi_adjoint* a, double* ms) {
     const size_t nj = a->T.jobs.size();
-    const size_t want = std::min<size_t>(nj, (size_t)a->dev_cu * 8);
+    const int64_t rr = std::min<int64_t>(a->range_rows, 4096);
+    const int32_t nr = (int32_t)std::max<int64_t>(1, std::min<int64_t>(256, a->eng->nlin / rr));
+    const size_t want = std::max<size_t>(1, std::min<size_t>(nj, (size_t)a->dev_cu * 64 / (size_t)nr));
     std::vector<int32_t> pick(want);
     for (size_t k = 0; k < want; ++k) pick[k] = a->order[k * nj / want];
-    ppi_status s = upload(a->dSample, pick, a->stream);
-    if (s) return s;
+    // The per-range sums are indexed by range: hold nr ranges (the full launch may use fewer).
+    const size_t nc = a->T.consts.size();
+    ppi_status s;
+    if ((s = a->dGpart.reserve((size_t)nr * nc)) || (s = a->dFpart.reserve((size_t)nr * a->n)) ||
+        (s = a->dFbad.reserve((size_t)nr * a->n)))
+        return s;
+    if ((s = upload(a->dSample, pick, a->stream))) return s;
     AdjArgs A = make_args(a);
     A.order = a->dSample.p;
     A.njobs = (int32_t)want;
-    A.nranges = 1;
-    A.range_rows = std::min<int64_t>(a->range_rows, 4096);
-    const int grid = (int)((want + 3) / 4);
+    A.nranges = nr;
+    A.range_rows = rr;
+    const int grid = (int)std::min<int64_t>(a->grid, ((int64_t)want * nr + 3) / 4);
     float best = 0.0f;
     for (int rep = 0; rep < 3; ++rep) {
         A_HIP(hipEventRecord(a->ev[2], a->stream));
@@ -334,55 +369,51 @@ ppi_status sample_ms(ppi_adjoint* a, double* ms) {
 }
 
 // Resolve PPI_PARTIALS_* and the rows per lane to a kernel: every candidate
-// (store: both for AUTO; rows per lane: 1 and 2 unless one was requested) whose
-// shape holds the programs is compiled and timed on a sample of the units, and
-// the fastest is prepared.  Timings are kept until the programs or data change.
+// (store: both for AUTO; rows per lane: 1 and 2 unless one was requested;
+// features in VGPRs or read from the dataset) whose shape holds the programs is
+// timed on a sample of the units, and the fastest is prepared.  Timings are kept
+// until the programs or data change.
 ppi_status choose(ppi_adjoint* a, int32_t partials, int* ps_out) {
     int ps_lo = 0, ps_hi = 1;
     if (partials == PPI_PARTIALS_RECOMPUTE) ps_lo = ps_hi = 0;
     else if (partials == PPI_PARTIALS_STORE) ps_lo = ps_hi = 1;
     else if (partials != PPI_PARTIALS_AUTO) { set_error("bad partials mode"); return PPI_E_ARG; }
     const int r_lo = a->want_r ? a->want_r : 1, r_hi = a->want_r ? a->want_r : 2;
-    if (ps_lo == ps_hi && r_lo == r_hi) {  // one candidate: nothing to time
-        ppi_status s = prepare(a, ps_lo, r_lo);
-        if (s) return s;
-        *ps_out = ps_lo;
-        a->stats.partials = ps_lo ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
-        return PPI_OK;
-    }
     int best = -1;
     double best_ms = 0.0;
     for (int r = r_lo; r <= r_hi; ++r)
-        for (int ps = ps_lo; ps <= ps_hi; ++ps) {
-            const int c = (r - 1) * 2 + ps;
-            if (a->cand_ms[c] == -1.0) {
-                ppi_status s = prepare(a, ps, r);
-                if (s == PPI_E_UNSUPPORTED) {
-                    a->cand_ms[c] = -2.0;
-                    a->stats.tune_ms[c] = -2.0;
-                    continue;
+        for (int ps = ps_lo; ps <= ps_hi; ++ps)
+            for (int fi = 0; fi < 2; ++fi) {
+                const int c = ((r - 1) * 2 + ps) * 2 + fi;
+                if (a->cand_ms[c] == -1.0) {
+                    ppi_status s = prepare(a, ps, r, fi ? 0 : 12);
+                    if (s == PPI_E_UNSUPPORTED) {
+                        a->cand_ms[c] = -2.0;
+                        a->stats.tune_ms[c] = -2.0;
+                        continue;
+                    }
+                    if (s) return s;
+                    double t = 0.0;
+                    if ((s = sample_ms(a, &t))) return s;
+                    a->cand_ms[c] = t;
+                    a->stats.tune_ms[c] = t;
                 }
-                if (s) return s;
-                double t = 0.0;
-                if ((s = sample_ms(a, &t))) return s;
-                a->cand_ms[c] = t;
-                a->stats.tune_ms[c] = t;
+                if (a->cand_ms[c] < 0.0) continue;
+                if (best < 0 || a->cand_ms[c] < best_ms) {
+                    best = c;
+                    best_ms = a->cand_ms[c];
+                }
             }
-            if (a->cand_ms[c] < 0.0) continue;
-            if (best < 0 || a->cand_ms[c] < best_ms) {
-                best = c;
-                best_ms = a->cand_ms[c];
-            }
-        }
     if (best < 0) {
         set_error(a->want_r == 2 ? "programs need more than 5 stack or adjoint slots: one row per lane"
                                  : "no adjoint kernel shape for these programs");
         return PPI_E_UNSUPPORTED;
     }
-    *ps_out = best & 1;
-    ppi_status s = prepare(a, best & 1, best / 2 + 1);
+    const int ps = (best >> 1) & 1, rows = best / 4 + 1, feat = (best & 1) ? 0 : 12;
+    ppi_status s = prepare(a, ps, rows, feat);
     if (s) return s;
-    a->stats.partials = (best & 1) ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
+    *ps_out = ps;
+    a->stats.partials = ps ? PPI_PARTIALS_STORE : PPI_PARTIALS_RECOMPUTE;
     return PPI_OK;
 }
 
@@ -437,7 +468,7 @@ void ppi_adjoint_destroy(ppi_adjoint* a) {
     a->dOrder.release(); a->dSample.release(); a->dMeta.release(); a->dConsts.release(); a->dProgConsts.release();
     a->dGpart.release(); a->dFpart.release(); a->dFit.release(); a->dGraw.release(); a->dGscale.release();
     a->dGout.release(); a->dCur.release(); a->dCbest.release(); a->dGbest.release(); a->dFbest.release();
-    a->dLam.release(); a->dFbad.release(); a->dBad.release();
+    a->dLam.release(); a->dFbad.release(); a->dBad.release(); a->dQueue.release();
     for (auto& ev : a->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (a->stream) (void)hipStreamDestroy(a->stream);

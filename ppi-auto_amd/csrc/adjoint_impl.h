@@ -141,7 +141,8 @@ __device__ __forceinline__ AdjJob load_job(const AdjJob* jobs, int i) {
     j.nconst = q->nconst;
     j.flags = q->flags;
     j.cost = 0;
-    j.pad_[0] = j.pad_[1] = j.pad_[2] = 0;
+    j.cols = q->cols;
+    j.pad_[0] = j.pad_[1] = 0;
     return j;
 }
 
@@ -299,8 +300,10 @@ struct AdjFeat {
 // R rows per lane, F features in VGPRs (0: read from the dataset), S stack
 // slots, AS adjoint slots (both register stacks: RegStack's layout), PS
 // partial store (0: VGPRs, 1: LDS).
+// One (job, row range) unit: unit u is range u / njobs of job order[u % njobs].
+// Called by a whole wave; pl is the lane's column of the wave's LDS partials.
 template <int R, int F, int S, int AS, int PS, bool P, bool N>
-__global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
+__device__ __forceinline__ void adjoint_unit(const AdjArgs& A, int64_t u, float* pl, int lane) {
     typedef AdjCore<R, S, PS, P, N> AC;
     typedef typename AC::Stk Stk;
     constexpr int B = AC::B;
@@ -308,15 +311,6 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     static_assert(AS <= kAdjSlots, "adjoint slots");
     typedef RegStack<R, AS> VA;
     typedef typename VecT<kAdjMaxConsts>::type VG;
-    constexpr int LDS_WAVE = PS == 1 ? B * R * 64 : 1;
-    __shared__ float lds[kWaves][LDS_WAVE];
-    stage_math<N>();
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63;
-    const int wave = wave_index();
-    const int64_t u = (int64_t)blockIdx.x * kWaves + wave;
-    if (u >= (int64_t)A.njobs * A.nranges) return;  // whole wave: no block-level synchronisation below
     const int rng = (int)(u / A.njobs);
     const int ji = as_const(A.order)[u - (int64_t)rng * A.njobs];
     const AdjJob J = load_job(A.jobs, ji);
@@ -324,7 +318,8 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
     const int64_t rend = min((int64_t)A.nlin, rbeg + A.range_rows);
 
     typename std::conditional<PS == 1, PStoreL<R>, PStoreV<R, B>>::type ps;
-    if constexpr (PS == 1) ps.base = &lds[wave][lane];
+    if constexpr (PS == 1) ps.base = pl;
+    else (void)pl;
 
     VG g = 0.0f;
     float part = 0.0f;
@@ -349,10 +344,12 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
 
         typedef typename AdjFeat<R, F>::type FeatT;
         typedef typename VgprFeat<R, (F > 0 ? F : 1)>::VF VF;
-        // Row block features and targets.  (Loading them one block ahead cost
+        // Row block features and targets: only the columns the program reads
+        // (a wave-uniform test per column).  (Loading them one block ahead cost
         // 13 VGPRs and a wave per SIMD, and measured slower.)
         VF xf[R];
         float y[R];
+        const uint32_t cols = J.cols;
         auto load_rows = [&](int64_t r0, VF (&xv)[R], float (&yv)[R]) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -360,7 +357,7 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
                 yv[r] = A.X[(int64_t)A.ycol * A.ldx + rw];
                 if constexpr (F > 0) {
 #pragma unroll
-                    for (int f = 0; f < F; ++f) xv[r][f] = f < A.ncol ? A.X[(int64_t)f * A.ldx + rw] : 0.0f;
+                    for (int f = 0; f < F; ++f) xv[r][f] = (cols >> f) & 1u ? A.X[(int64_t)f * A.ldx + rw] : 0.0f;
                 }
             }
         };
@@ -480,6 +477,36 @@ __global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
             A.fpart[(int64_t)rng * A.nprog + J.prog] = part;
             A.fbad[(int64_t)rng * A.nprog + J.prog] = any_bad ? 1u : 0u;
         }
+    }
+}
+
+// Persistent form: the grid holds as many workgroups as are resident at once and
+// every wave takes units from the queue counter (zeroed before the launch) until
+// the njobs * nranges units are used up, so a wave that finishes early does not
+// idle until the rest of its workgroup is done, and the launch's tail is one
+// unit long.  Units are taken in order: heaviest job first within each range.
+// Every wave leaves when the counter passes the end; nothing synchronises the
+// block after the table staging.
+template <int R, int F, int S, int AS, int PS, bool P, bool N>
+__global__ void __launch_bounds__(kBlock, 3) adjoint_kernel(AdjArgs A) {
+    constexpr int B = StoreCfg<PS, R>::B;
+    constexpr int LDS_WAVE = PS == 1 ? B * R * 64 : 1;
+    __shared__ float lds[kWaves][LDS_WAVE];
+    stage_math<N>();
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = wave_index();
+    const uint64_t total = (uint64_t)A.njobs * (uint64_t)A.nranges;
+    for (;;) {
+        // Every lane takes part in the atomic (lane 0 adds 1, the others 0) and lane
+        // 0's old value is broadcast.  No branch on the lane here: with an
+        // `if (lane == 0)` around the atomic, the optimiser threaded the epilogue's
+        // own lane-0 test into it and sent lanes 1-63 to the broadcast without the
+        // atomic, so they re-ran unit 0 without end.
+        const uint32_t t = atomicAdd(A.queue, lane == 0 ? 1u : 0u);
+        const uint64_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+        if (u >= total) break;
+        adjoint_unit<R, F, S, AS, PS, P, N>(A, (int64_t)u, &lds[wave][lane], lane);
     }
 }
 

@@ -82,7 +82,7 @@ struct BatchArgs {
     uint32_t* counter;      // work-queue head, reset to 0 before every launch
 };
 
-// Reverse-mode adjoint: one wave = one (job, row range) unit.
+// Reverse-mode adjoint: waves take (job, row range) units from a queue counter.
 struct AdjArgs {
     const AdjJob* jobs;
     const AdjSeg* segs;
@@ -102,6 +102,7 @@ struct AdjArgs {
     float* gpart;          // [range][nconst]: sum over the range's rows of dEF/dpred * dpred/dconst
     float* fpart;          // [range][nprog]: sum of finite errors
     uint32_t* fbad;        // [range][nprog]: 1 if a row's error is not finite
+    uint32_t* queue;       // next unit (zeroed before each launch); njobs * nranges < 2^32 - grid waves
 };
 
 // Per-program and per-constant epilogue of the adjoint.

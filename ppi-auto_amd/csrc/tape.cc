@@ -808,6 +808,11 @@ struct AdjBuilder {
             out->segs.push_back(sg);
         }
         J.cost = m.ncode * (uint32_t)nseg + (uint32_t)(out->rev.size() - rev0);
+        J.cols = 0;
+        for (uint32_t q = m.code_off; q < m.code_off + m.ncode; ++q) {
+            const uint32_t h = t.code[q].code & 0xffu, col = t.code[q].arg;
+            if ((h == H_LOAD_A || h == H_PUSH_A || (h >= H_BIN_A && h < H_BIN_S)) && col < 32) J.cols |= 1u << col;
+        }
         for (int k = 0; k < ng; ++k) {
             AdjConst ac{};
             ac.prog = prog;

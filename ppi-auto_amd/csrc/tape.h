@@ -285,7 +285,8 @@ struct AdjJob {
     uint32_t nconst;
     uint32_t flags;       // kEmpty / kInvalid of the program, kAdjFitness
     uint32_t cost;        // ncode * nseg + reverse instructions (scheduling weight)
-    uint32_t pad_[3];
+    uint32_t cols;        // bit c: the program reads input column c (c < 32)
+    uint32_t pad_[2];
 };
 
 struct AdjSeg {

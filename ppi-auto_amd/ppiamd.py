@@ -86,7 +86,7 @@ class AdjointStats(C.Structure):
         ("reverse_ins", C.c_double), ("n_jobs", C.c_int32), ("n_consts", C.c_int32),
         ("max_segments", C.c_int32), ("partials", C.c_int32), ("rows_per_lane", C.c_int32),
         ("ranges", C.c_int32), ("descend_steps", C.c_int32), ("descend_reverted", C.c_int32),
-        ("tune_ms", C.c_double * 4),
+        ("feature_slots", C.c_int32), ("pad_", C.c_int32), ("tune_ms", C.c_double * 8),
     ]
 
 
