@@ -4,12 +4,14 @@
 //
 // forward() evaluates with an internal engine (the plain drop-in path) and
 // keeps the programs.  backward() compiles them into the adjoint tape (forward
-// tape + per-instruction partial records + reverse program, tape.h), then
-// launches one wave per (job, row range) unit.  Units are ordered heaviest job
-// first within each range; the number of ranges is chosen so that the units
-// outnumber the device's wave slots several times over.  The partial store —
-// VGPRs with forward recomputation per segment, or LDS with one forward pass
-// for most programs — is picked by timing both on a sample of the units.
+// tape + per-segment store lists + reverse program, tape.h), then launches a
+// persistent grid whose waves take (job, row range) units from a queue counter,
+// heaviest job first within each range; the ranges are cut so that the units
+// outnumber the device's wave slots 32 times over.  The partial store (VGPRs with
+// forward recomputation per segment, or LDS with one forward pass for most
+// programs), the rows per lane (1, or 2 for programs of at most 5 stack and
+// adjoint slots) and, at one row per lane, whether a row's features sit in
+// registers are picked by timing the candidates on a sample of the units.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -279,9 +281,9 @@ ppi_status prepare(ppi_adjoint* a, int ps, int rows, int feat = -1) {
         a->S = T.fwd.stats.max_slots <= 12 ? 12 : 16;
         a->AS = kAdjSlots;
     }
-    const bool vgpr_ok = cols_fit && (rows == 2 || a->S == 12);
+    const bool vgpr_ok = cols_fit && rows == 1 && a->S == 12;  // two rows per lane: dataset reads only
     if (feat == 12 && !vgpr_ok) {
-        set_error("the programs read columns beyond the first 12 (or need 16 stack slots)");
+        set_error("features in registers: one row per lane, programs reading the first 12 columns, 12 stack slots");
         return PPI_E_UNSUPPORTED;
     }
     a->F = feat == 0 ? 0 : (vgpr_ok ? 12 : 0);
@@ -328,17 +330,18 @@ ppi_status enqueue_adjoint(ppi_adjoint* a, const AdjArgs& A, int grid) {
     return PPI_OK;
 }
 
-// Device ms of the adjoint kernel over a sample of 64 units per CU: an evenly
-// spaced subset of the jobs (heaviest first) times ranges of at most 4096 rows
-// spread over the whole dataset (up to 256 ranges).  That fills every wave slot of
-// any shape twice over, so shapes of different occupancy compare as in the full
-// launch, and the waves in flight stream rows from all over the dataset as the full
-// launch's long ranges do, rather than rereading a few cached ones.
+// Device ms of the adjoint kernel over a sample: an evenly spaced subset of the
+// jobs (heaviest first) times ranges of 1024 rows spread over the whole dataset
+// (up to 256 ranges), 32 units per wave slot of the persistent grid.  Short units,
+// many per wave, keep the sample's tail (one unit) small, so shapes of different
+// occupancy compare as in the full launch; the waves in flight stream rows from all
+// over the dataset as the full launch's long ranges do.
 ppi_status sample_ms(ppi_adjoint* a, double* ms) {
     const size_t nj = a->T.jobs.size();
-    const int64_t rr = std::min<int64_t>(a->range_rows, 4096);
+    const int64_t rr = std::min<int64_t>(a->range_rows, 1024);
     const int32_t nr = (int32_t)std::max<int64_t>(1, std::min<int64_t>(256, a->eng->nlin / rr));
-    const size_t want = std::max<size_t>(1, std::min<size_t>(nj, (size_t)a->dev_cu * 64 / (size_t)nr));
+    const size_t units = (size_t)a->grid * 4 * 32;
+    const size_t want = std::max<size_t>(1, std::min<size_t>(nj, units / (size_t)nr));
     std::vector<int32_t> pick(want);
     for (size_t k = 0; k < want; ++k) pick[k] = a->order[k * nj / want];
     // The per-range sums are indexed by range: hold nr ranges (the full launch may use fewer).

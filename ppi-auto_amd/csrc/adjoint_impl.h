@@ -1,24 +1,30 @@
 // adjoint_impl.h — reverse-mode adjoint kernels for gfx950: d(fitness)/d(T_CONST).
 //
 // fitness = sum_rows EF(pred, y) / nlin + alpha*size (sequential.cc:170-189), so
-// d fitness / d c = sum_rows EF'(pred, y) * d pred / d c / nlin.  One wave owns
-// one (job, row range) unit and walks the range 64*R rows at a time:
+// d fitness / d c = sum_rows EF'(pred, y) * d pred / d c / nlin.  Waves of a
+// persistent grid take (job, row range) units from a queue counter; a wave walks
+// its unit's range 64*R rows at a time (R = 1 or 2 rows per lane):
 //
 //   for each segment s of the job:
 //       forward pass over the program's accumulator tape (up to the last
-//       instruction that records a partial of segment s), storing those partials;
+//       instruction that stores a partial of segment s), plain steps between
+//       the segment's stores, which also write their partials;
 //       s == 0: seed the root adjoint with EF'(pred, y) (0 on rows past nlin)
 //               and add the row's error to the fitness sum;
 //       the segment's reverse instructions: adj[dst] = adj[src] * partial, or
 //       g[k] += adj[src] * partial for a constant k (a zero adjoint gives 0
 //       whatever the partial).
 //
-// The partial store is either VGPRs (16 per row: long programs re-run the
-// forward pass per segment — recomputation) or the wave's own LDS block (32 per
-// row: most programs need one forward pass — storage).  Adjoint slots and the
-// per-lane gradient accumulators live in VGPRs; wave-uniform slot indices use the
-// backend's indexed register moves.  Sums over a range are wave-reduced once per
-// unit; ranges are summed in a fixed order by adj_grad_kernel.
+// Tape, store list and reverse tape are fetched through 64-entry windows held
+// across the wave's lanes (v_readlane per instruction, no memory latency on the
+// dispatch path); a job whose program, store list and reverse tape each fit one
+// window keeps them in registers for the whole unit.  The partial store is either
+// VGPRs (16 / R per row: long programs re-run the forward pass per segment —
+// recomputation) or the wave's own LDS block (32 / R per row: most programs need
+// one forward pass — storage).  Adjoint slots and the per-lane gradient
+// accumulators live in VGPRs; wave-uniform slot indices use the backend's indexed
+// register moves.  Sums over a range are wave-reduced once per unit; ranges are
+// summed in a fixed order by adj_grad_kernel.
 #ifndef PPI_ADJOINT_IMPL_H
 #define PPI_ADJOINT_IMPL_H
 

@@ -4,7 +4,10 @@
 namespace ppi {
 // (features in VGPRs (0: read from the dataset), stack slots, adjoint slots, partial store (0: VGPR, 1: LDS)).
 // Five slots in one register tuple of 10 per stack: programs of up to five stack and adjoint slots.
-#define PPI_ADJ_SHAPES_R2(X) X(12, 5, 5, 0) X(12, 5, 5, 1) X(0, 5, 5, 0) X(0, 5, 5, 1)
+// Features are read from the dataset: holding 12 per row in VGPRs (24 registers) costs a wave per
+// SIMD (151 VGPRs, 3 waves) and measured 15% slower on the 1M-row adjoint workload (565 vs 494 ms),
+// although the tuner's short sample ranked it 4% faster.
+#define PPI_ADJ_SHAPES_R2(X) X(0, 5, 5, 0) X(0, 5, 5, 1)
 
 bool adjoint_shape_supported_r2(int F, int S, int AS, int PS) {
 #define PPI_X(f, s, as, ps) if (F == f && S == s && AS == as && PS == ps) return true;
