@@ -86,7 +86,7 @@ def bench_engine(eng, w, form, steps, warmup, rows_per_lane=0, programs_per_tile
     st = eng.stats()
     return {"ms_per_step": ms, "wall_s": wall, "main_kernel_ms": st.main_kernel_ms,
             "node_evals": st.node_evals, "executed_nodes": st.executed_nodes,
-            "rows_per_lane": st.rows_per_lane, "feature_slots": st.feature_slots, "programs_per_tile": st.programs_per_tile,
+            "rows_per_lane": st.rows_per_lane, "programs_per_tile": st.programs_per_tile,
             "form": st.form, "derived_columns": st0.derived_columns, "folded_subtrees": st0.folded_subtrees,
             "folded_nodes": st0.folded_nodes, "upload_ms": round(upload_s * 1e3, 2)}
 
@@ -148,7 +148,7 @@ def bench_batched(steps, warmup):
         plain_ms += eng.time(1)
     eng.close()
     return {"fp64_error": errs, "node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
-            "rows_per_lane": st.rows_per_lane, "feature_slots": st.feature_slots, "items": st.n_items, "grid": st.grid,
+            "rows_per_lane": st.rows_per_lane, "items": st.n_items, "grid": st.grid,
             "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs,
             "programs": int(sum(p.n for p in w.pops))}
 
