@@ -76,11 +76,11 @@ hipError_t launch_batch_recheck(const BatchArgs& B, const int32_t* pair_of, int3
 }
 
 hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st) {
-    dim3 grid((unsigned)((A.nprog + 255) / 256));
-    if (P && N) hipLaunchKernelGGL((dev::pp_kernel<true, true>), grid, dim3(256), 0, st, A);
-    else if (P) hipLaunchKernelGGL((dev::pp_kernel<true, false>), grid, dim3(256), 0, st, A);
-    else if (N) hipLaunchKernelGGL((dev::pp_kernel<false, true>), grid, dim3(256), 0, st, A);
-    else hipLaunchKernelGGL((dev::pp_kernel<false, false>), grid, dim3(256), 0, st, A);
+    const dim3 grid((unsigned)((A.nprog + dev::kPPBlock - 1) / dev::kPPBlock)), block(dev::kPPBlock);
+    if (P && N) hipLaunchKernelGGL((dev::pp_kernel<true, true>), grid, block, 0, st, A);
+    else if (P) hipLaunchKernelGGL((dev::pp_kernel<true, false>), grid, block, 0, st, A);
+    else if (N) hipLaunchKernelGGL((dev::pp_kernel<false, true>), grid, block, 0, st, A);
+    else hipLaunchKernelGGL((dev::pp_kernel<false, false>), grid, block, 0, st, A);
     return hipGetLastError();
 }
 

@@ -1149,17 +1149,43 @@ __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// PP form: one lane per program, rows in order (divergent dispatch).
+// PP form: one lane per program, rows in order (divergent dispatch).  A
+// workgroup is one wave (64 programs, longest first, so more workgroups than
+// CUs at the regression size); its programs' instructions are staged in LDS
+// when they fit (kPPTape entries), so each lane's per-row instruction fetch is
+// an LDS read rather than a divergent global one.
 // ---------------------------------------------------------------------------
+constexpr int kPPBlock = 64;
+constexpr int kPPTape = 4096;  // instructions staged per workgroup (32 KB)
+
 template <bool P, bool N>
-__global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
-    __shared__ float stk[kMaxSlots][256];
+__global__ void __launch_bounds__(kPPBlock) pp_kernel(EvalArgs A) {
+    __shared__ float stk[kMaxSlots][kPPBlock];
+    __shared__ Ins tape[kPPTape];
+    __shared__ uint32_t scan[kPPBlock];
     stage_math<N>();
+    const int t = blockIdx.x * kPPBlock + threadIdx.x;
+    const bool live = t < A.nprog;
+    const int p = live ? (A.order ? A.order[t] : t) : 0;
+    const ProgMeta m = live ? A.meta[p] : ProgMeta{0u, 0u, 0, kEmpty};
+    // Exclusive prefix sum of the instruction counts (Hillis-Steele over the workgroup).
+    const uint32_t nins = (m.flags & kEmpty) ? 0u : m.ncode;
+    uint32_t incl = nins;
+    scan[threadIdx.x] = incl;
     __syncthreads();
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= A.nprog) return;
-    const int p = A.order ? A.order[t] : t;
-    const ProgMeta m = A.meta[p];
+    for (int d = 1; d < kPPBlock; d <<= 1) {
+        const uint32_t add = (int)threadIdx.x >= d ? scan[threadIdx.x - d] : 0u;
+        __syncthreads();
+        incl += add;
+        scan[threadIdx.x] = incl;
+        __syncthreads();
+    }
+    const uint32_t total = scan[kPPBlock - 1], off = incl - nins;
+    const bool staged = total <= (uint32_t)kPPTape;  // the same in every lane
+    if (staged)
+        for (uint32_t k = 0; k < nins; ++k) tape[off + k] = A.code[m.code_off + k];
+    __syncthreads();  // tape and math tables staged
+    if (!live) return;
     float part = 0.0f;
     double dsum = 0.0;  // DP / PDP sums: the reference adds in its own tree order, not row order
     uint32_t kmin = 0xffffffffu, kmax = 0u;
@@ -1170,7 +1196,7 @@ __global__ void __launch_bounds__(256) pp_kernel(EvalArgs A) {
     const bool stop_at_bad = in_order || (A.strategy == PPI_STRATEGY_FP && !A.reducemax);
     const float fn = (float)A.nlin;
     if (!(m.flags & kEmpty)) {
-        const Ins* code = A.code + m.code_off;
+        const Ins* code = staged ? &tape[off] : A.code + m.code_off;
         for (int row = 0; row < A.nlin; ++row) {
             float acc = 0.0f;
             for (uint32_t pc = 0; pc < m.ncode; ++pc) {
