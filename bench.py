@@ -147,10 +147,104 @@ def bench_batched(steps, warmup):
         eng.time(1)
         plain_ms += eng.time(1)
     eng.close()
+    gen = batch_generations(w, st.rows_per_lane, 2)
     return {"fp64_error": errs, "node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
             "rows_per_lane": st.rows_per_lane, "items": st.n_items, "grid": st.grid,
             "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs,
-            "programs": int(sum(p.n for p in w.pops))}
+            "programs": int(sum(p.n for p in w.pops)), "generation": gen}
+
+
+def batch_generations(w, rows_per_lane, gens):
+    """Whole generations, wall clock, datasets resident: the batch re-adds every pair's population
+    (compile), rebuilds (plan + capture; rows per lane fixed to the tuned value), replays and reads
+    all 64 result vectors; the plain engine evaluates each pair's population through evaluate()
+    (compile + upload + launch + wait + download), one resident engine per pair, each pair's time
+    taken after a warm-up evaluation."""
+    b = ppiamd.Batch(0)
+    b.configure(w.grammars[0].table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True,
+                rows_per_lane=rows_per_lane)
+    ids = [b.add_dataset(d) for d in w.datas]
+    t_b = []
+    for g in range(gens + 1):
+        t0 = time.perf_counter()
+        b.clear_populations()
+        for k, p in enumerate(w.pops):
+            b.add_population(ids[k], p)
+        b.build()
+        b.run()
+        for k in range(len(w.pops)):
+            b.results(k)
+        t_b.append(time.perf_counter() - t0)
+    b.close()
+    plain = 0.0
+    eng = ppiamd.Engine(0)
+    for gr, p, d in zip(w.grammars, w.pops, w.datas):
+        eng.configure(gr.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+        eng.set_dataset(d)
+        eng.evaluate(p)
+        t0 = time.perf_counter()
+        for _ in range(gens):
+            eng.evaluate(p)
+        plain += (time.perf_counter() - t0) / gens
+    eng.close()
+    batched = min(t_b[1:])
+    return {"batched_ms": round(batched * 1e3, 3), "plain_ms": round(plain * 1e3, 3),
+            "vs_plain": round(plain / batched, 3)}
+
+
+def bench_batched_small(steps):
+    """Many small pairs (the island / many-problems case): 64 pairs of 500-2,000 programs (8-64 symbols)
+    on 1,000-4,000 rows x 2-16 features.  Batched graph replay against the plain engine serving the
+    same pairs from 64 resident engines (launch + wait each), device time and wall clock; then whole
+    generations as in batch_generations."""
+    w = ppiamd.batched_workload(n_pairs=64, seed=7, p_range=(500, 2000), len_range=(8, 64),
+                                rows_range=(1000, 4000), feat_range=(2, 16))
+    b = ppiamd.Batch(0)
+    b.configure(w.grammars[0].table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+    ids = [b.add_dataset(d) for d in w.datas]
+    for k, p in enumerate(w.pops):
+        b.add_population(ids[k], p)
+    b.build()
+    b.run()
+    dev_ms = b.time(steps)
+    st = b.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+        for k in range(len(w.pops)):
+            b.results(k)
+    wall_ms = (time.perf_counter() - t0) / steps * 1e3
+    vb = [b.results(k)[0] for k in range(len(w.pops))]
+    b.close()
+    engs = []
+    for gr, p, d in zip(w.grammars, w.pops, w.datas):
+        e = ppiamd.Engine(0)
+        e.configure(gr.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+        e.set_dataset(d)
+        e.upload(p)
+        e.time(1)
+        engs.append(e)
+    plain_dev = sum(e.time(1) for e in engs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs = []
+        for e, p in zip(engs, w.pops):
+            e.launch()
+            outs.append(e.wait(p.n)[0])
+    plain_wall = (time.perf_counter() - t0) / steps * 1e3
+    same = all(np.array_equal(x, y, equal_nan=True) for x, y in zip(vb, outs))
+    for e in engs:
+        e.close()
+    gen = batch_generations(w, st.rows_per_lane, 3)
+    ne = w.node_evals
+    return {"workload": f"64 pairs, {int(sum(p.n for p in w.pops))} programs (8-64 symbols), 1,000-4,000 rows x "
+                        f"2-16 features, {ne:.3e} node-evaluations per step",
+            "device_ms": round(dev_ms, 3), "plain_device_ms": round(plain_dev, 3),
+            "vs_plain_device": round(plain_dev / dev_ms, 3),
+            "wall_ms": round(wall_ms, 3), "plain_wall_ms": round(plain_wall, 3),
+            "vs_plain_wall": round(plain_wall / wall_ms, 3),
+            "gpops_wall": round(ne / (wall_ms * 1e-3) / 1e9, 3), "plain_gpops_wall": round(ne / (plain_wall * 1e-3) / 1e9, 3),
+            "results_identical_to_plain": bool(same), "rows_per_lane": st.rows_per_lane, "generation": gen}
 
 
 def bench_adjoint_pass(steps, warmup):
@@ -554,8 +648,13 @@ def main():
             "rows_per_lane": batched["rows_per_lane"], "items": batched["items"], "grid": batched["grid"],
             "autotune_ms_per_Gnode_R2_R4_R8": batched["tune_ms"],
             "fp64_error_sample": batched["fp64_error"]}
+        line["batched"]["generation_inclusive"] = dict(batched["generation"], note=(
+            "wall clock per generation with datasets resident: batch = re-add 64 populations (compile) + build + "
+            "replay + 64 result reads; plain = evaluate() per pair (compile + upload + launch + wait + download)"))
         if cpu_rate:
             line["batched"]["vs_cpu"] = round(bg / (cpu_rate / 1e9), 2)
+        log("batched: 64 small pairs (500-2,000 programs, 1,000-4,000 rows)")
+        line["batched_small"] = bench_batched_small(max(2, a.steps // 2))
     if "adjoint" in legs and (a.only == "adjoint" or extras):
         log("adjoint: 30,000 programs (<= 8 constants) x 1,000,000 rows x 8 features, backward pass + 20 descent steps")
         adjoint = bench_adjoint(2, 1, host_record()["threads"])

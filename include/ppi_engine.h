@@ -206,9 +206,11 @@ ppi_status ppi_engine_time(ppi_engine* eng, int32_t iters, double* ms_per_eval);
  * Batched evaluation: many (population, dataset) pairs in one persistent launch.
  * No counterpart in the reference (which evaluates one population on one dataset
  * per call); it serves GE runs that evaluate many islands or problems on one GPU.
- * Datasets are staged once; populations are compiled and appended; build() plans
+ * Datasets are staged once; populations are copied when added and compiled by
+ * build(), all pending ones at once on a pool of host threads, which then plans
  * the work (autotuning rows per lane and work-item size when
  * ppi_config.rows_per_lane is 0) and captures upload-free replay as a hipGraph.
+ * A program that cannot be compiled (more than 16 register slots) fails build().
  */
 typedef struct ppi_batch ppi_batch;
 
@@ -232,7 +234,8 @@ void ppi_batch_destroy(ppi_batch* b);
 ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable);
 /* Row-major nlin x ncol dataset, target in the last column; returns its id. */
 ppi_status ppi_batch_add_dataset(ppi_batch* b, const float* data, int32_t nlin, int32_t ncol, int32_t* dataset_id);
-/* A population evaluated on dataset `dataset_id` (ppi.cc:312-314 layout); returns its id. */
+/* A population evaluated on dataset `dataset_id` (ppi.cc:312-314 layout); returns its id.  The
+   arrays are copied (a size outside [0, stride] is rejected here); build() compiles them. */
 ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int32_t* phenotype,
                                     const float* ephemeral, const int32_t* size, int32_t stride, int32_t n,
                                     int32_t* pop_id);

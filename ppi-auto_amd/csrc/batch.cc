@@ -7,10 +7,12 @@
 // atomic counter by a grid sized to the kernel's occupancy.  Rows per lane R is
 // picked by timing the candidates on an evenly spaced sample of the items.  The
 // replay (queue reset, persistent kernel, per-program epilogue, download) is
-// captured once as a hipGraph.
+// captured once as a hipGraph.  Populations are copied when added and compiled
+// by build(), every pending one on its own thread of a pool.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -62,6 +64,12 @@ struct Pop {
     int32_t n = 0;
     int32_t prog_base = 0;
     int64_t total_size = 0;
+    // The caller's arrays (copied by add_population) until build() compiles them.
+    int32_t stride = 0;
+    std::vector<int32_t> ph, sz;
+    std::vector<float> eph;
+    bool compiled = false;
+    Tape t;
 };
 
 template <typename T>
@@ -217,6 +225,81 @@ void plan_items(const ppi_batch* b, int R, double items_per_slot, std::vector<Pa
     items->reserve(costed.size());
     for (auto& c : costed) items->push_back(c.w);
     *partials = part;
+}
+
+// Compiles the populations added since the last build: derived columns planned
+// per population (in the order they were added) and staged, then every pending
+// population compiled on its own thread of a pool (the derived-column tables are
+// only read there), then fold values, and the tapes appended to the batch tape in
+// order.  A population's copied arrays are released once it is compiled.
+ppi_status compile_pending(ppi_batch* b) {
+    std::vector<size_t> pend;
+    for (size_t k = 0; k < b->pops.size(); ++k)
+        if (!b->pops[k].compiled) pend.push_back(k);
+    if (pend.empty()) return PPI_OK;
+    const bool P = b->cfg.protected_ops != 0, N = b->cfg.native_math != 0;
+    const int hw = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    for (size_t k : pend) {
+        Pop& p = b->pops[k];
+        Dataset& d = b->datasets[p.dataset];
+        plan_derived(b->symtab, p.ph.data(), p.eph.data(), p.sz.data(), p.stride, p.n, d.ncol, hw, d.derived.get());
+    }
+    for (auto& d : b->datasets) {
+        if (d.derived_ready >= (int32_t)d.derived->cols.size()) continue;
+        ppi_status s = stage_derived_columns(&d.dX, &d.cap_X, d.ldx, d.ncol, *d.derived, &d.derived_ready, &d.dDerOps,
+                                             &d.dDerSrc, &d.cap_der, P, N, b->stream);
+        if (s) return s;
+    }
+    const bool fold = b->cfg.no_constant_folding == 0;
+    std::vector<std::string> errs(pend.size());
+    std::vector<char> ok(pend.size(), 1);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i = next++; i < pend.size(); i = next++) {
+            Pop& p = b->pops[pend[i]];
+            const Dataset& d = b->datasets[p.dataset];
+            ok[i] = compile_population_planned(b->symtab, p.ph.data(), p.eph.data(), p.sz.data(), p.stride, p.n,
+                                               d.ncol, &p.t, &errs[i], d.derived.get(), fold);
+        }
+    };
+    const int workers = (int)std::min<size_t>((size_t)hw, pend.size());
+    if (workers <= 1) {
+        work();
+    } else {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+        for (auto& th : pool) th.join();
+    }
+    for (size_t i = 0; i < pend.size(); ++i)
+        if (!ok[i]) {
+            set_error("population " + std::to_string(pend[i]) + ": " + errs[i]);
+            return PPI_E_PROGRAM;
+        }
+    for (size_t k : pend) {
+        Pop& p = b->pops[k];
+        Tape& t = p.t;
+        if (ppi_status fs = evaluate_folds(&t, &b->folds, P, N, b->stream)) return fs;
+        if (t.stats.max_slots > kMaxSlots) {
+            set_error("population " + std::to_string(k) + ": program needs more than 16 register slots");
+            return PPI_E_PROGRAM;
+        }
+        const uint32_t off = (uint32_t)b->tape.code.size();
+        for (auto m : t.meta) {
+            m.code_off += off;
+            b->tape.meta.push_back(m);
+        }
+        b->tape.code.insert(b->tape.code.end(), t.code.begin(), t.code.end());
+        b->tape.stats.total_size += t.stats.total_size;
+        b->tape.stats.total_code += t.stats.total_code;
+        b->tape.stats.max_slots = std::max(b->tape.stats.max_slots, t.stats.max_slots);
+        b->tape.stats.max_col = std::max(b->tape.stats.max_col, t.stats.max_col);
+        p.compiled = true;
+        p.t = Tape{};
+        std::vector<int32_t>().swap(p.ph);
+        std::vector<int32_t>().swap(p.sz);
+        std::vector<float>().swap(p.eph);
+    }
+    return PPI_OK;
 }
 
 ppi_status upload_items(ppi_batch* b) {
@@ -396,42 +479,24 @@ ppi_status ppi_batch_add_population(ppi_batch* b, int32_t dataset_id, const int3
     if (!b || !b->configured) { set_error("configure the batch first"); return PPI_E_STATE; }
     if (dataset_id < 0 || dataset_id >= (int32_t)b->datasets.size()) { set_error("unknown dataset"); return PPI_E_ARG; }
     if (!phenotype || !ephemeral || !size || stride <= 0 || n <= 0 || !pop_id) { set_error("bad program arrays"); return PPI_E_ARG; }
-    (void)hipSetDevice(b->device);
-    Dataset& d = b->datasets[dataset_id];
-    Tape t;
-    std::string err;
-    int threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (!compile_population_mt(b->symtab, phenotype, ephemeral, size, stride, n, d.ncol, false, threads, &t, &err,
-                               d.derived.get(), b->cfg.no_constant_folding == 0)) {
-        set_error(err);
-        return PPI_E_PROGRAM;
-    }
-    if (ppi_status fs = evaluate_folds(&t, &b->folds, b->cfg.protected_ops != 0, b->cfg.native_math != 0, b->stream))
-        return fs;
-    if (t.stats.max_slots > kMaxSlots) { set_error("program needs more than 16 register slots"); return PPI_E_PROGRAM; }
-    // New derived columns are computed now; the dataset buffer may move, which
-    // the next build picks up (a built graph is dropped below).
-    if (d.derived_ready < (int32_t)d.derived->cols.size()) {
-        (void)hipStreamSynchronize(b->stream);
-        free_build(b);
-        ppi_status s = stage_derived_columns(&d.dX, &d.cap_X, d.ldx, d.ncol, *d.derived, &d.derived_ready,
-                                             &d.dDerOps, &d.dDerSrc, &d.cap_der, b->cfg.protected_ops != 0,
-                                             b->cfg.native_math != 0, b->stream);
-        if (s) return s;
-    }
     Pop p;
     p.dataset = dataset_id;
     p.n = n;
     p.prog_base = total_programs(b);
-    p.total_size = t.stats.total_size;
-    const uint32_t off = (uint32_t)b->tape.code.size();
-    for (auto m : t.meta) { m.code_off += off; b->tape.meta.push_back(m); }
-    b->tape.code.insert(b->tape.code.end(), t.code.begin(), t.code.end());
-    b->tape.stats.total_size += t.stats.total_size;
-    b->tape.stats.total_code += t.stats.total_code;
-    b->tape.stats.max_slots = std::max(b->tape.stats.max_slots, t.stats.max_slots);
-    b->tape.stats.max_col = std::max(b->tape.stats.max_col, t.stats.max_col);
-    b->pops.push_back(p);
+    p.stride = stride;
+    for (int32_t i = 0; i < n; ++i) {
+        if (size[i] < 0 || size[i] > stride) {
+            set_error("program " + std::to_string(i) + ": size " + std::to_string(size[i]) + " outside [0, " +
+                      std::to_string(stride) + "]");
+            return PPI_E_PROGRAM;
+        }
+        p.total_size += size[i];
+    }
+    // Compiled by build(), all pending populations at once on a thread pool.
+    p.ph.assign(phenotype, phenotype + (size_t)n * stride);
+    p.eph.assign(ephemeral, ephemeral + (size_t)n * stride);
+    p.sz.assign(size, size + n);
+    b->pops.push_back(std::move(p));
     b->built = false;
     *pop_id = (int32_t)b->pops.size() - 1;
     return PPI_OK;
@@ -452,6 +517,7 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     const double t0 = now_ms();
     free_build(b);
     ppi_status s;
+    if ((s = compile_pending(b))) return s;
     const int32_t n = total_programs(b);
     b->P = b->cfg.protected_ops != 0;
     b->N = b->cfg.native_math != 0;

@@ -432,7 +432,7 @@ void apply_folds(Tape* t, const float* value) {
 void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                   int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived) {
     if (!derived || derived->cap <= 0 || (int32_t)derived->cols.size() >= derived->cap) return;
-    threads = std::max(1, std::min(threads, (n + 1023) / 1024));
+    threads = std::max(1, std::min(threads, n / 128));
     std::vector<std::unordered_map<uint64_t, uint32_t>> counts((size_t)threads);
     if (threads == 1) {
         count_chains(st, phenotype, ephemeral, size, stride, 0, n, ncol, &counts[0]);
@@ -475,6 +475,12 @@ void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* 
         derived->index.emplace(c.key, derived->base + (int32_t)derived->cols.size());
         derived->cols.push_back(DerivedColumns::Col{u, src});
     }
+}
+
+bool compile_population_planned(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
+                                const int32_t* size, int32_t stride, int32_t n, int32_t ncol, Tape* out,
+                                std::string* err, const DerivedColumns* derived, bool fold) {
+    return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, false, out, err, derived, fold);
 }
 
 bool compile_population(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
@@ -555,7 +561,10 @@ bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, cons
     out->stats = CompileStats{};
     out->folds = FoldSet{};
     if (derived) plan_derived(st, phenotype, ephemeral, size, stride, n, ncol, threads, derived);
-    if (threads <= 1 || n < 4096)
+    // At least 128 programs per worker: a worker's start-up costs about as much as compiling
+    // a few dozen programs.
+    threads = std::min(threads, n / 128);
+    if (threads <= 1)
         return compile_range(st, phenotype, ephemeral, size, stride, n, ncol, want_const_pos, out, err, derived,
                              fold);
     std::vector<Tape> parts((size_t)threads);

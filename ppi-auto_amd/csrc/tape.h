@@ -235,6 +235,12 @@ bool compile_population(const SymbolTable& st, const int32_t* phenotype, const f
                         const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,
                         Tape* out, std::string* err, DerivedColumns* derived = nullptr, bool fold = false);
 
+// As compile_population with the derived columns already planned: `derived` is only read, so
+// several populations sharing it may be compiled at once (one thread each).
+bool compile_population_planned(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
+                                const int32_t* size, int32_t stride, int32_t n, int32_t ncol, Tape* out,
+                                std::string* err, const DerivedColumns* derived, bool fold);
+
 // Multi-threaded front end: plans the derived columns and compiles with `threads` workers.
 bool compile_population_mt(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral,
                            const int32_t* size, int32_t stride, int32_t n, int32_t ncol, bool want_const_pos,

@@ -98,3 +98,37 @@ def test_batched_classification_bit_exact():
         ovec, _ = oracle.interpret(cfg, gr.table, pops2[k], datas[k], alpha=ALPHA)
         assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32))
     b.close()
+
+
+def test_batched_shared_dataset_incremental_build():
+    """Populations compiled together at build() on a thread pool, several sharing one dataset (and
+    its derived-column table), give the plain FP engine's bits at the same rows per lane; a population added after a build is
+    compiled by the next build and appended; a size outside [0, stride] is rejected when added."""
+    gr = ppiamd.Grammar(6)
+    data = ppiamd.dataset(ppiamd.DS_POLY_UNIFORM, 41, 6000, 6)
+    pops = [ppiamd.grow(gr, n, 80 + k, 8, 96) for k, n in enumerate((900, 300, 1500, 64))]
+    b = ppiamd.Batch(0)
+    b.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_SQUARED, rows_per_lane=4)
+    d = b.add_dataset(data)
+    ids = [b.add_population(d, p) for p in pops[:3]]
+    b.build()
+    b.run()
+    first = [b.results(i)[0] for i in ids]
+    ids.append(b.add_population(d, pops[3]))
+    b.build()
+    b.run()
+    e = ppiamd.Engine(0)  # same rows per lane: the same row tiles, so the same partial sums
+    e.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_SQUARED, rows_per_lane=4, form=ppiamd.FORM_FP)
+    e.set_dataset(data)
+    for k, p in enumerate(pops):
+        v, _ = b.results(ids[k])
+        pv, _ = e.evaluate(p)
+        assert np.array_equal(v.view(np.uint32), pv.view(np.uint32)), k
+        if k < 3:
+            assert np.array_equal(v.view(np.uint32), first[k].view(np.uint32)), k
+    e.close()
+    bad = pops[0].subset(4)
+    bad.size[2] = bad.stride + 1
+    with pytest.raises(ppiamd.PpiError):
+        b.add_population(d, bad)
+    b.close()
