@@ -83,7 +83,11 @@ typedef enum ppi_error_fn {
     PPI_EF_SQUARED = 1,    /* ((X)-(Y))*((X)-(Y)) */
     PPI_EF_NEQ = 2,        /* ((X)!=(Y))                          (problem/iris/README.md) */
     PPI_EF_COST = 3,       /* COST[(int)X][(int)Y]                (default when COST is given) */
-    PPI_EF_COST_CLAMP = 4  /* COST[(int)fminf(fmaxf(X,0),N-1)][(int)Y] */
+    PPI_EF_COST_CLAMP = 4, /* COST[(int)fminf(fmaxf(X,0),N-1)][(int)Y] */
+    PPI_EF_EXPR = 5        /* any other ERROR(X,Y): ppi_config.ef_expr, a C expression in X and Y compiled at
+                              configure (the types, operators and functions ppi_ef_evaluate documents); every
+                              program then takes the exact-order path (the reference's own partition and row
+                              order for the strategy), with the expression interpreted per row */
 } ppi_error_fn;
 
 /* Result semantics: which reference path's error vector and ranking to reproduce. */
@@ -125,6 +129,7 @@ typedef struct ppi_config {
                                   free device memory) of errors) */
     int32_t no_constant_folding; /* 1: evaluate constant subtrees on every row instead of once per upload
                                     (results are the same bits either way) */
+    const char* ef_expr;    /* PPI_EF_EXPR: the text of ERROR(X,Y) (copied at configure) */
 } ppi_config;
 
 typedef struct ppi_stats {
@@ -197,6 +202,21 @@ ppi_status ppi_engine_evaluate_noalloc(ppi_engine* eng, const int32_t* phenotype
                                       const int32_t* size, int32_t stride, int32_t n_programs, float alpha,
                                       float* vector, int32_t* index, int32_t* best_size);
 ppi_status ppi_engine_get_stats(const ppi_engine* eng, ppi_stats* out);
+
+/*
+ * User error function ERROR(X,Y) (definitions.h:18; -DEF=<expression>, CMakeLists.txt:26-33),
+ * evaluated on the host as the device evaluates it: out[i] = ERROR(X[i], Y[i]).  The
+ * expression is C over float X and Y with C's typing (int, float and double literals and the
+ * usual arithmetic conversions; + - * / %, comparisons, && || !, ?:, casts to int, float and
+ * double) and the functions sequential.cc sees through <cmath>: fabs sqrt exp log pow sin cos
+ * tan floor ceil round trunc fmin fmax (double), fabsf sqrtf floorf ceilf roundf truncf fminf
+ * fmaxf (float), abs (its argument's type).  PPI_E_ARG, with the reason in ppi_last_error(),
+ * for anything else (the C library's float transcendentals expf ... powf among them: they are
+ * not correctly rounded, so their results cannot be reproduced).  No reference counterpart (the
+ * reference compiles the expression into its kernels); the drop-in uses it to check, at init,
+ * that the compiled expression gives the macro's own values.
+ */
+ppi_status ppi_ef_evaluate(const char* text, const float* X, const float* Y, int32_t n, float* out);
 
 /* Device-resident timing loop used by benchmarks: run `iters` evaluations of the resident
    programs back to back on the engine stream, return mean device ms per evaluation. */

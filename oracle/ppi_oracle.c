@@ -202,6 +202,7 @@ static float ef_f32(const oracle_cfg* c, float X, float Y) {
         if (!(xx > -1.0f && xx < (float)n) || !(Y > -1.0f && Y < (float)n)) return NAN;
         return c->cost[(int)xx * n + (int)Y];
     }
+    case PPI_EF_EXPR: return c->ef_fn ? c->ef_fn(X, Y) : NAN;
     default: return NAN;
     }
 }

@@ -31,6 +31,9 @@ typedef struct oracle_cfg {
     int32_t dp_local;      /* evaluate_dp work-group size (accelerator.cc:304-313) */
     int32_t pdp_local;     /* evaluate_pdp work-group size (accelerator.cc:316-329) */
     int32_t best_local;    /* best_individual work-group size (accelerator.cc:344-355) */
+    /* PPI_EF_EXPR: the user's ERROR(X,Y) as a compiled C function of float X and Y (the
+       reference pastes the expression into sequential.cc, definitions.h:18) */
+    float (*ef_fn)(float X, float Y);
 } oracle_cfg;
 
 /* Error vector + ranking, exactly as the reference path named by cfg->strategy.

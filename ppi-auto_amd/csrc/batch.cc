@@ -411,6 +411,10 @@ void ppi_batch_destroy(ppi_batch* b) {
 ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_symbol* table, int32_t ntable) {
     if (!b || !cfg || ntable < 0 || (ntable > 0 && !table)) { set_error("bad arguments"); return PPI_E_ARG; }
     if (cfg->strategy < PPI_STRATEGY_SEQ || cfg->strategy > PPI_STRATEGY_PDP) { set_error("bad strategy"); return PPI_E_ARG; }
+    if (cfg->error_fn == PPI_EF_EXPR) {
+        set_error("the batched evaluator takes the built-in error functions (a user ERROR(X,Y) runs through the engine)");
+        return PPI_E_UNSUPPORTED;
+    }
     if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
     const bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
     if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs a matrix"); return PPI_E_ARG; }

@@ -364,7 +364,7 @@ void ppi_engine_destroy(ppi_engine* E) {
     (void)hipSetDevice(E->device);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void* dev[] = {E->dX, E->dMeta, E->dCode, E->dPartial, E->dBadLo, E->dBadHi, E->dRaw, E->dVec,
-                   E->dOrder, E->dCost, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop,
+                   E->dOrder, E->dCost, E->dEf, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop,
                    E->dRankVec, E->dRankGv, E->dRankGi, E->dRankIdx};
     for (void* p : dev)
         if (p) (void)hipFree(p);
@@ -394,7 +394,13 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
         set_error("this device's work-group limit exceeds " + std::to_string(kPdpMaxLanes) + "; set max_local_size");
         return PPI_E_ARG;
     }
-    if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_COST_CLAMP) { set_error("bad error_fn"); return PPI_E_ARG; }
+    if (cfg->error_fn < PPI_EF_ABS || cfg->error_fn > PPI_EF_EXPR) { set_error("bad error_fn"); return PPI_E_ARG; }
+    std::vector<EfIns> ef;
+    if (cfg->error_fn == PPI_EF_EXPR) {
+        std::string err;
+        if (!cfg->ef_expr) { set_error("PPI_EF_EXPR needs ef_expr"); return PPI_E_ARG; }
+        if (!ef_compile(cfg->ef_expr, &ef, &err)) { set_error("error function: " + err); return PPI_E_ARG; }
+    }
     bool cost = cfg->error_fn == PPI_EF_COST || cfg->error_fn == PPI_EF_COST_CLAMP;
     if (cost && (cfg->cost_n <= 0 || !cfg->cost)) { set_error("COST error function needs cost_n > 0 and cost"); return PPI_E_ARG; }
     if (cfg->rows_per_lane != 0 && cfg->rows_per_lane != 1 && cfg->rows_per_lane != 2 && cfg->rows_per_lane != 4 &&
@@ -417,6 +423,15 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
     E->cfg = *cfg;
     E->cost.assign(cfg->cost, cfg->cost + (cost ? (size_t)cfg->cost_n * cfg->cost_n : 0));
     E->cfg.cost = nullptr;
+    E->cfg.ef_expr = nullptr;
+    E->ef = ef;
+    if (!ef.empty()) {
+        size_t cap = E->cap_ef;
+        ppi_status s = ensure(&E->dEf, &cap, ef.size());
+        if (s) return s;
+        E->cap_ef = cap;
+        PPI_HIP(hipMemcpy(E->dEf, ef.data(), ef.size() * sizeof(EfIns), hipMemcpyHostToDevice));
+    }
     if (cost) {
         size_t cap = E->cap_cost;
         ppi_status s = ensure(&E->dCost, &cap, E->cost.size());
@@ -592,6 +607,8 @@ EvalArgs ppi::make_args(ppi_engine* E, float alpha) {
     A.rowerr = E->dRowErr;
     A.osum = E->dOSum;
     A.ostop = E->dOStop;
+    A.ef = E->dEf;
+    A.ef_n = (int32_t)E->ef.size();
     return A;
 }
 
@@ -602,6 +619,15 @@ ppi_status ppi::enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, boo
     PPI_HIP(launch_fill_u32(A.bad_lo, 0xffffffffu, A.nprog, E->stream));
     PPI_HIP(launch_fill_u32(A.bad_hi, 0u, A.nprog, E->stream));
     if (time_main) PPI_HIP(hipEventRecord(E->ev[2], E->stream));
+    if (A.error_fn == PPI_EF_EXPR) {
+        // A user ERROR(X,Y): every program through the exact-order recomputation (the
+        // reference's own partition and row order for the strategy), the expression
+        // interpreted per row.
+        PPI_HIP(launch_ef_prepare(A, E->stream));
+        PPI_HIP(launch_recheck(A, P, N, E->stream));
+        if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
+        return PPI_OK;
+    }
     if (L.form == PPI_FORM_FP) {
         PPI_HIP(launch_fp(A, L.R, L.F, L.S, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
@@ -910,6 +936,15 @@ ppi_status ppi_engine_resident_tape(const ppi_engine* E, uint32_t* meta, uint32_
     *n_code = (int64_t)E->tape.code.size();
     if (meta) std::memcpy(meta, E->tape.meta.data(), E->tape.meta.size() * sizeof(ProgMeta));
     if (code) std::memcpy(code, E->tape.code.data(), E->tape.code.size() * sizeof(Ins));
+    return PPI_OK;
+}
+
+ppi_status ppi_ef_evaluate(const char* text, const float* X, const float* Y, int32_t n, float* out) {
+    if (!text || n < 0 || (n > 0 && (!X || !Y || !out))) { set_error("bad arguments"); return PPI_E_ARG; }
+    std::vector<EfIns> ef;
+    std::string err;
+    if (!ef_compile(text, &ef, &err)) { set_error("error function: " + err); return PPI_E_ARG; }
+    for (int32_t i = 0; i < n; ++i) out[i] = ef_eval_host(ef.data(), (int)ef.size(), X[i], Y[i]);
     return PPI_OK;
 }
 

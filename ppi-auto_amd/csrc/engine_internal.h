@@ -91,6 +91,9 @@ struct ppi_engine {
     size_t cap_raw = 0;
     float* dCost = nullptr;
     size_t cap_cost = 0;
+    std::vector<ppi::EfIns> ef;  // PPI_EF_EXPR: the compiled ERROR(X,Y)
+    ppi::EfIns* dEf = nullptr;
+    size_t cap_ef = 0;
     float* dPred = nullptr;
     size_t cap_pred = 0;
     float* hRaw = nullptr;

@@ -116,6 +116,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(dev::fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
+
+hipError_t launch_ef_prepare(const EvalArgs& A, hipStream_t st) {
+    if (A.nprog <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dev::ef_prepare_kernel, dim3((unsigned)((A.nprog + 255) / 256)), dim3(256), 0, st, A);
+    return hipGetLastError();
+}
 }  // namespace ppi
 
 namespace ppi {

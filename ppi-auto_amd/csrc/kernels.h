@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "ef.h"
 #include "tape.h"
 
 namespace ppi {
@@ -52,6 +53,8 @@ struct EvalArgs {
     float* rowerr;        // [row block - rb0][nprog][64*R]: per-row errors of one pass
     float* osum;          // [nprog] running sum (REDUCEMAX: maximum) in row order
     uint32_t* ostop;      // [nprog] 1 once a row's error was not finite
+    const EfIns* ef;      // PPI_EF_EXPR: the compiled ERROR(X,Y) (ef.h), ef_n operations
+    int32_t ef_n;
 };
 
 // Batched form: one (population, dataset) pair as the device sees it.
@@ -178,6 +181,9 @@ hipError_t launch_batch_recheck(const BatchArgs& B, const int32_t* pair_of, int3
                                 hipStream_t st);
 hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+// PPI_EF_EXPR: empty programs get their result, every other one is marked for the exact-order
+// recomputation (launch_recheck), which evaluates the expression per row.
+hipError_t launch_ef_prepare(const EvalArgs& A, hipStream_t st);
 // Folded constant subtrees: out[j] = value of fold program j (meta[j], code).
 hipError_t launch_fold(const ProgMeta* meta, const Ins* code, int32_t n, float* out, bool P, bool N, hipStream_t st);
 // Derived columns first .. first+count-1 of X: column first+j = unary(ops[j], column srcs[j]).

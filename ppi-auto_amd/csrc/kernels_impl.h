@@ -837,6 +837,85 @@ __global__ void __launch_bounds__(256) ordered_sum_kernel(EvalArgs A, int rows_p
 // ---------------------------------------------------------------------------
 constexpr int kRecheckSpan = 16;  // programs per recheck workgroup
 
+// PPI_EF_EXPR: the compiled ERROR(X,Y) (ef.h) on one row, with the host interpreter's rules
+// (ef.cc, ef_eval_host): values held in double, a float-typed operation rounded to float after
+// it (exact for + - * / and sqrt: double has more than twice float's precision), int
+// operations wrapping in 32 bits, conversions to int truncating (INT_MIN out of range), the
+// double routines from the device library.  The stack lives in private memory: this runs in
+// the recomputation kernels only.
+static __device__ __noinline__ double ef_round_to(double v, uint32_t type) {
+    if (type == EF_FLOAT) return (double)(float)v;
+    if (type == EF_INT) return (v > -2147483649.0 && v < 2147483648.0) ? (double)(int32_t)v : -2147483648.0;
+    return v;
+}
+__device__ __forceinline__ double ef_wrap(int64_t v) { return (double)(int32_t)(uint32_t)(uint64_t)v; }
+
+static __device__ __noinline__ float ef_expr_eval(const EfIns* code, int n, float X, float Y) {
+    double st[kEfMaxDepth];
+    int sp = 0;
+    for (int k = 0; k < n; ++k) {
+        const EfIns e = code[k];
+        const uint32_t t = e.type;
+        switch (e.op) {
+        case EF_X: st[sp++] = X; break;
+        case EF_Y: st[sp++] = Y; break;
+        case EF_IMM: st[sp++] = e.imm; break;
+        case EF_CVT: st[sp - 1] = ef_round_to(st[sp - 1], t); break;
+        case EF_NEG: st[sp - 1] = t == EF_INT ? ef_wrap(-(int64_t)st[sp - 1]) : -st[sp - 1]; break;
+        case EF_NOT: st[sp - 1] = st[sp - 1] == 0.0 ? 1.0 : 0.0; break;
+        case EF_FABS: {
+            const double v = st[sp - 1];
+            st[sp - 1] = t == EF_INT ? ef_wrap(v < 0.0 ? -(int64_t)v : (int64_t)v) : fabs(v);
+            break;
+        }
+        case EF_SQRT: st[sp - 1] = ef_round_to(sqrt(st[sp - 1]), t); break;
+        case EF_EXP: st[sp - 1] = ef_round_to(exp(st[sp - 1]), t); break;
+        case EF_LOG: st[sp - 1] = ef_round_to(log(st[sp - 1]), t); break;
+        case EF_SIN: st[sp - 1] = ef_round_to(sin(st[sp - 1]), t); break;
+        case EF_COS: st[sp - 1] = ef_round_to(cos(st[sp - 1]), t); break;
+        case EF_TAN: st[sp - 1] = ef_round_to(tan(st[sp - 1]), t); break;
+        case EF_FLOOR: st[sp - 1] = floor(st[sp - 1]); break;
+        case EF_CEIL: st[sp - 1] = ceil(st[sp - 1]); break;
+        case EF_ROUND: st[sp - 1] = round(st[sp - 1]); break;
+        case EF_TRUNC: st[sp - 1] = trunc(st[sp - 1]); break;
+        case EF_SEL: {
+            const double b = st[--sp], a = st[--sp], c = st[--sp];
+            st[sp++] = c != 0.0 ? a : b;
+            break;
+        }
+        default: {
+            const double b = st[--sp], a = st[--sp];
+            double r = 0.0;
+            switch (e.op) {
+            case EF_ADD: r = t == EF_INT ? ef_wrap((int64_t)a + (int64_t)b) : ef_round_to(a + b, t); break;
+            case EF_SUB: r = t == EF_INT ? ef_wrap((int64_t)a - (int64_t)b) : ef_round_to(a - b, t); break;
+            case EF_MUL: r = t == EF_INT ? ef_wrap((int64_t)a * (int64_t)b) : ef_round_to(a * b, t); break;
+            case EF_DIV:
+                if (t == EF_INT) r = b == 0.0 ? 0.0 : ef_wrap((int64_t)a / (int64_t)b);
+                else r = ef_round_to(a / b, t);
+                break;
+            case EF_MOD: r = b == 0.0 ? 0.0 : ef_wrap((int64_t)a % (int64_t)b); break;
+            case EF_LT: r = a < b; break;
+            case EF_GT: r = a > b; break;
+            case EF_LE: r = a <= b; break;
+            case EF_GE: r = a >= b; break;
+            case EF_EQ: r = a == b; break;
+            case EF_NE: r = a != b; break;
+            case EF_AND: r = (a != 0.0) && (b != 0.0); break;
+            case EF_OR: r = (a != 0.0) || (b != 0.0); break;
+            case EF_POW: r = ef_round_to(pow(a, b), t); break;
+            case EF_FMIN: r = fmin(a, b); break;
+            case EF_FMAX: r = fmax(a, b); break;
+            default: break;
+            }
+            st[sp++] = r;
+            break;
+        }
+        }
+    }
+    return (float)st[sp - 1];
+}
+
 // Error of program m at row n.
 template <bool P, bool N>
 __device__ __forceinline__ float row_error(const EvalArgs& A, const ProgMeta& m, int n) {
@@ -845,7 +924,9 @@ __device__ __forceinline__ float row_error(const EvalArgs& A, const ProgMeta& m,
     float acc[1];
     typename C::Stk stk;
     C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<1>{A.X, A.ldx, row}, acc, stk);
-    return error_fn(A.error_fn, acc[0], A.X[(int64_t)A.ycol * A.ldx + n], A.cost, A.cost_n);
+    const float y = A.X[(int64_t)A.ycol * A.ldx + n];
+    if (A.error_fn == PPI_EF_EXPR) return ef_expr_eval(A.ef, A.ef_n, acc[0], y);
+    return error_fn(A.error_fn, acc[0], y, A.cost, A.cost_n);
 }
 
 // The reference's work-group reduction over PE[0, width) (accelerator.cl:157-166,
@@ -1055,6 +1136,21 @@ __global__ void __launch_bounds__(kBlock) recheck_kernel(EvalArgs A) {
 }
 
 // The same for the batched form: program g belongs to pair pair_of[g].
+// PPI_EF_EXPR: empty programs get their result here (as the main kernels' epilogue gives it);
+// every other program is marked for the recomputation with a bad-row record, so that
+// recheck_program takes the reference's own partition and row order for the strategy.
+__global__ void ef_prepare_kernel(EvalArgs A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.nprog) return;
+    const ProgMeta m = A.meta[p];
+    if (m.flags & kEmpty) {
+        semantics(A, 0.0f, m.flags, 0xffffffffu, 0u, m.size, &A.raw[p], &A.vec[p]);
+    } else {
+        A.raw[p] = __uint_as_float(kRecheckBits);
+        A.bad_lo[p] = 0u;
+    }
+}
+
 template <bool P, bool N>
 __global__ void __launch_bounds__(kBlock) batch_recheck_kernel(BatchArgs B, const int32_t* pair_of,
                                                               int32_t n_total) {

@@ -32,7 +32,7 @@ STRATEGIES = {"SEQ": SEQ, "PP": PP, "FP": FP, "DP": FP, "PDP": PDP}
 # ppi_form
 FORM_AUTO, FORM_PP, FORM_FP, FORM_PDP, FORM_ORDERED = 0, 1, 2, 3, 4
 # ppi_error_fn
-EF_ABS, EF_SQUARED, EF_NEQ, EF_COST, EF_COST_CLAMP = range(5)
+EF_ABS, EF_SQUARED, EF_NEQ, EF_COST, EF_COST_CLAMP, EF_EXPR = range(6)
 
 OPS = _grammar.OPS
 
@@ -53,7 +53,7 @@ class Config(C.Structure):
         ("protected_ops", C.c_int32), ("native_math", C.c_int32), ("cost_n", C.c_int32),
         ("cost", C.POINTER(C.c_float)), ("max_local_size", C.c_int32), ("compute_units", C.c_int32),
         ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32), ("ordered_pass_rows", C.c_int32),
-        ("no_constant_folding", C.c_int32),
+        ("no_constant_folding", C.c_int32), ("ef_expr", C.c_char_p),
     ]
 
 
@@ -151,7 +151,7 @@ ENGINE_SYMBOLS = [
     "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
     "ppi_adjoint_create", "ppi_adjoint_destroy", "ppi_adjoint_configure", "ppi_adjoint_set_dataset_rowmajor",
     "ppi_adjoint_forward", "ppi_adjoint_backward", "ppi_adjoint_descend", "ppi_adjoint_time",
-    "ppi_adjoint_get_stats", "ppi_adjoint_set_rows_per_lane",
+    "ppi_adjoint_get_stats", "ppi_adjoint_set_rows_per_lane", "ppi_ef_evaluate",
 ]
 
 
@@ -223,6 +223,7 @@ def lib() -> C.CDLL:
         L.ppi_adjoint_time.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double)]
         L.ppi_adjoint_get_stats.argtypes = [C.c_void_p, P(AdjointStats)]
         L.ppi_adjoint_set_rows_per_lane.argtypes = [C.c_void_p, C.c_int32]
+        L.ppi_ef_evaluate.argtypes = [C.c_char_p, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float)]
         for name in ENGINE_SYMBOLS:
             if name not in ("ppi_abi_version", "ppi_last_error", "ppi_engine_destroy", "ppi_batch_destroy",
                             "ppi_adjoint_destroy"):
@@ -300,8 +301,13 @@ class Engine:
                   reducemax: bool = False, protected: bool = True, native: bool = False,
                   cost: Optional[np.ndarray] = None, form: int = FORM_AUTO, max_local_size: int = 0,
                   compute_units: int = 0, rows_per_lane: int = 0, programs_per_tile: int = 0,
-                  ordered_pass_rows: int = 0, fold_constants: bool = True) -> None:
+                  ordered_pass_rows: int = 0, fold_constants: bool = True, ef_expr: Optional[str] = None) -> None:
+        """ef_expr: the text of a user ERROR(X,Y) (error_fn becomes EF_EXPR)."""
         cfg = Config()
+        if ef_expr is not None:
+            error_fn = EF_EXPR
+            self._ef_text = ef_expr.encode()
+            cfg.ef_expr = self._ef_text
         cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
         cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
         if cost is not None:
@@ -409,8 +415,11 @@ class Engine:
 
 def make_config(strategy=SEQ, error_fn=EF_ABS, reducemax=False, protected=True, native=False, cost=None,
                 form=FORM_AUTO, max_local_size=0, compute_units=0, rows_per_lane=0, programs_per_tile=0,
-                fold_constants=True):
+                fold_constants=True, ef_expr=None):
+    """ef_expr: the text of a user ERROR(X,Y) (error_fn becomes EF_EXPR)."""
     cfg = Config()
+    if ef_expr is not None:
+        error_fn = EF_EXPR
     cfg.strategy, cfg.form, cfg.error_fn = strategy, form, error_fn
     cfg.reducemax, cfg.protected_ops, cfg.native_math = int(reducemax), int(protected), int(native)
     keep = None
@@ -421,7 +430,21 @@ def make_config(strategy=SEQ, error_fn=EF_ABS, reducemax=False, protected=True, 
     cfg.max_local_size, cfg.compute_units = max_local_size, compute_units
     cfg.rows_per_lane, cfg.programs_per_tile = rows_per_lane, programs_per_tile
     cfg.no_constant_folding = 0 if fold_constants else 1
+    if ef_expr is not None:
+        text = ef_expr.encode()
+        cfg.ef_expr = text
+        keep = (keep, text)
     return cfg, keep
+
+
+def ef_evaluate(text: str, X: np.ndarray, Y: np.ndarray) -> np.ndarray:
+    """ppi_ef_evaluate: a user ERROR(X,Y) compiled and evaluated on the host, as the device does."""
+    x = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.ascontiguousarray(Y, dtype=np.float32)
+    out = np.empty(x.shape[0], dtype=np.float32)
+    _check(lib().ppi_ef_evaluate(text.encode(), _ptr(x, C.c_float), _ptr(y, C.c_float), x.shape[0],
+                                 _ptr(out, C.c_float)))
+    return out
 
 
 def _table_struct(table):

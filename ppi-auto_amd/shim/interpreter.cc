@@ -9,12 +9,17 @@
 //
 // Every evaluation runs on the GPU through the C ABI of include/ppi_engine.h; if
 // the engine cannot be created the init call fails (there is no CPU fallback).
+// The headers sequential.cc sees (sequential.cc:16-22), first: ERROR(X,Y) expands here with
+// the same overloads (libstdc++'s <stdlib.h> puts std::abs's float overloads in the global
+// namespace; <cmath> alone leaves the C library's double routines there).
+#include <stdio.h>
+#include <stdlib.h>
 #include <cmath>
-#include <cstdio>
-#include <cstdlib>
+#include <limits>
+#include <string>
+
 #include <cstring>
 #include <stdexcept>
-#include <string>
 #include <vector>
 
 #include <definitions.h>
@@ -40,6 +45,7 @@ struct ErrorFnInfo {
     int cost_n = 0;
     std::vector<float> cost;
     std::string text;
+    std::string why;  // why a user expression was not accepted
 };
 
 std::string squeeze(const char* s) {
@@ -51,7 +57,8 @@ std::string squeeze(const char* s) {
 
 // Identifies the compile-time ERROR(X,Y) (definitions.h:18) among the engine's
 // built-in error functions: by its expansion text for the COST family, then by
-// values on probe points for every candidate.
+// values on probe points for every candidate; any other expression is handed to the
+// engine as text (PPI_EF_EXPR) once its compiled form has matched the macro.
 ErrorFnInfo detect_error_fn() {
 #include "costmatrix"
     ErrorFnInfo info;
@@ -89,6 +96,43 @@ ErrorFnInfo detect_error_fn() {
         is_neq &= e == (float)(X != Y);
     }
     info.ef = is_abs ? PPI_EF_ABS : is_sq ? PPI_EF_SQUARED : is_neq ? PPI_EF_NEQ : -1;
+    if (info.ef >= 0) return info;
+    // Any other expression: compiled by the engine (PPI_EF_EXPR), accepted only if the compiled
+    // form gives this build's own ERROR(X,Y) bit for bit on a probe set (specials, small
+    // integers and halves, and a spread of magnitudes).
+    std::vector<float> vals = {0.0f, -0.0f, 0.5f, -0.5f, 1.0f, -1.0f, 1.5f, 2.0f, -2.5f, 3.0f, 7.0f, 1e-30f,
+                               1e-45f, 3.0e38f, -3.0e38f, 1e30f, std::numeric_limits<float>::infinity(),
+                               -std::numeric_limits<float>::infinity(), std::numeric_limits<float>::quiet_NaN()};
+    uint32_t r = 2463534242u;
+    for (int k = 0; k < 40; ++k) {
+        r ^= r << 13; r ^= r >> 17; r ^= r << 5;
+        vals.push_back(((float)(int32_t)r) * 0x1p-31f * (float)(1 << (k % 20)));
+    }
+    std::vector<float> xs, ys, want;
+    for (float X : vals)
+        for (float Y : vals) {
+            xs.push_back(X);
+            ys.push_back(Y);
+            want.push_back(ERROR(X, Y));
+        }
+    std::vector<float> got(xs.size());
+    if (ppi_ef_evaluate(info.text.c_str(), xs.data(), ys.data(), (int32_t)xs.size(), got.data()) != PPI_OK) {
+        info.why = ppi_last_error();
+        return info;
+    }
+    for (size_t k = 0; k < xs.size(); ++k) {
+        uint32_t a, b;
+        std::memcpy(&a, &want[k], 4);
+        std::memcpy(&b, &got[k], 4);
+        if (a != b && !(want[k] != want[k] && got[k] != got[k])) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "the compiled expression gives %a at X = %a, Y = %a where the build gives %a",
+                     (double)got[k], (double)xs[k], (double)ys[k], (double)want[k]);
+            info.why = buf;
+            return info;
+        }
+    }
+    info.ef = PPI_EF_EXPR;
     return info;
 }
 
@@ -177,9 +221,9 @@ int setup(Interp& I, int device, int strategy, int form, int mls, int rows_per_l
           float** input, int nlin, int ncol, bool verbose, bool native) {
     ErrorFnInfo ef = detect_error_fn();
     if (ef.ef < 0) {
-        fprintf(stderr, "Unsupported error function ERROR(X,Y) = %s: the engine evaluates fabs((X)-(Y)), "
-                        "((X)-(Y))*((X)-(Y)), ((X)!=(Y)), COST[(int)X][(int)Y] and its clamped form.\n",
-                ef.text.c_str());
+        fprintf(stderr, "Unsupported error function ERROR(X,Y) = %s: %s\n", ef.text.c_str(),
+                ef.why.empty() ? "COST is supported as COST[(int)X][(int)Y] and its clamped form"
+                               : ef.why.c_str());
         return 1;
     }
     ppi_status s = ppi_engine_create(&I.eng, device < 0 ? 0 : device);
@@ -201,6 +245,7 @@ int setup(Interp& I, int device, int strategy, int form, int mls, int rows_per_l
     cfg.native_math = native ? 1 : 0;
     cfg.cost_n = ef.cost_n;
     cfg.cost = ef.cost.empty() ? nullptr : ef.cost.data();
+    cfg.ef_expr = ef.ef == PPI_EF_EXPR ? ef.text.c_str() : nullptr;
     cfg.max_local_size = mls > 0 ? mls : 0;
     cfg.rows_per_lane = rows_per_lane;
     s = ppi_engine_configure(I.eng, &cfg, kSymbolTable, (int32_t)(sizeof(kSymbolTable) / sizeof(kSymbolTable[0])));

@@ -88,3 +88,40 @@ def sums_agree(vec, ovec, exact_sum, bad, nlin, alpha, size, rtol=1e-5):
     better = err_g <= err_o + 1e-6 * np.abs(ex)
     sentinel = (np.asarray(bad) != 0) | (size == 0) | (ovec >= 3.4e38) | (vec >= 3.4e38)
     return np.where(sentinel, vec == ovec, close | better)
+
+
+# The headers sequential.cc includes (sequential.cc:16-22): a user ERROR(X,Y) compiled after them
+# sees the overloads the reference's sequential path sees.
+_EF_PRELUDE = ("#include <stdio.h>\n#include <stdlib.h>\n#include <iostream>\n#include <cmath>\n#include <string>\n"
+               "#include <limits>\n#include <queue>\n")
+
+
+class CompiledEF:
+    """User error functions compiled by g++ as the reference compiles them into sequential.cc
+    (`float error = ERROR(X, Y);`, no contraction): `fn[text]` is a ctypes function
+    float(float, float) and `addr[text]` its address (for the oracle's ef_fn)."""
+
+    def __init__(self, texts, workdir):
+        import ctypes as C
+        import os
+        import subprocess
+        src = os.path.join(str(workdir), "ef_refs.cc")
+        lib = os.path.join(str(workdir), "libef_refs.so")
+        with open(src, "w") as f:
+            f.write(_EF_PRELUDE)
+            for k, t in enumerate(texts):
+                f.write(f'extern "C" float ef_ref_{k}(float X, float Y) {{ float error = {t}; return error; }}\n')
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", lib, src],
+                       check=True)
+        self._lib = C.CDLL(lib)
+        self.fn, self.addr = {}, {}
+        for k, t in enumerate(texts):
+            f = getattr(self._lib, f"ef_ref_{k}")
+            f.restype = C.c_float
+            f.argtypes = [C.c_float, C.c_float]
+            self.fn[t] = f
+            self.addr[t] = C.cast(f, C.c_void_p).value
+
+    def values(self, text, X, Y):
+        f = self.fn[text]
+        return np.array([f(float(x), float(y)) for x, y in zip(X, Y)], dtype=np.float32)

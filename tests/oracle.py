@@ -16,7 +16,7 @@ class OracleCfg(C.Structure):
     _fields_ = [
         ("strategy", C.c_int32), ("error_fn", C.c_int32), ("reducemax", C.c_int32), ("protected_ops", C.c_int32),
         ("cost_n", C.c_int32), ("cost", C.POINTER(C.c_float)), ("dp_local", C.c_int32), ("pdp_local", C.c_int32),
-        ("best_local", C.c_int32),
+        ("best_local", C.c_int32), ("ef_fn", C.c_void_p),
     ]
 
 
@@ -96,8 +96,12 @@ def reference_geometry(nlin, nprog, cu=256, max_wg=1024, lds=160 * 1024, mls=0):
     return dp_local, pdp_local, best_local
 
 
-def make_cfg(strategy, error_fn, reducemax=False, protected=True, cost=None, dp_local=1, pdp_local=1, best_local=1):
+def make_cfg(strategy, error_fn, reducemax=False, protected=True, cost=None, dp_local=1, pdp_local=1, best_local=1,
+             ef_fn=None):
+    """ef_fn: for EF_EXPR, the address of a compiled C function float(float X, float Y)."""
     c = OracleCfg()
+    if ef_fn is not None:
+        c.ef_fn = ef_fn
     c.strategy, c.error_fn, c.reducemax, c.protected_ops = strategy, error_fn, int(reducemax), int(protected)
     if cost is not None:
         c._cost = np.ascontiguousarray(cost, dtype=np.float32)

@@ -162,3 +162,30 @@ def test_profiling_variant_prints_reference_fields(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "time_kernel[1]:" in r.stdout and "gpops_kernel_communication:" in r.stdout
+
+
+EXPR_EF = "(((X)-(Y))*((X)-(Y))/(1.0f+fabs(Y)))"  # tests/shim/expr/definitions.h
+
+
+@pytest.mark.parametrize("mode,strategy,argv", [
+    ("acc", ppiamd.PDP, []),
+    ("acc", ppiamd.PP, ["-strategy", "PP"]),
+    ("acc", ppiamd.FP, ["-strategy", "DP"]),
+    ("seq", ppiamd.SEQ, []),
+])
+def test_user_error_function(tmp_path, mode, strategy, argv):
+    """A build with a user -DEF (not one of the engine's built-ins): the drop-in hands the expression
+    to the engine, which checks it against the build's own ERROR(X,Y) at init and evaluates every
+    program in the reference's partition and row order for the strategy, so the error vector is the
+    oracle's bit for bit (the oracle calling the same expression compiled by g++)."""
+    from helpers import CompiledEF
+    table, pop = population(600, 31)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 32, 1400, 4)
+    vec, idx, _ = run_harness(tmp_path, "expr", mode, data, pop, argv, best_size=2)
+    ref = CompiledEF([EXPR_EF], tmp_path)
+    dpl, ppl, bpl = oracle.reference_geometry(data.shape[0], pop.n)
+    cfg = oracle.make_cfg(strategy, ppiamd.EF_EXPR, False, True, None, dpl, ppl, bpl, ef_fn=ref.addr[EXPR_EF])
+    ovec, oidx = oracle.interpret(cfg, table, pop, data, alpha=ALPHA, best_size=2)
+    assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32)), \
+        f"{int((vec.view(np.uint32) != ovec.view(np.uint32)).sum())} differ"
+    assert list(idx) == list(oidx)
