@@ -110,7 +110,21 @@ def bench_generations(eng, w, gens):
         up += st.upload_ms
         ker += st.kernel_ms
     wall = (time.perf_counter() - t0) * 1e3 / gens
-    return {"ms_per_generation": round(wall, 3), "upload_ms": round(up / gens, 3), "device_ms": round(ker / gens, 3)}
+    # A fresh population every generation (never seen before: new chains for the derived-column
+    # table, which is refilled when it covers under half of a population's chain occurrences).
+    fresh = [ppiamd.workload("regression", seed=100 + g).pop for g in range(gens)]
+    up2 = ker2 = 0.0
+    t0 = time.perf_counter()
+    for p in fresh:
+        eng.evaluate(p, alpha=1e-6, best_size=1)
+        st = eng.stats()
+        up2 += st.upload_ms
+        ker2 += st.kernel_ms
+    wall2 = (time.perf_counter() - t0) * 1e3 / gens
+    return {"ms_per_generation": round(wall, 3), "upload_ms": round(up / gens, 3), "device_ms": round(ker / gens, 3),
+            "fresh_populations": {"ms_per_generation": round(wall2, 3), "upload_ms": round(up2 / gens, 3),
+                                  "device_ms": round(ker2 / gens, 3),
+                                  "derived_refills": eng.stats().derived_refills}}
 
 
 def bench_batched(steps, warmup):

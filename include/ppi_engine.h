@@ -146,6 +146,8 @@ typedef struct ppi_stats {
     int32_t derived_columns;/* unary-of-input columns precomputed for the FP form (this dataset) */
     int32_t folded_subtrees;/* constant subtrees of the last program set evaluated once at upload */
     int32_t folded_nodes;   /* operator nodes inside them */
+    int32_t derived_refills;/* times the full derived-column table was refilled for a population it no longer
+                               covered, since the dataset or configuration was set */
 } ppi_stats;
 
 typedef struct ppi_engine ppi_engine;

@@ -467,6 +467,7 @@ ppi_status ppi_batch_add_dataset(ppi_batch* b, const float* data, int32_t nlin, 
     d.cap_X = cm.size();
     d.derived.reset(new DerivedColumns());
     d.derived->reset(ncol, derived_capacity(d.ldx, ncol));
+    d.derived->refill = false;  // populations compiled by earlier builds keep reading the columns
     hipError_t e = hipMemcpy(d.dX, cm.data(), cm.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(d.dX);

@@ -46,6 +46,10 @@ int32_t derived_capacity(int64_t ldx, int32_t ncol) {
 ppi_status stage_derived_columns(float** X, size_t* cap_floats, int64_t ldx, int32_t ncol, DerivedColumns& table,
                                  int32_t* ready, uint32_t** dOps, int32_t** dSrc, size_t* cap_list, bool P, bool N,
                                  hipStream_t st) {
+    if (table.staged_epoch != table.epoch) {  // the table was refilled: every column is new
+        *ready = 0;
+        table.staged_epoch = table.epoch;
+    }
     const int32_t total = (int32_t)table.cols.size();
     const int32_t fresh = total - *ready;
     if (fresh <= 0) return PPI_OK;
@@ -452,6 +456,7 @@ ppi_status ppi_engine_configure(ppi_engine* E, const ppi_config* cfg, const ppi_
 static void reset_derived(ppi_engine* E) {
     E->derived.reset(E->ncol, derived_capacity(E->ldx, E->ncol));
     E->derived_ready = 0;
+    E->derived_epoch0 = E->derived.epoch;
 }
 
 // Computes the derived columns the last compile added.
@@ -955,6 +960,7 @@ ppi_status ppi_engine_get_stats(const ppi_engine* E, ppi_stats* out) {
     out->rows_per_lane = E->R;
     out->programs_per_tile = E->chunk;
     out->derived_columns = E->derived_ready;
+    out->derived_refills = (int32_t)(E->derived.epoch - E->derived_epoch0);
     return PPI_OK;
 }
 

@@ -66,6 +66,7 @@ struct ppi_engine {
     size_t cap_X = 0;
     ppi::DerivedColumns derived;
     int32_t derived_ready = 0;  // derived columns already computed on the device
+    uint32_t derived_epoch0 = 0; // table epoch when the dataset or configuration was set
     uint32_t* dDerOps = nullptr;
     int32_t* dDerSrc = nullptr;
     size_t cap_der = 0;

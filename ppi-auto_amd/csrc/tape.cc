@@ -431,7 +431,9 @@ void apply_folds(Tape* t, const float* value) {
 
 void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                   int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived) {
-    if (!derived || derived->cap <= 0 || (int32_t)derived->cols.size() >= derived->cap) return;
+    if (!derived || derived->cap <= 0) return;
+    const bool full = (int32_t)derived->cols.size() >= derived->cap;
+    if (full && !derived->refill) return;
     threads = std::max(1, std::min(threads, n / 128));
     std::vector<std::unordered_map<uint64_t, uint32_t>> counts((size_t)threads);
     if (threads == 1) {
@@ -449,6 +451,15 @@ void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* 
         for (auto& th : pool) th.join();
         for (int t = 1; t < threads; ++t)
             for (const auto& kv : counts[(size_t)t]) counts[0][kv.first] += kv.second;
+    }
+    if (full) {
+        uint64_t all = 0, covered = 0;
+        for (const auto& kv : counts[0]) {
+            all += kv.second;
+            if (derived->index.count(kv.first)) covered += kv.second;
+        }
+        if (all == 0 || (double)covered >= kDerivedKeepShare * (double)all) return;
+        derived->reset(derived->base, derived->cap);
     }
     struct Cand { uint64_t key; uint32_t count; int ops; };
     std::vector<Cand> cand;

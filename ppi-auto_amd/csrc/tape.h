@@ -179,6 +179,10 @@ constexpr int kChainMaxOps = 9;
 struct DerivedColumns {
     int32_t base = 0;  // first derived column (the dataset's column count)
     int32_t cap = 0;   // most derived columns (0: off)
+    uint32_t epoch = 0;        // counts resets: columns staged under an older epoch are stale
+    uint32_t staged_epoch = 0; // epoch of the columns on the device (stage_derived_columns)
+    bool refill = true;        // false: never reset by plan_derived (compiled tapes of earlier
+                               // populations still read the columns: the batched evaluator)
     struct Col { uint32_t op; int32_t src; };
     std::vector<Col> cols;                          // column base + j = op(column src), src < base + j
     std::unordered_map<uint64_t, int32_t> index;    // chain key -> column
@@ -187,6 +191,7 @@ struct DerivedColumns {
         cap = c;
         cols.clear();
         index.clear();
+        ++epoch;
     }
     // Column holding the chain, or -1.
     int32_t find(uint64_t key) const {
@@ -196,7 +201,11 @@ struct DerivedColumns {
 };
 
 // Adds columns for the unary chains of programs [0, n) to `derived` (see above);
-// `threads` workers count the chains.
+// `threads` workers count the chains.  A full table whose columns cover less than
+// kDerivedKeepShare of the population's chain occurrences is reset (its epoch
+// advances) and refilled from this population: over a long run the table follows
+// the chains the programs use now instead of the first generations'.
+constexpr double kDerivedKeepShare = 0.5;
 void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
                   int32_t stride, int32_t n, int32_t ncol, int threads, DerivedColumns* derived);
 

@@ -63,7 +63,7 @@ class Stats(C.Structure):
         ("download_ms", C.c_double), ("node_evals", C.c_double), ("executed_nodes", C.c_double),
         ("form", C.c_int32), ("rows_per_lane", C.c_int32), ("programs_per_tile", C.c_int32),
         ("max_slots", C.c_int32), ("derived_columns", C.c_int32), ("folded_subtrees", C.c_int32),
-        ("folded_nodes", C.c_int32),
+        ("folded_nodes", C.c_int32), ("derived_refills", C.c_int32),
     ]
 
 

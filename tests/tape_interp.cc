@@ -167,3 +167,34 @@ extern "C" int tape_eval(const ppi_symbol* table, int ntable, const int32_t* ph,
                          int32_t* slots, int32_t* ncode) {
     return tape_eval_ex(table, ntable, ph, eph, size, stride, n, data, nlin, ncol, P, out, slots, ncode, 0, nullptr);
 }
+
+// The derived-column table across populations (plan_derived's refill rule): out[0] columns and
+// out[1] epoch after population A; out[2] epoch after A again; out[3] epoch and out[4] columns
+// after B; out[5] 1 if every chain key of the table after B occurs in B's programs as counted by a
+// fresh table; out[6] epoch after A with refills turned off.
+extern "C" int derived_refill(const ppi_symbol* table, int ntable, const int32_t* phA, const float* ephA,
+                              const int32_t* sizeA, int strideA, int nA, const int32_t* phB, const float* ephB,
+                              const int32_t* sizeB, int strideB, int nB, int ncol, int cap, int32_t* out) {
+    SymbolTable st;
+    st.build(table, ntable);
+    DerivedColumns dc;
+    dc.reset(ncol, cap);
+    const uint32_t e0 = dc.epoch;
+    plan_derived(st, phA, ephA, sizeA, strideA, nA, ncol, 1, &dc);
+    out[0] = (int32_t)dc.cols.size();
+    out[1] = (int32_t)(dc.epoch - e0);
+    plan_derived(st, phA, ephA, sizeA, strideA, nA, ncol, 1, &dc);
+    out[2] = (int32_t)(dc.epoch - e0);
+    plan_derived(st, phB, ephB, sizeB, strideB, nB, ncol, 1, &dc);
+    out[3] = (int32_t)(dc.epoch - e0);
+    out[4] = (int32_t)dc.cols.size();
+    DerivedColumns fresh;
+    fresh.reset(ncol, cap);
+    plan_derived(st, phB, ephB, sizeB, strideB, nB, ncol, 1, &fresh);
+    out[5] = fresh.index == dc.index ? 1 : 0;
+    dc.refill = false;
+    const uint32_t e1 = dc.epoch;
+    plan_derived(st, phA, ephA, sizeA, strideA, nA, ncol, 1, &dc);
+    out[6] = (int32_t)(dc.epoch - e1);
+    return 0;
+}

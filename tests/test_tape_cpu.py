@@ -187,3 +187,29 @@ def test_folded_edge_programs(tape_lib):
         assert ok.all(), f"protected={prot}: programs {np.unique(np.nonzero(~ok)[0])}"
         assert info[2] >= 13
         assert (ncode[[0, 1, 2, 5, 6, 7, 12, 16, 17]] == 1).all()  # one load of the folded value
+
+
+def test_derived_table_follows_the_population(tape_lib):
+    """A full derived-column table that covers under half of a new population's chain occurrences is
+    refilled from that population (the same table a fresh start would give); one that covers it is
+    kept; with refills off (the batched evaluator) it is never reset."""
+    b = Builder(4)
+    a_progs = [["sin", "x0"], ["cos", "x1"], ["exp", "x0"], ["+", "sin", "x0", "cos", "x1"], ["log", "exp", "x0"]] * 4
+    b_progs = [["sin", "x2"], ["cos", "x3"], ["sqrt", "x2"], ["*", "sin", "x2", "cos", "x3"], ["exp", "sqrt", "x3"]] * 4
+    pa, pb = b.population(a_progs), b.population(b_progs)
+    L = tape_lib
+    P = C.POINTER
+    L.derived_refill.argtypes = [P(ppiamd.Symbol), C.c_int] + [P(C.c_int32), P(C.c_float), P(C.c_int32), C.c_int,
+                                                                C.c_int] * 2 + [C.c_int, C.c_int, P(C.c_int32)]
+    out = np.zeros(7, dtype=np.int32)
+    p = lambda a, t: a.ctypes.data_as(P(t))  # noqa: E731
+    args = []
+    for pop in (pa, pb):
+        args += [p(pop.phenotype, C.c_int32), p(pop.ephemeral, C.c_float), p(pop.size, C.c_int32), pop.stride, pop.n]
+    assert L.derived_refill(oracle._table(b.table), len(b.table), *args, 4, 4, p(out, C.c_int32)) == 0
+    cols_a, ep_a, ep_aa, ep_b, cols_b, same_as_fresh, ep_norefill = out
+    assert cols_a == 4 and ep_a == 0          # filled to the cap, no reset
+    assert ep_aa == 0                          # the same population is covered: kept
+    assert ep_b == 1 and cols_b == 4           # B's chains are not covered: refilled from B
+    assert same_as_fresh == 1
+    assert ep_norefill == 0
