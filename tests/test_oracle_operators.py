@@ -447,3 +447,39 @@ def test_float_computation_gradient_closed_forms():
     # the FP64 mode differs from it (the forward values are doubles there)
     g64, _ = oracle.grad_mag_f64(cfg, b.table, pop, data, max_consts=2, f32_values=False)
     assert g64[1, 0] != g[1, 0]
+
+
+def test_math_hard_inputs_fixture():
+    """The hard inputs of tests/golden/math_hard_inputs.npz are what their generator says: glibc's double
+    result (Python's math module calls the C library) lies within 256 ulps of a float rounding boundary,
+    and neighbouring inputs that are not listed do not."""
+    import math
+    import os
+    import struct
+
+    data = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "math_hard_inputs.npz"))
+    assert sorted(data.files) == sorted(["exp", "log", "sin", "cos", "tan", "exp10", "exp2", "log10", "log2"])
+
+    def hard(y):
+        if not (2.0 ** -126 <= abs(y) < 2.0 ** 128):
+            return False
+        d = (struct.unpack("<Q", struct.pack("<d", y))[0] & 0x1fffffff) - 0x10000000
+        return -256 <= d <= 256
+
+    fns = {"exp": math.exp, "log": math.log, "sin": math.sin, "cos": math.cos, "tan": math.tan,
+           "log10": math.log10, "log2": math.log2}
+    for name, arr in data.items():
+        assert arr.dtype == np.uint32 and arr.size > 400 and np.all(np.diff(arr.astype(np.int64)) > 0), name
+        x = arr.view(np.float32)
+        assert np.all(np.isfinite(x)), name
+        if name not in fns:
+            continue
+        listed = set(arr.tolist())
+        for bits in arr[:: max(1, arr.size // 64)].tolist():
+            xf = float(np.uint32(bits).view(np.float32))
+            assert hard(fns[name](xf)), (name, hex(bits))
+            nb = bits + 1
+            if nb not in listed:
+                xn = float(np.uint32(nb).view(np.float32))
+                if math.isfinite(xn) and (name not in ("log", "log10", "log2") or xn > 0):
+                    assert not hard(fns[name](xn)), (name, hex(nb))
