@@ -324,6 +324,7 @@ BatchArgs make_args(ppi_batch* b, int32_t n_items) {
     A.cost_n = b->cfg.cost_n;
     A.alpha = 0.0f;
     A.partial = b->dPartial;
+    A.part_rows = fp_rows_per_block(b->R);
     A.bad_lo = b->dBadLo;
     A.bad_hi = b->dBadHi;
     A.raw = b->dRaw;

@@ -634,6 +634,8 @@ ppi_status ppi::enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, boo
         return PPI_OK;
     }
     if (L.form == PPI_FORM_FP) {
+        A.part_nrb = L.nrb;
+        A.part_rows = fp_rows_per_block(L.R);
         PPI_HIP(launch_fp(A, L.R, L.F, L.S, P, N, E->stream));
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
         PPI_HIP(launch_finalize(A, L.nrb, E->stream));

@@ -55,6 +55,9 @@ struct EvalArgs {
     uint32_t* ostop;      // [nprog] 1 once a row's error was not finite
     const EfIns* ef;      // PPI_EF_EXPR: the compiled ERROR(X,Y) (ef.h), ef_n operations
     int32_t ef_n;
+    // The row-tile partials the recomputation may start from: part_nrb tiles of part_rows rows
+    // in `partial` (FP and batched forms); 0 where the form keeps none.
+    int32_t part_nrb, part_rows;
 };
 
 // Batched form: one (population, dataset) pair as the device sees it.

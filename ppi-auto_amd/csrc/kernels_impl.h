@@ -1089,17 +1089,53 @@ __device__ __forceinline__ double sum_rows_double(const EvalArgs& A, const ProgM
     return r;
 }
 
+// The same sum from the program's row-tile partials (pp[rb * A.nprog], A.part_nrb tiles of
+// A.part_rows rows): finite partials as they are, and the rows of every tile whose float partial
+// overflowed recomputed in double, so a program with one huge tile costs that tile's rows, not a
+// pass over the dataset.  The tiles are visited in order by every thread (the same address, so
+// the branch is uniform) and each thread's share is fixed, so the result does not vary between
+// runs.  Relative error against the exact sum: a partial's at most 14 float roundings (as in
+// reduce_partials) plus the double additions.
+template <bool P, bool N>
+__device__ __forceinline__ double sum_tiles_double(const EvalArgs& A, const ProgMeta& m, const float* pp,
+                                                   double* red) {
+    double s = 0.0;
+    for (int rb = (int)threadIdx.x; rb < A.part_nrb; rb += kBlock) {
+        const float v = pp[(int64_t)rb * A.nprog];
+        if (!isinf(v)) s += (double)v;
+    }
+    for (int rb = 0; rb < A.part_nrb; ++rb) {
+        if (!isinf(pp[(int64_t)rb * A.nprog])) continue;
+        const int r1 = min((rb + 1) * A.part_rows, A.nlin);
+        for (int n = rb * A.part_rows + (int)threadIdx.x; n < r1; n += kBlock) s += (double)row_error<P, N>(A, m, n);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
 // A program marked for its sum is first settled from the double sum when that sum
 // is clear of float overflow by more than any order's rounding (and, under DP,
 // finite: an overflow's sentinel depends on the work-groups); the rest take the
 // partition-exact path.  bad: the bad-row record where the form keeps one (FP and
 // batched forms); elsewhere a bad row makes the double sum NaN or infinite, which
-// also sends the program on.
+// also sends the program on.  pp: the program's first row-tile partial where the
+// form keeps them (A.part_nrb > 0), else null.
 template <bool P, bool N>
 __device__ __forceinline__ void recheck_program(const EvalArgs& A, const ProgMeta& m, bool bad, float* raw_out,
-                                                float* vec_out, float* PE, int* stop_s) {
+                                                float* vec_out, float* PE, int* stop_s, const float* pp) {
     if (!bad && !A.reducemax) {
-        const SumTotal t = decide_sum(A, sum_rows_double<P, N>(A, m, reinterpret_cast<double*>(PE)), 1e-9);
+        double* red = reinterpret_cast<double*>(PE);
+        const SumTotal t =
+            pp ? decide_sum(A, sum_tiles_double<P, N>(A, m, pp, red),
+                            14.0 * kUlp + (double)(A.part_nrb + A.nlin) * 0x1.0p-53)
+               : decide_sum(A, sum_rows_double<P, N>(A, m, red), 1e-9);
         if (!t.undecided && (A.strategy != PPI_STRATEGY_FP || finite_f(t.total))) {
             if (threadIdx.x == 0) semantics(A, t.total, m.flags, 0xffffffffu, 0u, m.size, raw_out, vec_out);
             return;
@@ -1130,7 +1166,7 @@ __global__ void __launch_bounds__(kBlock) recheck_kernel(EvalArgs A) {
         if (threadIdx.x == 0) stop_s = 0;
         __syncthreads();
         recheck_program<P, N>(A, load_meta(A.meta, p), A.bad_lo[p] != 0xffffffffu, &A.raw[p], &A.vec[p], PE,
-                              &stop_s);
+                              &stop_s, A.part_nrb > 0 ? A.partial + p : nullptr);
         __syncthreads();
     }
 }
@@ -1178,8 +1214,9 @@ __global__ void __launch_bounds__(kBlock) batch_recheck_kernel(BatchArgs B, cons
         A.pdp_lanes = pd.pdp_lanes;
         A.dp_groups = pd.dp_groups;
         A.dp_local = pd.dp_local;
+        A.part_nrb = pd.nrb;
         recheck_program<P, N>(A, load_meta(B.base.meta, g), B.base.bad_lo[g] != 0xffffffffu, &B.base.raw[g],
-                              &B.base.vec[g], PE, &stop_s);
+                              &B.base.vec[g], PE, &stop_s, B.base.partial + pd.partial_base + (g - pd.prog_base));
         __syncthreads();
     }
 }
