@@ -117,6 +117,12 @@ struct ppi_batch {
     float* dVec = nullptr;
     float* dCost = nullptr;
     uint32_t* dCounter = nullptr;
+    // the recheck's deep list (pairs of kDeepMinRows rows or more): kDeepCap entries of deep_stride rows
+    uint32_t* dDeepCount = nullptr;
+    int32_t* dDeepProg = nullptr;
+    float* dDeepErr = nullptr;
+    int64_t deep_stride = 0;
+    int32_t deep_rows = 0;
     float* hRaw = nullptr;
     size_t hRawCap = 0;
 
@@ -139,11 +145,13 @@ void free_build(ppi_batch* b) {
     b->graph = nullptr;
     // (the queue counter lives as long as the batch object)
     void* ptrs[] = {b->dMeta, b->dCode, b->dPairs, b->dItems, b->dPairOf, b->dPartial, b->dBadLo, b->dBadHi,
-                    b->dRaw, b->dVec};
+                    b->dRaw, b->dVec, b->dDeepCount, b->dDeepProg, b->dDeepErr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b->dMeta = nullptr; b->dCode = nullptr; b->dPairs = nullptr; b->dItems = nullptr; b->dPairOf = nullptr;
     b->dPartial = nullptr; b->dBadLo = nullptr; b->dBadHi = nullptr; b->dRaw = nullptr; b->dVec = nullptr;
+    b->dDeepCount = nullptr; b->dDeepProg = nullptr; b->dDeepErr = nullptr;
+    b->deep_stride = 0; b->deep_rows = 0;
     b->built = false;
 }
 
@@ -329,6 +337,11 @@ BatchArgs make_args(ppi_batch* b, int32_t n_items) {
     A.bad_hi = b->dBadHi;
     A.raw = b->dRaw;
     A.vec = b->dVec;
+    if (b->dDeepErr) {
+        A.deep_count = b->dDeepCount;
+        A.deep_prog = b->dDeepProg;
+        A.deep_cap = kDeepCap;
+    }
     B.pairs = b->dPairs;
     B.items = b->dItems;
     B.n_items = n_items;
@@ -338,6 +351,23 @@ BatchArgs make_args(ppi_batch* b, int32_t n_items) {
 
 int32_t total_programs(const ppi_batch* b) {
     return b->pops.empty() ? 0 : b->pops.back().prog_base + b->pops.back().n;
+}
+
+// The recheck and its deep list.
+ppi_status enqueue_recheck(ppi_batch* b, const BatchArgs& B, int32_t n) {
+    if (B.base.deep_cap > 0) B_HIP(launch_fill_u32(b->dDeepCount, 0u, 1, b->stream));
+    B_HIP(launch_batch_recheck(B, b->dPairOf, n, b->P, b->N, b->stream));
+    if (B.base.deep_cap > 0) {
+        DeepArgs D{};
+        D.base = B.base;
+        D.pairs = b->dPairs;
+        D.pair_of = b->dPairOf;
+        D.err = b->dDeepErr;
+        D.stride = b->deep_stride;
+        D.max_rows = b->deep_rows;
+        B_HIP(launch_deep(D, b->dev_cu * 4, b->P, b->N, b->stream));
+    }
+    return PPI_OK;
 }
 
 // Queue reset, persistent kernel, epilogue (optionally the download).
@@ -350,7 +380,8 @@ ppi_status enqueue_run(ppi_batch* b, int32_t n_items, bool with_finalize, bool w
     B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
     if (with_finalize) {
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
-        B_HIP(launch_batch_recheck(B, b->dPairOf, n, b->P, b->N, b->stream));
+        ppi_status s = enqueue_recheck(b, B, n);
+        if (s) return s;
     }
     if (with_download)
         B_HIP(hipMemcpyAsync(b->hRaw, b->dRaw, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
@@ -584,6 +615,18 @@ ppi_status ppi_batch_build(ppi_batch* b) {
         max_part = std::max(max_part, part);
     }
     if ((s = dalloc(&b->dPartial, (size_t)max_part))) return s;
+    b->deep_rows = 0;
+    b->deep_stride = 0;
+    for (const Pop& p : b->pops) {
+        const Dataset& d = b->datasets[p.dataset];
+        b->deep_rows = std::max(b->deep_rows, d.nlin);
+        b->deep_stride = std::max(b->deep_stride, d.ldx);
+    }
+    if (b->deep_rows >= kDeepMinRows) {
+        if ((s = dalloc(&b->dDeepCount, 1))) return s;
+        if ((s = dalloc(&b->dDeepProg, (size_t)kDeepCap))) return s;
+        if ((s = dalloc(&b->dDeepErr, (size_t)kDeepCap * b->deep_stride))) return s;
+    }
 
     const double items_per_slot = 6.0;
     std::memset(b->stats.tune_ms, 0, sizeof(b->stats.tune_ms));
@@ -731,7 +774,8 @@ ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
         B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
         B_HIP(hipEventRecord(b->ev[3], b->stream));
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
-        B_HIP(launch_batch_recheck(B, b->dPairOf, n, b->P, b->N, b->stream));
+        ppi_status s = enqueue_recheck(b, B, n);
+        if (s) return s;
         B_HIP(hipEventSynchronize(b->ev[3]));
         float t = 0.0f;
         B_HIP(hipEventElapsedTime(&t, b->ev[2], b->ev[3]));

@@ -369,7 +369,7 @@ void ppi_engine_destroy(ppi_engine* E) {
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     void* dev[] = {E->dX, E->dMeta, E->dCode, E->dPartial, E->dBadLo, E->dBadHi, E->dRaw, E->dVec,
                    E->dOrder, E->dCost, E->dEf, E->dPred, E->dDerOps, E->dDerSrc, E->dRowErr, E->dOSum, E->dOStop,
-                   E->dRankVec, E->dRankGv, E->dRankGi, E->dRankIdx};
+                   E->dRankVec, E->dRankGv, E->dRankGi, E->dRankIdx, E->dDeepCount, E->dDeepProg, E->dDeepErr};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     E->folds.release();
@@ -480,6 +480,13 @@ static ppi_status stage_dataset(ppi_engine* E, int32_t nlin, int32_t ncol, const
     if (s) return s;
     E->cap_X = cap;
     PPI_HIP(hipMemcpy(E->dX, colmajor.data(), colmajor.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (nlin >= kDeepMinRows) {
+        if (!E->dDeepCount && (s = ensure(&E->dDeepCount, &cap, 1))) return s;
+        if (!E->dDeepProg && (s = ensure(&E->dDeepProg, &cap, (size_t)kDeepCap))) return s;
+        cap = E->cap_deep;
+        if ((s = ensure(&E->dDeepErr, &cap, (size_t)kDeepCap * ldx))) return s;
+        E->cap_deep = cap;
+    }
     E->nlin = nlin;
     E->ncol = ncol;
     E->ldx = ldx;
@@ -614,7 +621,28 @@ EvalArgs ppi::make_args(ppi_engine* E, float alpha) {
     A.ostop = E->dOStop;
     A.ef = E->dEf;
     A.ef_n = (int32_t)E->ef.size();
+    if (E->nlin >= kDeepMinRows && E->dDeepErr) {
+        A.deep_count = E->dDeepCount;
+        A.deep_prog = E->dDeepProg;
+        A.deep_cap = kDeepCap;
+    }
     return A;
+}
+
+// The recheck and its deep list on the engine stream.
+static ppi_status enqueue_recheck(ppi_engine* E, const EvalArgs& A) {
+    const bool P = E->cfg.protected_ops != 0, N = E->cfg.native_math != 0;
+    if (A.deep_cap > 0) PPI_HIP(launch_fill_u32(A.deep_count, 0u, 1, E->stream));
+    PPI_HIP(launch_recheck(A, P, N, E->stream));
+    if (A.deep_cap > 0) {
+        DeepArgs D{};
+        D.base = A;
+        D.err = E->dDeepErr;
+        D.stride = E->ldx;
+        D.max_rows = A.nlin;
+        PPI_HIP(launch_deep(D, E->dev_cu * 4, P, N, E->stream));
+    }
+    return PPI_OK;
 }
 
 // Enqueue one evaluation on the engine stream: programs, work and result buffers as `A` names them,
@@ -629,7 +657,8 @@ ppi_status ppi::enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, boo
         // reference's own partition and row order for the strategy), the expression
         // interpreted per row.
         PPI_HIP(launch_ef_prepare(A, E->stream));
-        PPI_HIP(launch_recheck(A, P, N, E->stream));
+        ppi_status s = enqueue_recheck(E, A);
+        if (s) return s;
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
         return PPI_OK;
     }
@@ -658,8 +687,7 @@ ppi_status ppi::enqueue_with(ppi_engine* E, EvalArgs A, const LaunchPlan& L, boo
         if (time_main) PPI_HIP(hipEventRecord(E->ev[3], E->stream));
     }
     // The programs whose result depends on the reference's partition or order of the rows.
-    PPI_HIP(launch_recheck(A, P, N, E->stream));
-    return PPI_OK;
+    return enqueue_recheck(E, A);
 }
 
 // Enqueue one evaluation of the resident programs on the engine stream.
@@ -864,6 +892,7 @@ ppi_status ppi_engine_evaluate_noalloc(ppi_engine* E, const int32_t* phenotype, 
     A.bad_hi = reinterpret_cast<uint32_t*>(base + o_bhi);
     A.raw = reinterpret_cast<float*>(base + o_raw);
     A.vec = reinterpret_cast<float*>(base + o_vec);
+    A.deep_cap = 0;  // every buffer of this call lives in the scratch array: no deep list
     A.rb0 = 0;
     A.nrb_pass = L.nrb;
     A.rowerr = nullptr;

@@ -107,6 +107,11 @@ struct ppi_engine {
     uint32_t* dOStop = nullptr;
     size_t cap_osum = 0;
     int nrb_pass = 1;
+    // the recheck's deep list (datasets of kDeepMinRows rows or more): kDeepCap entries of ldx rows
+    uint32_t* dDeepCount = nullptr;
+    int32_t* dDeepProg = nullptr;
+    float* dDeepErr = nullptr;
+    size_t cap_deep = 0;
     // fold programs of the last upload (constant subtrees, evaluated once)
     ppi::FoldBuffers folds;
 

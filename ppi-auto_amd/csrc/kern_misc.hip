@@ -75,6 +75,19 @@ hipError_t launch_batch_recheck(const BatchArgs& B, const int32_t* pair_of, int3
     return hipGetLastError();
 }
 
+hipError_t launch_deep(const DeepArgs& D, int grid, bool P, bool N, hipStream_t st) {
+    if (D.base.deep_cap <= 0) return hipSuccess;
+    if (D.base.pdp_lanes > kPdpMaxLanes || D.base.dp_local > kPdpMaxLanes || D.max_rows > D.stride || grid < 1)
+        return hipErrorInvalidValue;
+    const dim3 g((unsigned)grid), b(dev::kBlock);
+    if (P && N) hipLaunchKernelGGL((dev::deep_rows_kernel<true, true>), g, b, 0, st, D);
+    else if (P) hipLaunchKernelGGL((dev::deep_rows_kernel<true, false>), g, b, 0, st, D);
+    else if (N) hipLaunchKernelGGL((dev::deep_rows_kernel<false, true>), g, b, 0, st, D);
+    else hipLaunchKernelGGL((dev::deep_rows_kernel<false, false>), g, b, 0, st, D);
+    hipLaunchKernelGGL(dev::deep_partition_kernel, dim3((unsigned)D.base.deep_cap), b, 0, st, D);
+    return hipGetLastError();
+}
+
 hipError_t launch_pp(const EvalArgs& A, bool P, bool N, hipStream_t st) {
     const dim3 grid((unsigned)((A.nprog + dev::kPPBlock - 1) / dev::kPPBlock)), block(dev::kPPBlock);
     if (P && N) hipLaunchKernelGGL((dev::pp_kernel<true, true>), grid, block, 0, st, A);

@@ -58,6 +58,12 @@ struct EvalArgs {
     // The row-tile partials the recomputation may start from: part_nrb tiles of part_rows rows
     // in `partial` (FP and batched forms); 0 where the form keeps none.
     int32_t part_nrb, part_rows;
+    // Deep list of the recomputation (deep_cap > 0): programs whose partition-exact result needs
+    // every row are queued here, their rows computed across the device and then reduced in the
+    // reference's partition, instead of by one workgroup each.
+    uint32_t* deep_count;  // entries queued (may exceed deep_cap: the rest stay in the recheck)
+    int32_t* deep_prog;    // [deep_cap] program id (the engine's index, or the batch's global one)
+    int32_t deep_cap;
 };
 
 // Batched form: one (population, dataset) pair as the device sees it.
@@ -86,6 +92,17 @@ struct BatchArgs {
     const WorkItem* items;
     int32_t n_items;
     uint32_t* counter;      // work-queue head, reset to 0 before every launch
+};
+
+// The deep list's row errors: entry k's rows at err + k * stride.  pairs / pair_of: the batched
+// form's pair table (null for the engine).
+struct DeepArgs {
+    EvalArgs base;
+    const PairDev* pairs;
+    const int32_t* pair_of;
+    float* err;
+    int64_t stride;     // >= the longest dataset's rows
+    int32_t max_rows;   // rows of the longest dataset
 };
 
 // Reverse-mode adjoint: waves take (job, row range) units from a queue counter.
@@ -182,6 +199,12 @@ hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st);
 hipError_t launch_recheck(const EvalArgs& A, bool P, bool N, hipStream_t st);
 hipError_t launch_batch_recheck(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, bool P, bool N,
                                 hipStream_t st);
+// The recheck's deep list (D.base.deep_cap > 0), after launch_recheck / launch_batch_recheck: the
+// queued programs' row errors over `grid` workgroups, then one workgroup per entry for the
+// partition-exact result.  Programs are queued only from datasets of at least kDeepMinRows rows.
+constexpr int32_t kDeepCap = 16;
+constexpr int32_t kDeepMinRows = 8192;
+hipError_t launch_deep(const DeepArgs& D, int grid, bool P, bool N, hipStream_t st);
 hipError_t launch_predict(const EvalArgs& A, int F, int S, bool P, bool N, float* out, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 // PPI_EF_EXPR: empty programs get their result, every other one is marked for the exact-order

@@ -929,6 +929,19 @@ __device__ __forceinline__ float row_error(const EvalArgs& A, const ProgMeta& m,
     return error_fn(A.error_fn, acc[0], y, A.cost, A.cost_n);
 }
 
+// Sources of a program's row errors for the partition-exact paths below: computed on the spot
+// (the recheck kernels), or read from a buffer that deep_rows_kernel filled (deep_partition_kernel).
+template <bool P, bool N>
+struct RowErrCompute {
+    const EvalArgs& A;
+    const ProgMeta& m;
+    __device__ __forceinline__ float operator()(int n) const { return row_error<P, N>(A, m, n); }
+};
+struct RowErrBuffer {
+    const float* e;
+    __device__ __forceinline__ float operator()(int n) const { return e[n]; }
+};
+
 // The reference's work-group reduction over PE[0, width) (accelerator.cl:157-166,
 // 251-260): strides from half the next power of two of width down to 1; at stride
 // s, lane l < s with l + s < width takes PE[l + s] (sum, or REDUCEMAX's
@@ -950,9 +963,9 @@ __device__ __forceinline__ void reference_tree(float* PE, int width, bool rmax) 
 // rows in order and stops at its first non-finite error with MAXFLOAT; an
 // infinite or NaN tree result becomes MAXFLOAT.  The host adds alpha*size
 // (accelerator.cc:849).
-template <bool P, bool N>
+template <class ErrF>
 __device__ __forceinline__ void pdp_recheck(const EvalArgs& A, const ProgMeta& m, float* raw_out, float* vec_out,
-                                            float* PE) {
+                                            float* PE, const ErrF& err) {
     const int lanes = A.pdp_lanes;
     const int passes = (A.nlin + lanes - 1) / lanes;
     const float fn = (float)A.nlin;
@@ -963,7 +976,7 @@ __device__ __forceinline__ void pdp_recheck(const EvalArgs& A, const ProgMeta& m
         for (int j = 0; j < passes; ++j) {
             const int n = j * lanes + l;
             const bool live = !stop && n < A.nlin;
-            const float e = row_error<P, N>(A, m, live ? n : 0);
+            const float e = err(live ? n : 0);
             if (live) {
                 if (!finite_f(e)) {
                     pe = FLT_MAX;
@@ -991,9 +1004,9 @@ __device__ __forceinline__ void pdp_recheck(const EvalArgs& A, const ProgMeta& m
 // tree; the host walks the groups in order, stops at the first non-finite group
 // result with FLT_MAX, and otherwise sums them (REDUCEMAX: maximum of
 // result*nlin); a non-finite total becomes FLT_MAX, else total/nlin + alpha*size.
-template <bool P, bool N>
+template <class ErrF>
 __device__ __forceinline__ void dp_recheck(const EvalArgs& A, const ProgMeta& m, float* raw_out, float* vec_out,
-                                           float* PE, int* stop_s) {
+                                           float* PE, int* stop_s, const ErrF& err) {
     const int width = A.dp_local;
     const int groups = (A.nlin + width - 1) / width;
     const float fn = (float)A.nlin;
@@ -1003,7 +1016,7 @@ __device__ __forceinline__ void dp_recheck(const EvalArgs& A, const ProgMeta& m,
             const int l = l0 + (int)threadIdx.x;
             const int n = g * width + l;
             const bool live = l < width && n < A.nlin;
-            const float e = row_error<P, N>(A, m, live ? n : 0);
+            const float e = err(live ? n : 0);
             if (l < width) PE[l] = live ? e : 0.0f;
         }
         __syncthreads();
@@ -1031,16 +1044,16 @@ __device__ __forceinline__ void dp_recheck(const EvalArgs& A, const ProgMeta& m,
 
 // SEQ / PP (sequential.cc:169-179, accelerator.cl:40-84): rows in order, the
 // errors of kBlock rows at a time staged in LDS and taken by thread 0.
-template <bool P, bool N>
+template <class ErrF>
 __device__ __forceinline__ void ordered_recheck(const EvalArgs& A, const ProgMeta& m, float* raw_out,
-                                                float* vec_out, float* PE, int* stop_s) {
+                                                float* vec_out, float* PE, int* stop_s, const ErrF& err) {
     const float fn = (float)A.nlin;
     const bool rmax = A.reducemax != 0;
     float part = 0.0f;  // thread 0's
     uint32_t stop = 0;
     for (int n0 = 0; n0 < A.nlin; n0 += kBlock) {
         const int n = n0 + (int)threadIdx.x;
-        const float e = row_error<P, N>(A, m, n < A.nlin ? n : 0);
+        const float e = err(n < A.nlin ? n : 0);
         PE[threadIdx.x] = e;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1126,10 +1139,22 @@ __device__ __forceinline__ double sum_tiles_double(const EvalArgs& A, const Prog
 // partition-exact path.  bad: the bad-row record where the form keeps one (FP and
 // batched forms); elsewhere a bad row makes the double sum NaN or infinite, which
 // also sends the program on.  pp: the program's first row-tile partial where the
-// form keeps them (A.part_nrb > 0), else null.
+// form keeps them (A.part_nrb > 0), else null.  A program still undecided then takes
+// the deep list when A has one with room (A.deep_cap > 0: datasets long enough that one
+// workgroup per program would dominate), else the partition-exact path right here.
+// The partition-exact result of program m, from the row errors err(n), for the strategy.
+template <class ErrF>
+__device__ __forceinline__ void partition_result(const EvalArgs& A, const ProgMeta& m, float* raw_out,
+                                                 float* vec_out, float* PE, int* stop_s, const ErrF& err) {
+    if (A.strategy == PPI_STRATEGY_PDP) pdp_recheck(A, m, raw_out, vec_out, PE, err);
+    else if (A.strategy == PPI_STRATEGY_FP) dp_recheck(A, m, raw_out, vec_out, PE, stop_s, err);
+    else ordered_recheck(A, m, raw_out, vec_out, PE, stop_s, err);
+}
+
 template <bool P, bool N>
-__device__ __forceinline__ void recheck_program(const EvalArgs& A, const ProgMeta& m, bool bad, float* raw_out,
-                                                float* vec_out, float* PE, int* stop_s, const float* pp) {
+__device__ __forceinline__ void recheck_program(const EvalArgs& A, const ProgMeta& m, int id, bool bad,
+                                                float* raw_out, float* vec_out, float* PE, int* stop_s,
+                                                int* deep_s, const float* pp) {
     if (!bad && !A.reducemax) {
         double* red = reinterpret_cast<double*>(PE);
         const SumTotal t =
@@ -1141,9 +1166,20 @@ __device__ __forceinline__ void recheck_program(const EvalArgs& A, const ProgMet
             return;
         }
     }
-    if (A.strategy == PPI_STRATEGY_PDP) pdp_recheck<P, N>(A, m, raw_out, vec_out, PE);
-    else if (A.strategy == PPI_STRATEGY_FP) dp_recheck<P, N>(A, m, raw_out, vec_out, PE, stop_s);
-    else ordered_recheck<P, N>(A, m, raw_out, vec_out, PE, stop_s);
+    if (A.deep_cap > 0) {
+        // Queued for the deep kernels when the list has room (id: where raw_out and vec_out
+        // point in A.raw and A.vec).
+        if (threadIdx.x == 0) {
+            const uint32_t k = atomicAdd(A.deep_count, 1u);
+            if (k < (uint32_t)A.deep_cap) A.deep_prog[k] = id;
+            *deep_s = k < (uint32_t)A.deep_cap ? 1 : 0;
+        }
+        __syncthreads();
+        const bool queued = *deep_s != 0;
+        __syncthreads();  // every thread has read the flag before it is written again
+        if (queued) return;
+    }
+    partition_result(A, m, raw_out, vec_out, PE, stop_s, RowErrCompute<P, N>{A, m});
 }
 
 __device__ __forceinline__ bool marked(const float* raw) { return __float_as_uint(*raw) == kRecheckBits; }
@@ -1153,7 +1189,7 @@ __device__ __forceinline__ bool marked(const float* raw) { return __float_as_uin
 template <bool P, bool N>
 __global__ void __launch_bounds__(kBlock) recheck_kernel(EvalArgs A) {
     __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
-    __shared__ int stop_s;
+    __shared__ int stop_s, deep_s;
     bool staged = false;
     for (int q = 0; q < kRecheckSpan; ++q) {
         const int p = (int)blockIdx.x * kRecheckSpan + q;
@@ -1165,13 +1201,12 @@ __global__ void __launch_bounds__(kBlock) recheck_kernel(EvalArgs A) {
         }
         if (threadIdx.x == 0) stop_s = 0;
         __syncthreads();
-        recheck_program<P, N>(A, load_meta(A.meta, p), A.bad_lo[p] != 0xffffffffu, &A.raw[p], &A.vec[p], PE,
-                              &stop_s, A.part_nrb > 0 ? A.partial + p : nullptr);
+        recheck_program<P, N>(A, load_meta(A.meta, p), p, A.bad_lo[p] != 0xffffffffu, &A.raw[p], &A.vec[p], PE,
+                              &stop_s, &deep_s, A.part_nrb > 0 ? A.partial + p : nullptr);
         __syncthreads();
     }
 }
 
-// The same for the batched form: program g belongs to pair pair_of[g].
 // PPI_EF_EXPR: empty programs get their result here (as the main kernels' epilogue gives it);
 // every other program is marked for the recomputation with a bad-row record, so that
 // recheck_program takes the reference's own partition and row order for the strategy.
@@ -1187,11 +1222,28 @@ __global__ void ef_prepare_kernel(EvalArgs A) {
     }
 }
 
+// The pair fields of EvalArgs for pair pd (the rest as in `base`).
+__device__ __forceinline__ EvalArgs pair_args(const EvalArgs& base, const PairDev& pd) {
+    EvalArgs A = base;
+    A.X = pd.X;
+    A.ldx = pd.ldx;
+    A.nlin = pd.nlin;
+    A.ncol = pd.ncol;
+    A.ycol = pd.ncol - 1;
+    A.nprog = pd.nprog;
+    A.pdp_lanes = pd.pdp_lanes;
+    A.dp_groups = pd.dp_groups;
+    A.dp_local = pd.dp_local;
+    A.part_nrb = pd.nrb;
+    return A;
+}
+
+// The recheck for the batched form: program g belongs to pair pair_of[g].
 template <bool P, bool N>
 __global__ void __launch_bounds__(kBlock) batch_recheck_kernel(BatchArgs B, const int32_t* pair_of,
                                                               int32_t n_total) {
     __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
-    __shared__ int stop_s;
+    __shared__ int stop_s, deep_s;
     bool staged = false;
     for (int q = 0; q < kRecheckSpan; ++q) {
         const int g = (int)blockIdx.x * kRecheckSpan + q;
@@ -1204,21 +1256,114 @@ __global__ void __launch_bounds__(kBlock) batch_recheck_kernel(BatchArgs B, cons
         if (threadIdx.x == 0) stop_s = 0;
         __syncthreads();
         const PairDev pd = B.pairs[pair_of[g]];
-        EvalArgs A = B.base;
-        A.X = pd.X;
-        A.ldx = pd.ldx;
-        A.nlin = pd.nlin;
-        A.ncol = pd.ncol;
-        A.ycol = pd.ncol - 1;
-        A.nprog = pd.nprog;
-        A.pdp_lanes = pd.pdp_lanes;
-        A.dp_groups = pd.dp_groups;
-        A.dp_local = pd.dp_local;
-        A.part_nrb = pd.nrb;
-        recheck_program<P, N>(A, load_meta(B.base.meta, g), B.base.bad_lo[g] != 0xffffffffu, &B.base.raw[g],
-                              &B.base.vec[g], PE, &stop_s, B.base.partial + pd.partial_base + (g - pd.prog_base));
+        const EvalArgs A = pair_args(B.base, pd);
+        recheck_program<P, N>(A, load_meta(B.base.meta, g), g, B.base.bad_lo[g] != 0xffffffffu, &B.base.raw[g],
+                              &B.base.vec[g], PE, &stop_s, &deep_s,
+                              B.base.partial + pd.partial_base + (g - pd.prog_base));
         __syncthreads();
     }
+}
+
+// Deep list, step 1: the row errors of every queued program, spread over the whole grid.  Item u
+// = (entry u / chunks, rows [c * kBlock, (c + 1) * kBlock) with c = u % chunks); a thread per row.
+template <bool P, bool N>
+__global__ void __launch_bounds__(kBlock) deep_rows_kernel(DeepArgs D) {
+    const uint32_t queued = *D.base.deep_count;
+    const int cnt = (int)min(queued, (uint32_t)D.base.deep_cap);
+    if (cnt == 0) return;
+    stage_math<N>();
+    __syncthreads();
+    const int64_t chunks = ((int64_t)D.max_rows + kBlock - 1) / kBlock;
+    for (int64_t u = blockIdx.x; u < cnt * chunks; u += gridDim.x) {
+        const int k = (int)(u / chunks);
+        const int n = (int)(u - (int64_t)k * chunks) * kBlock + (int)threadIdx.x;
+        const int id = D.base.deep_prog[k];
+        const EvalArgs A = D.pairs ? pair_args(D.base, D.pairs[D.pair_of[id]]) : D.base;
+        if (n < A.nlin) D.err[(int64_t)k * D.stride + n] = row_error<P, N>(A, load_meta(A.meta, id), n);
+    }
+}
+
+// SEQ / PP from a program's row errors e[0, nlin) (sequential.cc:169-179 and ordered_take's
+// rules).  With a bad row the result does not depend on the sum, and with none a REDUCEMAX
+// maximum does not depend on the order, so both are found by all threads; only the plain sum is
+// taken row by row, by lane 0 of wave 0, from LDS blocks that waves 1-3 stage one block ahead.
+constexpr int kDeepStage = 2048;  // floats per staged block
+__device__ __forceinline__ void ordered_from_buffer(const EvalArgs& A, const ProgMeta& m, const float* e,
+                                                    float* raw_out, float* vec_out,
+                                                    float (*stage)[kDeepStage], float* red) {
+    const float fn = (float)A.nlin;
+    const int tid = (int)threadIdx.x;
+    bool bad = false;
+    float mx = 0.0f;
+    for (int n = tid; n < A.nlin; n += kBlock) {
+        const float v = e[n];
+        bad = bad || !finite_f(v);
+        const float c = v * fn;
+        mx = (c > mx) ? c : mx;
+    }
+    if (__syncthreads_or(bad)) {
+        if (tid == 0) semantics(A, 0.0f, m.flags, 0u, 0u, m.size, raw_out, vec_out);
+        return;
+    }
+    if (A.reducemax) {
+        red[tid] = mx;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = fmaxf(red[tid], red[tid + w]);
+            __syncthreads();
+        }
+        if (tid == 0) semantics(A, red[0], m.flags, 0xffffffffu, 0u, m.size, raw_out, vec_out);
+        return;
+    }
+    const int nblk = (A.nlin + kDeepStage - 1) / kDeepStage;
+    for (int i = tid; i < kDeepStage; i += kBlock) stage[0][i] = i < A.nlin ? e[i] : 0.0f;
+    __syncthreads();
+    float part = 0.0f;  // lane 0 of wave 0's
+    for (int b = 0; b < nblk; ++b) {
+        if (tid >= 64) {
+            const int64_t base = (int64_t)(b + 1) * kDeepStage;
+            for (int i = tid - 64; i < kDeepStage; i += kBlock - 64)
+                stage[(b + 1) & 1][i] = base + i < A.nlin ? e[base + i] : 0.0f;
+        } else if (tid == 0) {
+            const float* s = stage[b & 1];
+            const int cnt = min(kDeepStage, A.nlin - b * kDeepStage);
+            const float4* s4 = reinterpret_cast<const float4*>(s);
+            int i = 0;
+#pragma unroll 8
+            for (; i < cnt / 4; ++i) {
+                const float4 q = s4[i];
+                part += q.x;
+                part += q.y;
+                part += q.z;
+                part += q.w;
+            }
+            for (i *= 4; i < cnt; ++i) part += s[i];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) semantics(A, part, m.flags, 0xffffffffu, 0u, m.size, raw_out, vec_out);
+}
+
+// Deep list, step 2: workgroup k reduces entry k's row errors in the reference's partition and
+// order for the strategy and writes the program's result.
+__global__ void __launch_bounds__(kBlock) deep_partition_kernel(DeepArgs D) {
+    __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
+    __shared__ __attribute__((aligned(16))) float stage[2][kDeepStage];
+    __shared__ int stop_s;
+    const int k = blockIdx.x;
+    const uint32_t queued = *D.base.deep_count;
+    if (k >= (int)min(queued, (uint32_t)D.base.deep_cap)) return;
+    const int id = D.base.deep_prog[k];
+    const EvalArgs A = D.pairs ? pair_args(D.base, D.pairs[D.pair_of[id]]) : D.base;
+    const float* err = D.err + (int64_t)k * D.stride;
+    const ProgMeta m = load_meta(A.meta, id);
+    if (A.strategy != PPI_STRATEGY_PDP && A.strategy != PPI_STRATEGY_FP) {
+        ordered_from_buffer(A, m, err, &A.raw[id], &A.vec[id], stage, PE);
+        return;
+    }
+    if (threadIdx.x == 0) stop_s = 0;
+    __syncthreads();
+    partition_result(A, m, &A.raw[id], &A.vec[id], PE, &stop_s, RowErrBuffer{err});
 }
 #endif  // PPI_KERNELS_MISC
 

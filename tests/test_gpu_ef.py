@@ -49,6 +49,28 @@ def test_user_error_function_matches_oracle(case, strategy, text):
         assert same.mean() > 0.95
 
 
+@pytest.mark.parametrize("strategy", [ppiamd.SEQ, ppiamd.PP, ppiamd.FP, ppiamd.PDP])
+def test_user_error_function_long_dataset(case, strategy):
+    """12,000 rows: the programs go through the recheck's deep list (16 at a time, row errors computed
+    across the device) and the rest through the recheck kernel; bit for bit either way."""
+    gr, _, _, ref = case
+    pop = ppiamd.grow(gr, 40, 44, 6, 64)
+    data = ppiamd.dataset(ppiamd.DS_FORMULA_SMALL, 45, 12000, 4)
+    text = EXACT[0]
+    print(f"user EF long dataset strategy {strategy}", flush=True)
+    e = ppiamd.Engine(0)
+    e.configure(gr.table, strategy=strategy, protected=True, ef_expr=text)
+    e.set_dataset(data)
+    vec, idx = e.evaluate(pop, alpha=ALPHA, best_size=3)
+    e.close()
+    geo = oracle.reference_geometry(data.shape[0], pop.n)
+    cfg = oracle.make_cfg(strategy, ppiamd.EF_EXPR, False, True, None, *geo, ef_fn=ref.addr[text])
+    ovec, oidx = oracle.interpret(cfg, gr.table, pop, data, alpha=ALPHA, best_size=3)
+    assert np.array_equal(vec.view(np.uint32), ovec.view(np.uint32)), \
+        f"{int((vec.view(np.uint32) != ovec.view(np.uint32)).sum())} differ"
+    assert list(idx) == list(oidx)
+
+
 def test_user_error_function_reducemax_and_sentinels(case):
     """REDUCEMAX and programs whose error overflows (sentinels) through a user expression, bit for bit."""
     b = Builder(4)

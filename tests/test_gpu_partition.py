@@ -174,3 +174,53 @@ def test_partition_sentinels_batched(strategy, reducemax):
                 same = agree(vec, ovec, d.shape[0])
                 assert same.all(), f"{case} rep {rep} pair {k} programs {np.nonzero(~same)[0]}: {vec[~same]} vs {ovec[~same]}"
         bt.close()
+
+
+# From 8,192 rows on, a program whose result needs all its rows in the reference's partition is
+# queued on the recheck's deep list (row errors computed across the device, then one workgroup
+# per program); the list holds 16 programs and the rest are recomputed in the recheck kernel.
+# Every program below appears three times, so both paths run.
+LONG = 3 * 8192 + 5
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("reducemax", [False, True])
+@pytest.mark.parametrize("form", [ppiamd.FORM_FP, ppiamd.FORM_PDP, ppiamd.FORM_PP])
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_partition_sentinels_long(eng, strategy, form, reducemax, case):
+    make, programs = CASES[case]
+    b = Builder(NATTR)
+    pop = b.population(programs * 3)
+    data = make(LONG)
+    print(f"case partition long {case} strategy={strategy} form={form} reducemax={reducemax}", flush=True)
+    eng.configure(b.table, strategy=strategy, error_fn=ppiamd.EF_SQUARED, reducemax=reducemax, protected=False,
+                  form=form, max_local_size=LANES, compute_units=256)
+    eng.set_dataset(data)
+    vec, _ = eng.evaluate(pop, alpha=ALPHA, best_size=1)
+    ovec = oracle_vec(b, pop, data, strategy, reducemax)
+    same = agree(vec, ovec, data.shape[0])
+    assert same.all(), f"programs {np.nonzero(~same)[0]}: {vec[~same]} vs {ovec[~same]}"
+
+
+@pytest.mark.parametrize("reducemax", [False, True])
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_partition_sentinels_long_batched(strategy, reducemax):
+    print(f"case partition long batched strategy={strategy} reducemax={reducemax}", flush=True)
+    b = Builder(NATTR)
+    for case in sorted(CASES):
+        make, programs = CASES[case]
+        pop = b.population(programs * 3)
+        datas = [make(LONG), make(700), make(LONG + 2048)]
+        bt = ppiamd.Batch(0)
+        bt.configure(b.table, strategy=strategy, error_fn=ppiamd.EF_SQUARED, reducemax=reducemax, protected=False,
+                     max_local_size=LANES, compute_units=256)
+        ids = [bt.add_population(bt.add_dataset(d), pop) for d in datas]
+        bt.build()
+        for rep in range(2):  # graph replays
+            bt.run()
+            for k, d in enumerate(datas):
+                vec, _ = bt.results(ids[k], alpha=ALPHA)
+                ovec = oracle_vec(b, pop, d, strategy, reducemax)
+                same = agree(vec, ovec, d.shape[0])
+                assert same.all(), f"{case} rep {rep} pair {k} programs {np.nonzero(~same)[0]}: {vec[~same]} vs {ovec[~same]}"
+        bt.close()
