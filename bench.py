@@ -140,6 +140,7 @@ def bench_batched(steps, warmup):
         b.run()
     ms = b.time(steps)
     st = b.stats()
+    shapes = shape_summary(b, len(w.pops), w)
     # FP64 error on three pairs (the first 100 programs of the largest, a middle and the smallest dataset)
     rows = [d.shape[0] for d in w.datas]
     picks = sorted({int(np.argmax(rows)), int(np.argsort(rows)[len(rows) // 2]), int(np.argmin(rows))})
@@ -161,22 +162,37 @@ def bench_batched(steps, warmup):
         eng.time(1)
         plain_ms += eng.time(1)
     eng.close()
-    gen = batch_generations(w, st.rows_per_lane, 2)
+    gen = batch_generations(w, 2)
     return {"fp64_error": errs, "node_evals": w.node_evals, "graph_ms": ms, "kernel_ms": st.kernel_ms, "plain_ms": plain_ms,
             "rows_per_lane": st.rows_per_lane, "items": st.n_items, "grid": st.grid,
-            "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs,
+            "tune_ms": [round(x, 3) for x in st.tune_ms[:3]], "pairs": st.n_pairs, "shapes": shapes,
             "programs": int(sum(p.n for p in w.pops)), "generation": gen}
 
 
-def batch_generations(w, rows_per_lane, gens):
+def shape_summary(b, n_pairs, w):
+    """The autotuner's outcome: kernel launches per replay, keys timed, and for each shape
+    (rows per lane, feature slots, stack slots) the pairs and node-evaluations it runs."""
+    st = b.stats()
+    per = {}
+    for k in range(n_pairs):
+        r, f, sl = b.pair_shape(k)
+        key = f"R{r}_F{f}_S{sl}"
+        e = per.setdefault(key, {"pairs": 0, "node_evals": 0.0})
+        e["pairs"] += 1
+        e["node_evals"] += float(w.pops[k].total_size) * w.datas[k].shape[0]
+    for e in per.values():
+        e["node_evals"] = float(f"{e['node_evals']:.4g}")
+    return {"launches": st.n_shapes, "tuned_keys": st.tuned_keys, "by_shape": per}
+
+
+def batch_generations(w, gens):
     """Whole generations, wall clock, datasets resident: the batch re-adds every pair's population
-    (compile), rebuilds (plan + capture; rows per lane fixed to the tuned value), replays and reads
-    all 64 result vectors; the plain engine evaluates each pair's population through evaluate()
-    (compile + upload + launch + wait + download), one resident engine per pair, each pair's time
-    taken after a warm-up evaluation."""
+    (compile), rebuilds (plan + capture; the autotuner's per-key shapes from the first, untimed,
+    generation are kept), replays and reads all 64 result vectors; the plain engine evaluates each
+    pair's population through evaluate() (compile + upload + launch + wait + download), one
+    resident engine per pair, each pair's time taken after a warm-up evaluation."""
     b = ppiamd.Batch(0)
-    b.configure(w.grammars[0].table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True,
-                rows_per_lane=rows_per_lane)
+    b.configure(w.grammars[0].table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
     ids = [b.add_dataset(d) for d in w.datas]
     t_b = []
     for g in range(gens + 1):
@@ -222,6 +238,8 @@ def bench_batched_small(steps):
     b.run()
     dev_ms = b.time(steps)
     st = b.stats()
+    shapes = shape_summary(b, len(w.pops), w)
+    pair_r = [b.pair_shape(k)[0] for k in range(len(w.pops))]
     t0 = time.perf_counter()
     for _ in range(steps):
         b.run()
@@ -231,9 +249,11 @@ def bench_batched_small(steps):
     vb = [b.results(k)[0] for k in range(len(w.pops))]
     b.close()
     engs = []
-    for gr, p, d in zip(w.grammars, w.pops, w.datas):
+    for gr, p, d, r in zip(w.grammars, w.pops, w.datas, pair_r):
         e = ppiamd.Engine(0)
-        e.configure(gr.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True)
+        # the pair's own rows per lane: the same row tiles, so the same bits as the batch
+        e.configure(gr.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=True, form=ppiamd.FORM_FP,
+                    rows_per_lane=r)
         e.set_dataset(d)
         e.upload(p)
         e.time(1)
@@ -249,7 +269,7 @@ def bench_batched_small(steps):
     same = all(np.array_equal(x, y, equal_nan=True) for x, y in zip(vb, outs))
     for e in engs:
         e.close()
-    gen = batch_generations(w, st.rows_per_lane, 3)
+    gen = batch_generations(w, 3)
     ne = w.node_evals
     return {"workload": f"64 pairs, {int(sum(p.n for p in w.pops))} programs (8-64 symbols), 1,000-4,000 rows x "
                         f"2-16 features, {ne:.3e} node-evaluations per step",
@@ -258,7 +278,8 @@ def bench_batched_small(steps):
             "wall_ms": round(wall_ms, 3), "plain_wall_ms": round(plain_wall, 3),
             "vs_plain_wall": round(plain_wall / wall_ms, 3),
             "gpops_wall": round(ne / (wall_ms * 1e-3) / 1e9, 3), "plain_gpops_wall": round(ne / (plain_wall * 1e-3) / 1e9, 3),
-            "results_identical_to_plain": bool(same), "rows_per_lane": st.rows_per_lane, "generation": gen}
+            "results_identical_to_plain": bool(same), "rows_per_lane": st.rows_per_lane, "shapes": shapes,
+            "generation": gen}
 
 
 def bench_adjoint_pass(steps, warmup):
@@ -660,7 +681,7 @@ def main():
             "vs_plain": round(bg / pg, 3),
             "kernel_frac_of_valu_peak": round(batched["node_evals"] / (batched["kernel_ms"] * 1e-3) / VALU_PEAK, 5),
             "rows_per_lane": batched["rows_per_lane"], "items": batched["items"], "grid": batched["grid"],
-            "autotune_ms_per_Gnode_R2_R4_R8": batched["tune_ms"],
+            "autotune_ms_per_Gnode_R2_R4_R8": batched["tune_ms"], "shapes": batched["shapes"],
             "fp64_error_sample": batched["fp64_error"]}
         line["batched"]["generation_inclusive"] = dict(batched["generation"], note=(
             "wall clock per generation with datasets resident: batch = re-add 64 populations (compile) + build + "

@@ -243,12 +243,16 @@ typedef struct ppi_batch_stats {
     double build_ms;        /* compile + upload + plan + autotune + capture */
     int32_t n_pairs;
     int32_t n_items;
-    int32_t grid;           /* persistent workgroups */
-    int32_t rows_per_lane;  /* chosen R */
-    int32_t feature_slots;  /* F of the kernel shape (0 = attributes read from global memory) */
-    int32_t autotuned;      /* 1 if R was picked by measurement */
-    double tune_ms[4];      /* sampled ms per 10^9 node-evaluations, candidates R = 2, 4, 8 (0 = not tried) */
+    int32_t grid;           /* persistent workgroups of the launch carrying the most work */
+    int32_t rows_per_lane;  /* R of that launch (each pair's own: ppi_batch_pair_shape) */
+    int32_t feature_slots;  /* F of that launch's kernel shape (0 = attributes read from global memory) */
+    int32_t autotuned;      /* 1 if some pair's shape was picked by measurement */
+    double tune_ms[4];      /* sampled ms per 10^9 instruction x rows, candidates R = 2, 4, 8 (0 = not
+                               tried), of the tuning key carrying the most work */
     int32_t derived_columns; /* derived columns (unary operators over input columns) computed, all datasets */
+    int32_t n_shapes;       /* persistent launches per replay: one per kernel shape in use */
+    int32_t tuned_keys;     /* tuning keys (stack class x program length x rows) timed by this build
+                               (keys decided by an earlier build of the batch keep their shape) */
 } ppi_batch_stats;
 
 ppi_status ppi_batch_create(ppi_batch** out, int32_t device);
@@ -272,6 +276,10 @@ ppi_status ppi_batch_results(ppi_batch* b, int32_t pop_id, float alpha, float* v
 /* `iters` graph replays back to back; mean device ms per replay (also times the kernel alone). */
 ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run);
 ppi_status ppi_batch_get_stats(const ppi_batch* b, ppi_batch_stats* out);
+/* The kernel shape the autotuner gave population `pop_id` after build(): rows per lane, LDS
+   feature slots, register-stack slots (any pointer may be NULL).  No reference counterpart. */
+ppi_status ppi_batch_pair_shape(const ppi_batch* b, int32_t pop_id, int32_t* rows_per_lane,
+                                int32_t* feature_slots, int32_t* stack_slots);
 
 /*
  * Reverse-mode derivatives of the fitness with respect to the ephemeral constants.

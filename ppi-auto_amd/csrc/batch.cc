@@ -4,11 +4,15 @@
 // Work items are (pair, row tile of 64*R rows, program range), where each pair's
 // programs are cut into ranges of roughly equal tape length so that every item
 // costs about the same; items are ordered longest first and pulled from an
-// atomic counter by a grid sized to the kernel's occupancy.  Rows per lane R is
-// picked by timing the candidates on an evenly spaced sample of the items.  The
-// replay (queue reset, persistent kernel, per-program epilogue, download) is
-// captured once as a hipGraph.  Populations are copied when added and compiled
-// by build(), every pending one on its own thread of a pool.
+// atomic counter by a grid sized to the kernel's occupancy.  Each pair runs in a
+// kernel shape (rows per lane R, LDS feature slots F, register-stack slots S)
+// chosen by the autotuner: pairs are keyed by their stack class, mean program
+// length and row count, and each key's candidate shapes are timed on a sample of
+// that key's own items.  Pairs that share a shape form one persistent launch; the
+// launches run back to back.  The replay (queue resets, persistent kernels,
+// per-program epilogue, download) is captured once as a hipGraph.  Populations are
+// copied when added and compiled by build(), every pending one on its own thread
+// of a pool.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,6 +68,8 @@ struct Pop {
     int32_t n = 0;
     int32_t prog_base = 0;
     int64_t total_size = 0;
+    int32_t max_slots = 0;   // register slots of the population's largest program (after compile)
+    double mean_code = 0.0;  // tape instructions per program, its H_END included
     // The caller's arrays (copied by add_population) until build() compiles them.
     int32_t stride = 0;
     std::vector<int32_t> ph, sz;
@@ -71,6 +77,22 @@ struct Pop {
     bool compiled = false;
     Tape t;
 };
+
+// A kernel shape of the persistent form: rows per lane, LDS feature slots, stack slots.
+struct Shape {
+    int R = 4, F = 16, S = 4;
+    bool operator==(const Shape& o) const { return R == o.R && F == o.F && S == o.S; }
+};
+
+// The pairs that run in one shape: one persistent launch over their items.
+struct Group {
+    Shape shape;
+    int grid = 0;
+    int32_t item_off = 0, n_items = 0;
+    double work = 0.0;  // instructions x rows of the group's items
+};
+
+constexpr int kMaxGroups = 16;  // queue counters allocated per batch
 
 template <typename T>
 ppi_status dalloc(T** p, size_t n) {
@@ -128,7 +150,11 @@ struct ppi_batch {
 
     std::vector<PairDev> pairs;
     std::vector<WorkItem> items;
-    int R = 4, F = 32, S = 9, grid = 0;
+    std::vector<Shape> pair_shape;  // per pair (population)
+    std::vector<Group> groups;
+    // The autotuner's decisions by key (stack class, length class, rows class), kept across
+    // builds until configure(): a rebuild with new populations times only keys it has not seen.
+    std::vector<std::pair<int, Shape>> tuned;
     bool P = false, N = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -177,29 +203,44 @@ void pair_geometry(const ppi_batch* b, int32_t nlin, int32_t nprog, int32_t* pdp
     *best_local = std::max(1, std::min(mls2, (int)std::ceil(nprog / (float)cu)));
 }
 
-// Work items for rows per lane R: each pair's programs are cut into ranges whose
-// tape length (instructions + one per program for its epilogue) is about
-// `target`, and every range is crossed with every row tile of the pair.
-void plan_items(const ppi_batch* b, int R, double items_per_slot, std::vector<PairDev>* pairs,
-                std::vector<WorkItem>* items, int64_t* partials) {
+// Work items for the pairs' shapes (pair_shape): each pair's programs are cut into
+// ranges whose tape length (instructions + one per program for its epilogue) is about
+// its group's `target`, and every range is crossed with every row tile of the pair.
+// Pairs with include[k] == 0 get their partial block but no items, and with row_limit
+// only the row tiles below row_limit[k] get items (the tuner's samples; ranges are then
+// sized for that work).  Groups hold the pairs of one shape, largest first; a group's
+// items are contiguous in `items`, longest first.
+void plan_items(const ppi_batch* b, const std::vector<char>& include, double items_per_slot,
+                std::vector<PairDev>* pairs, std::vector<WorkItem>* items, std::vector<Group>* groups,
+                int64_t* partials, const std::vector<int64_t>* row_limit = nullptr) {
     pairs->clear();
     items->clear();
-    double total = 0.0;
-    std::vector<int32_t> nrb(b->pops.size());
-    for (size_t k = 0; k < b->pops.size(); ++k) {
+    groups->clear();
+    const size_t np = b->pops.size();
+    std::vector<int32_t> nrb(np), tiles(np);
+    std::vector<int> gof(np, -1);
+    for (size_t k = 0; k < np; ++k) {
         const Pop& p = b->pops[k];
-        const Dataset& d = b->datasets[p.dataset];
-        nrb[k] = (d.nlin + 64 * R - 1) / (64 * R);
-        double len = 0.0;
-        for (int32_t j = 0; j < p.n; ++j) len += b->tape.meta[p.prog_base + j].ncode + 1.0;
-        total += len * nrb[k];
+        const Shape& sh = b->pair_shape[k];
+        nrb[k] = (b->datasets[p.dataset].nlin + 64 * sh.R - 1) / (64 * sh.R);
+        tiles[k] = row_limit ? (int32_t)std::min<int64_t>(nrb[k], ((*row_limit)[k] + 64 * sh.R - 1) / (64 * sh.R))
+                             : nrb[k];
+        if (!include[k]) continue;
+        int g = 0;
+        while (g < (int)groups->size() && !((*groups)[g].shape == sh)) ++g;
+        if (g == (int)groups->size()) {
+            Group G;
+            G.shape = sh;
+            G.grid = b->dev_cu * std::max(1, batch_occupancy(sh.R, sh.F, sh.S, b->P, b->N));
+            groups->push_back(G);
+        }
+        gof[k] = g;
+        (*groups)[g].work += p.mean_code * p.n * tiles[k] * 64.0 * sh.R;
     }
-    const double slots = std::max(1.0, (double)b->grid * items_per_slot);
-    const double target = std::max(64.0, total / slots);
     int64_t part = 0;
     struct Cost { double c; WorkItem w; };
-    std::vector<Cost> costed;
-    for (size_t k = 0; k < b->pops.size(); ++k) {
+    std::vector<std::vector<Cost>> costed(groups->size());
+    for (size_t k = 0; k < np; ++k) {
         const Pop& p = b->pops[k];
         const Dataset& d = b->datasets[p.dataset];
         PairDev pd{};
@@ -211,28 +252,55 @@ void plan_items(const ppi_batch* b, int R, double items_per_slot, std::vector<Pa
         pd.nprog = p.n;
         pd.partial_base = part;
         pd.nrb = nrb[k];
+        pd.rows_per_tile = 64 * b->pair_shape[k].R;
         int32_t bl;
         pair_geometry(b, d.nlin, p.n, &pd.pdp_lanes, &pd.dp_groups, &pd.dp_local, &bl);
         pairs->push_back(pd);
         part += (int64_t)nrb[k] * p.n;
+        if (gof[k] < 0) continue;
+        const Group& G = (*groups)[gof[k]];
         // Program ranges of about `target` instructions each (an item covers one row tile).
-        const double per_range = std::max(16.0, target);
+        const double target = std::max(64.0, G.work / (64.0 * G.shape.R) / std::max(1.0, G.grid * items_per_slot));
         int32_t j0 = 0;
         double acc = 0.0;
         for (int32_t j = 0; j < p.n; ++j) {
             acc += b->tape.meta[p.prog_base + j].ncode + 1.0;
-            const bool cut = acc >= per_range || j + 1 == p.n;
-            if (cut) {
-                for (int32_t t = 0; t < nrb[k]; ++t) costed.push_back({acc, WorkItem{(int32_t)k, t, j0, j + 1}});
+            if (acc >= target || j + 1 == p.n) {
+                for (int32_t t = 0; t < tiles[k]; ++t)
+                    costed[gof[k]].push_back({acc, WorkItem{(int32_t)k, t, j0, j + 1}});
                 j0 = j + 1;
                 acc = 0.0;
             }
         }
     }
-    std::stable_sort(costed.begin(), costed.end(), [](const Cost& x, const Cost& y) { return x.c > y.c; });
-    items->reserve(costed.size());
-    for (auto& c : costed) items->push_back(c.w);
+    std::vector<int> order(groups->size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return (*groups)[x].work > (*groups)[y].work; });
+    std::vector<Group> sorted;
+    for (int g : order) {
+        auto& c = costed[g];
+        std::stable_sort(c.begin(), c.end(), [](const Cost& x, const Cost& y) { return x.c > y.c; });
+        Group G = (*groups)[g];
+        G.item_off = (int32_t)items->size();
+        G.n_items = (int32_t)c.size();
+        for (auto& e : c) items->push_back(e.w);
+        sorted.push_back(G);
+    }
+    *groups = sorted;
     *partials = part;
+}
+
+// Instructions x rows of a list of items (the tuner's unit of work).
+double items_work(const ppi_batch* b, const WorkItem* w, size_t n) {
+    double t = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const Pop& p = b->pops[w[i].pair];
+        double len = 0.0;
+        for (int32_t j = w[i].p0; j < w[i].p1; ++j) len += b->tape.meta[p.prog_base + j].ncode + 1.0;
+        t += len * 64.0 * b->pair_shape[w[i].pair].R;
+    }
+    return t;
 }
 
 // Compiles the populations added since the last build: derived columns planned
@@ -291,6 +359,8 @@ ppi_status compile_pending(ppi_batch* b) {
             set_error("population " + std::to_string(k) + ": program needs more than 16 register slots");
             return PPI_E_PROGRAM;
         }
+        p.max_slots = t.stats.max_slots;
+        p.mean_code = (double)(t.stats.total_code + p.n) / std::max(1, p.n);
         const uint32_t off = (uint32_t)b->tape.code.size();
         for (auto m : t.meta) {
             m.code_off += off;
@@ -332,7 +402,6 @@ BatchArgs make_args(ppi_batch* b, int32_t n_items) {
     A.cost_n = b->cfg.cost_n;
     A.alpha = 0.0f;
     A.partial = b->dPartial;
-    A.part_rows = fp_rows_per_block(b->R);
     A.bad_lo = b->dBadLo;
     A.bad_hi = b->dBadHi;
     A.raw = b->dRaw;
@@ -370,14 +439,29 @@ ppi_status enqueue_recheck(ppi_batch* b, const BatchArgs& B, int32_t n) {
     return PPI_OK;
 }
 
-// Queue reset, persistent kernel, epilogue (optionally the download).
-ppi_status enqueue_run(ppi_batch* b, int32_t n_items, bool with_finalize, bool with_download) {
+// The persistent launches of the groups, back to back, each with its own queue counter
+// (reset beforehand by the caller).
+ppi_status enqueue_groups(ppi_batch* b, const std::vector<Group>& groups) {
+    for (size_t g = 0; g < groups.size(); ++g) {
+        const Group& G = groups[g];
+        if (G.n_items == 0) continue;
+        BatchArgs B = make_args(b, G.n_items);
+        B.items = b->dItems + G.item_off;
+        B.counter = b->dCounter + g;
+        B_HIP(launch_batch(B, G.grid, G.shape.R, G.shape.F, G.shape.S, b->P, b->N, b->stream));
+    }
+    return PPI_OK;
+}
+
+// Queue resets, persistent kernels, epilogue (optionally the download).
+ppi_status enqueue_run(ppi_batch* b, const std::vector<Group>& groups, bool with_finalize, bool with_download) {
     const int32_t n = total_programs(b);
-    BatchArgs B = make_args(b, n_items);
+    BatchArgs B = make_args(b, 0);
     B_HIP(launch_fill_u32(b->dBadLo, 0xffffffffu, n, b->stream));
     B_HIP(launch_fill_u32(b->dBadHi, 0u, n, b->stream));
-    B_HIP(launch_fill_u32(b->dCounter, 0u, 1, b->stream));
-    B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
+    B_HIP(launch_fill_u32(b->dCounter, 0u, (int32_t)groups.size(), b->stream));
+    ppi_status gs = enqueue_groups(b, groups);
+    if (gs) return gs;
     if (with_finalize) {
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
         ppi_status s = enqueue_recheck(b, B, n);
@@ -405,7 +489,7 @@ ppi_status ppi_batch_create(ppi_batch** out, int32_t device) {
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->dCounter, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dCounter, kMaxGroups * sizeof(uint32_t));
     if (e != hipSuccess) {
         set_error(std::string("device init: ") + hipGetErrorString(e));
         ppi_batch_destroy(b);
@@ -473,6 +557,7 @@ ppi_status ppi_batch_configure(ppi_batch* b, const ppi_config* cfg, const ppi_sy
     }
     b->symtab.build(table, ntable);
     b->configured = true;
+    b->tuned.clear();
     // Derived columns depend on PROTECTED / NATIVE: recompute them on demand.
     for (auto& d : b->datasets) {
         d.derived->reset(d.ncol, derived_capacity(d.ldx, d.ncol));
@@ -558,9 +643,7 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     const int32_t n = total_programs(b);
     b->P = b->cfg.protected_ops != 0;
     b->N = b->cfg.native_math != 0;
-    b->F = feature_slots(b->tape.stats.max_col);
-    const int ms = b->tape.stats.max_slots;
-    b->S = ms <= 4 ? 4 : (ms <= 9 ? 9 : 16);
+    const size_t np = b->pops.size();
 
     std::vector<Ins> code = b->tape.code;
     code.insert(code.end(), kTapePad, Ins{H_END, kEmpty});
@@ -569,7 +652,7 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     B_HIP(hipMemcpy(b->dMeta, b->tape.meta.data(), b->tape.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice));
     B_HIP(hipMemcpy(b->dCode, code.data(), code.size() * sizeof(Ins), hipMemcpyHostToDevice));
     std::vector<int32_t> pair_of((size_t)n);
-    for (size_t k = 0; k < b->pops.size(); ++k)
+    for (size_t k = 0; k < np; ++k)
         std::fill(pair_of.begin() + b->pops[k].prog_base, pair_of.begin() + b->pops[k].prog_base + b->pops[k].n,
                   (int32_t)k);
     if ((s = dalloc(&b->dPairOf, (size_t)n))) return s;
@@ -585,34 +668,39 @@ ppi_status ppi_batch_build(ppi_batch* b) {
         b->hRawCap = (size_t)n;
     }
 
-    // Candidates for rows per lane; the partial buffer is sized for the smallest R.
-    std::vector<int> cands;
-    if (b->cfg.rows_per_lane > 0) cands.push_back(b->cfg.rows_per_lane);
-    else cands = {2, 4, 8};
-    cands.erase(std::remove_if(cands.begin(), cands.end(),
-                               [&](int r) { return !batch_shape_supported(r, b->F, b->S); }),
-                cands.end());
-    if (cands.empty() && b->S == 4) {  // the 4-slot stack exists at 4 and 8 rows only
-        b->S = 9;
-        if (b->cfg.rows_per_lane > 0) cands.push_back(b->cfg.rows_per_lane);
-        else cands = {2, 4, 8};
-        cands.erase(std::remove_if(cands.begin(), cands.end(),
-                                   [&](int r) { return !batch_shape_supported(r, b->F, b->S); }),
-                    cands.end());
-    }
-    if (cands.empty()) {
-        // Only the 16-slot programs reach here; they run at 2 rows with 32 feature slots.
-        if (b->S == 16 && b->F > 0 && b->F <= 32 && batch_shape_supported(2, 32, 16)) { cands.push_back(2); b->F = 32; }
-        else { set_error("no batched kernel shape for these programs"); return PPI_E_PROGRAM; }
-    }
-    int64_t max_part = 0;
-    for (int r : cands) {
-        int64_t part = 0;
-        for (size_t k = 0; k < b->pops.size(); ++k) {
-            const Dataset& d = b->datasets[b->pops[k].dataset];
-            part += (int64_t)((d.nlin + 64 * r - 1) / (64 * r)) * b->pops[k].n;
+    // Candidate shapes of each pair, from its largest program's register slots: the 4-slot
+    // stack at 4 and 8 rows per lane (its 2-row form is the 9-slot one), the 9-slot stack at
+    // 2, 4 and 8, the 16-slot stack at 2 rows; 16 LDS feature slots (feature_slots), 32 at
+    // 2 rows.  The shape at 4 rows per lane comes first (the default where nothing is timed).
+    std::vector<std::vector<Shape>> cand(np);
+    for (size_t k = 0; k < np; ++k) {
+        const int ms = b->pops[k].max_slots;
+        const int F = feature_slots(b->tape.stats.max_col);
+        std::vector<Shape> c;
+        if (ms <= 4) c = {{4, F, 4}, {8, F, 4}, {2, 32, 9}};
+        else if (ms <= 9) c = {{4, F, 9}, {8, F, 9}, {2, 32, 9}};
+        else c = {{2, 32, 16}};
+        c.erase(std::remove_if(c.begin(), c.end(),
+                               [&](const Shape& x) {
+                                   return !batch_shape_supported(x.R, x.F, x.S) ||
+                                          (b->cfg.rows_per_lane > 0 && x.R != b->cfg.rows_per_lane);
+                               }),
+                c.end());
+        if (c.empty()) {
+            set_error("population " + std::to_string(k) + ": no batched kernel shape for its programs (" +
+                      std::to_string(ms) + " register slots) at rows_per_lane " +
+                      std::to_string(b->cfg.rows_per_lane));
+            return PPI_E_PROGRAM;
         }
-        max_part = std::max(max_part, part);
+        cand[k] = c;
+    }
+    // Partials sized for each pair's smallest candidate R.
+    int64_t max_part = 0;
+    for (size_t k = 0; k < np; ++k) {
+        int rmin = 8;
+        for (const Shape& x : cand[k]) rmin = std::min(rmin, x.R);
+        const Dataset& d = b->datasets[b->pops[k].dataset];
+        max_part += (int64_t)((d.nlin + 64 * rmin - 1) / (64 * rmin)) * b->pops[k].n;
     }
     if ((s = dalloc(&b->dPartial, (size_t)max_part))) return s;
     b->deep_rows = 0;
@@ -628,59 +716,109 @@ ppi_status ppi_batch_build(ppi_batch* b) {
         if ((s = dalloc(&b->dDeepErr, (size_t)kDeepCap * b->deep_stride))) return s;
     }
 
+    // Autotuner.  Pairs are keyed by (stack class, mean tape length per program, rows):
+    // lengths below 24, 24-63 and 64 or more instructions; rows below 16,384, below 262,144,
+    // and more.  For each key with more than one candidate shape, every candidate is timed
+    // on the same sample of that key's work (a row prefix of each of its pairs, all their
+    // programs), and the key's pairs take the fastest.  A key decided by an earlier build of
+    // this batch keeps its shape.
     const double items_per_slot = 6.0;
+    b->pair_shape.resize(np);
+    for (size_t k = 0; k < np; ++k) b->pair_shape[k] = cand[k][0];
+    std::vector<std::pair<int, std::vector<size_t>>> cells;
+    for (size_t k = 0; k < np; ++k) {
+        const Pop& p = b->pops[k];
+        const int sc = cand[k][0].S;
+        const int lc = p.mean_code < 24.0 ? 0 : (p.mean_code < 64.0 ? 1 : 2);
+        const int32_t nl = b->datasets[p.dataset].nlin;
+        const int rc = nl < 16384 ? 0 : (nl < 262144 ? 1 : 2);
+        const int key = (sc * 3 + lc) * 3 + rc;
+        auto it = std::find_if(cells.begin(), cells.end(), [&](const auto& c) { return c.first == key; });
+        if (it == cells.end()) cells.push_back({key, {k}});
+        else it->second.push_back(k);
+    }
     std::memset(b->stats.tune_ms, 0, sizeof(b->stats.tune_ms));
     b->stats.autotuned = 0;
-    if (cands.size() > 1) {
-        // Time each candidate on an evenly spaced sample of its own item list.
-        double best = 1e300;
-        int best_r = cands[0];
-        for (int r : cands) {
-            b->R = r;
-            const int occ = std::max(1, batch_occupancy(r, b->F, b->S, b->P, b->N));
-            b->grid = b->dev_cu * occ;
-            int64_t part;
-            plan_items(b, r, items_per_slot, &b->pairs, &b->items, &part);
-            const size_t want = std::min<size_t>(b->items.size(), (size_t)b->grid * 2);
-            const size_t stride = std::max<size_t>(1, b->items.size() / std::max<size_t>(1, want));
-            std::vector<WorkItem> sample;
-            double sample_cost = 0.0;
-            for (size_t i = 0; i < b->items.size() && sample.size() < want; i += stride) {
-                sample.push_back(b->items[i]);
-                const WorkItem& w = b->items[i];
-                const Pop& p = b->pops[w.pair];
-                double len = 0.0;
-                for (int32_t j = w.p0; j < w.p1; ++j) len += b->tape.meta[p.prog_base + j].ncode + 1.0;
-                sample_cost += len * 64.0 * r;
-            }
-            b->items = sample;
-            if ((s = upload_items(b))) return s;
-            if ((s = enqueue_run(b, (int32_t)sample.size(), false, false))) return s;  // warm-up
-            B_HIP(hipEventRecord(b->ev[2], b->stream));
-            if ((s = enqueue_run(b, (int32_t)sample.size(), false, false))) return s;
-            B_HIP(hipEventRecord(b->ev[3], b->stream));
-            B_HIP(hipEventSynchronize(b->ev[3]));
-            float ms = 0.0f;
-            B_HIP(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
-            const double per_unit = ms / std::max(1.0, sample_cost);
-            b->stats.tune_ms[r == 2 ? 0 : (r == 4 ? 1 : 2)] = per_unit * 1e9;
-            if (per_unit < best) { best = per_unit; best_r = r; }
-        }
-        b->R = best_r;
-        b->stats.autotuned = 1;
-    } else {
-        b->R = cands[0];
-    }
-    const int occ = std::max(1, batch_occupancy(b->R, b->F, b->S, b->P, b->N));
-    b->grid = b->dev_cu * occ;
+    b->stats.tuned_keys = 0;
+    double top_work = -1.0;
     int64_t part;
-    plan_items(b, b->R, items_per_slot, &b->pairs, &b->items, &part);
+    for (auto& cell : cells) {
+        const std::vector<size_t>& ks = cell.second;
+        const std::vector<Shape>& cs = cand[ks[0]];
+        if (cs.size() < 2) continue;
+        auto known = std::find_if(b->tuned.begin(), b->tuned.end(), [&](const auto& t) { return t.first == cell.first; });
+        if (known != b->tuned.end()) {
+            for (size_t k : ks) b->pair_shape[k] = known->second;
+            b->stats.autotuned = 1;
+            continue;
+        }
+        std::vector<char> inc(np, 0);
+        double cell_work = 0.0;
+        for (size_t k : ks) {
+            inc[k] = 1;
+            const Pop& p = b->pops[k];
+            cell_work += p.mean_code * p.n * b->datasets[p.dataset].nlin;
+        }
+        // The same work for every candidate: rows [0, T_k) of each of the key's pairs, all their
+        // programs, T_k a multiple of 512 rows (whole row tiles at 2, 4 and 8 rows per lane), the
+        // prefix sized to about 8e9 instruction x rows (a few ms) so that the timed span runs
+        // many items per workgroup, not just the ramp and the tail.
+        const double frac = std::min(1.0, 8e9 / std::max(1.0, cell_work));
+        std::vector<int64_t> rows_of(np, 0);
+        for (size_t k : ks) {
+            const int32_t nl = b->datasets[b->pops[k].dataset].nlin;
+            rows_of[k] = std::min<int64_t>(nl, ((int64_t)std::ceil(frac * nl / 512.0)) * 512);
+        }
+        double tms[3] = {0.0, 0.0, 0.0}, tms_raw[3] = {0.0, 0.0, 0.0};
+        for (const Shape& sh : cs) {
+            for (size_t k : ks) b->pair_shape[k] = sh;
+            plan_items(b, inc, items_per_slot, &b->pairs, &b->items, &b->groups, &part, &rows_of);
+            const double work = items_work(b, b->items.data(), b->items.size());
+            if ((s = upload_items(b))) return s;
+            if ((s = enqueue_run(b, b->groups, false, false))) return s;  // warm-up
+            // Repeat short samples so that the timed span is well above launch jitter.
+            int reps = 1;
+            float ms = 0.0f;
+            for (;;) {
+                B_HIP(hipEventRecord(b->ev[2], b->stream));
+                for (int r = 0; r < reps; ++r)
+                    if ((s = enqueue_run(b, b->groups, false, false))) return s;
+                B_HIP(hipEventRecord(b->ev[3], b->stream));
+                B_HIP(hipEventSynchronize(b->ev[3]));
+                B_HIP(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+                if (ms >= 2.0f || reps >= 64) break;
+                reps *= 4;
+            }
+            const int slot = sh.R == 2 ? 0 : (sh.R == 4 ? 1 : 2);
+            tms_raw[slot] = ms / reps;
+            tms[slot] = ms / reps / std::max(1.0, work) * 1e9;
+        }
+        // The first candidate (4 rows per lane) stays unless another is faster by 3% on the
+        // same work: the samples cannot resolve smaller differences reliably.
+        Shape bs = cs[0];
+        double best = tms_raw[cs[0].R == 2 ? 0 : (cs[0].R == 4 ? 1 : 2)];
+        for (const Shape& sh : cs) {
+            const double t = tms_raw[sh.R == 2 ? 0 : (sh.R == 4 ? 1 : 2)];
+            if (t < 0.97 * best) { best = t; bs = sh; }
+        }
+        for (size_t k : ks) b->pair_shape[k] = bs;
+        b->tuned.push_back({cell.first, bs});
+        b->stats.autotuned = 1;
+        ++b->stats.tuned_keys;
+        if (cell_work > top_work) {
+            top_work = cell_work;
+            for (int i = 0; i < 3; ++i) b->stats.tune_ms[i] = tms[i];
+        }
+    }
+    std::vector<char> all(np, 1);
+    plan_items(b, all, items_per_slot, &b->pairs, &b->items, &b->groups, &part);
     if ((s = upload_items(b))) return s;
+    if ((int)b->groups.size() > kMaxGroups) { set_error("too many kernel shapes"); return PPI_E_PROGRAM; }
 
     // Capture the replay: resets, persistent kernel, epilogue, download.
     B_HIP(hipStreamSynchronize(b->stream));
     B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    s = enqueue_run(b, (int32_t)b->items.size(), true, true);
+    s = enqueue_run(b, b->groups, true, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (s) return s;
@@ -690,9 +828,11 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     b->built = true;
     b->stats.n_pairs = (int32_t)b->pops.size();
     b->stats.n_items = (int32_t)b->items.size();
-    b->stats.grid = b->grid;
-    b->stats.rows_per_lane = b->R;
-    b->stats.feature_slots = b->F;
+    // The group carrying the most work (groups are sorted by work).
+    b->stats.grid = b->groups[0].grid;
+    b->stats.rows_per_lane = b->groups[0].shape.R;
+    b->stats.feature_slots = b->groups[0].shape.F;
+    b->stats.n_shapes = (int32_t)b->groups.size();
     double w = 0.0;
     for (auto& p : b->pops) w += (double)p.total_size * b->datasets[p.dataset].nlin;
     b->stats.node_evals = w;
@@ -762,16 +902,17 @@ ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
     B_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     *ms_per_run = ms / iters;
     b->stats.graph_ms = *ms_per_run;
-    // The persistent kernel alone, outside the graph, for the roofline figure.
+    // The persistent kernels alone, outside the graph, for the roofline figure.
     double kms = 0.0;
     for (int32_t k = 0; k < iters; ++k) {
         const int32_t n = total_programs(b);
-        BatchArgs B = make_args(b, (int32_t)b->items.size());
+        BatchArgs B = make_args(b, 0);
         B_HIP(launch_fill_u32(b->dBadLo, 0xffffffffu, n, b->stream));
         B_HIP(launch_fill_u32(b->dBadHi, 0u, n, b->stream));
-        B_HIP(launch_fill_u32(b->dCounter, 0u, 1, b->stream));
+        B_HIP(launch_fill_u32(b->dCounter, 0u, (int32_t)b->groups.size(), b->stream));
         B_HIP(hipEventRecord(b->ev[2], b->stream));
-        B_HIP(launch_batch(B, b->grid, b->R, b->F, b->S, b->P, b->N, b->stream));
+        ppi_status gs = enqueue_groups(b, b->groups);
+        if (gs) return gs;
         B_HIP(hipEventRecord(b->ev[3], b->stream));
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
         ppi_status s = enqueue_recheck(b, B, n);
@@ -783,6 +924,17 @@ ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
     }
     B_HIP(hipStreamSynchronize(b->stream));
     b->stats.kernel_ms = kms / iters;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_pair_shape(const ppi_batch* b, int32_t pop_id, int32_t* rows_per_lane, int32_t* feature_slots,
+                                int32_t* stack_slots) {
+    if (!b || !b->built) { set_error("build the batch first"); return PPI_E_STATE; }
+    if (pop_id < 0 || pop_id >= (int32_t)b->pops.size()) { set_error("unknown population"); return PPI_E_ARG; }
+    const Shape& sh = b->pair_shape[pop_id];
+    if (rows_per_lane) *rows_per_lane = sh.R;
+    if (feature_slots) *feature_slots = sh.F;
+    if (stack_slots) *stack_slots = sh.S;
     return PPI_OK;
 }
 

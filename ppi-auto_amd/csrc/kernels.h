@@ -74,9 +74,10 @@ struct PairDev {
     int32_t prog_base;     // first program of the pair in the concatenated population
     int32_t nprog;
     int64_t partial_base;  // first partial of the pair: [row_block][program] block of nrb*nprog
-    int32_t nrb;           // row tiles of 64*R rows
+    int32_t nrb;           // row tiles of rows_per_tile = 64*R rows (R: the pair's rows per lane)
     int32_t pdp_lanes, dp_groups;
     int32_t dp_local;
+    int32_t rows_per_tile;
 };
 
 // One work item: a row tile of a pair x a program range [p0, p1) of that pair.

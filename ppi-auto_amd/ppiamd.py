@@ -75,7 +75,7 @@ class BatchStats(C.Structure):
         ("graph_ms", C.c_double), ("kernel_ms", C.c_double), ("node_evals", C.c_double), ("build_ms", C.c_double),
         ("n_pairs", C.c_int32), ("n_items", C.c_int32), ("grid", C.c_int32), ("rows_per_lane", C.c_int32),
         ("feature_slots", C.c_int32), ("autotuned", C.c_int32), ("tune_ms", C.c_double * 4),
-        ("derived_columns", C.c_int32),
+        ("derived_columns", C.c_int32), ("n_shapes", C.c_int32), ("tuned_keys", C.c_int32),
     ]
 
 
@@ -148,7 +148,7 @@ ENGINE_SYMBOLS = [
     "ppi_ge_get_stats",
     "ppi_engine_time", "ppi_batch_create", "ppi_batch_destroy", "ppi_batch_configure", "ppi_batch_add_dataset",
     "ppi_batch_add_population", "ppi_batch_clear_populations", "ppi_batch_build", "ppi_batch_launch",
-    "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats",
+    "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats", "ppi_batch_pair_shape",
     "ppi_adjoint_create", "ppi_adjoint_destroy", "ppi_adjoint_configure", "ppi_adjoint_set_dataset_rowmajor",
     "ppi_adjoint_forward", "ppi_adjoint_backward", "ppi_adjoint_descend", "ppi_adjoint_time",
     "ppi_adjoint_get_stats", "ppi_adjoint_set_rows_per_lane", "ppi_ef_evaluate",
@@ -211,6 +211,7 @@ def lib() -> C.CDLL:
         L.ppi_batch_results.argtypes = [C.c_void_p, C.c_int32, C.c_float, P(C.c_float), P(C.c_int32), P(C.c_int32)]
         L.ppi_batch_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
         L.ppi_batch_get_stats.argtypes = [C.c_void_p, P(BatchStats)]
+        L.ppi_batch_pair_shape.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32)]
         L.ppi_adjoint_create.argtypes = [P(C.c_void_p), C.c_int32]
         L.ppi_adjoint_destroy.argtypes = [C.c_void_p]
         L.ppi_adjoint_destroy.restype = None
@@ -524,6 +525,12 @@ class Batch:
         s = BatchStats()
         _check(lib().ppi_batch_get_stats(self._h, C.byref(s)))
         return s
+
+    def pair_shape(self, pop_id: int) -> Tuple[int, int, int]:
+        """(rows per lane, LDS feature slots, register-stack slots) the autotuner gave a population."""
+        r, f, s = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        _check(lib().ppi_batch_pair_shape(self._h, pop_id, C.byref(r), C.byref(f), C.byref(s)))
+        return r.value, f.value, s.value
 
 
 class Adjoint:

@@ -162,56 +162,81 @@ def test_rows_per_lane_needs_small_programs():
     a.close()
 
 
+def fd_check(w, g, fit, want=60, per_program=3):
+    """Central differences of the FP64 fitness (h = 1e-3) on a sample of constants, all
+    perturbed copies evaluated in one oracle call.  Two bars, each on the constants where it
+    is meaningful:
+
+    * magnitude bar, |g - fd| <= 1e-3 (|fd| + M): where the difference quotient is accurate
+      (it agrees with the FP64 analytic gradient to 1e-4 (|G| + M)) and the float
+      computation's own derivative agrees with FP64 (as in check_both); M is the magnitude
+      of the summed per-row terms;
+    * relative bar, |g - fd| <= 1e-3 |fd|: where the quotient agrees with the FP64 gradient
+      to 1e-5 relative and the per-row terms cancel by less than 100x (M <= 100 |G|), so a
+      float sum of them can be held to a relative bar at all.
+
+    Returns (checked, strict) counts; asserts every checked constant passes its bars."""
+    nlin = w.data.shape[0]
+    cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_SQUARED, protected=True)
+    G, M, _ = oracle_layout(w.grammar.table, w.pop, w.data, True, ppiamd.EF_SQUARED, w.grammar.t_const)
+    Gv, _, _ = oracle_layout(w.grammar.table, w.pop, w.data, True, ppiamd.EF_SQUARED, w.grammar.t_const,
+                             f32_values=True)
+    G, M, Gv = (x.reshape(w.pop.n, w.pop.stride) for x in (G, M, Gv))
+    g = g.reshape(w.pop.n, w.pop.stride)
+    st = w.pop.stride
+    picks = []
+    for i in range(w.pop.n):
+        if not np.isfinite(fit[i]) or fit[i] >= 3.0e38 / nlin:
+            continue
+        pos = np.nonzero(w.pop.phenotype[i * st:i * st + w.pop.size[i]] == w.grammar.t_const)[0]
+        for p in pos[:per_program]:
+            if np.isfinite(G[i, p]) and np.isfinite(Gv[i, p]) and np.isfinite(M[i, p]) and M[i, p] > 0:
+                picks.append((i, p))
+        if len(picks) >= 4 * want:
+            break
+    h = 1e-3
+    k = len(picks)
+    ph = np.concatenate([w.pop.phenotype[i * st:(i + 1) * st] for i, _ in picks] * 2)
+    sz = np.concatenate([w.pop.size[i:i + 1] for i, _ in picks] * 2)
+    e64 = np.concatenate([w.pop.ephemeral[i * st:(i + 1) * st].astype(np.float64) for i, _ in picks] * 2)
+    for j, (_, p) in enumerate(picks):
+        e64[j * st + p] += h
+        e64[(k + j) * st + p] -= h
+    pert = ppiamd.Population(ph, e64.astype(np.float32), sz, st)
+    f = oracle.fitness_f64(cfg, w.grammar.table, pert, w.data, eph64=e64)
+    checked = strict = 0
+    misses = []
+    for j, (i, p) in enumerate(picks):
+        fd = (f[j] - f[k + j]) / (2 * h)
+        if not np.isfinite(fd):
+            continue
+        Gi, Mi, gi = G[i, p], M[i, p], float(g[i, p])
+        if abs(fd - Gi) > 1e-4 * (abs(Gi) + Mi):
+            continue  # curvature or a kink within h: the quotient is not a reference here
+        if abs(Gv[i, p] - Gi) > 0.5e-3 * (abs(Gi) + Mi):
+            continue  # the float computation's own derivative departs from FP64 here
+        if checked >= want:
+            break
+        checked += 1
+        ok = abs(gi - fd) <= 1e-3 * (abs(fd) + Mi)
+        if abs(fd - Gi) <= 1e-5 * abs(Gi) and Mi <= 100 * abs(Gi):
+            strict += 1
+            ok = ok and abs(gi - fd) <= 1e-3 * abs(fd)
+        if not ok:
+            misses.append((i, p, gi, fd, float(Gi), float(Gv[i, p]), float(Mi)))
+    assert not misses, f"{len(misses)} of {checked} gradients off the central differences: {misses}"
+    return checked, strict
+
+
 def test_gradient_against_finite_differences():
-    """Central differences of the FP64 fitness (h = 1e-3) on a sample of constants.  Judged
-    where the difference quotient is accurate (it agrees with the FP64 analytic gradient to
-    1e-4) and the float computation's derivative agrees with FP64 (as in check_both): there
-    the device gradient agrees with the differences to 1e-3."""
     w = ppiamd.workload("adjoint", n_programs=120, nlin=5000, seed=11)
     a = adjoint_for(w)
     fit = a.forward(w.pop)
-    g = a.backward(partials=ppiamd.PARTIALS_AUTO).reshape(w.pop.n, w.pop.stride)
-    cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_SQUARED, protected=True)
-    G, M, _ = oracle_layout(w.grammar.table, w.pop, w.data, True, ppiamd.EF_SQUARED, w.grammar.t_const)
-    Gv, Mv, _ = oracle_layout(w.grammar.table, w.pop, w.data, True, ppiamd.EF_SQUARED, w.grammar.t_const,
-                              f32_values=True)
-    G, M, Gv = (x.reshape(w.pop.n, w.pop.stride) for x in (G, M, Gv))
-    misses = []
-    h = 1e-3
-    base = w.pop.ephemeral.astype(np.float64)
-    checked = agree = 0
-    for i in range(w.pop.n):
-        if not np.isfinite(fit[i]) or fit[i] >= 3.0e38 / 5000:
-            continue
-        pos = np.nonzero(w.pop.phenotype[i * w.pop.stride:i * w.pop.stride + w.pop.size[i]] == w.grammar.t_const)[0]
-        sl = slice(i * w.pop.stride, (i + 1) * w.pop.stride)
-        one = ppiamd.Population(w.pop.phenotype[sl].copy(), w.pop.ephemeral[sl].copy(), w.pop.size[i:i + 1].copy(),
-                                w.pop.stride)
-        for p in pos[:3]:
-            e_plus = base[sl].copy()
-            e_minus = base[sl].copy()
-            e_plus[p] += h
-            e_minus[p] -= h
-            fp = oracle.fitness_f64(cfg, w.grammar.table, one, w.data, eph64=e_plus)[0]
-            fm = oracle.fitness_f64(cfg, w.grammar.table, one, w.data, eph64=e_minus)[0]
-            fd = (fp - fm) / (2 * h)
-            if not (np.isfinite(fd) and np.isfinite(G[i, p]) and np.isfinite(Gv[i, p]) and np.isfinite(M[i, p])):
-                continue
-            if M[i, p] == 0:
-                continue
-            if abs(fd - G[i, p]) > 1e-4 * (abs(G[i, p]) + M[i, p]):
-                continue  # curvature or a kink within h: the quotient is not a reference here
-            if abs(Gv[i, p] - G[i, p]) > 0.5e-3 * (abs(G[i, p]) + M[i, p]):
-                continue  # the float computation's own derivative departs from FP64 here
-            checked += 1
-            ok = abs(g[i, p] - fd) <= 1e-3 * (abs(fd) + M[i, p])
-            agree += ok
-            if not ok:
-                misses.append((i, p, float(g[i, p]), fd, float(G[i, p]), float(Gv[i, p]), float(M[i, p])))
-        if checked >= 60:
-            break
+    g = a.backward(partials=ppiamd.PARTIALS_AUTO)
+    checked, strict = fd_check(w, g, fit)
+    print(f"finite differences: {checked} constants, {strict} under the relative 1e-3 bar", flush=True)
     assert checked >= 40, f"only {checked} constants with a reliable difference quotient"
-    assert agree == checked, f"{checked - agree} of {checked} gradients off the central differences: {misses}"
+    assert strict >= 20, f"only {strict} constants under the relative bar"
 
 
 def test_descent_on_device():

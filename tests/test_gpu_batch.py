@@ -47,6 +47,33 @@ def test_batched_equals_plain_fp_form_bit_for_bit(small):
     b.close()
 
 
+def test_autotuned_shapes_per_pair():
+    """The autotuner keys pairs by program length and rows and may give them different shapes;
+    every pair's results equal the plain FP form's at the pair's own rows per lane, bit for bit."""
+    w = ppiamd.batched_workload(10, seed=23, p_range=(200, 3000), len_range=(8, 200),
+                                rows_range=(500, 300000))
+    # one pair of very short programs, so that at least two length classes occur
+    w.pops[0] = ppiamd.exact(w.grammars[0], 2000, 77, 3, 9)
+    b, ids = run_batch(w, 0)
+    st = b.stats()
+    shapes = [b.pair_shape(i) for i in ids]
+    print(f"shapes={shapes} launches={st.n_shapes} keys={st.tuned_keys} tune_ms={list(st.tune_ms)}",
+          flush=True)
+    assert st.autotuned == 1 and st.tuned_keys >= 2 and st.n_shapes >= 1
+    assert st.n_shapes == len(set(shapes))
+    eng = ppiamd.Engine(0)
+    for k, (p, d) in enumerate(zip(w.pops, w.datas)):
+        vec, idx = b.results(ids[k], alpha=ALPHA, best_size=2)
+        eng.configure(w.grammars[k].table, strategy=ppiamd.SEQ, error_fn=w.error_fn, form=ppiamd.FORM_FP,
+                      rows_per_lane=shapes[k][0], compute_units=256)
+        eng.set_dataset(d)
+        pv, pidx = eng.evaluate(p, alpha=ALPHA, best_size=2)
+        assert np.array_equal(vec.view(np.uint32), pv.view(np.uint32)), f"pair {k} shape {shapes[k]}"
+        assert np.array_equal(idx, pidx)
+    eng.close()
+    b.close()
+
+
 @pytest.mark.parametrize("strategy", [ppiamd.SEQ, ppiamd.PDP])
 def test_batched_autotuned_against_oracle(small, strategy):
     w = small

@@ -10,7 +10,7 @@
   4 slots), one persistent launch: bit-identical to the plain engine per pair at the same row tile, and
   the oracle's bar;
 * adjoint: 256 programs x 1,000,000 rows: gradients against the oracle's float-computation and FP64
-  forward-mode gradients.
+  forward-mode gradients; 64 programs x 1,000,000 rows against central differences of the FP64 fitness.
 The oracle runs OpenMP over programs on up to 16 host threads."""
 import os
 
@@ -128,11 +128,12 @@ def test_batched_long_programs_large_rows():
     b.build()
     b.run()
     res = [b.results(i, alpha=ALPHA, best_size=1) for i in ids]
-    R = b.stats().rows_per_lane
+    shapes = [b.pair_shape(i) for i in ids]
     b.close()
     with ppiamd.Engine(0) as e:
-        for gr, p, d, (vec, idx) in zip(grammars, pops, datas, res):
-            # the same row tile (64 x R rows) as the batched launch: the same float partials
+        for gr, p, d, (vec, idx), (R, _, S) in zip(grammars, pops, datas, res, shapes):
+            assert S > 4, "the pair must run with the 9-slot stack"
+            # the same row tile (64 x R rows) as the pair's batched launch: the same float partials
             e.configure(gr.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_SQUARED, protected=True,
                         form=ppiamd.FORM_FP, rows_per_lane=R)
             e.set_dataset(d)
@@ -167,3 +168,22 @@ def test_adjoint_full_rows():
         assert not off_64.any(), f"{off_64.sum()} off FP64"
     print(f"adjoint 1M rows: FP64 judges {judged:.3f} of the constants", flush=True)
     assert judged >= 0.7
+
+
+def test_adjoint_finite_differences_full_rows():
+    """The adjoint workload's 1,000,000 rows: device gradients against central differences of
+    the FP64 fitness, under the magnitude bar and, where the terms cancel by less than 100x,
+    the relative bar 1e-3 |fd| (test_gpu_adjoint.fd_check)."""
+    from test_gpu_adjoint import fd_check
+    w = ppiamd.workload("adjoint", n_programs=64, nlin=1_000_000, seed=7)
+    print("case full adjoint finite differences: 64 programs x 1,000,000 rows", flush=True)
+    a = ppiamd.Adjoint(0)
+    a.configure(w.grammar.table, strategy=ppiamd.SEQ, error_fn=w.error_fn, protected=w.protected)
+    a.set_dataset(w.data)
+    fit = a.forward(w.pop)
+    g = a.backward()
+    a.close()
+    checked, strict = fd_check(w, g, fit, want=40)
+    print(f"adjoint 1M rows, finite differences: {checked} constants, {strict} under the relative bar",
+          flush=True)
+    assert checked >= 25 and strict >= 10
