@@ -41,8 +41,13 @@ hipError_t launch_fp(const EvalArgs& A, int R, int F, int S, bool P, bool N, hip
 }
 
 hipError_t launch_ordered_sum(const EvalArgs& A, int rows_per_block, bool last, hipStream_t st) {
-    hipLaunchKernelGGL(dev::ordered_sum_kernel, dim3((unsigned)((A.nprog + 255) / 256)), dim3(256), 0, st, A,
-                       rows_per_block, last ? 1 : 0);
+    // One wave per workgroup: 30,000 programs are 469 waves, spread over every CU (256-thread
+    // workgroups put them on 118 CUs).
+    const dim3 grid((unsigned)((A.nprog + 63) / 64)), block(64);
+    if (A.reducemax)
+        hipLaunchKernelGGL(dev::ordered_sum_kernel<true>, grid, block, 0, st, A, rows_per_block, last ? 1 : 0);
+    else
+        hipLaunchKernelGGL(dev::ordered_sum_kernel<false>, grid, block, 0, st, A, rows_per_block, last ? 1 : 0);
     return hipGetLastError();
 }
 

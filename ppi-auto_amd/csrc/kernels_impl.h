@@ -808,45 +808,106 @@ __device__ __forceinline__ void ordered_take(float e, float fn, bool rmax, float
     stop = (stop || !fin) ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256) ordered_sum_kernel(EvalArgs A, int rows_per_block, int last) {
-    constexpr int V = 16;  // float4 per batch
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.nprog) return;
-    const ProgMeta m = A.meta[p];
-    float part = A.osum[p];
-    uint32_t stop = A.ostop[p];
-    const float fn = (float)A.nlin;
-    const bool rmax = A.reducemax != 0;
-    if (!(m.flags & kEmpty)) {
-        for (int b = 0; b < A.nrb_pass && !stop; ++b) {
-            const int64_t row0 = (int64_t)(A.rb0 + b) * rows_per_block;
-            const int nrows = (int)min<int64_t>(rows_per_block, (int64_t)A.nlin - row0);
-            const int nv = (nrows + 3) >> 2;
-            const float4* src = reinterpret_cast<const float4*>(A.rowerr + ((int64_t)b * A.nprog + p) * rows_per_block);
-            float4 cur[V];
+// Ordered sums: one thread per program, one wave per workgroup (30,000 programs are 469 waves,
+// spread over every CU).  Each lane streams its program's row errors of the pass in row order
+// (rowerr[tile][program][row of the tile]) in batches of 64 (16 float4), the next batch loading
+// while the current one is added.
+//
+// Full tiles take a fast path: the 64 errors of a batch are added with one instruction each on
+// the dependency chain (REDUCEMAX: a compare and a select), with the sequential loop's
+// non-finite test moved off the chain: for sums, a sum that is still finite at the end of the
+// batch proves that every error in it was finite (an inf or NaN error leaves the sum
+// non-finite); REDUCEMAX tests every error's class.  A batch that fails the test (a non-finite
+// error, or a sum that overflowed) is added again from its starting value with the exact
+// per-row rules (ordered_take), which also sets `stop`.  The last tile of the dataset, when
+// partial, is added row by row with the same rules.
+constexpr int kOrdV = 16;  // float4 per batch
+
+template <bool RMAX>
+__device__ __forceinline__ void ordered_batch(const float4 (&c)[kOrdV], float fn, float& part, uint32_t& stop) {
+    float q = part;
+    bool bad = false;
 #pragma unroll
-            for (int k = 0; k < V; ++k) cur[k] = k < nv ? src[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int base = 0; base < nv && !stop; base += V) {
-                float4 nxt[V];
+    for (int k = 0; k < kOrdV; ++k) {
+        const float e[4] = {c[k].x, c[k].y, c[k].z, c[k].w};
 #pragma unroll
-                for (int k = 0; k < V; ++k)
-                    nxt[k] = base + V + k < nv ? src[base + V + k] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int k = 0; k < V; ++k) {
-                    const int j = (base + k) * 4;
-                    if (j + 0 < nrows) ordered_take(cur[k].x, fn, rmax, part, stop);
-                    if (j + 1 < nrows) ordered_take(cur[k].y, fn, rmax, part, stop);
-                    if (j + 2 < nrows) ordered_take(cur[k].z, fn, rmax, part, stop);
-                    if (j + 3 < nrows) ordered_take(cur[k].w, fn, rmax, part, stop);
-                }
-#pragma unroll
-                for (int k = 0; k < V; ++k) cur[k] = nxt[k];
+        for (int i = 0; i < 4; ++i) {
+            if constexpr (RMAX) {
+                bad = bad || !finite_f(e[i]);
+                const float x = e[i] * fn;
+                q = (x > q) ? x : q;
+            } else {
+                q = q + e[i];
             }
         }
     }
+    if constexpr (!RMAX) bad = !finite_f(q);
+    if (__builtin_expect(bad && !stop, 0)) {
+        q = part;
+        uint32_t st = 0;
+#pragma unroll
+        for (int k = 0; k < kOrdV; ++k) {
+            ordered_take(c[k].x, fn, RMAX, q, st);
+            ordered_take(c[k].y, fn, RMAX, q, st);
+            ordered_take(c[k].z, fn, RMAX, q, st);
+            ordered_take(c[k].w, fn, RMAX, q, st);
+        }
+        stop = st;
+    }
+    part = stop ? part : q;
+}
+
+template <bool RMAX>
+__global__ void __launch_bounds__(64) ordered_sum_kernel(EvalArgs A, int rows_per_block, int last) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < A.nprog;
+    const int pp = live ? p : A.nprog - 1;  // the tail wave's spare lanes shadow the last program
+    const ProgMeta m = A.meta[pp];
+    float part = A.osum[pp];
+    uint32_t stop = A.ostop[pp] | ((m.flags & kEmpty) ? 1u : 0u);
+    const float fn = (float)A.nlin;
+    // Full tiles of the pass, then at most one partial tile (the dataset's last).
+    const int64_t row_end = (int64_t)(A.rb0 + A.nrb_pass) * rows_per_block;
+    const bool partial_last = row_end > (int64_t)A.nlin;
+    const int full = A.nrb_pass - (partial_last ? 1 : 0);
+    const int bpt = rows_per_block / (4 * kOrdV);  // batches per tile (rows_per_block = 64 R)
+    const int nb = full * bpt;
+    const int64_t tstride = (int64_t)A.nprog * rows_per_block;  // floats between a program's tiles
+    const float* base = A.rowerr + (int64_t)pp * rows_per_block;
+    auto batch_ptr = [&](int b) {
+        const int t = b / bpt;
+        return reinterpret_cast<const float4*>(base + t * tstride + (b - t * bpt) * (4 * kOrdV));
+    };
+    float4 a[kOrdV], c[kOrdV];
+    if (nb > 0) {
+        const float4* src = batch_ptr(0);
+#pragma unroll
+        for (int k = 0; k < kOrdV; ++k) a[k] = src[k];
+    }
+    for (int b = 0; b < nb; b += 2) {
+        // The loads are unconditional (a batch past the end reloads the last one) so that the
+        // compiler waits only for the batch about to be added.
+        const float4* sc = batch_ptr(min(b + 1, nb - 1));
+#pragma unroll
+        for (int k = 0; k < kOrdV; ++k) c[k] = sc[k];
+        ordered_batch<RMAX>(a, fn, part, stop);
+        const float4* sa = batch_ptr(min(b + 2, nb - 1));
+#pragma unroll
+        for (int k = 0; k < kOrdV; ++k) a[k] = sa[k];
+        if (b + 1 < nb) ordered_batch<RMAX>(c, fn, part, stop);
+        if (__all(stop)) break;
+    }
+    if (partial_last && !stop) {
+        const int64_t row0 = row_end - rows_per_block;
+        const int nrows = (int)((int64_t)A.nlin - row0);
+        const float* src = base + (int64_t)full * tstride;
+        for (int j = 0; j < nrows && !stop; ++j) ordered_take(src[j], fn, RMAX, part, stop);
+    }
+    if (!live) return;
+    const uint32_t stopped = stop && !(m.flags & kEmpty) ? 1u : A.ostop[p];
     A.osum[p] = part;
-    A.ostop[p] = stop;
-    if (last) semantics(A, part, m.flags, stop ? 0u : 0xffffffffu, 0u, m.size, &A.raw[p], &A.vec[p]);
+    A.ostop[p] = stopped;
+    if (last) semantics(A, part, m.flags, stopped ? 0u : 0xffffffffu, 0u, m.size, &A.raw[p], &A.vec[p]);
 }
 
 // ---------------------------------------------------------------------------

@@ -101,3 +101,43 @@ def test_rejects_tree_strategies(eng):
     for strategy in (ppiamd.FP, ppiamd.PDP):
         with pytest.raises(ppiamd.PpiError):
             eng.configure(b.table, strategy=strategy, form=ppiamd.FORM_ORDERED)
+
+
+@pytest.mark.parametrize("rows_per_lane", [1, 2, 4, 8])
+def test_planted_rows_bit_exact(eng, rows_per_lane):
+    """Program k returns column k; the target is 0, so the row errors are the column's values.
+    Each column plants a non-finite value (or a run of huge ones) at a chosen row: the first row,
+    both ends of a 64-row batch, a row tile boundary, mid-batch, the partial last tile, a later
+    pass; sums that overflow with finite errors only; -0 errors.  Sum and REDUCEMAX, one pass and
+    several: bit-identical to the oracle's sequential loop."""
+    nlin = 5000 + 37  # the last row tile is partial at every rows per lane
+    rng = np.random.default_rng(rows_per_lane)
+    plants = [(0, np.inf), (63, np.nan), (64, -np.inf), (127, np.nan), (255, np.inf), (256, np.nan),
+              (511, np.inf), (100, np.nan), (nlin - 1, np.inf), (nlin - 20, np.nan), (4000, np.inf),
+              (None, None), (None, "huge"), (None, "negzero")]
+    cols = []
+    for row, v in plants:
+        c = rng.uniform(0.0, 2.0, nlin).astype(np.float32)
+        if v == "huge":
+            c[1000:1200] = np.float32(3.0e38)  # finite errors whose row-order sum overflows
+        elif v == "negzero":
+            c[:] = np.float32(-0.0)
+        elif row is not None:
+            c[row] = np.float32(v)
+        cols.append(c)
+    data = np.stack(cols + [np.zeros(nlin, np.float32)], axis=1)
+    b = Builder(len(cols))
+    pop = b.population([[f"x{k}"] for k in range(len(cols))] + [["+", f"x{k}", 1.0] for k in range(len(cols))])
+    for reducemax in (False, True):
+        for pass_rows in (0, 1024):
+            eng.configure(b.table, strategy=ppiamd.SEQ, error_fn=ppiamd.EF_ABS, reducemax=reducemax,
+                          form=ppiamd.FORM_ORDERED, ordered_pass_rows=pass_rows, rows_per_lane=rows_per_lane,
+                          compute_units=256)
+            eng.set_dataset(data)
+            vec, idx = eng.evaluate(pop, alpha=ALPHA, best_size=1)
+            dpl, ppl, bpl = oracle.reference_geometry(nlin, pop.n)
+            cfg = oracle.make_cfg(ppiamd.SEQ, ppiamd.EF_ABS, reducemax, True, None, dpl, ppl, bpl)
+            ovec, oidx = oracle.interpret(cfg, b.table, pop, data, alpha=ALPHA, best_size=1)
+            print(f"case planted R={rows_per_lane} rm={reducemax} pass_rows={pass_rows}", flush=True)
+            assert_bits(vec, ovec, f"planted rows R={rows_per_lane} rm={reducemax} pass_rows={pass_rows}")
+            assert np.array_equal(idx, oidx)
