@@ -511,7 +511,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU-baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline workload and CPU baseline only")
-    ap.add_argument("--only", choices=["fp", "batched", "adjoint", "adjoint_pass", "ge"],
+    ap.add_argument("--only", choices=["fp", "ordered", "batched", "adjoint", "adjoint_pass", "ge"],
                     help="one leg, no CPU baseline (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the 1 GiB copy kernel used to scale the memory counters, then exit")
@@ -655,6 +655,16 @@ def main():
                                         "sample (the first programs, all rows), for the GPU and for the reference's "
                                         "sequential path (the oracle), with every operator and the error function "
                                         "in double for FP64; sentinel (non-finite) programs are counted apart")
+    if a.only == "ordered":
+        log("ordered form (sums bit-identical to the sequential path): regression workload")
+        w = ppiamd.workload("regression")
+        eng = ppiamd.Engine(0)
+        od = bench_engine(eng, w, ppiamd.FORM_ORDERED, a.steps, a.warmup)
+        eng.close()
+        line.update({"value": round(od["node_evals"] / (od["ms_per_step"] * 1e-3) / 1e9, 3),
+                     "ms_per_step": round(od["ms_per_step"], 4),
+                     "config": {"workload": "regression: 30,000 programs x 100,000 rows x 16 features, ordered form "
+                                            "(per-row errors stored, row-order sums)", "form": "ORDERED"}})
     if a.only == "adjoint_pass":
         log("adjoint backward pass: 30,000 programs x 1,000,000 rows")
         line["adjoint_pass"] = bench_adjoint_pass(a.steps, a.warmup)

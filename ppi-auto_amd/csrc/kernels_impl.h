@@ -557,10 +557,12 @@ __device__ __forceinline__ void accumulate(const EvalArgs& A, const float (&acc)
     }
 }
 
-// Wave-reduces a lane partial (returned, the same in every lane) and records program
-// p's bad rows (lane 0, atomics; rare).
-__device__ __forceinline__ float reduce_record(const EvalArgs& A, int p, int lane, float part, uint32_t kmin,
-                                               uint32_t kmax) {
+// Wave-reduce a lane partial and bad-row record; lane 0 stores them for program p.
+// (Buffering a wave's partials one program per lane and storing 64 at a time cuts the
+// kernel's write traffic by a third, but the lane select costs two more VALU instructions
+// per program end than this store and the kernel measured 1.3% slower.)
+__device__ __forceinline__ void publish(const EvalArgs& A, int64_t slot, int p, int lane, float part, uint32_t kmin,
+                                        uint32_t kmax) {
     part = A.reducemax ? wave_max(part) : wave_sum(part);
     if (__any(kmin != 0xffffffffu)) {
         kmin = wave_umin(kmin);
@@ -570,28 +572,8 @@ __device__ __forceinline__ float reduce_record(const EvalArgs& A, int p, int lan
             atomicMax(&A.bad_hi[p], kmax);
         }
     }
-    return part;
+    if (lane == 0) A.partial[slot] = part;
 }
-
-// A wave's row-tile partials, one program per lane: lane j holds the partial of program
-// first + j.  The wave's programs are consecutive, so their partials are consecutive in
-// partial[row_block][program]: one coalesced store per 64 programs instead of one
-// single-float store per program (each of which wrote back a partly filled line).
-struct PartialBuffer {
-    float v = 0.0f;
-    int first;  // wave-uniform
-    __device__ __forceinline__ explicit PartialBuffer(int p0) : first(p0) {}
-    // Program p's total (wave-uniform); stores the buffer when it is full or p is the
-    // wave's last program.
-    __device__ __forceinline__ void put(const EvalArgs& A, int rb, int p, int last, int lane, float total) {
-        const int k = p - first;
-        v = lane == k ? total : v;
-        if (k == 63 || p == last) {
-            if (lane <= k) A.partial[(int64_t)rb * A.nprog + first + lane] = v;
-            first = p + 1;
-        }
-    }
-};
 
 // ---------------------------------------------------------------------------
 // FP form
@@ -661,7 +643,6 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
     uint32_t wc = code[blk + lane].code, wa = code[blk + lane].arg;
     int k = (int)(gbeg - blk);
     int p = pa;
-    PartialBuffer pbuf(pa);
     float acc[R];
     typename C::Stk stk;
     for (; blk < gend; blk += 64) {
@@ -678,7 +659,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
                     float part = 0.0f;
                     uint32_t kmin = 0xffffffffu, kmax = 0u;
                     if (!(a & kEmpty)) accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
-                    pbuf.put(A, rb, p, pb - 1, lane, reduce_record(A, p, lane, part, kmin, kmax));
+                    publish(A, (int64_t)rb * A.nprog + p, p, lane, part, kmin, kmax);
                 }
                 ++p;
                 continue;
