@@ -525,20 +525,41 @@ __device__ __forceinline__ void tile_features(const EvalArgs& A, const int (&row
     }
 }
 
-// Error of the R rows of one lane: accumulate finite errors, record bad rows.
+// Error of the R rows of one lane: accumulate finite errors, record bad rows.  `full`: every
+// row of the tile is a dataset row (wave-uniform).
 template <int R>
 __device__ __forceinline__ void accumulate(const EvalArgs& A, const float (&acc)[R], const float (&y)[R],
-                                           const bool (&valid)[R], const int (&row)[R], float& part,
+                                           const bool (&valid)[R], const int (&row)[R], bool full, float& part,
                                            uint32_t& kmin, uint32_t& kmax) {
+    const float fn = (float)A.nlin;
+    float e[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) e[r] = error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n);
+    if (!A.reducemax) {
+        // Sums: the rows' errors are added in row order with no per-row test.  A finite result
+        // proves every error finite (an inf or NaN error leaves the sum non-finite), and is then
+        // the exact rules' result bit for bit; otherwise (a non-finite error, or a sum that
+        // overflowed) the rules below recompute it.  Rows past the end, only in the dataset's
+        // last tile, add 0.
+        float q = part;
+        if (full) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) q = q + e[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) q = q + (valid[r] ? e[r] : 0.0f);
+        }
+        if (__builtin_expect(finite_f(q), 1)) {
+            part = q;
+            return;
+        }
+    }
     // Branch-free over the rows (a row that is past the end or not finite adds 0, which
     // leaves the partial's bits unchanged: it is never -0 and a maximum of non-negative
     // values is at least 0); the bad-row record is one rarely taken branch.
-    const float fn = (float)A.nlin;
-    float e[R];
     bool any_bad = false;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        e[r] = error_fn(A.error_fn, acc[r], y[r], A.cost, A.cost_n);
         const bool fin = finite_f(e[r]);
         any_bad = any_bad || (valid[r] && !fin);
         const bool take = valid[r] && fin;
@@ -619,6 +640,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
     float y[R];
     bool valid[R];
     tile_rows<R>(A, row0, lane, row, y, valid);
+    const bool full = row0 + 64 * R <= (int64_t)A.nlin;
     __syncthreads();
 
     const int per = (c1 - c0 + kWaves - 1) / kWaves;
@@ -658,7 +680,7 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
                 } else {
                     float part = 0.0f;
                     uint32_t kmin = 0xffffffffu, kmax = 0u;
-                    if (!(a & kEmpty)) accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
+                    if (!(a & kEmpty)) accumulate<R>(A, acc, y, valid, row, full, part, kmin, kmax);
                     publish(A, (int64_t)rb * A.nprog + p, p, lane, part, kmin, kmax);
                 }
                 ++p;
@@ -1464,7 +1486,7 @@ __global__ void __launch_bounds__(kBlock) pdp_kernel(EvalArgs A) {
             } else {
                 C::run(as_const(A.code) + m.code_off, m.ncode, GlobalFeat<R>{A.X, A.ldx, row}, acc, stk);
             }
-            accumulate<R>(A, acc, y, valid, row, part, kmin, kmax);
+            accumulate<R>(A, acc, y, valid, row, row0 + 64 * R <= (int64_t)A.nlin, part, kmin, kmax);
         }
     }
     part = A.reducemax ? wave_max(part) : wave_sum(part);
