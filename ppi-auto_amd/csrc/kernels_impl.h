@@ -643,9 +643,24 @@ __device__ __forceinline__ void fp_tile_body(const EvalArgs& A, int rb, int c0, 
     const bool full = row0 + 64 * R <= (int64_t)A.nlin;
     __syncthreads();
 
-    const int per = (c1 - c0 + kWaves - 1) / kWaves;
-    const int pa = c0 + wave * per;
-    const int pb = min(pa + per, c1);
+    // The waves split the programs by tape length, not by count: a wave that finishes early
+    // idles its slot until the workgroup ends.  Wave w takes the programs whose first
+    // instruction lies in [t_w, t_{w+1}), t_w = g0 + w (g1 - g0) / kWaves (binary searches on
+    // the programs' tape offsets, wave-uniform).
+    auto first_at = [&](int64_t t) {  // first program in [c0, c1) starting at or after t (c1 if none)
+        int lo = c0, hi = c1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)as_const(A.meta)[mid].code_off < t) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const ProgMeta mlast = load_meta(A.meta, c1 - 1);
+    const int64_t g0 = as_const(A.meta)[c0].code_off;
+    const int64_t g1 = (int64_t)mlast.code_off + mlast.ncode + 1;
+    const int pa = wave == 0 ? c0 : first_at(g0 + (g1 - g0) * wave / kWaves);
+    const int pb = wave == kWaves - 1 ? c1 : first_at(g0 + (g1 - g0) * (wave + 1) / kWaves);
     if (pa >= pb) return;
 
     const ProgMeta first = load_meta(A.meta, pa);
