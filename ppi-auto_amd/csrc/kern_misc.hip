@@ -52,7 +52,8 @@ hipError_t launch_ordered_sum(const EvalArgs& A, int rows_per_block, bool last, 
 }
 
 hipError_t launch_finalize(const EvalArgs& A, int nrb, hipStream_t st) {
-    hipLaunchKernelGGL(dev::finalize_kernel, dim3((unsigned)((A.nprog + 255) / 256)), dim3(256), 0, st, A, nrb);
+    // One wave per workgroup: 30,000 programs are 469 waves, over every CU.
+    hipLaunchKernelGGL(dev::finalize_kernel, dim3((unsigned)((A.nprog + 63) / 64)), dim3(64), 0, st, A, nrb);
     return hipGetLastError();
 }
 
@@ -144,7 +145,7 @@ hipError_t launch_ef_prepare(const EvalArgs& A, hipStream_t st) {
 
 namespace ppi {
 hipError_t launch_batch_finalize(const BatchArgs& B, const int32_t* pair_of, int32_t n_total, hipStream_t st) {
-    hipLaunchKernelGGL(dev::batch_finalize_kernel, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, st, B,
+    hipLaunchKernelGGL(dev::batch_finalize_kernel, dim3((unsigned)((n_total + 63) / 64)), dim3(64), 0, st, B,
                        pair_of, n_total);
     return hipGetLastError();
 }

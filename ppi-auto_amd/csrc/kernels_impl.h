@@ -773,18 +773,38 @@ __global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S>::value)) batch_kern
 // roundings before the double sum.
 __device__ __forceinline__ SumTotal reduce_partials(const EvalArgs& A, const float* partial, int nrb,
                                                     int64_t stride) {
+    // The partials are loaded 16 at a time (independent loads in flight together) and taken
+    // in tile order, one after another: the order of the additions, and so the bits, do not
+    // depend on the batching.
+    constexpr int kB = 16;
     if (A.reducemax) {
         float mx = 0.0f;
-        for (int rb = 0; rb < nrb; ++rb) mx = fmaxf(mx, partial[(int64_t)rb * stride]);
+        int rb = 0;
+        for (; rb + kB <= nrb; rb += kB) {
+            float v[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) v[k] = partial[(int64_t)(rb + k) * stride];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) mx = fmaxf(mx, v[k]);
+        }
+        for (; rb < nrb; ++rb) mx = fmaxf(mx, partial[(int64_t)rb * stride]);
         return {mx, false};
     }
     double sum = 0.0;
     int ninf = 0;
-    for (int rb = 0; rb < nrb; ++rb) {
-        const float v = partial[(int64_t)rb * stride];
+    auto take = [&](float v) {
         if (isinf(v) && v > 0.0f) ++ninf;
         else sum += (double)v;
+    };
+    int rb = 0;
+    for (; rb + kB <= nrb; rb += kB) {
+        float v[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) v[k] = partial[(int64_t)(rb + k) * stride];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) take(v[k]);
     }
+    for (; rb < nrb; ++rb) take(partial[(int64_t)rb * stride]);
     return decide_sum(A, sum, 14.0 * kUlp + (double)nrb * 0x1.0p-53, ninf);
 }
 
@@ -1285,17 +1305,35 @@ __device__ __forceinline__ void recheck_program(const EvalArgs& A, const ProgMet
 
 __device__ __forceinline__ bool marked(const float* raw) { return __float_as_uint(*raw) == kRecheckBits; }
 
+// Which of programs [p0, p0 + kRecheckSpan) (below n) are marked, as a bit mask: lane q of
+// wave 0 reads program p0 + q's mark, so the span costs one load round trip.  Called by every
+// thread of the workgroup (a barrier inside); the marks are read before any recheck rewrites
+// its program's result.
+__device__ __forceinline__ uint32_t span_marks(const float* raw, int p0, int n, uint32_t* marks_s) {
+    static_assert(kRecheckSpan <= 32, "the span's marks fit one 32-bit mask");
+    if (threadIdx.x < 64) {
+        const int p = p0 + (int)threadIdx.x;
+        const bool m = threadIdx.x < (unsigned)kRecheckSpan && p < n && marked(&raw[p]);
+        const uint64_t bal = __ballot(m);
+        if (threadIdx.x == 0) *marks_s = (uint32_t)bal;
+    }
+    __syncthreads();
+    return *marks_s;
+}
+
 // Workgroup b looks at programs [b*kRecheckSpan, (b+1)*kRecheckSpan); the mark is
 // read by every thread from the same address, so the branches are uniform.
 template <bool P, bool N>
 __global__ void __launch_bounds__(kBlock) recheck_kernel(EvalArgs A) {
     __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
     __shared__ int stop_s, deep_s;
+    __shared__ uint32_t marks_s;
     bool staged = false;
-    for (int q = 0; q < kRecheckSpan; ++q) {
+    uint32_t marks = span_marks(A.raw, (int)blockIdx.x * kRecheckSpan, A.nprog, &marks_s);
+    while (marks) {
+        const int q = __builtin_ctz(marks);
+        marks &= marks - 1u;
         const int p = (int)blockIdx.x * kRecheckSpan + q;
-        if (p >= A.nprog) break;
-        if (!marked(&A.raw[p])) continue;
         if (!staged) {
             stage_math<N>();
             staged = true;
@@ -1346,11 +1384,13 @@ __global__ void __launch_bounds__(kBlock) batch_recheck_kernel(BatchArgs B, cons
                                                               int32_t n_total) {
     __shared__ __attribute__((aligned(16))) float PE[kPdpMaxLanes];
     __shared__ int stop_s, deep_s;
+    __shared__ uint32_t marks_s;
     bool staged = false;
-    for (int q = 0; q < kRecheckSpan; ++q) {
+    uint32_t marks = span_marks(B.base.raw, (int)blockIdx.x * kRecheckSpan, n_total, &marks_s);
+    while (marks) {
+        const int q = __builtin_ctz(marks);
+        marks &= marks - 1u;
         const int g = (int)blockIdx.x * kRecheckSpan + q;
-        if (g >= n_total) break;
-        if (!marked(&B.base.raw[g])) continue;
         if (!staged) {
             stage_math<N>();
             staged = true;
