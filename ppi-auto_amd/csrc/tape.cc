@@ -374,13 +374,14 @@ void put_end(Tape* out, bool want_const_pos, uint32_t flags) {
 
 namespace {
 
-// Chains of unary operators over input columns in programs [lo, hi), counted into `counts`.
+// Chains of unary operators over input columns in programs [lo, hi) (every `step`-th program),
+// counted into `counts`.
 void count_chains(const SymbolTable& st, const int32_t* phenotype, const float* ephemeral, const int32_t* size,
-                  int32_t stride, int32_t lo, int32_t hi, int32_t ncol,
-                  std::unordered_map<uint64_t, uint32_t>* counts) {
+                  int32_t stride, int32_t lo, int32_t hi, int32_t ncol, int32_t step, FlatMap64<uint32_t>* counts) {
     struct Item { uint64_t key; int ops; };
     std::vector<Item> stk;
     for (int32_t p = lo; p < hi; ++p) {
+        if (p % step) continue;
         const int32_t* ph = phenotype + (size_t)p * stride;
         const float* eph = ephemeral + (size_t)p * stride;
         stk.clear();
@@ -400,7 +401,7 @@ void count_chains(const SymbolTable& st, const int32_t* phenotype, const float* 
                 if (it.key && it.ops < kChainMaxOps) {
                     it.key = chain_extend(it.key, it.ops, unop_of(op));
                     it.ops++;
-                    ++(*counts)[it.key];
+                    ++counts->at(it.key);
                 } else {
                     it.key = 0;
                 }
@@ -435,37 +436,46 @@ void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* 
     const bool full = (int32_t)derived->cols.size() >= derived->cap;
     if (full && !derived->refill) return;
     threads = std::max(1, std::min(threads, n / 128));
-    std::vector<std::unordered_map<uint64_t, uint32_t>> counts((size_t)threads);
-    if (threads == 1) {
-        count_chains(st, phenotype, ephemeral, size, stride, 0, n, ncol, &counts[0]);
-    } else {
-        std::vector<std::thread> pool;
-        const int32_t chunk = (n + threads - 1) / threads;
-        for (int t = 0; t < threads; ++t) {
-            const int32_t lo = t * chunk, hi = std::min(n, lo + chunk);
-            if (lo < hi)
-                pool.emplace_back([&, t, lo, hi] {
-                    count_chains(st, phenotype, ephemeral, size, stride, lo, hi, ncol, &counts[(size_t)t]);
-                });
+    // Chain counts of every `step`-th program, on `threads` workers, merged in worker order.
+    auto count_all = [&](int32_t step) {
+        std::vector<FlatMap64<uint32_t>> counts((size_t)threads);
+        if (threads == 1) {
+            count_chains(st, phenotype, ephemeral, size, stride, 0, n, ncol, step, &counts[0]);
+        } else {
+            std::vector<std::thread> pool;
+            const int32_t chunk = (n + threads - 1) / threads;
+            for (int t = 0; t < threads; ++t) {
+                const int32_t lo = t * chunk, hi = std::min(n, lo + chunk);
+                if (lo < hi)
+                    pool.emplace_back([&, t, lo, hi] {
+                        count_chains(st, phenotype, ephemeral, size, stride, lo, hi, ncol, step, &counts[(size_t)t]);
+                    });
+            }
+            for (auto& th : pool) th.join();
+            for (int t = 1; t < threads; ++t)
+                counts[(size_t)t].for_each([&](uint64_t k, uint32_t v) { counts[0].at(k) += v; });
         }
-        for (auto& th : pool) th.join();
-        for (int t = 1; t < threads; ++t)
-            for (const auto& kv : counts[(size_t)t]) counts[0][kv.first] += kv.second;
-    }
+        return std::move(counts[0]);
+    };
     if (full) {
+        // Is the table still used?  Judged on every 8th program of a large population (the
+        // share of chain occurrences the table covers; a full count follows only a refill).
+        const FlatMap64<uint32_t> sample = count_all(n >= 1024 ? 8 : 1);
         uint64_t all = 0, covered = 0;
-        for (const auto& kv : counts[0]) {
-            all += kv.second;
-            if (derived->index.count(kv.first)) covered += kv.second;
-        }
+        sample.for_each([&](uint64_t k, uint32_t v) {
+            all += v;
+            if (derived->index.contains(k)) covered += v;
+        });
         if (all == 0 || (double)covered >= kDerivedKeepShare * (double)all) return;
         derived->reset(derived->base, derived->cap);
     }
+    const FlatMap64<uint32_t> counts = count_all(1);
     struct Cand { uint64_t key; uint32_t count; int ops; };
     std::vector<Cand> cand;
-    cand.reserve(counts[0].size());
-    for (const auto& kv : counts[0])
-        if (!derived->index.count(kv.first)) cand.push_back(Cand{kv.first, kv.second, chain_ops_of(kv.first)});
+    cand.reserve(counts.size());
+    counts.for_each([&](uint64_t k, uint32_t v) {
+        if (!derived->index.contains(k)) cand.push_back(Cand{k, v, chain_ops_of(k)});
+    });
     std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) {
         if (a.count != b.count) return a.count > b.count;
         if (a.ops != b.ops) return a.ops < b.ops;
@@ -483,7 +493,7 @@ void plan_derived(const SymbolTable& st, const int32_t* phenotype, const float* 
             src = derived->find(prefix);
             if (src < 0) continue;  // the prefix got no column
         }
-        derived->index.emplace(c.key, derived->base + (int32_t)derived->cols.size());
+        derived->index.at(c.key) = derived->base + (int32_t)derived->cols.size();
         derived->cols.push_back(DerivedColumns::Col{u, src});
     }
 }

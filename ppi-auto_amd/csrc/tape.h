@@ -176,6 +176,75 @@ struct CompileStats {
 // first), so the table is deterministic and, while compiling, read-only.
 constexpr int kChainMaxOps = 9;
 
+// Open-addressing hash map from non-zero 64-bit keys (chain keys always carry a column + 1 in
+// their low bits) to V: linear probing, power-of-two capacity at most half full.  The chain
+// tables are looked up once per unary node over a column while compiling and counted once per
+// such node while planning; a node-based std::unordered_map made that the compiler's largest cost.
+template <typename V>
+struct FlatMap64 {
+    std::vector<uint64_t> keys;  // 0 = empty slot
+    std::vector<V> vals;
+    size_t count = 0;
+    int shift = 64;
+
+    size_t slot_of(uint64_t k) const { return (size_t)((k * 0x9E3779B97F4A7C15ull) >> shift); }
+    const V* find(uint64_t k) const {
+        if (keys.empty()) return nullptr;
+        const size_t mask = keys.size() - 1;
+        for (size_t i = slot_of(k);; i = (i + 1) & mask) {
+            if (keys[i] == k) return &vals[i];
+            if (keys[i] == 0) return nullptr;
+        }
+    }
+    bool contains(uint64_t k) const { return find(k) != nullptr; }
+    // The value of key k, inserted as V{} if absent.
+    V& at(uint64_t k) {
+        if ((count + 1) * 2 > keys.size()) rehash(keys.empty() ? 1024 : keys.size() * 2);
+        const size_t mask = keys.size() - 1;
+        size_t i = slot_of(k);
+        for (; keys[i] != 0 && keys[i] != k; i = (i + 1) & mask) {}
+        if (keys[i] == 0) {
+            keys[i] = k;
+            vals[i] = V{};
+            ++count;
+        }
+        return vals[i];
+    }
+    void rehash(size_t cap) {
+        std::vector<uint64_t> ok;
+        std::vector<V> ov;
+        ok.swap(keys);
+        ov.swap(vals);
+        keys.assign(cap, 0);
+        vals.assign(cap, V{});
+        shift = 64 - __builtin_ctzll((unsigned long long)cap);
+        count = 0;
+        for (size_t j = 0; j < ok.size(); ++j)
+            if (ok[j]) at(ok[j]) = ov[j];
+    }
+    void clear() {
+        keys.clear();
+        vals.clear();
+        count = 0;
+        shift = 64;
+    }
+    size_t size() const { return count; }
+    template <class F>
+    void for_each(F&& f) const {
+        for (size_t j = 0; j < keys.size(); ++j)
+            if (keys[j]) f(keys[j], vals[j]);
+    }
+    bool operator==(const FlatMap64& o) const {
+        if (count != o.count) return false;
+        bool same = true;
+        for_each([&](uint64_t k, const V& v) {
+            const V* w = o.find(k);
+            same = same && w && *w == v;
+        });
+        return same;
+    }
+};
+
 struct DerivedColumns {
     int32_t base = 0;  // first derived column (the dataset's column count)
     int32_t cap = 0;   // most derived columns (0: off)
@@ -185,7 +254,7 @@ struct DerivedColumns {
                                // populations still read the columns: the batched evaluator)
     struct Col { uint32_t op; int32_t src; };
     std::vector<Col> cols;                          // column base + j = op(column src), src < base + j
-    std::unordered_map<uint64_t, int32_t> index;    // chain key -> column
+    FlatMap64<int32_t> index;                       // chain key -> column
     void reset(int32_t b, int32_t c) {
         base = b;
         cap = c;
@@ -195,8 +264,8 @@ struct DerivedColumns {
     }
     // Column holding the chain, or -1.
     int32_t find(uint64_t key) const {
-        auto it = index.find(key);
-        return it == index.end() ? -1 : it->second;
+        const int32_t* c = index.find(key);
+        return c ? *c : -1;
     }
 };
 
