@@ -34,8 +34,11 @@ constexpr int kBlock = 64 * kWaves;
 // Waves per SIMD the register allocator must leave room for (the second
 // __launch_bounds__ argument caps VGPRs at 512 / waves) in the FP and batched
 // forms.  Measured on gfx950 with the regression workload (ms per launch):
-// 4 rows per lane with the 4-slot stack 6 waves (80 VGPRs, no spills; 6.8%
-// faster than 5 waves, and 7 waves spill), with the 9-slot stack 5 (96 VGPRs;
+// 4 rows per lane with the 4-slot stack 7 waves (72 VGPRs; the spills left
+// sit after exact-math calls and in the non-finite epilogue, and the launch
+// is 1.5% faster than at 6 waves, which was 6.8% faster than 5), but 6 waves
+// in the persistent batched kernel (its queue state spills on the hot path at
+// 7), with the 9-slot stack 5 (96 VGPRs;
 // 10% faster than 4 waves without the bound); 2 rows per lane 5 (76 VGPRs: 6
 // waves run); 1 row per lane (the 16-slot fallback) takes 4; at 8 rows any
 // bound spills, so none is set.  More waves hide more of the scalar dispatch.
@@ -44,9 +47,9 @@ constexpr int kBlock = 64 * kWaves;
 // tables): the bound never asks for more, since spilling for waves that LDS
 // cannot hold only costs (32 feature slots at 4 rows: 4 waves).  The PDP form
 // keeps its rows' features in VGPRs and is left unbounded.
-template <int R, int F, int S>
+template <int R, int F, int S, bool Queue = false>
 struct MinWaves {
-    static constexpr int pref = R == 4 ? (S <= 4 ? 6 : 5) : (R == 2 ? 5 : (R == 1 ? 4 : 1));
+    static constexpr int pref = R == 4 ? (S <= 4 ? (Queue ? 6 : 7) : 5) : (R == 2 ? 5 : (R == 1 ? 4 : 1));
     static constexpr int lds = (F > 0 ? F : 1) * 64 * R * 4 + (int)sizeof(ExactTables) + 64;
     static constexpr int fit = (160 * 1024) / lds;
     static constexpr int value = pref < fit ? pref : (fit > 0 ? fit : 1);
@@ -106,10 +109,10 @@ struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP for
     typedef typename VecT<R>::type VR;
     typedef __attribute__((address_space(3))) const VR LdsVR;
     LdsVR* base;  // &tile[0][lane][0]: a lane's R rows of column c are one vector at base[c * 64]
-    const float* __restrict__ X;
+    const float* __restrict__ X0;  // the tile's first row of column 0 (wave-uniform)
     int64_t ldx;
     uint32_t ncol;
-    const int (&row)[R];
+    int lane;
     __device__ __forceinline__ void get(uint32_t col, float (&b)[R]) const {
         if (col < ncol) {
             // One typed LDS vector load (ds_read_b128 at R = 4); kept apart from the global
@@ -117,18 +120,36 @@ struct LdsFeat {  // tile shared by the workgroup in LDS, [col][lane][r] (FP for
             const VR v = base[col * 64];
 #pragma unroll
             for (int r = 0; r < R; ++r) b[r] = v[r];
-        } else {
+        } else if constexpr (R <= 4) {
+            // A buffer load: the column's first row of the tile as a wave-uniform descriptor
+            // (third word 0x00020000, the gfx9 raw-buffer format) and a 32-bit lane offset, so no
+            // 64-bit address per lane is held in registers (at 4 rows per lane this is what lets
+            // 7 waves per SIMD fit without a spill on the hot path).  The descriptor's range is
+            // the tile's 64 R rows, all inside the dataset, whose columns are padded to a
+            // multiple of 2,048 rows.
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(X0 + (int64_t)col * ldx), (short)0, 64 * R * 4, 0x00020000);
 #pragma unroll
-            for (int r = 0; r < R; ++r) b[r] = X[(int64_t)col * ldx + row[r]];
+            for (int r = 0; r < R; ++r)
+                b[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, 256 * r, 0));
+        } else {
+            const float* __restrict__ c = X0 + (int64_t)col * ldx + lane;
+#pragma unroll
+            for (int r = 0; r < R; ++r) b[r] = c[64 * r];
         }
     }
 };
 
 template <int R, int F>
 __device__ __forceinline__ auto make_feat(const EvalArgs& A, const float* lds_lane, const int (&row)[R]) {
-    if constexpr (F > 0)
-        return LdsFeat<R, F>{(typename LdsFeat<R, F>::LdsVR*)lds_lane, A.X, A.ldx, (uint32_t)min(A.ncol, F), row};
-    else return GlobalFeat<R>{A.X, A.ldx, row};
+    if constexpr (F > 0) {
+        const int lane = (int)(threadIdx.x & 63);
+        const int64_t row0 = __builtin_amdgcn_readfirstlane(row[0] - lane);  // row[r] = row0 + 64 r + lane
+        return LdsFeat<R, F>{(typename LdsFeat<R, F>::LdsVR*)lds_lane, A.X + row0, A.ldx, (uint32_t)min(A.ncol, F),
+                             lane};
+    } else {
+        return GlobalFeat<R>{A.X, A.ldx, row};
+    }
 }
 
 // Register stack of S slots for each of a lane's R rows.  The slot index is
@@ -736,7 +757,7 @@ __global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S>::value)) fp_tile_ke
 // workgroup leaves the loop when the counter passes the end.
 // ---------------------------------------------------------------------------
 template <int R, int F, int S, bool P, bool N>
-__global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S>::value)) batch_kernel(BatchArgs B) {
+__global__ void __launch_bounds__(kBlock, (MinWaves<R, F, S, true>::value)) batch_kernel(BatchArgs B) {
     constexpr int FL = F > 0 ? F : 1;
     __shared__ __attribute__((aligned(16))) float tile[FL][64][R];
     __shared__ int item_s;
