@@ -182,7 +182,7 @@ def shape_summary(b, n_pairs, w):
         e["node_evals"] += float(w.pops[k].total_size) * w.datas[k].shape[0]
     for e in per.values():
         e["node_evals"] = float(f"{e['node_evals']:.4g}")
-    return {"launches": st.n_shapes, "tuned_keys": st.tuned_keys, "by_shape": per}
+    return {"launches": st.n_shapes, "direct_pairs": st.direct_pairs, "tuned_keys": st.tuned_keys, "by_shape": per}
 
 
 def batch_generations(w, gens):

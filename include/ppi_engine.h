@@ -253,6 +253,7 @@ typedef struct ppi_batch_stats {
     int32_t n_shapes;       /* persistent launches per replay: one per kernel shape in use */
     int32_t tuned_keys;     /* tuning keys (stack class x program length x rows) timed by this build
                                (keys decided by an earlier build of the batch keep their shape) */
+    int32_t direct_pairs;   /* pairs evaluated by their own FP launch instead of the persistent queue */
 } ppi_batch_stats;
 
 ppi_status ppi_batch_create(ppi_batch** out, int32_t device);
@@ -280,6 +281,12 @@ ppi_status ppi_batch_get_stats(const ppi_batch* b, ppi_batch_stats* out);
    feature slots, register-stack slots (any pointer may be NULL).  No reference counterpart. */
 ppi_status ppi_batch_pair_shape(const ppi_batch* b, int32_t pop_id, int32_t* rows_per_lane,
                                 int32_t* feature_slots, int32_t* stack_slots);
+/* Which pairs the next build() evaluates by their own FP launch (the engine's kernel and
+   chunking) instead of the persistent queue: those with at least `min_items` work items of
+   (row tile, program), row tiles of 64 x rows-per-lane rows.  0 = the default (96 x 64 items
+   per compute unit: the pair's launch fills the device), a negative value = none, 1 = every
+   pair.  Results are the same either way.  No reference counterpart. */
+ppi_status ppi_batch_set_direct_threshold(ppi_batch* b, int64_t min_items);
 
 /*
  * Reverse-mode derivatives of the fitness with respect to the ephemeral constants.

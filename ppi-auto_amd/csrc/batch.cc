@@ -152,6 +152,12 @@ struct ppi_batch {
     std::vector<WorkItem> items;
     std::vector<Shape> pair_shape;  // per pair (population)
     std::vector<Group> groups;
+    // Pairs evaluated by plain FP launches (the engine's kernel and chunking) instead of the queue:
+    // those whose own launch fills the device (direct_min_items work items of a row tile x 64
+    // programs or more); their arguments, built at build().
+    int64_t direct_min_items = 0;  // 0: the default; < 0: none
+    struct Direct { EvalArgs A; Shape shape; };
+    std::vector<Direct> direct;
     // The autotuner's decisions by key (stack class, length class, rows class), kept across
     // builds until configure(): a rebuild with new populations times only keys it has not seen.
     std::vector<std::pair<int, Shape>> tuned;
@@ -439,9 +445,12 @@ ppi_status enqueue_recheck(ppi_batch* b, const BatchArgs& B, int32_t n) {
     return PPI_OK;
 }
 
-// The persistent launches of the groups, back to back, each with its own queue counter
-// (reset beforehand by the caller).
-ppi_status enqueue_groups(ppi_batch* b, const std::vector<Group>& groups) {
+// The direct pairs' FP launches (with_direct), then the persistent launches of the groups, back
+// to back, each with its own queue counter (reset beforehand by the caller).
+ppi_status enqueue_groups(ppi_batch* b, const std::vector<Group>& groups, bool with_direct = true) {
+    if (with_direct)
+        for (const auto& d : b->direct)
+            B_HIP(launch_fp(d.A, d.shape.R, d.shape.F, d.shape.S, b->P, b->N, b->stream));
     for (size_t g = 0; g < groups.size(); ++g) {
         const Group& G = groups[g];
         if (G.n_items == 0) continue;
@@ -454,13 +463,14 @@ ppi_status enqueue_groups(ppi_batch* b, const std::vector<Group>& groups) {
 }
 
 // Queue resets, persistent kernels, epilogue (optionally the download).
-ppi_status enqueue_run(ppi_batch* b, const std::vector<Group>& groups, bool with_finalize, bool with_download) {
+ppi_status enqueue_run(ppi_batch* b, const std::vector<Group>& groups, bool with_finalize, bool with_download,
+                       bool with_direct = true) {
     const int32_t n = total_programs(b);
     BatchArgs B = make_args(b, 0);
     B_HIP(launch_fill_u32(b->dBadLo, 0xffffffffu, n, b->stream));
     B_HIP(launch_fill_u32(b->dBadHi, 0u, n, b->stream));
     B_HIP(launch_fill_u32(b->dCounter, 0u, (int32_t)groups.size(), b->stream));
-    ppi_status gs = enqueue_groups(b, groups);
+    ppi_status gs = enqueue_groups(b, groups, with_direct);
     if (gs) return gs;
     if (with_finalize) {
         B_HIP(launch_batch_finalize(B, b->dPairOf, n, b->stream));
@@ -775,14 +785,14 @@ ppi_status ppi_batch_build(ppi_batch* b) {
             plan_items(b, inc, items_per_slot, &b->pairs, &b->items, &b->groups, &part, &rows_of);
             const double work = items_work(b, b->items.data(), b->items.size());
             if ((s = upload_items(b))) return s;
-            if ((s = enqueue_run(b, b->groups, false, false))) return s;  // warm-up
+            if ((s = enqueue_run(b, b->groups, false, false, false))) return s;  // warm-up
             // Repeat short samples so that the timed span is well above launch jitter.
             int reps = 1;
             float ms = 0.0f;
             for (;;) {
                 B_HIP(hipEventRecord(b->ev[2], b->stream));
                 for (int r = 0; r < reps; ++r)
-                    if ((s = enqueue_run(b, b->groups, false, false))) return s;
+                    if ((s = enqueue_run(b, b->groups, false, false, false))) return s;
                 B_HIP(hipEventRecord(b->ev[3], b->stream));
                 B_HIP(hipEventSynchronize(b->ev[3]));
                 B_HIP(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
@@ -810,8 +820,47 @@ ppi_status ppi_batch_build(ppi_batch* b) {
             for (int i = 0; i < 3; ++i) b->stats.tune_ms[i] = tms[i];
         }
     }
-    std::vector<char> all(np, 1);
-    plan_items(b, all, items_per_slot, &b->pairs, &b->items, &b->groups, &part);
+    // Pairs whose own FP launch fills the device run as one: the engine's kernel, chunking and
+    // XCD-aware tile order over the pair alone, with the partials in the pair's block (the layout
+    // the queue's items write), so the epilogue below is the same for both.  The queue keeps the
+    // rest, where its packing of many pairs into one launch pays.  On the 64-pair workload the
+    // pairs' FP launches run at the plain engine's rate (7 waves per SIMD against the persistent
+    // kernel's 6).
+    std::vector<char> queued(np, 1);
+    b->direct.clear();
+    const int64_t min_items = b->direct_min_items != 0 ? b->direct_min_items : (int64_t)b->dev_cu * 96 * 64;
+    if (min_items > 0) {
+        for (size_t k = 0; k < np; ++k) {
+            const Shape& sh = b->pair_shape[k];
+            const Pop& p = b->pops[k];
+            const Dataset& d = b->datasets[p.dataset];
+            const int64_t nrb = (d.nlin + 64 * sh.R - 1) / (64 * sh.R);
+            if ((int64_t)p.n * nrb < min_items || !shape_supported(sh.R, sh.F, sh.S)) continue;
+            queued[k] = 0;
+        }
+    }
+    plan_items(b, queued, items_per_slot, &b->pairs, &b->items, &b->groups, &part);
+    for (size_t k = 0; k < np; ++k) {
+        if (queued[k]) continue;
+        const PairDev& pd = b->pairs[k];
+        EvalArgs A = make_args(b, 0).base;  // the per-pair fields as batch_kernel sets them
+        A.meta += pd.prog_base;
+        A.X = pd.X;
+        A.ldx = pd.ldx;
+        A.nlin = pd.nlin;
+        A.ncol = pd.ncol;
+        A.ycol = pd.ncol - 1;
+        A.nprog = pd.nprog;
+        A.pdp_lanes = pd.pdp_lanes;
+        A.dp_groups = pd.dp_groups;
+        A.partial += pd.partial_base;
+        A.bad_lo += pd.prog_base;
+        A.bad_hi += pd.prog_base;
+        A.chunk = fp_default_chunk(pd.nprog, pd.nrb, b->dev_cu);
+        A.rb0 = 0;
+        A.nrb_pass = pd.nrb;
+        b->direct.push_back({A, b->pair_shape[k]});
+    }
     if ((s = upload_items(b))) return s;
     if ((int)b->groups.size() > kMaxGroups) { set_error("too many kernel shapes"); return PPI_E_PROGRAM; }
 
@@ -828,10 +877,12 @@ ppi_status ppi_batch_build(ppi_batch* b) {
     b->built = true;
     b->stats.n_pairs = (int32_t)b->pops.size();
     b->stats.n_items = (int32_t)b->items.size();
-    // The group carrying the most work (groups are sorted by work).
-    b->stats.grid = b->groups[0].grid;
-    b->stats.rows_per_lane = b->groups[0].shape.R;
-    b->stats.feature_slots = b->groups[0].shape.F;
+    b->stats.direct_pairs = (int32_t)b->direct.size();
+    // The group carrying the most work (groups are sorted by work), else the first direct pair.
+    const Shape top = b->groups.empty() ? b->direct[0].shape : b->groups[0].shape;
+    b->stats.grid = b->groups.empty() ? 0 : b->groups[0].grid;
+    b->stats.rows_per_lane = top.R;
+    b->stats.feature_slots = top.F;
     b->stats.n_shapes = (int32_t)b->groups.size();
     double w = 0.0;
     for (auto& p : b->pops) w += (double)p.total_size * b->datasets[p.dataset].nlin;
@@ -935,6 +986,12 @@ ppi_status ppi_batch_pair_shape(const ppi_batch* b, int32_t pop_id, int32_t* row
     if (rows_per_lane) *rows_per_lane = sh.R;
     if (feature_slots) *feature_slots = sh.F;
     if (stack_slots) *stack_slots = sh.S;
+    return PPI_OK;
+}
+
+ppi_status ppi_batch_set_direct_threshold(ppi_batch* b, int64_t min_items) {
+    if (!b) { set_error("batch is NULL"); return PPI_E_ARG; }
+    b->direct_min_items = min_items;
     return PPI_OK;
 }
 

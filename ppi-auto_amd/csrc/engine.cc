@@ -229,6 +229,24 @@ static void reset_derived(ppi_engine* E);
 
 namespace ppi {
 
+// Programs per FP tile: about 96 workgroups per CU, so that the launch ends in a short tail (a
+// workgroup is a row tile x a chunk; on the regression workload 1,024 programs per chunk, 30
+// workgroups per CU, measured 3% slower than 384-512), and at most 400 programs (at 7 waves per
+// SIMD 400 measured 0.4% faster than 477 on the regression workload and 0.8% faster than 715 on
+// the classification one, with 600 and more slower on both); at least 64 programs where that still
+// leaves 4 workgroups per CU (fewer, and the LDS tile is staged for too little work); then equal
+// chunks.
+int32_t fp_default_chunk(int64_t nprog, int64_t nrb, int cu) {
+    const int64_t items = nprog * nrb;
+    const int64_t target = (int64_t)cu * 96;
+    const int64_t floor = items >= (int64_t)cu * 4 * 64 ? 64 : 8;
+    const int64_t per = (items + target - 1) / target;
+    int64_t chunk = std::max<int64_t>(floor, std::min<int64_t>(kDefaultMaxChunk, per));
+    const int64_t nch = (nprog + chunk - 1) / chunk;
+    chunk = std::max<int64_t>(1, (nprog + nch - 1) / std::max<int64_t>(1, nch));
+    return (int32_t)std::min<int64_t>(chunk, kMaxChunk);
+}
+
 bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t nprog, int requested_form,
                          LaunchPlan* L, std::string* why) {
     int need_f = cs.max_col + 1;
@@ -269,23 +287,10 @@ bool compute_plan(const ppi_engine* E, const CompileStats& cs, int32_t nprog, in
     L->R = R;
     L->F = F;
     L->S = S;
-    // Programs per FP tile: about 96 workgroups per CU, so that the launch ends in a short tail
-    // (a workgroup is a row tile x a chunk; on the regression workload 1,024 programs per chunk,
-    // 30 workgroups per CU, measured 3% slower than 384-512), at least 64 programs where that
-    // still leaves 4 workgroups per CU (fewer, and the LDS tile is staged for too little work),
-    // then equal chunks.
     int64_t rows_per_block = fp_rows_per_block(R);
     int64_t nrb = (E->nlin + rows_per_block - 1) / rows_per_block;
     int chunk = E->cfg.programs_per_tile;
-    if (chunk <= 0) {
-        const int64_t items = (int64_t)nprog * nrb;
-        const int64_t target = (int64_t)E->dev_cu * 96;
-        const int64_t floor = items >= (int64_t)E->dev_cu * 4 * 64 ? 64 : 8;
-        const int64_t per = (items + target - 1) / target;
-        chunk = (int)std::max<int64_t>(floor, std::min<int64_t>(kMaxChunk, per));
-        const int64_t nch = ((int64_t)nprog + chunk - 1) / chunk;
-        chunk = (int)std::max<int64_t>(1, ((int64_t)nprog + nch - 1) / nch);
-    }
+    if (chunk <= 0) chunk = fp_default_chunk(nprog, nrb, E->dev_cu);
     L->chunk = std::min(chunk, kMaxChunk);
     L->nrb = (int)nrb;
     return true;

@@ -20,6 +20,9 @@ int32_t derived_capacity(int64_t ldx, int32_t ncol);
 ppi_status stage_derived_columns(float** X, size_t* cap_floats, int64_t ldx, int32_t ncol, DerivedColumns& table,
                                  int32_t* ready, uint32_t** dOps, int32_t** dSrc, size_t* cap_list, bool P, bool N,
                                  hipStream_t st);
+// Programs per FP workgroup of the default plan for nprog programs over nrb row tiles on a device
+// of cu compute units (the FP form's launch is nrb x ceil(nprog / chunk) workgroups).
+int32_t fp_default_chunk(int64_t nprog, int64_t nrb, int cu);
 // Device buffers of a tape's fold programs (tape.h: FoldSet), kept across uploads.
 struct FoldBuffers {
     ProgMeta* meta = nullptr;

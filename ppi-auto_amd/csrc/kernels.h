@@ -12,6 +12,7 @@
 namespace ppi {
 
 constexpr int kMaxChunk = 1024;  // programs per FP workgroup
+constexpr int kDefaultMaxChunk = 400;  // the default plan's cap on programs per FP workgroup
 constexpr int kPdpMaxLanes = 1024;  // PDP / DP work-group width at most (the reference's -cl-mls)
 
 // Rows of one FP-form workgroup tile (64 lanes x R rows per lane, shared by its waves).

@@ -76,6 +76,7 @@ class BatchStats(C.Structure):
         ("n_pairs", C.c_int32), ("n_items", C.c_int32), ("grid", C.c_int32), ("rows_per_lane", C.c_int32),
         ("feature_slots", C.c_int32), ("autotuned", C.c_int32), ("tune_ms", C.c_double * 4),
         ("derived_columns", C.c_int32), ("n_shapes", C.c_int32), ("tuned_keys", C.c_int32),
+        ("direct_pairs", C.c_int32),
     ]
 
 
@@ -149,6 +150,7 @@ ENGINE_SYMBOLS = [
     "ppi_engine_time", "ppi_batch_create", "ppi_batch_destroy", "ppi_batch_configure", "ppi_batch_add_dataset",
     "ppi_batch_add_population", "ppi_batch_clear_populations", "ppi_batch_build", "ppi_batch_launch",
     "ppi_batch_wait", "ppi_batch_results", "ppi_batch_time", "ppi_batch_get_stats", "ppi_batch_pair_shape",
+    "ppi_batch_set_direct_threshold",
     "ppi_adjoint_create", "ppi_adjoint_destroy", "ppi_adjoint_configure", "ppi_adjoint_set_dataset_rowmajor",
     "ppi_adjoint_forward", "ppi_adjoint_backward", "ppi_adjoint_descend", "ppi_adjoint_time",
     "ppi_adjoint_get_stats", "ppi_adjoint_set_rows_per_lane", "ppi_ef_evaluate",
@@ -212,6 +214,7 @@ def lib() -> C.CDLL:
         L.ppi_batch_time.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
         L.ppi_batch_get_stats.argtypes = [C.c_void_p, P(BatchStats)]
         L.ppi_batch_pair_shape.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32)]
+        L.ppi_batch_set_direct_threshold.argtypes = [C.c_void_p, C.c_int64]
         L.ppi_adjoint_create.argtypes = [P(C.c_void_p), C.c_int32]
         L.ppi_adjoint_destroy.argtypes = [C.c_void_p]
         L.ppi_adjoint_destroy.restype = None
@@ -498,6 +501,11 @@ class Batch:
     def clear_populations(self):
         _check(lib().ppi_batch_clear_populations(self._h))
         self.sizes = []
+
+    def set_direct_threshold(self, min_items: int):
+        """Pairs with at least min_items (row tile, program) items run as their own FP launch at the
+        next build(): 0 = default (the pair fills the device), negative = none, 1 = every pair."""
+        _check(lib().ppi_batch_set_direct_threshold(self._h, min_items))
 
     def build(self):
         _check(lib().ppi_batch_build(self._h))

@@ -18,11 +18,12 @@ def small():
     return w
 
 
-def run_batch(w, rows_per_lane, **kw):
+def run_batch(w, rows_per_lane, direct=0, **kw):
     b = ppiamd.Batch(0)
     b.configure(w.grammars[0].table, strategy=kw.pop("strategy", ppiamd.SEQ), error_fn=w.error_fn,
                 rows_per_lane=rows_per_lane, compute_units=256, **kw)
     ids = [b.add_population(b.add_dataset(d), p) for p, d in zip(w.pops, w.datas)]
+    b.set_direct_threshold(direct)
     b.build()
     b.run()
     return b, ids
@@ -43,6 +44,39 @@ def test_batched_equals_plain_fp_form_bit_for_bit(small):
         pv, pidx = eng.evaluate(p, alpha=ALPHA, best_size=2)
         assert np.array_equal(vec.view(np.uint32), pv.view(np.uint32)), f"pair {k}"
         assert np.array_equal(idx, pidx)
+    eng.close()
+    b.close()
+
+
+@pytest.mark.parametrize("strategy", [ppiamd.SEQ, ppiamd.PDP])
+@pytest.mark.parametrize("direct", [1, 3000, -1])
+def test_direct_pairs_equal_plain(small, direct, strategy):
+    """Pairs run as their own FP launch (all: threshold 1; the larger ones: 3,000 items; none: -1)
+    give the plain engine's FP-form results bit for bit, through the batch's own epilogue."""
+    w = small
+    b, ids = run_batch(w, 4, direct=direct, strategy=strategy)
+    st = b.stats()
+    items = [p.n * ((d.shape[0] + 255) // 256) for p, d in zip(w.pops, w.datas)]
+    want = len(items) if direct == 1 else (0 if direct < 0 else sum(i >= direct for i in items))
+    print(f"direct={st.direct_pairs} of {len(items)} items={items} launches={st.n_shapes}", flush=True)
+    assert st.direct_pairs == want
+    if direct == 3000:
+        assert 0 < want < len(items)
+    eng = ppiamd.Engine(0)
+    for k, (p, d) in enumerate(zip(w.pops, w.datas)):
+        vec, idx = b.results(ids[k], alpha=ALPHA, best_size=2)
+        eng.configure(w.grammars[k].table, strategy=strategy, error_fn=w.error_fn, form=ppiamd.FORM_FP,
+                      rows_per_lane=4, compute_units=256)
+        eng.set_dataset(d)
+        pv, pidx = eng.evaluate(p, alpha=ALPHA, best_size=2)
+        assert np.array_equal(vec.view(np.uint32), pv.view(np.uint32)), f"pair {k}"
+        assert np.array_equal(idx, pidx)
+    # a second replay gives the same bits
+    first = [b.results(i, alpha=ALPHA)[0] for i in ids]
+    b.run()
+    for k in range(len(ids)):
+        again, _ = b.results(ids[k], alpha=ALPHA)
+        assert np.array_equal(again.view(np.uint32), first[k].view(np.uint32))
     eng.close()
     b.close()
 
