@@ -71,11 +71,14 @@ __device__ __forceinline__ void stage_exact_tables() {
 }
 
 // Ziv's test for rounding a double y (with |y| in the float normal range, or 0)
-// to float: true when bits 0-28 of y's significand are more than kGuard away from
-// the midpoint pattern 0x10000000.
+// to float: true when bits 0-28 of y's significand (lo) are more than kGuard away
+// from the midpoint pattern 0x10000000, i.e. lo outside [2^28 - G, 2^28 + G].
+// With w = (lo + 2^28 + G) mod 2^29 that is w > 2G (lo below the band wraps to
+// w >= 2^28 + G), and 8w = (8 y.lo + 2^31 + 8G) mod 2^32 exactly: one shift-add
+// and one compare per row instead of a mask, an add and a compare.
 __device__ __forceinline__ bool rounds_safely(double y) {
-    const uint32_t lo = (uint32_t)__double2loint(y) & 0x1fffffffu;
-    return lo - (0x10000000u - kGuard) > 2u * kGuard;
+    const uint32_t w8 = (uint32_t)__double2loint(y) * 8u + (0x80000000u + 8u * kGuard);
+    return w8 > 16u * kGuard;
 }
 
 // exp.  x = k ln2/128 + r with |r| <= ln2/256 (+ rounding); exp(x) = 2^(k>>7) T[k&127] (1 + p(r)).
