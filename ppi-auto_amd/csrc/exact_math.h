@@ -101,7 +101,9 @@ __device__ __forceinline__ float exact_exp_fast(float x, bool& ok) {
     float res = (float)y;
     if (x >= 128.0f) res = __builtin_inff();
     if (x <= -104.0f) res = 0.0f;
-    ok = (x > -87.0f && rounds_safely(y)) || x >= 128.0f || x <= -104.0f;
+    // Bitwise rather than short-circuit operators: the same value, without the exec-mask
+    // branches the compiler builds around a short-circuited rounds_safely.
+    ok = ((x > -87.0f) & rounds_safely(y)) | (x >= 128.0f) | (x <= -104.0f);
     return res;
 }
 
