@@ -238,7 +238,7 @@ typedef struct ppi_batch ppi_batch;
 
 typedef struct ppi_batch_stats {
     double graph_ms;        /* last replay: whole graph (reset, kernel, finalize, D2H), device time */
-    double kernel_ms;       /* persistent kernel alone (ppi_batch_time) */
+    double kernel_ms;       /* evaluation kernels alone: direct FP launches + persistent queue (ppi_batch_time) */
     double node_evals;      /* sum over pairs of sum(size) * nlin */
     double build_ms;        /* compile + upload + plan + autotune + capture */
     int32_t n_pairs;

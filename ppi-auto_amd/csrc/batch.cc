@@ -953,7 +953,8 @@ ppi_status ppi_batch_time(ppi_batch* b, int32_t iters, double* ms_per_run) {
     B_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     *ms_per_run = ms / iters;
     b->stats.graph_ms = *ms_per_run;
-    // The persistent kernels alone, outside the graph, for the roofline figure.
+    // The evaluation kernels alone (direct FP launches and the persistent queue), outside the graph, for the
+    // roofline figure.
     double kms = 0.0;
     for (int32_t k = 0; k < iters; ++k) {
         const int32_t n = total_programs(b);
