@@ -25,6 +25,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ppi_engine.h"
@@ -147,6 +148,10 @@ struct ppi_batch {
     int32_t deep_rows = 0;
     float* hRaw = nullptr;
     size_t hRawCap = 0;
+    bool use_deep = false;  // this build queues deep-list programs (a dataset of kDeepMinRows rows or more)
+    // Element capacity of each per-build device buffer above, by the buffer member's address: the
+    // buffers are kept across builds and grown only when a build needs more.
+    std::unordered_map<const void*, size_t> cap;
 
     std::vector<PairDev> pairs;
     std::vector<WorkItem> items;
@@ -170,11 +175,20 @@ struct ppi_batch {
 
 namespace {
 
+// The graph of the last build.  The per-build device buffers stay allocated for the next build
+// (free_buffers releases them): freeing and re-allocating them every generation — the partials
+// alone can be hundreds of MB — cost a device synchronization and fresh page mappings each time.
 void free_build(ppi_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+    b->deep_stride = 0; b->deep_rows = 0;
+    b->use_deep = false;
+    b->built = false;
+}
+
+void free_buffers(ppi_batch* b) {
     // (the queue counter lives as long as the batch object)
     void* ptrs[] = {b->dMeta, b->dCode, b->dPairs, b->dItems, b->dPairOf, b->dPartial, b->dBadLo, b->dBadHi,
                     b->dRaw, b->dVec, b->dDeepCount, b->dDeepProg, b->dDeepErr};
@@ -183,8 +197,17 @@ void free_build(ppi_batch* b) {
     b->dMeta = nullptr; b->dCode = nullptr; b->dPairs = nullptr; b->dItems = nullptr; b->dPairOf = nullptr;
     b->dPartial = nullptr; b->dBadLo = nullptr; b->dBadHi = nullptr; b->dRaw = nullptr; b->dVec = nullptr;
     b->dDeepCount = nullptr; b->dDeepProg = nullptr; b->dDeepErr = nullptr;
-    b->deep_stride = 0; b->deep_rows = 0;
-    b->built = false;
+    b->cap.clear();
+}
+
+// A per-build buffer of at least n elements: kept if it already holds that many, else replaced.
+template <typename T>
+ppi_status dgrow(ppi_batch* b, T** p, size_t n) {
+    size_t& c = b->cap[(const void*)p];
+    if (*p && n <= c) return PPI_OK;
+    ppi_status s = dalloc(p, n);
+    c = s == PPI_OK ? std::max<size_t>(n, 1) : 0;
+    return s;
 }
 
 // LDS feature slots of the persistent kernel: 16 whatever the pairs' widths.  Wider datasets read
@@ -388,8 +411,8 @@ ppi_status compile_pending(ppi_batch* b) {
 
 ppi_status upload_items(ppi_batch* b) {
     ppi_status s;
-    if ((s = dalloc(&b->dPairs, b->pairs.size()))) return s;
-    if ((s = dalloc(&b->dItems, b->items.size()))) return s;
+    if ((s = dgrow(b, &b->dPairs, b->pairs.size()))) return s;
+    if ((s = dgrow(b, &b->dItems, b->items.size()))) return s;
     B_HIP(hipMemcpy(b->dPairs, b->pairs.data(), b->pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice));
     B_HIP(hipMemcpy(b->dItems, b->items.data(), b->items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     return PPI_OK;
@@ -412,7 +435,7 @@ BatchArgs make_args(ppi_batch* b, int32_t n_items) {
     A.bad_hi = b->dBadHi;
     A.raw = b->dRaw;
     A.vec = b->dVec;
-    if (b->dDeepErr) {
+    if (b->use_deep) {
         A.deep_count = b->dDeepCount;
         A.deep_prog = b->dDeepProg;
         A.deep_cap = kDeepCap;
@@ -519,6 +542,7 @@ void ppi_batch_destroy(ppi_batch* b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     free_build(b);
+    free_buffers(b);
     if (b->dCounter) (void)hipFree(b->dCounter);
     for (auto& d : b->datasets) {
         if (d.dX) (void)hipFree(d.dX);
@@ -657,20 +681,20 @@ ppi_status ppi_batch_build(ppi_batch* b) {
 
     std::vector<Ins> code = b->tape.code;
     code.insert(code.end(), kTapePad, Ins{H_END, kEmpty});
-    if ((s = dalloc(&b->dMeta, b->tape.meta.size()))) return s;
-    if ((s = dalloc(&b->dCode, code.size()))) return s;
+    if ((s = dgrow(b, &b->dMeta, b->tape.meta.size()))) return s;
+    if ((s = dgrow(b, &b->dCode, code.size()))) return s;
     B_HIP(hipMemcpy(b->dMeta, b->tape.meta.data(), b->tape.meta.size() * sizeof(ProgMeta), hipMemcpyHostToDevice));
     B_HIP(hipMemcpy(b->dCode, code.data(), code.size() * sizeof(Ins), hipMemcpyHostToDevice));
     std::vector<int32_t> pair_of((size_t)n);
     for (size_t k = 0; k < np; ++k)
         std::fill(pair_of.begin() + b->pops[k].prog_base, pair_of.begin() + b->pops[k].prog_base + b->pops[k].n,
                   (int32_t)k);
-    if ((s = dalloc(&b->dPairOf, (size_t)n))) return s;
+    if ((s = dgrow(b, &b->dPairOf, (size_t)n))) return s;
     B_HIP(hipMemcpy(b->dPairOf, pair_of.data(), pair_of.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    if ((s = dalloc(&b->dBadLo, (size_t)n))) return s;
-    if ((s = dalloc(&b->dBadHi, (size_t)n))) return s;
-    if ((s = dalloc(&b->dRaw, (size_t)n))) return s;
-    if ((s = dalloc(&b->dVec, (size_t)n))) return s;
+    if ((s = dgrow(b, &b->dBadLo, (size_t)n))) return s;
+    if ((s = dgrow(b, &b->dBadHi, (size_t)n))) return s;
+    if ((s = dgrow(b, &b->dRaw, (size_t)n))) return s;
+    if ((s = dgrow(b, &b->dVec, (size_t)n))) return s;
     if (b->hRawCap < (size_t)n) {
         if (b->hRaw) (void)hipHostFree(b->hRaw);
         b->hRaw = nullptr;
@@ -712,7 +736,7 @@ ppi_status ppi_batch_build(ppi_batch* b) {
         const Dataset& d = b->datasets[b->pops[k].dataset];
         max_part += (int64_t)((d.nlin + 64 * rmin - 1) / (64 * rmin)) * b->pops[k].n;
     }
-    if ((s = dalloc(&b->dPartial, (size_t)max_part))) return s;
+    if ((s = dgrow(b, &b->dPartial, (size_t)max_part))) return s;
     b->deep_rows = 0;
     b->deep_stride = 0;
     for (const Pop& p : b->pops) {
@@ -721,9 +745,10 @@ ppi_status ppi_batch_build(ppi_batch* b) {
         b->deep_stride = std::max(b->deep_stride, d.ldx);
     }
     if (b->deep_rows >= kDeepMinRows) {
-        if ((s = dalloc(&b->dDeepCount, 1))) return s;
-        if ((s = dalloc(&b->dDeepProg, (size_t)kDeepCap))) return s;
-        if ((s = dalloc(&b->dDeepErr, (size_t)kDeepCap * b->deep_stride))) return s;
+        if ((s = dgrow(b, &b->dDeepCount, 1))) return s;
+        if ((s = dgrow(b, &b->dDeepProg, (size_t)kDeepCap))) return s;
+        if ((s = dgrow(b, &b->dDeepErr, (size_t)kDeepCap * b->deep_stride))) return s;
+        b->use_deep = true;
     }
 
     // Autotuner.  Pairs are keyed by (stack class, mean tape length per program, rows):
